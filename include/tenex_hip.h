@@ -1,0 +1,277 @@
+/*
+ * tenex_hip.h -- C ABI of libtenex_hip.so, the MI355X (gfx950) transfer-entropy
+ * engine.
+ *
+ * Conventions
+ *  - Every entry point returns int: 0 = OK, negative = error; the message is
+ *    in tnx_last_error() (thread-local, valid until the next call on that
+ *    thread).
+ *  - All array arguments are device pointers unless the comment says "host".
+ *  - The library never allocates or frees device memory and never
+ *    synchronises: callers own every buffer and pass a hipStream_t as void*.
+ *    Kernels run on that stream, on the calling thread's current device.
+ *  - Bins are dense per variable (0 <= bins[v, t] < n_per_var[v] <= 127),
+ *    int8, K = 1, row stride `ld` bytes (ld >= T, ld % 16 == 0).
+ *  - Pair convention of the reference: a pair is (target, source); its TE is
+ *    TE(source -> target).  Pair-free linear order:
+ *       id = target * (n - 1) + s,   source = s + (s >= target)
+ *    (tenex/csrc/full_smem.cu:190-195).
+ *
+ * Reference interfaces replaced (paths in the cxinsys/tenex tree):
+ *  - get_smem_optin                      tenex/csrc/full_smem.cu:150-155
+ *  - bin_fsbw_l_numpy + clamp            tenex/preprocess.py:61-88, :622
+ *  - remap_bins / _compact_dtype         tenex/preprocess.py:401-459, :335-354
+ *  - coarsen_bins_cuda                   tenex/csrc/adaptive_smem.cu:557-592
+ *  - te_smem_launch(_pairfree)           tenex/csrc/full_smem.cu:319-381
+ *  - te_adaptive_smem_launch(_pairfree)  tenex/csrc/adaptive_smem.cu:307-376
+ *  - te_gmem_launch                      tenex/csrc/gmem.cu:190-230
+ *  - compute_te_gemm_b2 (torch.mm x3 + Triton epilogue)
+ *                                        tenex/kernels/gemm_b2.py:307-425
+ *  - full/adaptive_smem_surrogate_test_launch
+ *                                        tenex/csrc/full_smem_surrogate_test.cu:297-394
+ *                                        tenex/csrc/adaptive_smem_surrogate_test.cu:306-400
+ */
+#ifndef TENEX_HIP_H
+#define TENEX_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define TNX_API __attribute__((visibility("default")))
+#else
+#define TNX_API
+#endif
+
+#define TNX_ABI_VERSION 1
+
+/* Status codes. */
+#define TNX_OK 0
+#define TNX_E_ARG (-1)     /* invalid argument / shape */
+#define TNX_E_HIP (-2)     /* HIP runtime error */
+#define TNX_E_RANGE (-3)   /* value outside the supported range */
+#define TNX_E_CAP (-4)     /* caller-provided capacity too small */
+
+/* Library ABI version (== TNX_ABI_VERSION). */
+TNX_API int tnx_abi_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+TNX_API const char *tnx_last_error(void);
+
+typedef struct {
+    int64_t lds_bytes_per_block; /* max dynamic LDS per workgroup (160 KiB on gfx950) */
+    int32_t n_cu;                /* compute units */
+    int32_t wave_size;           /* 64 */
+    int32_t clock_khz;
+    int32_t gcn_arch;            /* e.g. 950 */
+    char name[64];               /* gcnArchName */
+} tnx_device_props_t;
+
+/* Device properties of `device` (host struct out).
+ * Replaces get_smem_optin (tenex/csrc/full_smem.cu:150-155). */
+TNX_API int tnx_device_props(int device, tnx_device_props_t *out);
+
+/* ── Discretisation ───────────────────────────────────────────────────── */
+
+/* FSBW-L with numpy-exact statistics: per variable, std (ddof=1, numpy's
+ * float32 pairwise sums), min, max; n_bins = ceil((max-min)/std);
+ * raw[v,t] = max(0, floor((x - (min - kp*std)) / std)), std == 0 -> 1.
+ * x: float32 (n_vars, T) row-major.  raw_out int32 (n_vars, T).  Any of
+ * n_bins_out / std_out / smin_out may be NULL.
+ * Replaces bin_fsbw_l_numpy + clamp (tenex/preprocess.py:61-88, :622). */
+TNX_API int tnx_fsbw_l(const float *x, int64_t n_vars, int64_t T, float kp, int32_t *raw_out,
+               int32_t *n_bins_out, float *std_out, float *smin_out, void *stream);
+
+/* Dense per-variable relabelling (inverse of sorted unique).  raw int32
+ * (n_vars, T) with 0 <= raw < 65536; writes int8 bins (row stride ld) and
+ * n_per_var.  status (device int32, caller-zeroed) gets bit 1 if a raw value
+ * is >= 65536 or < 0, bit 2 if a variable has more than 127 levels.
+ * Replaces remap_bins + _compact_dtype (tenex/preprocess.py:401-459, :335-354). */
+TNX_API int tnx_remap_dense(const int32_t *raw, int64_t n_vars, int64_t T, int8_t *bins_out,
+                    int64_t ld, int32_t *n_per_var_out, int32_t *status, void *stream);
+
+/* In-place lossy coarsening: for variables with n_per_var > limit,
+ * bin = bin * limit / n_per_var and n_per_var = limit.
+ * Replaces coarsen_bins_cuda (tenex/csrc/adaptive_smem.cu:557-592). */
+TNX_API int tnx_coarsen_bins(int8_t *bins, int64_t n_vars, int64_t T, int64_t ld,
+                     int32_t *n_per_var, int32_t limit, void *stream);
+
+/* ── Per-variable preparation ─────────────────────────────────────────── */
+
+/* c*log2(c) for c in [0, N] as float hi/lo pairs (tbl[2c], tbl[2c+1]) with
+ * hi + lo == the FP64 value to ~2^-48 relative.  tbl: float (2*(N+1)). */
+TNX_API int tnx_xlogx_table(float *tbl, int64_t N, void *stream);
+
+/* Per variable v with b = n_per_var[v], L = T - tau:
+ *   states[v, t] = x[t+tau] * b + x[t]          (uint16, row stride lds)
+ *   S1[v]  = sum_b  c1 log2 c1   over x_t,             t < L   (FP64)
+ *   S2a[v] = sum_ab c2a log2 c2a over (x_{t+tau}, x_t), t < L   (FP64)
+ * feat (optional, float (n_vars, 8)): b, H(X_t), H(X_t+1|X_t),
+ * fraction of x_t == 0, fraction of x_t == mode, log2 b, mean level / b,
+ * lag-1 self agreement. */
+TNX_API int tnx_gene_prep(const int8_t *bins, int64_t n_vars, int64_t T, int64_t ld, int32_t tau,
+                  const int32_t *n_per_var, uint16_t *states, int64_t lds,
+                  double *S1, double *S2a, float *feat, void *stream);
+
+/* ── Pairwise TE (LDS histogram kernels) ─────────────────────────────── */
+
+/* A work item: targets tgt_order[tgt_beg .. tgt_beg+tgt_cnt) against sources
+ * src_order[src_beg .. src_end). */
+typedef struct {
+    int32_t tgt_beg, tgt_cnt, src_beg, src_end;
+} tnx_work_t;
+
+typedef struct {
+    int64_t n_wave_items;   /* items for the wave-per-pair kernel */
+    int64_t n_block_items;  /* items for the workgroup-per-pair kernel */
+    int64_t n_pairs;        /* pairs covered (excludes target == source) */
+    int32_t cap_wave;       /* histogram cells per wave */
+    int32_t cap_block;      /* histogram cells per workgroup pass */
+    int32_t max_block_cols; /* max b_t * b_s over block-kernel pairs */
+    int32_t n_tgt, n_src;
+    double est_cost_wave;   /* relative cost estimates */
+    double est_cost_block;
+} tnx_plan_info_t;
+
+/* Host-only planner.  n_per_var (host, n_vars); tgt_list / src_list (host)
+ * choose targets and sources (NULL = all variables).  Writes tgt_order
+ * (n_tgt), src_order (n_src) and work (wave items first, then block items)
+ * into host buffers; work_cap is the capacity of `work` in items.
+ * cap_wave / cap_block <= 0 select the gfx950 defaults.  A pair (x, y) is
+ * counted by the wave kernel iff b_x^2 * b_y <= cap_wave.
+ * This is the scheduling half of te_(adaptive_)smem_launch_pairfree
+ * (tenex/csrc/full_smem.cu:350-381, tenex/csrc/adaptive_smem.cu:351-376). */
+TNX_API int tnx_te_plan(const int32_t *n_per_var, int64_t n_vars, int64_t L,
+                const int32_t *tgt_list, int64_t n_tgt, const int32_t *src_list,
+                int64_t n_src, int32_t cap_wave, int32_t cap_block,
+                int32_t *tgt_order, int32_t *src_order, tnx_work_t *work, int64_t work_cap,
+                tnx_plan_info_t *info);
+
+/* Output modes for tnx_te_run. */
+#define TNX_OUT_PAIRFREE 0 /* out[id - pair_offset], id in [pair_offset, pair_offset+n_pairs_local) */
+#define TNX_OUT_MATRIX 1   /* out[target * ld_out + source] (internal layout) */
+
+typedef struct {
+    const int8_t *bins;       /* (n_vars, ld) */
+    const uint16_t *states;   /* (n_vars, lds) from tnx_gene_prep */
+    const int32_t *n_per_var;
+    const double *S1, *S2a;   /* from tnx_gene_prep */
+    const float *xlogx;       /* from tnx_xlogx_table(N = T - tau) */
+    const int32_t *tgt_order, *src_order; /* device copies of the plan */
+    const tnx_work_t *work;   /* device copy of the plan */
+    int64_t n_vars, T, ld, lds;
+    int32_t tau, out_mode;
+    int64_t n_wave_items, n_block_items;
+    int32_t cap_wave, cap_block, max_block_cols;
+    float *out;
+    int64_t ld_out, pair_offset, n_pairs_local;
+} tnx_te_args_t;
+
+/* Pairwise TE for every pair of the plan: FP32 entropy terms with
+ * compensated summation, FP64 final combination
+ * (S3 + S1 - S2a - S2b) / N.  Replaces te_smem_launch(_pairfree),
+ * te_adaptive_smem_launch(_pairfree) and te_gmem_launch
+ * (tenex/csrc/full_smem.cu:319-381, adaptive_smem.cu:307-376,
+ * gmem.cu:190-230). */
+TNX_API int tnx_te_run(const tnx_te_args_t *args, void *stream);
+
+/* Debug: raw joint counts of explicit pairs (host-side count n_pairs,
+ * pairs device int32 [target, source]) through the same accumulation code
+ * as tnx_te_run; counts_out int32 (n_pairs, max_cells) in layout
+ * ((a * b_t + b) * b_s + c).  Pairs with b_t^2 b_s > max_cells are an error.
+ * Also writes te_out[p] (float) computed by the wave path. */
+TNX_API int tnx_te_counts_debug(const int8_t *bins, const uint16_t *states, const int32_t *n_per_var,
+                        const double *S1, const double *S2a, const float *xlogx,
+                        int64_t n_vars, int64_t T, int64_t ld, int64_t lds, int32_t tau,
+                        const int32_t *pairs, int64_t n_pairs, int32_t max_cells,
+                        int32_t *counts_out, float *te_out, void *stream);
+
+/* ── GEMM-B2: binary data on INT8 MFMA ───────────────────────────────── */
+
+/* Workspace bytes for tnx_te_gemm_b2 at (n, T). */
+TNX_API int64_t tnx_gemm_b2_workspace(int64_t n, int64_t T, int32_t tau);
+
+/* All n*(n-1) TE values for binary bins (0/1) via exact INT8 MFMA
+ * contractions with INT32 accumulation (C = fp.p^T, D = p.p^T, E = f.p^T)
+ * and the fused 8-state epilogue.  te_out float (n, ld_out), internal
+ * layout te_out[i * ld_out + j] = TE(j -> i), diagonal 0.
+ * S1, S2a from tnx_gene_prep.  Replaces compute_te_gemm_b2
+ * (tenex/kernels/gemm_b2.py:307-425, :66-187, :210-285). */
+TNX_API int tnx_te_gemm_b2(const int8_t *bins, int64_t n, int64_t T, int64_t ld, int32_t tau,
+                   const double *S1, const double *S2a, const float *xlogx,
+                   void *workspace, int64_t workspace_bytes, float *te_out, int64_t ld_out,
+                   int32_t *counts_dbg, void *stream);
+
+/* ── Surrogate test ──────────────────────────────────────────────────── */
+
+typedef struct {
+    const int8_t *bins;       /* (n_vars, ld) */
+    const int32_t *n_per_var;
+    const float *xlogx;       /* N = T - tau */
+    const int32_t *perm;      /* (n_surr, n_vars, n_blocks) block order per surrogate */
+    const int32_t *tgt_order, *src_order;
+    const tnx_work_t *work;   /* plan from tnx_te_plan (same cap values) */
+    const float *observed;    /* public layout M[src * n_vars + tgt] */
+    int64_t n_vars, T, ld;
+    int32_t tau, n_surr, n_blocks, block_length;
+    int64_t n_wave_items, n_block_items;
+    int32_t cap_wave, cap_block, max_block_cols;
+    double *sum_te, *sumsq_te; /* (n_vars, n_vars) public layout, accumulated (+=) */
+    int32_t *count_ge;         /* (n_vars, n_vars) public layout, accumulated (+=) */
+    float *te_k_dbg;           /* optional (n_vars*n_vars, n_surr): te_k per slot */
+} tnx_surr_args_t;
+
+/* Block-shuffle surrogate statistics for every pair of the plan: for each
+ * surrogate k, each variable's head [0, n_blocks*block_length) is read in
+ * block order perm[k, v, :], the tail untouched; TE per (pair, k) with the
+ * same arithmetic as tnx_te_run; FP64 sum / sum of squares and the count of
+ * te_k >= observed (float32 compare).  Replaces
+ * full_smem_surrogate_test_launch / adaptive_smem_surrogate_test_launch
+ * (tenex/csrc/full_smem_surrogate_test.cu:297-394,
+ * tenex/csrc/adaptive_smem_surrogate_test.cu:306-400). */
+TNX_API int tnx_surrogate_run(const tnx_surr_args_t *args, void *stream);
+
+/* ── Learned null: 3-layer MLP (no reference counterpart) ────────────── */
+
+/* Network: h1 = relu(W1 x + b1), h2 = relu(W2 h1 + b2), y = W3 h2 + b3.
+ * Weights bf16 row-major W1 [H x D], W2 [H x H], W3 [O x H]; biases fp32.
+ * D <= 64, H == 64, O <= 4.  Accumulation in fp32. */
+typedef struct {
+    const uint16_t *W1, *W2, *W3; /* bf16 */
+    const float *b1, *b2, *b3;
+    int32_t D, H, O;
+} tnx_mlp_t;
+
+/* Y[r, o] for rows r < n_rows of X (float, (n_rows, D)). */
+TNX_API int tnx_mlp_forward(const tnx_mlp_t *m, const float *X, int64_t n_rows, float *Y,
+                    void *stream);
+
+/* Loss 0.5 * w_o * (y - tgt)^2 averaged over rows and outputs; gradients
+ * (fp32, params layout W1, b1, W2, b2, W3, b3) accumulated into grad
+ * (caller zeroes); loss_out (device float, accumulated). */
+TNX_API int tnx_mlp_backward(const tnx_mlp_t *m, const float *X, const float *tgt,
+                     const float *w_out, int64_t n_rows, float *grad, float *loss_out,
+                     void *stream);
+
+/* Adam step on fp32 master weights `master` (params layout), refreshing the
+ * bf16 copies in m; moments m1, m2 fp32; step t >= 1. */
+TNX_API int tnx_mlp_adam(const tnx_mlp_t *m, float *master, const float *grad, float *m1,
+                 float *m2, int64_t n_params, float lr, float beta1, float beta2,
+                 float eps, int32_t t, void *stream);
+
+/* ── Hardware ceilings (roofline peaks) ────────────────────────────────── */
+
+/* Chip-wide non-returning LDS atomic increments: n_blocks workgroups of 256
+ * threads, each thread doing iters * 8 increments; mode 0 conflict-free,
+ * mode 1 random keys in 2048 cells.  out: uint32 (n_blocks,) sink.
+ * No reference counterpart (the reference states no LDS ceiling). */
+TNX_API int tnx_bench_lds_atomics(int32_t mode, int64_t n_blocks, int32_t iters, uint32_t *out,
+                                  void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TENEX_HIP_H */

@@ -1,0 +1,317 @@
+"""Device-resident data and launch wrappers over libtenex_hip (torch = memory).
+
+Layout in HBM (one GPU):
+  bins    int8   (n_vars, ld)   ld = round_up(T, 64), padding zero
+  n_per_var int32 (n_vars,)
+  states  uint16 (n_vars, lds)  lds = round_up(T - tau, 64): x[t+tau]*b + x[t]
+  S1, S2a float64 (n_vars,)     per-target entropy sums
+  xlogx   float32 (2*(N+1),)    c*log2(c) as hi/lo pairs, N = T - tau
+  feat    float32 (n_vars, 8)   per-variable features of the learned null
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, ptr
+
+ROW_ALIGN = 64
+
+
+def round_up(v: int, m: int) -> int:
+    return (v + m - 1) // m * m
+
+
+def require_gpu(device=None) -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("tenex_amd needs a ROCm GPU (gfx950); none is available")
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    if dev.type != "cuda":
+        raise RuntimeError(f"tenex_amd runs on the GPU only (got device {dev})")
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    _lib.load()
+    return dev
+
+
+def _stream(dev) -> int:
+    return torch.cuda.current_stream(dev).cuda_stream
+
+
+def device_props(dev) -> _lib.DeviceProps:
+    p = _lib.DeviceProps()
+    call("tnx_device_props", int(dev.index), C.byref(p))
+    return p
+
+
+# ── discretisation ────────────────────────────────────────────────────────
+
+def fsbw_l_gpu(x: torch.Tensor, kp: float = 0.5):
+    """FSBW-L with numpy-exact statistics on the GPU.  x float32 (n, T) on
+    device.  Returns (raw int32 (n, T), n_bins int32 (n,))."""
+    n, T = x.shape
+    x = x.contiguous()
+    raw = torch.empty((n, T), dtype=torch.int32, device=x.device)
+    nb = torch.empty(n, dtype=torch.int32, device=x.device)
+    sd = torch.empty(n, dtype=torch.float32, device=x.device)
+    sm = torch.empty(n, dtype=torch.float32, device=x.device)
+    call("tnx_fsbw_l", ptr(x), n, T, float(kp), ptr(raw), ptr(nb), ptr(sd), ptr(sm),
+         _stream(x.device))
+    return raw, nb
+
+
+def remap_gpu(raw: torch.Tensor):
+    """Dense per-variable relabelling into padded int8 rows.
+    Returns (bins int8 (n, ld), n_per_var int32 (n,) device, b_max)."""
+    n, T = raw.shape
+    ld = round_up(T, ROW_ALIGN)
+    bins = torch.empty((n, ld), dtype=torch.int8, device=raw.device)
+    npv = torch.empty(n, dtype=torch.int32, device=raw.device)
+    status = torch.zeros(1, dtype=torch.int32, device=raw.device)
+    call("tnx_remap_dense", ptr(raw.contiguous()), n, T, ptr(bins), ld, ptr(npv), ptr(status),
+         _stream(raw.device))
+    st = int(status.item())
+    if st & 1:
+        raise ValueError("remap: a raw bin index is negative or >= 65536")
+    if st & 2:
+        raise ValueError("remap: a variable has more than 127 distinct levels; "
+                         "int8 bins cannot hold it (use coarsening or a smaller kp)")
+    b_max = int(npv.max().item()) if n else 0
+    return bins, npv, b_max
+
+
+def upload_bins(bin_arr, device) -> tuple[torch.Tensor, torch.Tensor, int, int]:
+    """Dense bins (n, T) or (n, T, 1) from the host or device -> padded int8
+    device rows.  Bins must already be dense per variable (0..b-1 with every
+    level present) as remap_bins produces.  Returns (bins, npv, b_max, T)."""
+    t = torch.as_tensor(np.asarray(bin_arr) if not isinstance(bin_arr, torch.Tensor) else bin_arr)
+    if t.dim() == 3:
+        if t.shape[2] != 1:
+            raise ValueError(f"only K=1 bin arrays are supported (got K={t.shape[2]})")
+        t = t[:, :, 0]
+    n, T = t.shape
+    ld = round_up(T, ROW_ALIGN)
+    bins = torch.zeros((n, ld), dtype=torch.int8, device=device)
+    bins[:, :T] = t.to(device=device, dtype=torch.int8)
+    npv = (bins[:, :T].to(torch.int32).amax(dim=1) + 1).to(torch.int32) if T else \
+        torch.ones(n, dtype=torch.int32, device=device)
+    b_max = int(npv.max().item()) if n else 0
+    return bins, npv.contiguous(), b_max, T
+
+
+def coarsen_gpu(bins: torch.Tensor, npv: torch.Tensor, T: int, limit: int):
+    call("tnx_coarsen_bins", ptr(bins), bins.shape[0], T, bins.shape[1], ptr(npv), int(limit),
+         _stream(bins.device))
+
+
+# ── per-variable preparation ──────────────────────────────────────────────
+
+_xlogx_cache: dict = {}
+
+
+def xlogx_table(N: int, device) -> torch.Tensor:
+    key = (int(N), str(device))
+    t = _xlogx_cache.get(key)
+    if t is None:
+        t = torch.empty(2 * (N + 1), dtype=torch.float32, device=device)
+        call("tnx_xlogx_table", ptr(t), int(N), _stream(device))
+        _xlogx_cache[key] = t
+    return t
+
+
+@dataclass
+class Prepared:
+    """Device-resident discretised data ready for the TE kernels."""
+    bins: torch.Tensor       # int8 (n, ld)
+    npv: torch.Tensor        # int32 (n,)
+    npv_host: np.ndarray     # int32 (n,)
+    T: int
+    tau: int
+    states: torch.Tensor     # uint16 as int16 storage (n, lds)
+    S1: torch.Tensor
+    S2a: torch.Tensor
+    feat: torch.Tensor       # float32 (n, 8)
+    xlogx: torch.Tensor
+    plans: dict = field(default_factory=dict)
+
+    @property
+    def n_vars(self) -> int:
+        return self.bins.shape[0]
+
+    @property
+    def ld(self) -> int:
+        return self.bins.shape[1]
+
+    @property
+    def lds(self) -> int:
+        return self.states.shape[1]
+
+    @property
+    def L(self) -> int:
+        return self.T - self.tau
+
+    @property
+    def device(self):
+        return self.bins.device
+
+
+def prepare(bins: torch.Tensor, npv: torch.Tensor, T: int, tau: int) -> Prepared:
+    n, ld = bins.shape
+    if not (1 <= tau < T):
+        raise ValueError(f"need 1 <= tau < T (tau={tau}, T={T})")
+    dev = bins.device
+    L = T - tau
+    lds = round_up(L, ROW_ALIGN)
+    states = torch.empty((n, lds), dtype=torch.int16, device=dev)
+    S1 = torch.empty(n, dtype=torch.float64, device=dev)
+    S2a = torch.empty(n, dtype=torch.float64, device=dev)
+    feat = torch.empty((n, 8), dtype=torch.float32, device=dev)
+    call("tnx_gene_prep", ptr(bins), n, T, ld, int(tau), ptr(npv), ptr(states), lds, ptr(S1),
+         ptr(S2a), ptr(feat), _stream(dev))
+    return Prepared(bins=bins, npv=npv, npv_host=npv.cpu().numpy().astype(np.int32), T=T,
+                    tau=tau, states=states, S1=S1, S2a=S2a, feat=feat,
+                    xlogx=xlogx_table(L, dev))
+
+
+# ── plan ─────────────────────────────────────────────────────────────────
+
+@dataclass
+class Plan:
+    info: _lib.PlanInfo
+    tgt_order: torch.Tensor
+    src_order: torch.Tensor
+    work: torch.Tensor  # int32 (n_items, 4)
+    work_host: np.ndarray
+    tgt_host: np.ndarray
+    src_host: np.ndarray
+
+
+def make_plan(npv_host: np.ndarray, L: int, device, targets=None, sources=None,
+              cap_wave: int = 0, cap_block: int = 0) -> Plan:
+    npv_host = np.ascontiguousarray(npv_host, dtype=np.int32)
+    n = len(npv_host)
+    tg = None if targets is None else np.ascontiguousarray(targets, dtype=np.int32)
+    sr = None if sources is None else np.ascontiguousarray(sources, dtype=np.int32)
+    info = _lib.PlanInfo()
+    args = (ptr(npv_host), n, int(L), ptr(tg), 0 if tg is None else len(tg), ptr(sr),
+            0 if sr is None else len(sr), int(cap_wave), int(cap_block))
+    call("tnx_te_plan", *args, None, None, None, 0, C.byref(info))
+    n_items = info.n_wave_items + info.n_block_items
+    tgt = np.empty(info.n_tgt, np.int32)
+    src = np.empty(info.n_src, np.int32)
+    work = np.empty((max(n_items, 1), 4), np.int32)
+    call("tnx_te_plan", *args, ptr(tgt), ptr(src), ptr(work), n_items, C.byref(info))
+    return Plan(info=info, tgt_order=torch.from_numpy(tgt).to(device),
+                src_order=torch.from_numpy(src).to(device),
+                work=torch.from_numpy(work).to(device), work_host=work[:n_items],
+                tgt_host=tgt, src_host=src)
+
+
+def get_plan(prep: Prepared, targets=None, sources=None, cap_wave: int = 0,
+             cap_block: int = 0) -> Plan:
+    key = (None if targets is None else hash(np.asarray(targets).tobytes()),
+           None if sources is None else hash(np.asarray(sources).tobytes()), cap_wave, cap_block)
+    p = prep.plans.get(key)
+    if p is None:
+        p = make_plan(prep.npv_host, prep.L, prep.device, targets, sources, cap_wave, cap_block)
+        prep.plans[key] = p
+    return p
+
+
+# ── TE run ────────────────────────────────────────────────────────────────
+
+def te_args(prep: Prepared, plan: Plan, out: torch.Tensor, out_mode: int, ld_out: int = 0,
+            pair_offset: int = 0, n_pairs_local: int = 0) -> _lib.TeArgs:
+    i = plan.info
+    return _lib.TeArgs(
+        bins=ptr(prep.bins), states=ptr(prep.states), n_per_var=ptr(prep.npv), S1=ptr(prep.S1),
+        S2a=ptr(prep.S2a), xlogx=ptr(prep.xlogx), tgt_order=ptr(plan.tgt_order),
+        src_order=ptr(plan.src_order), work=ptr(plan.work), n_vars=prep.n_vars, T=prep.T,
+        ld=prep.ld, lds=prep.lds, tau=prep.tau, out_mode=out_mode,
+        n_wave_items=i.n_wave_items, n_block_items=i.n_block_items, cap_wave=i.cap_wave,
+        cap_block=i.cap_block, max_block_cols=max(i.max_block_cols, 1), out=ptr(out),
+        ld_out=ld_out, pair_offset=pair_offset, n_pairs_local=n_pairs_local)
+
+
+def te_run(prep: Prepared, plan: Plan, out: torch.Tensor, out_mode: int, ld_out: int = 0,
+           pair_offset: int = 0, n_pairs_local: int = 0, args: _lib.TeArgs | None = None):
+    if args is None:
+        args = te_args(prep, plan, out, out_mode, ld_out, pair_offset, n_pairs_local)
+    call("tnx_te_run", C.byref(args), _stream(prep.device))
+
+
+def te_matrix_internal(prep: Prepared, targets=None, sources=None, cap_wave: int = 0,
+                       cap_block: int = 0) -> torch.Tensor:
+    """Internal-layout TE matrix M[target, source] = TE(source -> target) on the
+    device (zeros where not computed)."""
+    n = prep.n_vars
+    plan = get_plan(prep, targets, sources, cap_wave, cap_block)
+    out = torch.zeros((n, n), dtype=torch.float32, device=prep.device)
+    te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n)
+    return out
+
+
+def te_pairfree(prep: Prepared, pair_offset: int = 0, n_pairs_local: int | None = None,
+                cap_wave: int = 0, cap_block: int = 0) -> torch.Tensor:
+    """TE in pair-free linear order for ids [pair_offset, pair_offset + n_local)."""
+    n = prep.n_vars
+    total = n * (n - 1)
+    if n_pairs_local is None:
+        n_pairs_local = total - pair_offset
+    if pair_offset < 0 or n_pairs_local < 0 or pair_offset + n_pairs_local > total:
+        raise ValueError("pair range outside [0, n*(n-1))")
+    out = torch.empty(n_pairs_local, dtype=torch.float32, device=prep.device)
+    if n_pairs_local == 0:
+        return out
+    nm1 = n - 1
+    t0 = pair_offset // nm1
+    t1 = (pair_offset + n_pairs_local - 1) // nm1 + 1
+    plan = get_plan(prep, np.arange(t0, t1, dtype=np.int32), None, cap_wave, cap_block)
+    te_run(prep, plan, out, _lib.TNX_OUT_PAIRFREE, pair_offset=pair_offset,
+           n_pairs_local=n_pairs_local)
+    return out
+
+
+def counts_debug(prep: Prepared, pairs: np.ndarray, max_cells: int = 8192):
+    """Raw joint counts of explicit [target, source] pairs (wave path)."""
+    pairs = np.ascontiguousarray(pairs, dtype=np.int32)
+    p_d = torch.from_numpy(pairs).to(prep.device)
+    P = len(pairs)
+    counts = torch.zeros((P, max_cells), dtype=torch.int32, device=prep.device)
+    te = torch.empty(P, dtype=torch.float32, device=prep.device)
+    call("tnx_te_counts_debug", ptr(prep.bins), ptr(prep.states), ptr(prep.npv), ptr(prep.S1),
+         ptr(prep.S2a), ptr(prep.xlogx), prep.n_vars, prep.T, prep.ld, prep.lds, prep.tau,
+         ptr(p_d), P, int(max_cells), ptr(counts), ptr(te), _stream(prep.device))
+    return counts, te
+
+
+def gene_prep_into(prep: Prepared):
+    """Re-run the per-variable preparation into prep's existing buffers."""
+    call("tnx_gene_prep", ptr(prep.bins), prep.n_vars, prep.T, prep.ld, int(prep.tau),
+         ptr(prep.npv), ptr(prep.states), prep.lds, ptr(prep.S1), ptr(prep.S2a), ptr(prep.feat),
+         _stream(prep.device))
+
+
+def lds_atomic_rate(device, mode: int = 0, n_blocks: int | None = None, iters: int = 4096,
+                    reps: int = 5) -> float:
+    """Measured chip-wide LDS atomic increments per second (median of reps)."""
+    props = device_props(device)
+    if n_blocks is None:
+        n_blocks = int(props.n_cu) * 8
+    out = torch.empty(n_blocks, dtype=torch.int32, device=device)
+    s = _stream(device)
+    call("tnx_bench_lds_atomics", int(mode), int(n_blocks), int(iters), ptr(out), s)
+    rates = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        call("tnx_bench_lds_atomics", int(mode), int(n_blocks), int(iters), ptr(out), s)
+        e1.record()
+        e1.synchronize()
+        rates.append(n_blocks * 256.0 * iters * 8 / (e0.elapsed_time(e1) / 1e3))
+    return float(np.median(rates))

