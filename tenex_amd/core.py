@@ -315,3 +315,64 @@ def lds_atomic_rate(device, mode: int = 0, n_blocks: int | None = None, iters: i
         e1.synchronize()
         rates.append(n_blocks * 256.0 * iters * 8 / (e0.elapsed_time(e1) / 1e3))
     return float(np.median(rates))
+
+
+# ── surrogates ────────────────────────────────────────────────────────────
+
+SURR_MAX_CELLS = 32768
+
+
+def surrogate_cap(npv_host: np.ndarray, targets=None, sources=None) -> int:
+    """Histogram cells needed by the surrogate kernel: max b_t^2 * b_s over
+    the pairs, rounded up to an even count >= 256."""
+    bt = npv_host if targets is None else npv_host[np.asarray(targets)]
+    bs = npv_host if sources is None else npv_host[np.asarray(sources)]
+    need = int(bt.max(initial=1)) ** 2 * int(bs.max(initial=1))
+    return max(256, need + (need & 1))
+
+
+def block_perm_device(n_vars: int, T: int, shuffle_method: str, block_length: int, k_start: int,
+                      k_end: int, seed: int, rng_device, out_device):
+    """Block orders for surrogates [k_start, k_end): perm[k] =
+    argsort(rand(n_vars, n_blocks)) after manual_seed(seed + k) on a torch
+    Generator on rng_device (tenex/inference/surrogate_test.py:165-207).
+    Returns (perm int32 (k, n_vars, n_blocks) on out_device, eff_bl, n_blocks)."""
+    rnd_all = shuffle_method == "random" or block_length <= 1 or block_length >= T
+    eff, nb = (1, T) if rnd_all else (int(block_length), T // int(block_length))
+    out = torch.empty((k_end - k_start, n_vars, nb), dtype=torch.int32, device=out_device)
+    gen = torch.Generator(device=rng_device)
+    for ki, k in enumerate(range(k_start, k_end)):
+        gen.manual_seed(seed + k)
+        r = torch.rand(n_vars, nb, device=rng_device, generator=gen)
+        out[ki].copy_(torch.argsort(r, dim=1).to(torch.int32))
+    return out, eff, nb
+
+
+def surrogate_run(prep: Prepared, plan: Plan, perm: torch.Tensor, block_length: int,
+                  observed: torch.Tensor, sum_te: torch.Tensor, sumsq_te: torch.Tensor,
+                  count_ge: torch.Tensor, te_k_dbg: torch.Tensor | None = None):
+    """Accumulate surrogate statistics for every pair of the plan into the
+    public-layout (n, n) sum / sumsq (float64) and count_ge (int32)."""
+    n_surr, n, nb = perm.shape
+    if n != prep.n_vars:
+        raise ValueError("perm does not match n_vars")
+    if plan.info.n_block_items:
+        raise ValueError("surrogate plans must put every pair in the wave kernel")
+    args = _lib.SurrArgs(
+        bins=ptr(prep.bins), n_per_var=ptr(prep.npv), xlogx=ptr(prep.xlogx), perm=ptr(perm),
+        tgt_order=ptr(plan.tgt_order), src_order=ptr(plan.src_order), work=ptr(plan.work),
+        observed=ptr(observed), n_vars=n, T=prep.T, ld=prep.ld, tau=prep.tau, n_surr=n_surr,
+        n_blocks=nb, block_length=int(block_length), n_wave_items=plan.info.n_wave_items,
+        n_block_items=0, cap_wave=plan.info.cap_wave, cap_block=plan.info.cap_block,
+        max_block_cols=max(plan.info.max_block_cols, 1), sum_te=ptr(sum_te),
+        sumsq_te=ptr(sumsq_te), count_ge=ptr(count_ge), te_k_dbg=ptr(te_k_dbg))
+    call("tnx_surrogate_run", C.byref(args), _stream(prep.device))
+
+
+def surrogate_plan(prep: Prepared, targets=None, sources=None) -> Plan:
+    cap = surrogate_cap(prep.npv_host, targets, sources)
+    if cap > SURR_MAX_CELLS:
+        raise NotImplementedError(
+            f"surrogate test needs b_t^2*b_s <= {SURR_MAX_CELLS} histogram cells (got {cap}); "
+            f"use coarsening=True or a smaller kp")
+    return get_plan(prep, targets, sources, cap_wave=cap, cap_block=0)

@@ -1,0 +1,210 @@
+// gemm_b2.hip -- TE for binary data from exact INT8 MFMA contractions (gfx950).
+//
+// For binary bins (values 0/1), with f_i[t] = x_i[t+tau], p_i[t] = x_i[t],
+// t < L = T - tau, the three contractions
+//     C[i,j] = sum_t f_i p_i p_j,  D[i,j] = sum_t p_i p_j,  E[i,j] = sum_t f_i p_j
+// plus per-gene marginals nf, np, nfp give all eight joint counts of
+// (x_i[t+tau], x_i[t], x_j[t]) by inclusion-exclusion
+// (tenex/kernels/gemm_b2.py:240-249).  The reference evaluates C, D, E as
+// three FP32 torch.mm calls and a Triton epilogue (gemm_b2.py:389-391,
+// :66-187); here one kernel contracts int8 operands on
+// v_mfma_i32_32x32x32_i8 with INT32 accumulation (exact for any L < 2^31)
+// and applies the epilogue in registers:
+//     TE(j -> i) = (S3 + S1[i] - S2a[i] - S2b) / L
+// with S3 over the 8 joint counts and S2b over the 4 (x_i[t], x_j[t]) counts,
+// c*log2(c) from the hi/lo table with compensated fp32 sums and an fp64
+// final combination.
+//
+// Workspace (tnx_gemm_b2_workspace): F and P operand rows (n_pad x Lp int8,
+// zero padded), then nf, np, nfp (int32 n_pad each).
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace tnx {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int kB2TileI = 64;    // rows (targets) per workgroup
+constexpr int kB2TileJ = 128;   // cols (sources) per workgroup
+constexpr int kB2K = 64;        // bytes of t per main-loop step (2 MFMAs)
+
+__host__ __device__ inline int64_t b2_npad(int64_t n) { return (n + 127) / 128 * 128; }
+__host__ __device__ inline int64_t b2_lp(int64_t L) { return (L + kB2K - 1) / kB2K * kB2K; }
+
+// One workgroup per variable: operand rows and marginals.
+__global__ __launch_bounds__(256) void b2_pack_kernel(const int8_t *__restrict__ bins, int64_t n,
+                                                      int64_t ld, int32_t tau, int64_t L,
+                                                      int64_t Lp, int8_t *__restrict__ F,
+                                                      int8_t *__restrict__ P,
+                                                      int32_t *__restrict__ nf,
+                                                      int32_t *__restrict__ np_,
+                                                      int32_t *__restrict__ nfp) {
+    __shared__ int32_t red[3][4];
+    const int64_t i = blockIdx.x;
+    const int tid = threadIdx.x;
+    int32_t sf = 0, sp = 0, sfp = 0;
+    for (int64_t t = tid; t < Lp; t += 256) {
+        int8_t f = 0, p = 0;
+        if (i < n && t < L) {
+            f = bins[i * ld + t + tau];
+            p = bins[i * ld + t];
+        }
+        F[i * Lp + t] = f;
+        P[i * Lp + t] = p;
+        sf += f;
+        sp += p;
+        sfp += f & p;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        sf += __shfl_xor(sf, off, 64);
+        sp += __shfl_xor(sp, off, 64);
+        sfp += __shfl_xor(sfp, off, 64);
+    }
+    if ((tid & 63) == 0) {
+        red[0][tid >> 6] = sf;
+        red[1][tid >> 6] = sp;
+        red[2][tid >> 6] = sfp;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        nf[i] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        np_[i] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        nfp[i] = red[2][0] + red[2][1] + red[2][2] + red[2][3];
+    }
+}
+
+// c*log2(c) of an inclusion-exclusion count; counts outside [0, N] can only
+// come from non-binary input and are ignored (the table has N + 1 entries).
+__device__ __forceinline__ void xl_add(DF &acc, const float2 *__restrict__ tbl, int32_t c,
+                                       int32_t N) {
+    if (c > 1 && c <= N) {
+        float2 t = tbl[c];
+        df_add(acc, t.x, t.y);
+    }
+}
+
+// Workgroup: 4 waves as 2 (i) x 2 (j); wave tile 32 (i) x 64 (j).
+__global__ __launch_bounds__(256) void b2_mfma_kernel(
+    const int8_t *__restrict__ F, const int8_t *__restrict__ P, const int32_t *__restrict__ nf,
+    const int32_t *__restrict__ np_, const int32_t *__restrict__ nfp,
+    const double *__restrict__ S1, const double *__restrict__ S2a,
+    const float *__restrict__ xlogx, int64_t n, int64_t L, int64_t Lp,
+    float *__restrict__ te_out, int64_t ld_out, int32_t *__restrict__ counts_dbg) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t i0 = (int64_t)blockIdx.y * kB2TileI + (wave >> 1) * 32;
+    const int64_t j0 = (int64_t)blockIdx.x * kB2TileJ + (wave & 1) * 64;
+    const int r = lane & 31, h = lane >> 5;
+    // operand rows held by this lane (padded rows are zero, so no masking)
+    const int8_t *fa = F + (i0 + r) * Lp + 16 * h;
+    const int8_t *pa = P + (i0 + r) * Lp + 16 * h;
+    const int8_t *pb0 = P + (j0 + r) * Lp + 16 * h;
+    const int8_t *pb1 = P + (j0 + 32 + r) * Lp + 16 * h;
+    v16i accC0 = {}, accC1 = {}, accD0 = {}, accD1 = {}, accE0 = {}, accE1 = {};
+    for (int64_t k = 0; k < Lp; k += kB2K) {
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int64_t kk = k + 32 * u;
+            const v4i f = *reinterpret_cast<const v4i *>(fa + kk);
+            const v4i p = *reinterpret_cast<const v4i *>(pa + kk);
+            const v4i q0 = *reinterpret_cast<const v4i *>(pb0 + kk);
+            const v4i q1 = *reinterpret_cast<const v4i *>(pb1 + kk);
+            const v4i fp = f & p;
+            accC0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fp, q0, accC0, 0, 0, 0);
+            accC1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fp, q1, accC1, 0, 0, 0);
+            accD0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(p, q0, accD0, 0, 0, 0);
+            accD1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(p, q1, accD1, 0, 0, 0);
+            accE0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(f, q0, accE0, 0, 0, 0);
+            accE1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(f, q1, accE1, 0, 0, 0);
+        }
+    }
+    // Epilogue.  32x32 accumulator layout: lane holds column (lane & 31),
+    // register q holds row 8*(q/4) + 4*(lane/32) + (q%4).
+    const float2 *tbl = reinterpret_cast<const float2 *>(xlogx);
+    const int32_t N = (int32_t)L;
+    const double invN = 1.0 / (double)L;
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+        const int64_t j = j0 + 32 * half + (lane & 31);
+        const v16i &aC = half ? accC1 : accC0;
+        const v16i &aD = half ? accD1 : accD0;
+        const v16i &aE = half ? accE1 : accE0;
+        const int32_t npj = j < n ? np_[j] : 0;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int64_t i = i0 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+            if (i >= n || j >= n) continue;
+            const int32_t Cc = aC[q], Dd = aD[q], Ee = aE[q];
+            if (counts_dbg) {
+                counts_dbg[i * n + j] = Cc;
+                counts_dbg[n * n + i * n + j] = Dd;
+                counts_dbg[2 * n * n + i * n + j] = Ee;
+            }
+            if (i == j) {
+                te_out[i * ld_out + j] = 0.0f;
+                continue;
+            }
+            const int32_t nfi = nf[i], npi = np_[i], nfpi = nfp[i];
+            DF s3 = {0.f, 0.f}, s2 = {0.f, 0.f};
+            xl_add(s3, tbl, Cc, N);
+            xl_add(s3, tbl, Dd - Cc, N);
+            xl_add(s3, tbl, Ee - Cc, N);
+            xl_add(s3, tbl, npj - Dd - Ee + Cc, N);
+            xl_add(s3, tbl, nfpi - Cc, N);
+            xl_add(s3, tbl, npi - Dd - nfpi + Cc, N);
+            xl_add(s3, tbl, nfi - Ee - nfpi + Cc, N);
+            xl_add(s3, tbl, N - nfi - npi - npj + nfpi + Dd + Ee - Cc, N);
+            xl_add(s2, tbl, Dd, N);
+            xl_add(s2, tbl, npj - Dd, N);
+            xl_add(s2, tbl, npi - Dd, N);
+            xl_add(s2, tbl, N - npi - npj + Dd, N);
+            const double te = (df_to_double(s3) + S1[i] - S2a[i] - df_to_double(s2)) * invN;
+            te_out[i * ld_out + j] = (float)te;
+        }
+    }
+}
+
+}  // namespace tnx
+
+using namespace tnx;
+
+extern "C" {
+
+int64_t tnx_gemm_b2_workspace(int64_t n, int64_t T, int32_t tau) {
+    if (n <= 0 || tau < 1 || tau >= T) return 0;
+    const int64_t np = b2_npad(n), Lp = b2_lp(T - tau);
+    return 2 * np * Lp + 3 * np * 4 + 256;
+}
+
+int tnx_te_gemm_b2(const int8_t *bins, int64_t n, int64_t T, int64_t ld, int32_t tau,
+                   const double *S1, const double *S2a, const float *xlogx, void *workspace,
+                   int64_t workspace_bytes, float *te_out, int64_t ld_out, int32_t *counts_dbg,
+                   void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(bins && S1 && S2a && xlogx && workspace && te_out, "tnx_te_gemm_b2: null pointer");
+    TNX_CHECK_ARG(tau >= 1 && tau < T && ld >= T && n >= 1 && ld_out >= n,
+                  "tnx_te_gemm_b2: bad shape (n=%lld T=%lld ld=%lld tau=%d)", (long long)n,
+                  (long long)T, (long long)ld, tau);
+    TNX_CHECK_ARG(T - tau < (1LL << 31) - 64, "tnx_te_gemm_b2: L too large");
+    const int64_t need = tnx_gemm_b2_workspace(n, T, tau);
+    TNX_CHECK_ARG(workspace_bytes >= need, "tnx_te_gemm_b2: workspace %lld < %lld bytes",
+                  (long long)workspace_bytes, (long long)need);
+    const int64_t np = b2_npad(n), L = T - tau, Lp = b2_lp(L);
+    int8_t *F = reinterpret_cast<int8_t *>(workspace);
+    int8_t *P = F + np * Lp;
+    int32_t *nf = reinterpret_cast<int32_t *>(P + np * Lp);
+    int32_t *npp = nf + np;
+    int32_t *nfp = npp + np;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(b2_pack_kernel, dim3((unsigned)np), dim3(256), 0, s, bins, n, ld, tau, L,
+                       Lp, F, P, nf, npp, nfp);
+    TNX_CHECK_LAUNCH("b2_pack_kernel");
+    dim3 grid((unsigned)((n + kB2TileJ - 1) / kB2TileJ), (unsigned)((n + kB2TileI - 1) / kB2TileI));
+    hipLaunchKernelGGL(b2_mfma_kernel, grid, dim3(256), 0, s, F, P, nf, npp, nfp, S1, S2a, xlogx, n,
+                       L, Lp, te_out, ld_out, counts_dbg);
+    TNX_CHECK_LAUNCH("b2_mfma_kernel");
+    return TNX_OK;
+}
+
+}  // extern "C"

@@ -2,11 +2,6 @@
 #include "common.hpp"
 using namespace tnx;
 extern "C" {
-int64_t tnx_gemm_b2_workspace(int64_t, int64_t, int32_t) { return 0; }
-int tnx_te_gemm_b2(const int8_t *, int64_t, int64_t, int64_t, int32_t, const double *,
-                   const double *, const float *, void *, int64_t, float *, int64_t, int32_t *,
-                   void *) { set_error("tnx_te_gemm_b2: not built"); return TNX_E_RANGE; }
-int tnx_surrogate_run(const tnx_surr_args_t *, void *) { set_error("tnx_surrogate_run: not built"); return TNX_E_RANGE; }
 int tnx_mlp_forward(const tnx_mlp_t *, const float *, int64_t, float *, void *) { set_error("not built"); return TNX_E_RANGE; }
 int tnx_mlp_backward(const tnx_mlp_t *, const float *, const float *, const float *, int64_t, float *, float *, void *) { set_error("not built"); return TNX_E_RANGE; }
 int tnx_mlp_adam(const tnx_mlp_t *, float *, const float *, float *, float *, int64_t, float, float, float, float, int32_t, void *) { set_error("not built"); return TNX_E_RANGE; }

@@ -2,59 +2,185 @@
 #pragma once
 #include "common.hpp"
 
+// Cells of privatised histogram copies a pair may use (see wave_copies).
+#ifndef TNX_WAVE_COPY_BUDGET
+#define TNX_WAVE_COPY_BUDGET 2048
+#endif
+#ifndef TNX_BLOCK_COPY_BUDGET
+#define TNX_BLOCK_COPY_BUDGET 1
+#endif
+
 namespace tnx {
 
 __device__ __forceinline__ void lds_inc(uint32_t *p) {
     __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Phase 1 (one wavefront): histogram of key = state[t] * by + y[t], t < L.
-// st: target states (uint16, 8-byte aligned row), yr: source bins (int8,
-// 4-byte aligned row).  Each lane takes 4 consecutive samples per step so a
-// wave reads 256 contiguous source bytes and 512 state bytes per step.
-__device__ __forceinline__ void hist_accumulate_wave(uint32_t *hist,
-                                                     const uint16_t *__restrict__ st,
-                                                     const int8_t *__restrict__ yr, int by, int L,
-                                                     int lane) {
-    const int Lfull = L & ~255;
-    int t0 = lane * 4;
-#pragma unroll 2
-    for (; t0 < Lfull; t0 += 256) {
-        const uint32_t yw = *reinterpret_cast<const uint32_t *>(yr + t0);
-        const uint2 sw = *reinterpret_cast<const uint2 *>(st + t0);
-        lds_inc(hist + (sw.x & 0xFFFFu) * by + (yw & 0xFFu));
-        lds_inc(hist + (sw.x >> 16) * by + ((yw >> 8) & 0xFFu));
-        lds_inc(hist + (sw.y & 0xFFFFu) * by + ((yw >> 16) & 0xFFu));
-        lds_inc(hist + (sw.y >> 16) * by + (yw >> 24));
-    }
-    if (t0 < L) {
-        const uint32_t yw = *reinterpret_cast<const uint32_t *>(yr + t0);
-        const uint2 sw = *reinterpret_cast<const uint2 *>(st + t0);
-        lds_inc(hist + (sw.x & 0xFFFFu) * by + (yw & 0xFFu));
-        if (t0 + 1 < L) lds_inc(hist + (sw.x >> 16) * by + ((yw >> 8) & 0xFFu));
-        if (t0 + 2 < L) lds_inc(hist + (sw.y & 0xFFFFu) * by + ((yw >> 16) & 0xFFu));
-        if (t0 + 3 < L) lds_inc(hist + (sw.y >> 16) * by + (yw >> 24));
+// 16 samples of one lane: 8 packed 16-bit states, 4 source dwords.
+__device__ __forceinline__ void acc16(uint32_t *hist, const uint32_t *sreg, const uint4 y4,
+                                      uint32_t by) {
+    const uint32_t yw[4] = {y4.x, y4.y, y4.z, y4.w};
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const uint32_t sv = (sreg[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
+        const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+        lds_inc(hist + sv * by + yv);
     }
 }
 
-// Phases 2-4 (one wavefront): S3 over the b_x^2 b_y cells and S2b over the
-// b_x b_y (x_t, y_t) columns, zeroing the histogram for the next pair.
-// Cell (a, b, c) sits at a * (b_x b_y) + (b * b_y + c).
-__device__ __forceinline__ void hist_reduce_wave(uint32_t *hist, int bx, int by,
-                                                 const float2 *__restrict__ tbl, int lane, DF &s3,
-                                                 DF &s2b) {
+__device__ __forceinline__ void acc16_tail(uint32_t *hist, const uint32_t *sreg, const uint4 y4,
+                                           uint32_t by, int n_valid) {
+    const uint32_t yw[4] = {y4.x, y4.y, y4.z, y4.w};
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        if (q < n_valid) {
+            const uint32_t sv = (sreg[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
+            const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+            lds_inc(hist + sv * by + yv);
+        }
+    }
+}
+
+// Phase 1 (one wavefront), target states read from memory: key =
+// state[t] * by + y[t] for t < L.  Lane `lane` takes 16 consecutive samples
+// per step (one dwordx4 of source bytes, two of states), two steps in
+// flight.  Rows must be padded to a multiple of 64 elements.
+__device__ __forceinline__ void hist_accumulate_mem(uint32_t *hist, const uint16_t *__restrict__ st,
+                                                    const int8_t *__restrict__ yr, uint32_t by,
+                                                    int L, int lane) {
+    int t = 16 * lane;
+    for (; t + 1024 + 16 <= L; t += 2048) {
+        const uint4 ya = *reinterpret_cast<const uint4 *>(yr + t);
+        const uint4 sa0 = *reinterpret_cast<const uint4 *>(st + t);
+        const uint4 sa1 = *reinterpret_cast<const uint4 *>(st + t + 8);
+        const uint4 yb = *reinterpret_cast<const uint4 *>(yr + t + 1024);
+        const uint4 sb0 = *reinterpret_cast<const uint4 *>(st + t + 1024);
+        const uint4 sb1 = *reinterpret_cast<const uint4 *>(st + t + 1032);
+        const uint32_t sa[8] = {sa0.x, sa0.y, sa0.z, sa0.w, sa1.x, sa1.y, sa1.z, sa1.w};
+        const uint32_t sb[8] = {sb0.x, sb0.y, sb0.z, sb0.w, sb1.x, sb1.y, sb1.z, sb1.w};
+        acc16(hist, sa, ya, by);
+        acc16(hist, sb, yb, by);
+    }
+    for (; t < L; t += 1024) {
+        const uint4 ya = *reinterpret_cast<const uint4 *>(yr + t);
+        const uint4 sa0 = *reinterpret_cast<const uint4 *>(st + t);
+        const uint4 sa1 = *reinterpret_cast<const uint4 *>(st + t + 8);
+        const uint32_t sa[8] = {sa0.x, sa0.y, sa0.z, sa0.w, sa1.x, sa1.y, sa1.z, sa1.w};
+        acc16_tail(hist, sa, ya, by, L - t);
+    }
+}
+
+// Register-resident target: lane `lane` owns samples
+// t = 16 * lane + 1024 * i + q (q < 16) for chunks i < NI, holding their
+// 16-bit states as 8 packed dwords per chunk.  Rows must be padded so that
+// a 16-sample chunk starting below L stays inside the row (true for rows
+// padded to a multiple of 64 elements).
+template <int NI>
+struct TargetRegs {
+    uint32_t s[NI * 8];
+};
+
+template <int NI>
+__device__ __forceinline__ void load_target_regs(TargetRegs<NI> &r,
+                                                 const uint16_t *__restrict__ st, int L,
+                                                 int lane) {
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+        const int t = 16 * lane + 1024 * i;
+        if (t < L) {
+            const uint4 a = *reinterpret_cast<const uint4 *>(st + t);
+            const uint4 b = *reinterpret_cast<const uint4 *>(st + t + 8);
+            r.s[8 * i + 0] = a.x; r.s[8 * i + 1] = a.y; r.s[8 * i + 2] = a.z; r.s[8 * i + 3] = a.w;
+            r.s[8 * i + 4] = b.x; r.s[8 * i + 5] = b.y; r.s[8 * i + 6] = b.z; r.s[8 * i + 7] = b.w;
+        }
+    }
+}
+
+// Phase 1 with the target in registers: 16 source bytes per lane per chunk
+// (one dwordx4 load), 16 LDS increments.
+template <int NI>
+__device__ __forceinline__ void hist_accumulate_regs(uint32_t *hist, TargetRegs<NI> &r,
+                                                     const int8_t *__restrict__ yr, uint32_t by,
+                                                     int L, int lane) {
+    // Keep the packed states opaque per call: otherwise the compiler hoists
+    // the per-sample 16-bit extractions out of the caller's source loop and
+    // spends a register on every sample.
+#pragma unroll
+    for (int k = 0; k < NI * 8; k++) asm volatile("" : "+v"(r.s[k]));
+    const uint32_t *sreg = r.s;
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+        const int t = 16 * lane + 1024 * i;
+        if (t >= L) break;
+        const uint4 y4 = *reinterpret_cast<const uint4 *>(yr + t);
+        const uint32_t yw[4] = {y4.x, y4.y, y4.z, y4.w};
+        if (t + 16 <= L) {
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
+                const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                lds_inc(hist + sv * by + yv);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                if (t + q < L) {
+                    const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
+                    const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                    lds_inc(hist + sv * by + yv);
+                }
+            }
+        }
+    }
+}
+
+// Privatised histograms: a pair with `cells` cells gets R copies at an odd
+// stride S >= cells inside the wave's cap-cell LDS slice; lane l counts into
+// copy l % R.  Lanes whose samples share a key then mostly hit different
+// copies, and the odd stride puts one key's copies in different LDS banks,
+// which removes the same-address serialisation that dominates small
+// histograms (a pair with 8 cells would otherwise see ~20 lanes of a wave
+// on one address).
+struct WaveCopies {
+    int S, R;
+};
+
+// budget bounds R * S: the reduction reads and zeroes R * S cells per pair,
+// so copies are only worth it for small histograms.
+__device__ __forceinline__ WaveCopies wave_copies(int cells, int cap, int budget) {
+    WaveCopies c;
+    c.S = cells | 1;
+    int lim = cap < budget ? cap : budget;
+    c.R = lim / c.S;
+    if (c.R > 32) c.R = 32;
+    if (c.R < 1) c.R = 1;
+    return c;
+}
+
+// Sum of the R copies of a cell; zeroes them.
+__device__ __forceinline__ uint32_t take_cell(uint32_t *hist, int cell, const WaveCopies &cp) {
+    uint32_t v = 0;
+    uint32_t *p = hist + cell;
+    for (int r = 0; r < cp.R; r++, p += cp.S) {
+        v += *p;
+        *p = 0;
+    }
+    return v;
+}
+
+// Phases 2-4 with copies: as hist_reduce_wave.
+__device__ __forceinline__ void hist_reduce_wave_copies(uint32_t *hist, int bx, int by,
+                                                        const WaveCopies &cp,
+                                                        const float2 *__restrict__ tbl, int lane,
+                                                        DF &s3, DF &s2b) {
     const int ncol = bx * by;
     DF a3 = {0.f, 0.f}, a2 = {0.f, 0.f};
     for (int col = lane; col < ncol; col += kWave) {
         uint32_t c2 = 0;
-        uint32_t *p = hist + col;
-        for (int aa = 0; aa < bx; aa++, p += ncol) {
-            const uint32_t v = *p;
-            if (v) {
-                *p = 0;
-                c2 += v;
-                xlogx_add(a3, tbl, v);
-            }
+        for (int aa = 0; aa < bx; aa++) {
+            const uint32_t v = take_cell(hist, aa * ncol + col, cp);
+            c2 += v;
+            xlogx_add(a3, tbl, v);
         }
         xlogx_add(a2, tbl, c2);
     }
@@ -89,10 +215,13 @@ __device__ __forceinline__ void df_block_sum2(DF &u, DF &v, float *red) {
 
 // Whole workgroup on one pair with a histogram of cap cells: passes over
 // ranges [a0, a1) of x_{t+tau} so any b_x^2 b_y fits; c2b (b_x b_y cells)
-// accumulates the (x_t, y_t) marginal across passes.  S3, S2b valid in
-// thread 0.  hist must be zero on entry and is zero on exit.
+// accumulates the (x_t, y_t) marginal across passes.  The target's states
+// are in registers (thread tid owns samples 16 tid + 16 nth i + q, NB chunks,
+// L <= 16 nth NB); a pass with few cells counts into privatised copies.
+// S3, S2b valid in thread 0.  hist must be zero on entry and is zero on exit.
+template <int NB>
 __device__ __forceinline__ void hist_pair_block(uint32_t *hist, uint32_t *c2b, int cap,
-                                                const uint16_t *__restrict__ st,
+                                                const TargetRegs<NB> &regs,
                                                 const int8_t *__restrict__ yr, int bx, int by,
                                                 int L, const float2 *__restrict__ tbl, float *red,
                                                 DF &s3, DF &s2b) {
@@ -105,32 +234,36 @@ __device__ __forceinline__ void hist_pair_block(uint32_t *hist, uint32_t *c2b, i
     for (int a0 = 0; a0 < bx; a0 += apass) {
         const int a1 = (a0 + apass < bx) ? (a0 + apass) : bx;
         const uint32_t lo = (uint32_t)(a0 * bx), hi = (uint32_t)(a1 * bx);
-        const uint32_t base = (uint32_t)(a0 * ncol);
+        const int cells = (a1 - a0) * ncol;
+        const WaveCopies cp = wave_copies(cells, cap, TNX_BLOCK_COPY_BUDGET);
+        uint32_t *hc = hist + (tid % cp.R) * cp.S - a0 * ncol;  // cell index of the full pair
         __syncthreads();
-        for (int t0 = tid * 4; t0 < L; t0 += 4 * nth) {
-            const uint32_t yw = *reinterpret_cast<const uint32_t *>(yr + t0);
-            const uint2 sw = *reinterpret_cast<const uint2 *>(st + t0);
-            const uint32_t s[4] = {sw.x & 0xFFFFu, sw.x >> 16, sw.y & 0xFFFFu, sw.y >> 16};
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                if (t0 + q < L && s[q] >= lo && s[q] < hi)
-                    lds_inc(hist + s[q] * by + ((yw >> (8 * q)) & 0xFFu) - base);
+        for (int i = 0; i < NB; i++) {
+            const int t = 16 * tid + 16 * nth * i;
+            if (t < L) {
+                const uint4 y4 = *reinterpret_cast<const uint4 *>(yr + t);
+                const uint32_t yw[4] = {y4.x, y4.y, y4.z, y4.w};
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    const uint32_t sv = (regs.s[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
+                    const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                    if (t + q < L && sv >= lo && sv < hi) lds_inc(hc + sv * by + yv);
+                }
             }
         }
         __syncthreads();
-        const int rows = a1 - a0;
-        for (int col = tid; col < ncol; col += nth) {
-            uint32_t c2 = c2b[col];
-            uint32_t *p = hist + col;
-            for (int aa = 0; aa < rows; aa++, p += ncol) {
-                const uint32_t v = *p;
-                if (v) {
-                    *p = 0;
-                    c2 += v;
-                    xlogx_add(a3, tbl, v);
-                }
-            }
-            c2b[col] = c2;
+        // flattened scan over the pass's cells: coalesced, independent lookups
+        const int step = nth % ncol;
+        int col = tid % ncol;
+#pragma unroll 4
+        for (int i = tid; i < cells; i += nth) {
+            const uint32_t v = take_cell(hist, i, cp);
+            const float2 tv = tbl[v];  // tbl[0] = tbl[1] = 0
+            df_add(a3, tv.x, tv.y);
+            if (v) __hip_atomic_fetch_add(c2b + col, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            col += step;
+            if (col >= ncol) col -= ncol;
         }
     }
     __syncthreads();

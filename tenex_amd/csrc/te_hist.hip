@@ -34,8 +34,11 @@ namespace tnx {
 
 constexpr int kWavesPerWG = 4;
 constexpr int kBlockThreads = 512;
-constexpr int kDefaultCapWave = 2048;
-constexpr int kDefaultCapBlock = 16384;
+#ifndef TNX_DEFAULT_CAP_WAVE
+#define TNX_DEFAULT_CAP_WAVE 3200  // 12.8 KiB per wave: 3 workgroups of 4 waves per CU
+#endif
+constexpr int kDefaultCapWave = TNX_DEFAULT_CAP_WAVE;
+constexpr int kDefaultCapBlock = 16384;  // 64 KiB + the b_t b_s marginal
 constexpr int kWaveItemPairs = 256;  // pairs per wave-kernel work item
 constexpr int kBlockItemPairs = 16;  // pairs per block-kernel work item
 
@@ -50,7 +53,16 @@ __device__ __forceinline__ bool out_slot(const tnx_te_args_t &a, int x, int y, i
     return slot >= 0 && slot < a.n_pairs_local;
 }
 
-__global__ __launch_bounds__(256) void te_wave_kernel(tnx_te_args_t a) {
+// Wave-per-pair histogram kernel.  The four waves of a workgroup take the
+// (up to four) targets of a work item and stream the same source rows, which
+// stay in L1.  NI > 0: each wave keeps its target's states in registers
+// (L <= NI * 1024) so only the source row is read per pair; NI == 0: states
+// are re-read from memory (any L).  Every pair counts into privatised copies
+// of its histogram (wave_copies) inside the wave's cap_wave-cell LDS slice.
+// The register variants are limited to 3 waves per SIMD so the state
+// registers do not spill; the LDS slice is sized for that occupancy.
+template <int NI>
+__device__ __forceinline__ void te_wave_body(const tnx_te_args_t &a) {
     extern __shared__ uint32_t lds_u32[];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -60,27 +72,37 @@ __global__ __launch_bounds__(256) void te_wave_kernel(tnx_te_args_t a) {
     const int L = (int)(a.T - a.tau);
     const float2 *tbl = reinterpret_cast<const float2 *>(a.xlogx);
     const double invN = 1.0 / (double)L;
+    TargetRegs<(NI > 0 ? NI : 1)> regs;
     for (int64_t w = blockIdx.x; w < a.n_wave_items; w += gridDim.x) {
         const tnx_work_t it = a.work[w];
         const int nsrc = it.src_end - it.src_beg;
         const int npair = it.tgt_cnt * nsrc;
+        int cur_x = -1;
         for (int q = wave; q < npair; q += kWavesPerWG) {
             const int ti = q % it.tgt_cnt, sj = q / it.tgt_cnt;
             const int x = a.tgt_order[it.tgt_beg + ti];
             const int y = a.src_order[it.src_beg + sj];
-            if (x == y) continue;
             int64_t slot;
-            if (!out_slot(a, x, y, slot)) continue;
+            const bool want = (x != y) && out_slot(a, x, y, slot);
             const int bx = a.n_per_var[x], by = a.n_per_var[y];
-            if (bx * bx * by > a.cap_wave) {  // plan does not match n_per_var
-                if (lane == 0) a.out[slot] = __builtin_nanf("");
-                continue;
+            const bool fits = bx * bx * by <= a.cap_wave;
+            if (want && !fits && lane == 0) a.out[slot] = __builtin_nanf("");  // plan mismatch
+            if (!(want && fits)) continue;
+            const WaveCopies cp = wave_copies(bx * bx * by, a.cap_wave, TNX_WAVE_COPY_BUDGET);
+            uint32_t *hc = hist + (lane % cp.R) * cp.S;
+            const int8_t *yr = a.bins + (int64_t)y * a.ld;
+            if constexpr (NI > 0) {
+                if (x != cur_x) {
+                    load_target_regs<NI>(regs, a.states + (int64_t)x * a.lds, L, lane);
+                    cur_x = x;
+                }
+                hist_accumulate_regs<NI>(hc, regs, yr, (uint32_t)by, L, lane);
+            } else {
+                hist_accumulate_mem(hc, a.states + (int64_t)x * a.lds, yr, (uint32_t)by, L, lane);
             }
-            hist_accumulate_wave(hist, a.states + (int64_t)x * a.lds, a.bins + (int64_t)y * a.ld,
-                                 by, L, lane);
             wave_lds_fence();
             DF s3, s2b;
-            hist_reduce_wave(hist, bx, by, tbl, lane, s3, s2b);
+            hist_reduce_wave_copies(hist, bx, by, cp, tbl, lane, s3, s2b);
             wave_lds_fence();
             if (lane == 0) {
                 double te = (df_to_double(s3) + a.S1[x] - a.S2a[x] - df_to_double(s2b)) * invN;
@@ -90,9 +112,21 @@ __global__ __launch_bounds__(256) void te_wave_kernel(tnx_te_args_t a) {
     }
 }
 
+__global__ __launch_bounds__(256) void te_wave_kernel(tnx_te_args_t a) { te_wave_body<0>(a); }
+
+template <int NI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void te_wave_reg_kernel(
+    tnx_te_args_t a) {
+    te_wave_body<NI>(a);
+}
+
+// Workgroup-per-pair kernel for pairs above cap_wave cells (multi-pass, exact).
+// The target's states are loaded into registers once per target and reused
+// for every source and pass.  Requires L <= 16 * kBlockThreads * NB.
+template <int NB>
 __global__ __launch_bounds__(kBlockThreads) void te_block_kernel(tnx_te_args_t a) {
     extern __shared__ uint32_t lds_u32[];
-    __shared__ float red[2 * 2 * (kBlockThreads / kWave)];
+    __shared__ float red[4 * (kBlockThreads / kWave)];
     uint32_t *hist = lds_u32;
     uint32_t *c2b = lds_u32 + a.cap_block;
     const int tid = threadIdx.x;
@@ -102,23 +136,38 @@ __global__ __launch_bounds__(kBlockThreads) void te_block_kernel(tnx_te_args_t a
     const float2 *tbl = reinterpret_cast<const float2 *>(a.xlogx);
     const double invN = 1.0 / (double)L;
     const tnx_work_t *work = a.work + a.n_wave_items;
+    TargetRegs<NB> regs;
     for (int64_t w = blockIdx.x; w < a.n_block_items; w += gridDim.x) {
         const tnx_work_t it = work[w];
         for (int ti = 0; ti < it.tgt_cnt; ti++) {
             const int x = a.tgt_order[it.tgt_beg + ti];
+            const int bx = a.n_per_var[x];
+            {
+                const uint16_t *st = a.states + (int64_t)x * a.lds;
+#pragma unroll
+                for (int i = 0; i < NB; i++) {
+                    const int t = 16 * tid + 16 * kBlockThreads * i;
+                    if (t < L) {
+                        const uint4 u = *reinterpret_cast<const uint4 *>(st + t);
+                        const uint4 v = *reinterpret_cast<const uint4 *>(st + t + 8);
+                        regs.s[8 * i + 0] = u.x; regs.s[8 * i + 1] = u.y;
+                        regs.s[8 * i + 2] = u.z; regs.s[8 * i + 3] = u.w;
+                        regs.s[8 * i + 4] = v.x; regs.s[8 * i + 5] = v.y;
+                        regs.s[8 * i + 6] = v.z; regs.s[8 * i + 7] = v.w;
+                    }
+                }
+            }
             for (int sj = it.src_beg; sj < it.src_end; sj++) {
                 const int y = a.src_order[sj];
-                if (x == y) continue;
                 int64_t slot;
-                if (!out_slot(a, x, y, slot)) continue;
-                const int bx = a.n_per_var[x], by = a.n_per_var[y];
-                if (bx * by > a.max_block_cols) {  // plan does not match n_per_var
-                    if (tid == 0) a.out[slot] = __builtin_nanf("");
-                    continue;
-                }
+                const bool want = (x != y) && out_slot(a, x, y, slot);
+                const int by = a.n_per_var[y];
+                const bool fits = bx * by <= a.max_block_cols;
+                if (want && !fits && tid == 0) a.out[slot] = __builtin_nanf("");  // plan mismatch
+                if (!(want && fits)) continue;
                 DF s3, s2b;
-                hist_pair_block(hist, c2b, a.cap_block, a.states + (int64_t)x * a.lds,
-                                a.bins + (int64_t)y * a.ld, bx, by, L, tbl, red, s3, s2b);
+                hist_pair_block<NB>(hist, c2b, a.cap_block, regs, a.bins + (int64_t)y * a.ld, bx,
+                                    by, L, tbl, red, s3, s2b);
                 if (tid == 0) {
                     double te =
                         (df_to_double(s3) + a.S1[x] - a.S2a[x] - df_to_double(s2b)) * invN;
@@ -129,7 +178,9 @@ __global__ __launch_bounds__(kBlockThreads) void te_block_kernel(tnx_te_args_t a
     }
 }
 
-// Debug: counts of explicit pairs through the wave accumulation path.
+// Debug: counts of explicit pairs through the same accumulation code as the
+// wave kernel tnx_te_run launches at this L (NI as in wave_variant()).
+template <int NI>
 __global__ __launch_bounds__(256) void te_counts_kernel(
     const int8_t *__restrict__ bins, const uint16_t *__restrict__ states,
     const int32_t *__restrict__ npv, const double *__restrict__ S1,
@@ -152,15 +203,37 @@ __global__ __launch_bounds__(256) void te_counts_kernel(
         if (lane == 0) te_out[p] = __builtin_nanf("");
         return;
     }
-    hist_accumulate_wave(hist, states + (int64_t)x * lds, bins + (int64_t)y * ld, by, L, lane);
-    wave_lds_fence();
     const int cells = bx * bx * by;
-    for (int i = lane; i < cells; i += kWave) counts[p * cap + i] = (int32_t)hist[i];
+    const WaveCopies cp = wave_copies(cells, cap, TNX_WAVE_COPY_BUDGET);
+    uint32_t *hc = hist + (lane % cp.R) * cp.S;
+    if constexpr (NI > 0) {
+        TargetRegs<NI> regs;
+        load_target_regs<NI>(regs, states + (int64_t)x * lds, L, lane);
+        hist_accumulate_regs<NI>(hc, regs, bins + (int64_t)y * ld, (uint32_t)by, L, lane);
+    } else {
+        hist_accumulate_mem(hc, states + (int64_t)x * lds, bins + (int64_t)y * ld, (uint32_t)by, L,
+                            lane);
+    }
+    wave_lds_fence();
+    for (int i = lane; i < cells; i += kWave) {
+        uint32_t v = 0;
+        for (int r = 0; r < cp.R; r++) v += hist[r * cp.S + i];
+        counts[p * cap + i] = (int32_t)v;
+    }
     wave_lds_fence();
     DF s3, s2b;
-    hist_reduce_wave(hist, bx, by, tbl, lane, s3, s2b);
+    hist_reduce_wave_copies(hist, bx, by, cp, tbl, lane, s3, s2b);
     if (lane == 0)
         te_out[p] = (float)((df_to_double(s3) + S1[x] - S2a[x] - df_to_double(s2b)) / (double)L);
+}
+
+// Register-variant choice for a series of L samples: the smallest
+// instantiated NI with NI * 1024 >= L, 0 (memory variant) above 12288.
+static int wave_variant(int64_t L) {
+    static const int kNI[] = {1, 2, 3, 4, 6, 8, 10, 12};
+    for (int ni : kNI)
+        if (L <= (int64_t)ni * 1024) return ni;
+    return 0;
 }
 
 }  // namespace tnx
@@ -273,15 +346,32 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
     TNX_CHECK_ARG(a.out_mode == TNX_OUT_MATRIX || a.out_mode == TNX_OUT_PAIRFREE,
                   "tnx_te_run: bad out_mode");
     TNX_CHECK_ARG(a.cap_wave >= 8 && a.cap_wave <= 8192, "tnx_te_run: cap_wave out of range");
+    TNX_CHECK_ARG((size_t)kWavesPerWG * a.cap_wave * 4 <= 160 * 1024, "tnx_te_run: cap_wave too large");
     TNX_CHECK_ARG(a.cap_block >= 127 * 127, "tnx_te_run: cap_block too small");
     hipStream_t s = as_stream(stream);
     if (a.n_wave_items > 0) {
         TNX_CHECK_ARG(a.work, "tnx_te_run: null work");
         size_t shm = (size_t)kWavesPerWG * a.cap_wave * 4;
-        TNX_CHECK_HIP(hipFuncSetAttribute((const void *)te_wave_kernel,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         unsigned grid = (unsigned)std::min<int64_t>(a.n_wave_items, 1 << 20);
-        hipLaunchKernelGGL(te_wave_kernel, dim3(grid), dim3(256), shm, s, a);
+        const int ni = wave_variant(a.T - a.tau);
+        const void *kfn = (const void *)te_wave_kernel;
+        switch (ni) {
+#define TNX_WAVE_FN(NI) \
+    case NI: kfn = (const void *)te_wave_reg_kernel<NI>; break;
+            TNX_WAVE_FN(1) TNX_WAVE_FN(2) TNX_WAVE_FN(3) TNX_WAVE_FN(4) TNX_WAVE_FN(6)
+            TNX_WAVE_FN(8) TNX_WAVE_FN(10) TNX_WAVE_FN(12)
+#undef TNX_WAVE_FN
+            default: break;
+        }
+        TNX_CHECK_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        switch (ni) {
+#define TNX_WAVE_LAUNCH(NI) \
+    case NI: hipLaunchKernelGGL(te_wave_reg_kernel<NI>, dim3(grid), dim3(256), shm, s, a); break;
+            TNX_WAVE_LAUNCH(1) TNX_WAVE_LAUNCH(2) TNX_WAVE_LAUNCH(3) TNX_WAVE_LAUNCH(4)
+            TNX_WAVE_LAUNCH(6) TNX_WAVE_LAUNCH(8) TNX_WAVE_LAUNCH(10) TNX_WAVE_LAUNCH(12)
+#undef TNX_WAVE_LAUNCH
+            default: hipLaunchKernelGGL(te_wave_kernel, dim3(grid), dim3(256), shm, s, a); break;
+        }
         TNX_CHECK_LAUNCH("te_wave_kernel");
     }
     if (a.n_block_items > 0) {
@@ -290,10 +380,25 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
                       "tnx_te_run: bad max_block_cols");
         size_t shm = (size_t)(a.cap_block + a.max_block_cols) * 4;
         TNX_CHECK_ARG(shm <= 160 * 1024, "tnx_te_run: block kernel LDS %zu exceeds 160 KiB", shm);
-        TNX_CHECK_HIP(hipFuncSetAttribute((const void *)te_block_kernel,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         unsigned grid = (unsigned)std::min<int64_t>(a.n_block_items, 1 << 20);
-        hipLaunchKernelGGL(te_block_kernel, dim3(grid), dim3(kBlockThreads), shm, s, a);
+        const int64_t L = a.T - a.tau;
+        const int nb = (int)((L + 16 * kBlockThreads - 1) / (16 * kBlockThreads));
+        TNX_CHECK_ARG(nb <= 16, "tnx_te_run: L = %lld too long for the block kernel (max %d)",
+                      (long long)L, 16 * 16 * kBlockThreads);
+        const void *kfn = nullptr;
+        const int nbi = nb <= 1 ? 1 : nb <= 2 ? 2 : nb <= 4 ? 4 : nb <= 8 ? 8 : 16;
+        switch (nbi) {
+#define TNX_BLOCK(NB)                                                                     \
+    case NB:                                                                              \
+        kfn = (const void *)te_block_kernel<NB>;                                          \
+        TNX_CHECK_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                          (int)shm));                                     \
+        hipLaunchKernelGGL(te_block_kernel<NB>, dim3(grid), dim3(kBlockThreads), shm, s, a); \
+        break;
+            TNX_BLOCK(1) TNX_BLOCK(2) TNX_BLOCK(4) TNX_BLOCK(8) TNX_BLOCK(16)
+#undef TNX_BLOCK
+            default: break;
+        }
         TNX_CHECK_LAUNCH("te_block_kernel");
     }
     return TNX_OK;
@@ -307,17 +412,27 @@ int tnx_te_counts_debug(const int8_t *bins, const uint16_t *states, const int32_
     clear_error();
     TNX_CHECK_ARG(bins && states && n_per_var && S1 && S2a && xlogx && pairs && counts_out && te_out,
                   "tnx_te_counts_debug: null pointer");
-    TNX_CHECK_ARG(max_cells >= 8 && max_cells <= 8192, "tnx_te_counts_debug: max_cells in [8, 8192]");
+    TNX_CHECK_ARG(max_cells >= 8 && max_cells <= 8192 && max_cells % 2 == 0, "tnx_te_counts_debug: max_cells in [8, 8192]");
     TNX_CHECK_ARG(ld % 16 == 0 && lds % 8 == 0 && tau >= 1 && tau < T,
                   "tnx_te_counts_debug: bad strides / tau");
     if (n_pairs == 0) return TNX_OK;
     size_t shm = (size_t)kWavesPerWG * max_cells * 4;
-    TNX_CHECK_HIP(hipFuncSetAttribute((const void *)te_counts_kernel,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     unsigned grid = (unsigned)((n_pairs + kWavesPerWG - 1) / kWavesPerWG);
-    hipLaunchKernelGGL(te_counts_kernel, dim3(grid), dim3(256), shm, as_stream(stream), bins,
-                       states, n_per_var, S1, S2a, xlogx, T, ld, lds, tau, pairs, n_pairs,
-                       max_cells, counts_out, te_out);
+    const int ni = wave_variant(T - tau);
+    switch (ni) {
+#define TNX_COUNTS(NI)                                                                         \
+    case NI:                                                                                   \
+        TNX_CHECK_HIP(hipFuncSetAttribute((const void *)te_counts_kernel<NI>,                  \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm)); \
+        hipLaunchKernelGGL(te_counts_kernel<NI>, dim3(grid), dim3(256), shm, as_stream(stream),  \
+                           bins, states, n_per_var, S1, S2a, xlogx, T, ld, lds, tau, pairs,      \
+                           n_pairs, max_cells, counts_out, te_out);                            \
+        break;
+        TNX_COUNTS(0) TNX_COUNTS(1) TNX_COUNTS(2) TNX_COUNTS(3) TNX_COUNTS(4) TNX_COUNTS(6)
+        TNX_COUNTS(8) TNX_COUNTS(10) TNX_COUNTS(12)
+#undef TNX_COUNTS
+        default: break;
+    }
     TNX_CHECK_LAUNCH("te_counts_kernel");
     return TNX_OK;
 }

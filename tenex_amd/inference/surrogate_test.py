@@ -1,0 +1,243 @@
+"""Time-axis shuffle surrogate test (tenex/inference/surrogate_test.py:586-948).
+
+For every ordered pair, n_surrogates block-shuffled copies of the data give a
+null distribution of TE; effective TE = observed - mean(surrogate TE); a
+per-pair Gaussian z-test (parametric) or the Monte Carlo rank (mc) gives
+p-values; BH-FDR over the tested pairs and effective TE > 0 give the GRN.
+
+Paths:
+  fused (default)  the gfx950 kernel surrogate_wave_kernel computes every
+                   (pair, surrogate) histogram and accumulates FP64
+                   statistics on the device (the role of the reference's
+                   fused Full/Adaptive-SMEM surrogate kernels).
+  fused=False      loop path: per surrogate, shuffle the bins on the device
+                   (torch gather) and run the TE kernels on the shuffled data,
+                   accumulating in FP64 (the reference's loop path,
+                   surrogate_test.py:322-468).  Same numbers, kept as an
+                   independent cross-check.
+  learned='mlp'    learned null (no reference counterpart): see
+                   tenex_amd/inference/learned_null.py.
+
+Permutations follow the reference schedule exactly: surrogate k uses
+torch.Generator(rng_device).manual_seed(seed + k), rand(n_vars, n_blocks),
+argsort (surrogate_test.py:165-207); rng_device defaults to the compute
+device as in the reference's GPU path.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .. import core
+from .._log import vprint
+from ..kernels import get_kernel
+from . import GRN, InferenceMethod, bh_fdr_gpu, build_pairs, make_grn
+
+
+@dataclass
+class SurrogateTestResult:
+    """observed_te, mean_surrogate_te, std_surrogate_te, effective_te,
+    p_values: (n, n) float32 in the public layout [i, j] = TE(i -> j);
+    grn; n_surrogates; shuffle_method; block_length; p_method; fdr
+    (tenex/inference/surrogate_test.py:74-104)."""
+    observed_te: np.ndarray
+    mean_surrogate_te: np.ndarray
+    std_surrogate_te: np.ndarray
+    effective_te: np.ndarray
+    p_values: np.ndarray
+    grn: GRN
+    n_surrogates: int
+    shuffle_method: str
+    block_length: int | None
+    p_method: str
+    fdr: float
+
+
+def shuffle_bins(bins: torch.Tensor, T: int, perm_k: torch.Tensor, eff_bl: int) -> torch.Tensor:
+    """Shuffled copy of dense bins (n, T) by per-variable block order perm_k
+    (n, n_blocks) (_block_shuffle_gpu / _random_shuffle_gpu,
+    surrogate_test.py:107-162): head blocks reordered, tail untouched."""
+    n = bins.shape[0]
+    nb = perm_k.shape[1]
+    head = nb * eff_bl
+    src = bins[:, :T]
+    h = src[:, :head].reshape(n, nb, eff_bl)
+    idx = perm_k.to(torch.int64)[:, :, None].expand(-1, -1, eff_bl)
+    out = torch.gather(h, 1, idx).reshape(n, head)
+    if T > head:
+        out = torch.cat([out, src[:, head:]], dim=1)
+    return out
+
+
+def surrogate_accumulate(prep: core.Prepared, observed_pub: torch.Tensor, n_surrogates: int,
+                         shuffle_method: str, block_length: int, seed: int, rng_device,
+                         fused: bool = True, targets=None, sources=None, perm_batch: int = 32,
+                         k_start: int = 0, te_k_dbg: torch.Tensor | None = None):
+    """Device (n, n) public-layout sum, sumsq (float64) and count_ge (int32)
+    over surrogates [k_start, k_start + n_surrogates)."""
+    n, T = prep.n_vars, prep.T
+    dev = prep.device
+    s = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    s2 = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    cg = torch.zeros((n, n), dtype=torch.int32, device=dev)
+    k_end = k_start + n_surrogates
+    if fused:
+        plan = core.surrogate_plan(prep, targets, sources)
+        for k0 in range(k_start, k_end, perm_batch):
+            k1 = min(k0 + perm_batch, k_end)
+            perm, eff_bl, nb = core.block_perm_device(n, T, shuffle_method, block_length, k0, k1,
+                                                      seed, rng_device, dev)
+            dbg = None if te_k_dbg is None else te_k_dbg[:, k0 - k_start:k1 - k_start]
+            if dbg is not None:
+                tmp = torch.zeros((n * n, k1 - k0), dtype=torch.float32, device=dev)
+                core.surrogate_run(prep, plan, perm, eff_bl, observed_pub, s, s2, cg, tmp)
+                dbg.copy_(tmp)
+            else:
+                core.surrogate_run(prep, plan, perm, eff_bl, observed_pub, s, s2, cg)
+        return s, s2, cg
+    # loop path: shuffle then the TE kernels
+    for k in range(k_start, k_end):
+        perm, eff_bl, nb = core.block_perm_device(n, T, shuffle_method, block_length, k, k + 1,
+                                                  seed, rng_device, dev)
+        shuf = shuffle_bins(prep.bins, T, perm[0], eff_bl)
+        sb = torch.zeros_like(prep.bins)
+        sb[:, :T] = shuf
+        sp = core.prepare(sb, prep.npv, T, prep.tau)
+        te_int = core.te_matrix_internal(sp, targets, sources)
+        te_k = te_int.t()  # public layout
+        te_d = te_k.to(torch.float64)
+        s += te_d
+        s2 += te_d * te_d
+        cg += (te_k >= observed_pub).to(torch.int32)
+    return s, s2, cg
+
+
+def surrogate_decision(observed_te, sum_s, sumsq_s, count_ge, n_surr, p_method, fdr, mask,
+                       device):
+    """Statistics -> p-values -> BH -> edges, with the reference's arithmetic
+    (surrogate_test.py:890-929): float32 mean/std from FP64 sums, z in
+    float32, erfc in float64, BH in float32 on the device."""
+    mean = (sum_s / n_surr).astype(np.float32)
+    var = sumsq_s / n_surr - (sum_s / n_surr) ** 2
+    std = np.sqrt(np.maximum(var, 0.0)).astype(np.float32)
+    eff = (observed_te - mean).astype(np.float32)
+    if p_method == "parametric":
+        from scipy.special import erfc
+        eps = np.float32(1e-12)
+        z = (observed_te - mean) / np.maximum(std, eps)
+        p = (0.5 * erfc(z / np.sqrt(2.0))).astype(np.float32)
+    else:
+        p = ((1.0 + count_ge) / (1.0 + n_surr)).astype(np.float32)
+    np.fill_diagonal(p, 1.0)
+    p_off = p[mask]
+    adj_off = bh_fdr_gpu(torch.from_numpy(p_off).to(device), alpha=fdr).cpu().numpy()
+    sig = np.zeros(p.shape, dtype=bool)
+    sig[mask] = adj_off < fdr
+    sig &= eff > 0
+    return mean, std, eff, p, sig
+
+
+class SurrogateTestMethod(InferenceMethod):
+    """Surrogate-based test with effective-TE bias correction."""
+
+    @property
+    def name(self) -> str:
+        return "surrogate_test"
+
+    def infer(self, te_matrix, variable_names, device, **kwargs) -> SurrogateTestResult:
+        """kwargs as the reference (bin_data, n_per_var, b_max, kernel, tau,
+        n_surrogates=100, shuffle_method='block', block_length=None,
+        p_method='parametric', fdr=0.01, seed=42, fused=None, sources) plus
+        rng_device (generator device, default: the compute device),
+        pairs_scope ('all' | 'sources': compute only source pairs) and
+        learned (None | 'mlp', with learned_kwargs)."""
+        bin_data = kwargs.get("bin_data")
+        n_per_var = kwargs.get("n_per_var")
+        b_max = kwargs.get("b_max")
+        kernel_name = kwargs.get("kernel")
+        if bin_data is None or n_per_var is None or b_max is None or not kernel_name:
+            raise ValueError("surrogate_test requires bin_data, n_per_var, b_max, and kernel. "
+                             "Pass a TransferEntropyResult to NetWeaver (these are auto-filled).")
+        tau = int(kwargs.get("tau", 1))
+        n_surrogates = int(kwargs.get("n_surrogates", 100))
+        shuffle_method = kwargs.get("shuffle_method", "block")
+        block_length = kwargs.get("block_length")
+        p_method = kwargs.get("p_method", "parametric")
+        fdr = float(kwargs.get("fdr", 0.01))
+        seed = int(kwargs.get("seed", 42))
+        if shuffle_method not in ("block", "random"):
+            raise ValueError(f"shuffle_method must be 'block' or 'random' (got {shuffle_method!r})")
+        if n_surrogates < 2:
+            raise ValueError(f"n_surrogates must be >= 2 (got {n_surrogates}); parametric z-test "
+                             f"needs a non-degenerate std estimate and MC p-values need at least "
+                             f"two samples to be informative.")
+        if p_method not in ("parametric", "mc"):
+            raise ValueError(f"p_method must be 'parametric' or 'mc' (got {p_method!r})")
+        bin_data = np.asarray(bin_data)
+        if bin_data.ndim == 2:
+            bin_data = bin_data[..., None]
+        n_vars, T, K = bin_data.shape
+        if K != 1:
+            raise ValueError(f"only K=1 bin arrays are supported (got K={K})")
+        if T - tau < 2:
+            raise ValueError(f"T - tau = {T - tau} is too short for TE computation")
+        if block_length is None:
+            block_length = max(2, int(np.sqrt(T - tau)))
+        block_length = int(block_length)
+        get_kernel(kernel_name)  # validates the name (KeyError if unknown)
+        fused_kw = kwargs.get("fused", None)
+        observed_te = np.asarray(te_matrix, dtype=np.float32)
+        dev = core.require_gpu(device if device is not None and torch.device(device).type == "cuda"
+                               else None)
+        rng_device = kwargs.get("rng_device") or dev
+        sources = kwargs.get("sources")
+        scope = kwargs.get("pairs_scope", "all")
+        if sources is not None:
+            pairs_bh = build_pairs(np.asarray(variable_names), sources)
+            mask = np.zeros((n_vars, n_vars), dtype=bool)
+            mask[pairs_bh[:, 0], pairs_bh[:, 1]] = True
+        else:
+            mask = ~np.eye(n_vars, dtype=bool)
+        src_idx = None
+        if scope == "sources" and sources is not None:
+            _, src_idx, _ = np.intersect1d(np.asarray(variable_names), sources,
+                                           return_indices=True)
+            src_idx = src_idx.astype(np.int32)
+        vprint(f"[TENEX] surrogate_test: {n_surrogates} shuffles (method={shuffle_method}, "
+               f"block_length={block_length}, kernel={kernel_name}, p={p_method}, "
+               f"device={dev})")
+        bins, npv_d, _, T_ = core.upload_bins(bin_data, dev)
+        npv_in = torch.as_tensor(np.asarray(n_per_var), dtype=torch.int32).to(dev)
+        if npv_in.shape[0] == n_vars and bool((npv_in >= npv_d).all()):
+            npv_d = npv_in.contiguous()
+        prep = core.prepare(bins, npv_d, T, tau)
+        obs_d = torch.from_numpy(observed_te).to(dev)
+        learned = kwargs.get("learned")
+        if learned is not None:
+            from .learned_null import learned_surrogate_test
+            return learned_surrogate_test(
+                prep, observed_te, obs_d, variable_names, n_surrogates, shuffle_method,
+                block_length, seed, rng_device, p_method, fdr, mask, src_idx, dev,
+                **(kwargs.get("learned_kwargs") or {}))
+        fused = fused_kw is not False
+        s, s2, cg = surrogate_accumulate(prep, obs_d, n_surrogates, shuffle_method, block_length,
+                                         seed, rng_device, fused=fused, sources=src_idx)
+        sum_s = s.cpu().numpy()
+        sumsq_s = s2.cpu().numpy()
+        count_ge = cg.cpu().numpy()
+        if fused:
+            d = np.arange(n_vars)
+            count_ge[d, d] += n_surrogates  # the loop path's diagonal (te_k = obs = 0)
+        mean, std, eff, p, sig = surrogate_decision(observed_te, sum_s, sumsq_s, count_ge,
+                                                    n_surrogates, p_method, fdr, mask, dev)
+        pairs = np.argwhere(sig).astype(np.int64)
+        grn = make_grn(np.asarray(variable_names), pairs, eff[pairs[:, 0], pairs[:, 1]])
+        vprint(f"[TENEX] surrogate_test: {len(grn)} significant edges (FDR<{fdr})")
+        return SurrogateTestResult(
+            observed_te=observed_te, mean_surrogate_te=mean, std_surrogate_te=std,
+            effective_te=eff, p_values=p, grn=grn, n_surrogates=n_surrogates,
+            shuffle_method=shuffle_method,
+            block_length=block_length if shuffle_method == "block" else None,
+            p_method=p_method, fdr=fdr)

@@ -1,0 +1,116 @@
+"""GPU parity of the fused surrogate kernel and the surrogate test."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tenex_amd import core, synthetic
+from tenex_amd.inference.surrogate_test import surrogate_accumulate
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-6
+
+
+def _small(n=30, T=400, seed=0):
+    x = synthetic.zero_inflated_lognormal(n, T, seed=seed)
+    d, nb, npv, bm = O.discretize(x)
+    return d, npv
+
+
+def _prep(d, npv, dev, tau=1):
+    bins, _, _, T = core.upload_bins(d, dev)
+    return core.prepare(bins, torch.from_numpy(npv).to(dev), T, tau)
+
+
+@pytest.mark.parametrize("bl,method,tau", [(20, "block", 1), (33, "block", 1), (1, "random", 1),
+                                           (25, "block", 2)])
+def test_fused_te_k_vs_oracle(dev, bl, method, tau):
+    d, npv = _small()
+    n, T = d.shape
+    prep = _prep(d, npv, dev, tau)
+    obs_int = core.te_matrix_internal(prep).cpu().numpy()
+    obs_pub = obs_int.T.copy()
+    n_surr = 6
+    perm, eff_bl, nb = core.block_perm_device(n, T, method, bl, 0, n_surr, 42, dev, dev)
+    plan = core.surrogate_plan(prep)
+    s = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    s2 = torch.zeros_like(s)
+    cg = torch.zeros((n, n), dtype=torch.int32, device=dev)
+    tk = torch.zeros((n * n, n_surr), dtype=torch.float32, device=dev)
+    core.surrogate_run(prep, plan, perm, eff_bl, torch.from_numpy(obs_pub).to(dev), s, s2, cg, tk)
+    pairs = O.pairfree_pairs(n)
+    obs_pairs = obs_int[pairs[:, 0], pairs[:, 1]]
+    os_, os2, ocg, otk = O.surrogate_stats(d, npv, perm.cpu().numpy(), eff_bl, pairs, obs_pairs,
+                                           tau=tau, return_te_k=True)
+    slot = pairs[:, 1] * n + pairs[:, 0]
+    tk = tk.cpu().numpy()[slot]
+    assert np.abs(tk - otk).max() < TOL
+    sp = s.cpu().numpy().ravel()[slot]
+    assert np.abs(sp - os_).max() < n_surr * TOL
+    assert np.abs(s2.cpu().numpy().ravel()[slot] - os2).max() < n_surr * 2 * TOL
+    # counts: identical except where a te_k sits within rounding of te_obs
+    cgp = cg.cpu().numpy().ravel()[slot]
+    near = np.any(np.abs(otk - obs_pairs[:, None]) < 1e-5, axis=1)
+    assert np.array_equal(cgp[~near], ocg[~near])
+
+
+def test_identity_permutation_gives_observed(dev):
+    d, npv = _small(n=20, T=300, seed=3)
+    n, T = d.shape
+    prep = _prep(d, npv, dev)
+    obs = core.te_matrix_internal(prep).cpu().numpy().T.copy()
+    perm = torch.zeros((3, n, 1), dtype=torch.int32, device=dev)  # one block of length T
+    plan = core.surrogate_plan(prep)
+    s = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    s2 = torch.zeros_like(s)
+    cg = torch.zeros((n, n), dtype=torch.int32, device=dev)
+    core.surrogate_run(prep, plan, perm, T, torch.from_numpy(obs).to(dev), s, s2, cg)
+    off = ~np.eye(n, dtype=bool)
+    assert np.abs(s.cpu().numpy()[off] / 3 - obs[off]).max() < TOL
+    assert np.all(cg.cpu().numpy()[off] == 3)
+
+
+def test_fused_matches_loop(dev):
+    d, npv = _small(n=24, T=500, seed=5)
+    prep = _prep(d, npv, dev)
+    obs = torch.from_numpy(core.te_matrix_internal(prep).cpu().numpy().T.copy()).to(dev)
+    a = surrogate_accumulate(prep, obs, 8, "block", 22, 42, dev, fused=True)
+    b = surrogate_accumulate(prep, obs, 8, "block", 22, 42, dev, fused=False)
+    off = ~np.eye(24, dtype=bool)
+    # mean surrogate TE of the two paths (the reference states <= 2e-9 for its
+    # loop vs fused kernels); both are within 1e-6 of the FP64 oracle
+    assert (np.abs(a[0].cpu().numpy() - b[0].cpu().numpy())[off] / 8).max() < 1e-8
+    assert (np.abs(a[1].cpu().numpy() - b[1].cpu().numpy())[off] / 8).max() < 1e-8
+    ca, cb = a[2].cpu().numpy(), b[2].cpu().numpy()
+    assert np.array_equal(ca[off], cb[off])
+
+
+def test_surrogate_test_end_to_end(dev):
+    import tenex_amd as tnx
+    x = synthetic.zero_inflated_lognormal(40, 600, seed=8)
+    # plant a lagged dependence 3 -> 7
+    x[7, 1:] = x[3, :-1] + 0.1 * np.random.default_rng(1).random(599).astype(np.float32)
+    names = np.array([f"g{i}" for i in range(40)])
+    res = tnx.TransferEntropyEngine(x, names).compute(accelerator="gpu")
+    st = tnx.NetWeaver(res, fdr=0.05).infer(method="surrogate_test", n_surrogates=20,
+                                            block_length=25)
+    # oracle decision from oracle statistics on the same permutations
+    d, nb, npv, bm = O.discretize(x)
+    perm, eff_bl, nbk = O.block_perm(40, 600, "block", 25, 0, 20, 42, device=dev)
+    pairs = O.pairfree_pairs(40)
+    obs = res.matrix  # public layout [src, tgt]
+    obs_pairs = obs[pairs[:, 1], pairs[:, 0]]
+    s, s2, cg = O.surrogate_stats(d, npv, perm, eff_bl, pairs, obs_pairs)
+    S = np.zeros((40, 40)); S2 = np.zeros((40, 40)); CG = np.zeros((40, 40), np.int32)
+    S[pairs[:, 1], pairs[:, 0]] = s
+    S2[pairs[:, 1], pairs[:, 0]] = s2
+    CG[pairs[:, 1], pairs[:, 0]] = cg
+    dec = O.surrogate_decision(obs, S, S2, CG, 20, fdr=0.05)
+    assert np.abs(st.mean_surrogate_te - dec["mean"]).max() < 1e-6
+    assert np.abs(st.std_surrogate_te - dec["std"]).max() < 1e-5
+    off = ~np.eye(40, dtype=bool)
+    assert np.abs(st.p_values - dec["p"])[off].max() < 1e-3
+    got = set(map(tuple, st.grn.pairs.tolist()))
+    exp = set(map(tuple, np.argwhere(dec["sig"]).tolist()))
+    assert len(got ^ exp) <= 1
+    assert (3, 7) in got
