@@ -1,0 +1,231 @@
+"""Host-side logic without a GPU: planner (C-ABI, host-only), registry,
+device contract, inference utilities."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import tenex_amd as tnx
+from oracle import oracle as O
+from tenex_amd import _lib, core
+from tenex_amd.inference import (bh_fdr_gpu, build_pairs, fdr_threshold_cpu, make_grn,
+                                 threshold_pairs)
+from tenex_amd.kernels import auto_select, get_kernel
+
+
+def _plan_host(npv, L, tg=None, sr=None, cap_wave=0, cap_block=0):
+    npv = np.ascontiguousarray(npv, np.int32)
+    info = _lib.PlanInfo()
+    tgp = None if tg is None else np.ascontiguousarray(tg, np.int32)
+    srp = None if sr is None else np.ascontiguousarray(sr, np.int32)
+    args = (_lib.ptr(npv), len(npv), L, _lib.ptr(tgp), 0 if tg is None else len(tg),
+            _lib.ptr(srp), 0 if sr is None else len(sr), cap_wave, cap_block)
+    _lib.call("tnx_te_plan", *args, None, None, None, 0, C.byref(info))
+    n_items = info.n_wave_items + info.n_block_items
+    tgo = np.empty(info.n_tgt, np.int32)
+    sro = np.empty(info.n_src, np.int32)
+    work = np.empty((max(1, n_items), 4), np.int32)
+    _lib.call("tnx_te_plan", *args, _lib.ptr(tgo), _lib.ptr(sro), _lib.ptr(work), n_items,
+              C.byref(info))
+    return info, tgo, sro, work[:n_items]
+
+
+def _expand(info, tgo, sro, work):
+    wave, block = [], []
+    for k, (tb, tc, sb, se) in enumerate(work):
+        dst = wave if k < info.n_wave_items else block
+        for ti in range(tb, tb + tc):
+            for sj in range(sb, se):
+                if tgo[ti] != sro[sj]:
+                    dst.append((int(tgo[ti]), int(sro[sj])))
+    return wave, block
+
+
+@pytest.mark.parametrize("cap_wave", [0, 64, 512, 2048])
+def test_planner_covers_every_pair_once(cap_wave):
+    npv = synthetic_levels(300)
+    info, tgo, sro, work = _plan_host(npv, 999, cap_wave=cap_wave)
+    wave, block = _expand(info, tgo, sro, work)
+    allp = wave + block
+    assert len(allp) == len(set(allp)) == 300 * 299 == info.n_pairs
+    cap = info.cap_wave
+    for x, y in wave:
+        assert npv[x] ** 2 * npv[y] <= cap
+    for x, y in block:
+        assert npv[x] ** 2 * npv[y] > cap
+        assert npv[x] * npv[y] <= info.max_block_cols
+    # sources ascending by levels, targets descending
+    assert np.all(np.diff(npv[sro]) >= 0) and np.all(np.diff(npv[tgo]) <= 0)
+
+
+def test_planner_sources_and_targets():
+    npv = synthetic_levels(50)
+    src = np.array([3, 9, 20, 41], np.int32)
+    info, tgo, sro, work = _plan_host(npv, 500, sr=src)
+    wave, block = _expand(info, tgo, sro, work)
+    exp = {(t, s) for t in range(50) for s in src if s != t}
+    assert set(wave + block) == exp and len(wave + block) == len(exp)
+    tg = np.array([0, 1, 2], np.int32)
+    info, tgo, sro, work = _plan_host(npv, 500, tg=tg)
+    wave, block = _expand(info, tgo, sro, work)
+    assert {p[0] for p in wave + block} == {0, 1, 2}
+
+
+def test_planner_rejects_bad_levels():
+    with pytest.raises(ValueError):
+        _plan_host(np.array([2, 0, 3], np.int32), 10)
+    with pytest.raises(ValueError):
+        _plan_host(np.array([2, 200], np.int32), 10)
+
+
+def synthetic_levels(n):
+    from tenex_amd import synthetic
+    return synthetic.mixed_cardinality_levels(n, seed=n)
+
+
+# ── registry and device contract (tests/test_cpu_fallback.py:12-16,
+#    tests/test_device_contract.py:22-60 of the reference) ──────────────────
+
+def test_registry_names_and_lookup():
+    names = [k.name for k in tnx.registered_kernels()]
+    assert names == ["GEMM-B2", "Full-SMEM", "Adaptive-SMEM"]
+    assert get_kernel("full-smem").name == "Full-SMEM"
+    with pytest.raises(KeyError):
+        get_kernel("scatter_add")
+
+
+def test_auto_select_priorities():
+    lds = 160 * 1024
+    from tenex_amd.kernels import compute_smem_bytes
+    sel = lambda b, src=False: auto_select(b, True, lds, compute_smem_bytes(b, 100, 1, 1), None,
+                                           src).name
+    assert sel(2) == "GEMM-B2"
+    assert sel(2, src=True) == "Full-SMEM"
+    assert sel(8) == "Full-SMEM"
+    assert sel(34) == "Full-SMEM"   # (34^3 + 34^2) * 4 B fits 160 KiB
+    assert sel(35) == "Adaptive-SMEM"
+    assert sel(127) == "Adaptive-SMEM"
+    with pytest.raises(RuntimeError):
+        auto_select(8, False, 0, 0, None, False)
+
+
+def test_available_methods_match_reference():
+    assert tnx.available_methods() == ["clr", "fdr", "nd", "point", "surrogate_test", "trace"]
+
+
+def test_engine_validation():
+    x = np.zeros((3, 10), np.float32)
+    names = np.array(["a", "b", "c"])
+    with pytest.raises(ValueError):
+        tnx.TransferEntropyEngine(x[0], names)
+    with pytest.raises(ValueError):
+        tnx.TransferEntropyEngine(np.full((3, 10), np.nan, np.float32), names)
+    with pytest.raises(ValueError):
+        tnx.TransferEntropyEngine(x, np.array(["a", "a", "c"]))
+    with pytest.raises(ValueError):
+        tnx.TransferEntropyEngine(x, names, sources=["zz"])
+    with pytest.raises(ValueError):
+        tnx.TransferEntropyEngine(x, names, sources=["a", "a"])
+
+
+@pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-GPU contract")
+def test_device_contract_without_gpu():
+    x = np.random.default_rng(0).random((4, 30)).astype(np.float32)
+    eng = tnx.TransferEntropyEngine(x, np.array(list("abcd")))
+    with pytest.raises(RuntimeError):
+        eng.compute(accelerator="gpu")
+    with pytest.raises(RuntimeError):   # GPU-only build: no silent CPU path
+        eng.compute(accelerator="cpu")
+    with pytest.raises(RuntimeError):
+        eng.compute(accelerator="auto")
+    with pytest.raises(ValueError):
+        eng.compute(accelerator="tpu")
+    with pytest.raises(RuntimeError):
+        core.require_gpu()
+
+
+def test_tau_validation_before_device():
+    x = np.random.default_rng(0).random((4, 30)).astype(np.float32)
+    eng = tnx.TransferEntropyEngine(x, np.array(list("abcd")))
+    with pytest.raises(ValueError):
+        eng._resolve_accelerator_and_devices("bogus")
+
+
+# ── inference utilities ───────────────────────────────────────────────────
+
+def test_build_pairs_orders():
+    names = np.array(list("abcd"))
+    p = build_pairs(names)
+    assert p.shape == (12, 2) and not np.any(p[:, 0] == p[:, 1])
+    assert np.array_equal(p[:3], [[0, 1], [0, 2], [0, 3]])
+    ps = build_pairs(names, ["c", "a"])
+    assert set(map(tuple, ps)) == {(0, 1), (0, 2), (0, 3), (2, 0), (2, 1), (2, 3)}
+
+
+def test_fdr_threshold_cpu_and_topk():
+    rng = np.random.default_rng(1)
+    te = rng.random(1000).astype(np.float32) * 0.01
+    te[:5] = 1.0
+    pairs = np.stack([np.arange(1000), np.arange(1000)], 1)
+    t, p = fdr_threshold_cpu(te, pairs, 0.05)
+    assert set(p[:, 0]) == set(range(5))
+    t, p = threshold_pairs(te, pairs, torch.device("cpu"), links=3)
+    assert len(t) == 3 and np.all(t == 1.0)
+    tt = torch.from_numpy(te)
+    t2, p2 = threshold_pairs(tt, torch.from_numpy(pairs), torch.device("cpu"), links=3)
+    assert len(t2) == 3
+
+
+def test_netweaver_fdr_on_cpu_matrix():
+    rng = np.random.default_rng(2)
+    M = (rng.random((20, 20)) * 0.01).astype(np.float32)
+    M[3, 7] = 1.0
+    np.fill_diagonal(M, 0)
+    names = np.array([f"g{i}" for i in range(20)])
+    grn, trimmed = tnx.NetWeaver((M, names), fdr=0.05).infer("fdr", device="cpu")
+    assert (3, 7) in set(map(tuple, grn.pairs.tolist()))
+    assert len(trimmed) <= len(grn)
+    assert grn.to_sif().shape == (len(grn), 3)
+
+
+def test_surrogate_decision_restatement():
+    from tenex_amd.inference.surrogate_test import surrogate_decision
+    rng = np.random.default_rng(3)
+    n, ns = 15, 20
+    obs = (rng.random((n, n)) * 0.05).astype(np.float32)
+    np.fill_diagonal(obs, 0)
+    tk = rng.random((n, n, ns)) * 0.02
+    s, s2 = tk.sum(-1), (tk ** 2).sum(-1)
+    cg = (tk.astype(np.float32) >= obs[..., None]).sum(-1).astype(np.int32)
+    mask = ~np.eye(n, dtype=bool)
+    for pm in ("parametric", "mc"):
+        mean, std, eff, p, sig = surrogate_decision(obs, s, s2, cg, ns, pm, 0.05, mask,
+                                                    torch.device("cpu"))
+        ref = O.surrogate_decision(obs, s, s2, cg, ns, p_method=pm, fdr=0.05)
+        assert np.array_equal(mean, ref["mean"]) and np.array_equal(std, ref["std"])
+        assert np.array_equal(p, ref["p"]) and np.array_equal(sig, ref["sig"])
+
+
+def test_multi_lag_combine():
+    from tenex_amd.inference.surrogate_test import SurrogateTestResult
+    from tenex_amd.pipeline import combine_surrogate_multi_lag
+    rng = np.random.default_rng(4)
+    n = 6
+    res = []
+    for k in range(3):
+        eff = rng.random((n, n)).astype(np.float32) - 0.3
+        p = rng.random((n, n)).astype(np.float32) * 0.01
+        np.fill_diagonal(p, 1.0)
+        grn = make_grn(np.array(list("abcdef")), np.empty((0, 2), np.int64), np.empty(0))
+        res.append(SurrogateTestResult(eff + 0.3, np.full((n, n), 0.3, np.float32),
+                                       np.ones((n, n), np.float32), eff, p, grn, 10, "block",
+                                       5, "parametric", 0.05))
+    for comb in ("max", "adaptive_select", "weighted_sum"):
+        r = combine_surrogate_multi_lag(res, [1, 2, 3], comb, np.array(list("abcdef")),
+                                        torch.device("cpu"))
+        assert r.p_values.shape == (n, n) and np.all(np.diag(r.p_values) == 1)
+        pv = np.stack([x.p_values for x in res])
+        exp = np.minimum(1.0, pv.min(0) * 3).astype(np.float32)
+        np.fill_diagonal(exp, 1)
+        assert np.array_equal(r.p_values, exp)
