@@ -376,3 +376,38 @@ def surrogate_plan(prep: Prepared, targets=None, sources=None) -> Plan:
             f"surrogate test needs b_t^2*b_s <= {SURR_MAX_CELLS} histogram cells (got {cap}); "
             f"use coarsening=True or a smaller kp")
     return get_plan(prep, targets, sources, cap_wave=cap, cap_block=0)
+
+
+def pairlist_plan(npv_host: np.ndarray, pairs: np.ndarray, device, cap_wave: int,
+                  per_item: int = 256) -> Plan:
+    """Plan for an explicit [target, source] pair list with every pair in the
+    wave kernel: pairs grouped by target, one item per target chunk of at most
+    per_item sources (src_order is the concatenation of per-target source
+    lists).  Every pair must satisfy b_t^2 * b_s <= cap_wave."""
+    pairs = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    info = _lib.PlanInfo()
+    if len(pairs) == 0:
+        info.cap_wave, info.cap_block = int(cap_wave), 127 * 127
+        z = torch.zeros(1, dtype=torch.int32, device=device)
+        return Plan(info=info, tgt_order=z, src_order=z, work=torch.zeros((1, 4), dtype=torch.int32,
+                    device=device), work_host=np.zeros((0, 4), np.int32),
+                    tgt_host=np.zeros(0, np.int32), src_host=np.zeros(0, np.int32))
+    cells = npv_host[pairs[:, 0]].astype(np.int64) ** 2 * npv_host[pairs[:, 1]]
+    if cells.max() > cap_wave:
+        raise ValueError("a pair exceeds cap_wave histogram cells")
+    order = np.lexsort((pairs[:, 1], pairs[:, 0]))
+    p = pairs[order]
+    tg, start, cnt = np.unique(p[:, 0], return_index=True, return_counts=True)
+    items = []
+    for ti, (s0, c) in enumerate(zip(start, cnt)):
+        for a in range(0, c, per_item):
+            items.append((ti, 1, s0 + a, s0 + min(c, a + per_item)))
+    work = np.asarray(items, np.int32)
+    info.n_wave_items, info.n_block_items = len(work), 0
+    info.n_pairs = int((p[:, 0] != p[:, 1]).sum())
+    info.cap_wave, info.cap_block, info.max_block_cols = int(cap_wave), 127 * 127, 1
+    info.n_tgt, info.n_src = len(tg), len(p)
+    return Plan(info=info, tgt_order=torch.from_numpy(tg.astype(np.int32)).to(device),
+                src_order=torch.from_numpy(p[:, 1].astype(np.int32)).to(device),
+                work=torch.from_numpy(work).to(device), work_host=work,
+                tgt_host=tg.astype(np.int32), src_host=p[:, 1].astype(np.int32))

@@ -1,0 +1,91 @@
+"""Parity against fixtures written by the reference implementation itself
+(tests/golden/make_golden.py, CPU scatter_add path and CPU surrogate loop)."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from oracle import scatter_add_ref as S
+
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return dict(np.load(os.path.join(G, name), allow_pickle=False))
+
+
+def test_golden_c0_bins_and_te_oracle():
+    g = load("te_c0_100x500.npz")
+    d, nb, npv, bm = O.discretize(g["data"])
+    assert np.array_equal(d.astype(np.int8), g["bins"])
+    assert np.array_equal(npv, g["n_per_var"]) and bm == int(g["b_max"])
+    M = O.te_matrix(d, npv)
+    assert np.abs(M - g["matrix"]).max() < 1e-6
+    assert str(g["kernel"]) == "scatter_add"
+
+
+def test_golden_c0_scatter_add_restatement_bit_exact():
+    g = load("te_c0_100x500.npz")
+    d = g["bins"].astype(np.int32)
+    pairs = O.build_pairs_engine(100)
+    te = S.te_pairs(d, int(g["b_max"]), pairs)
+    M = np.zeros((100, 100), np.float32)
+    M[pairs[:, 0], pairs[:, 1]] = te
+    assert np.array_equal(M.T, g["matrix"])
+
+
+def test_golden_surrogate_oracle():
+    g = load("surrogate_40x400.npz")
+    d = g["bins"].astype(np.int32)
+    npv = g["n_per_var"]
+    n = d.shape[0]
+    obs = g["observed"]
+    ns, bl, seed = int(g["n_surrogates"]), int(g["block_length"]), int(g["seed"])
+    perm, eff, nbk = O.block_perm(n, d.shape[1], "block", bl, 0, ns, seed, device="cpu")
+    pairs = O.pairfree_pairs(n)
+    s, s2, cg = O.surrogate_stats(d, npv, perm, eff, pairs, obs[pairs[:, 1], pairs[:, 0]])
+    Sm = np.zeros((n, n)); S2 = np.zeros((n, n)); CG = np.zeros((n, n), np.int32)
+    Sm[pairs[:, 1], pairs[:, 0]] = s
+    S2[pairs[:, 1], pairs[:, 0]] = s2
+    CG[pairs[:, 1], pairs[:, 0]] = cg
+    dec = O.surrogate_decision(obs, Sm, S2, CG, ns, fdr=float(g["fdr"]))
+    off = ~np.eye(n, dtype=bool)
+    assert np.abs(dec["mean"] - g["mean"])[off].max() < 1e-6
+    assert np.abs(dec["std"] - g["std"])[off].max() < 1e-5
+    assert np.abs(dec["p"] - g["p"])[off].max() < 1e-3
+    got = set(map(tuple, np.argwhere(dec["sig"]).tolist()))
+    exp = set(map(tuple, g["edges"].tolist()))
+    assert got == exp
+
+
+@pytest.mark.gpu
+def test_golden_c0_gpu_engine():
+    import tenex_amd as tnx
+    g = load("te_c0_100x500.npz")
+    names = np.array([f"g{i}" for i in range(100)])
+    res = tnx.TransferEntropyEngine(g["data"], names).compute(accelerator="gpu")
+    assert np.array_equal(res.bin_arrs[:, :, 0], g["bins"])
+    assert np.array_equal(res.n_per_var, g["n_per_var"])
+    assert np.abs(res.matrix - g["matrix"]).max() < 1e-6
+
+
+@pytest.mark.gpu
+def test_golden_surrogate_gpu():
+    import tenex_amd as tnx
+    g = load("surrogate_40x400.npz")
+    n = g["bins"].shape[0]
+    names = np.array([f"g{i}" for i in range(n)])
+    res = tnx.TransferEntropyEngine(g["data"], names).compute(accelerator="gpu")
+    assert np.abs(res.matrix - g["observed"]).max() < 1e-6
+    st = tnx.NetWeaver(res, fdr=float(g["fdr"])).infer(
+        method="surrogate_test", n_surrogates=int(g["n_surrogates"]),
+        block_length=int(g["block_length"]), seed=int(g["seed"]), rng_device="cpu")
+    off = ~np.eye(n, dtype=bool)
+    assert np.abs(st.mean_surrogate_te - g["mean"])[off].max() < 1e-6
+    assert np.abs(st.std_surrogate_te - g["std"])[off].max() < 1e-5
+    assert np.abs(st.p_values - g["p"])[off].max() < 1e-3
+    got = set(map(tuple, st.grn.pairs.tolist()))
+    exp = set(map(tuple, g["edges"].tolist()))
+    assert got == exp

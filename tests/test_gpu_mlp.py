@@ -1,0 +1,83 @@
+"""Learned null: MLP kernels against the FP64 oracle, training, end to end."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tenex_amd import core, synthetic
+from tenex_amd.inference.learned_null import NullMLP
+
+pytestmark = pytest.mark.gpu
+
+
+def _params_as_used(model):
+    """Master parameters with weights rounded to the bf16 the kernels read."""
+    w1, b1, w2, b2, w3, b3 = [v.cpu().double().numpy() for v in model.views()]
+    to = lambda t: (t.view(torch.int32) & -65536).view(torch.float32) if False else None  # noqa
+    def bf(x):
+        u = torch.from_numpy(x.astype(np.float32)).view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+        u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+        return u.to(torch.int32).view(torch.float32).double().numpy()
+    return np.concatenate([bf(w1), b1, bf(w2), b2, bf(w3), b3])
+
+
+def test_mlp_forward_and_gradient_vs_oracle(dev):
+    D, Oo, N = 17, 2, 300
+    m = NullMLP(D, Oo, dev, seed=3)
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((N, D)).astype(np.float32)
+    T = rng.standard_normal((N, Oo)).astype(np.float32)
+    Xd, Td = torch.from_numpy(X).to(dev), torch.from_numpy(T).to(dev)
+    p = _params_as_used(m)
+    Y = m.forward(Xd).cpu().numpy()
+    Yo = O.mlp_forward(p, D, 64, Oo, X.astype(np.float64))
+    assert np.abs(Y - Yo).max() < 1e-4 * max(1.0, np.abs(Yo).max())
+    loss, g = m.loss_grad(Xd, Td)
+    lo, go = O.mlp_loss_grad(p, D, 64, Oo, X.astype(np.float64), T.astype(np.float64))
+    assert abs(loss.item() - lo) < 1e-4 * max(1.0, lo)
+    g = g.cpu().numpy()
+    scale = np.abs(go).max()
+    assert np.abs(g - go).max() < 1e-4 * scale
+
+
+def test_mlp_adam_and_bf16_copies(dev):
+    m = NullMLP(8, 2, dev, seed=1)
+    g = torch.ones(m.n_params, device=dev) * 0.5
+    before = m.master.clone()
+    m.adam_step(g, lr=1e-2)
+    torch.cuda.synchronize()
+    assert torch.allclose(m.master, before - 1e-2, atol=1e-6)  # first Adam step = -lr*sign
+    w1 = m.views()[0]
+    u = w1.view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    exp = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF).to(torch.int32).to(torch.int16)
+    assert torch.equal(m.W1, exp)
+
+
+def test_mlp_fit_learns_smooth_target(dev):
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((6000, 6)).astype(np.float32)
+    T = np.stack([np.sin(X[:, 0]) + 0.5 * X[:, 1] ** 2, X[:, 2] * X[:, 3]], 1).astype(np.float32)
+    m = NullMLP(6, 2, dev, seed=0)
+    losses = m.fit(torch.from_numpy(X).to(dev), torch.from_numpy(T).to(dev), steps=600, lr=3e-3)
+    assert losses[-1] < 0.1 * losses[0]
+
+
+def test_learned_surrogate_end_to_end(dev):
+    import tenex_amd as tnx
+    n, T = 120, 1200
+    x = synthetic.zero_inflated_lognormal(n, T, seed=12)
+    rng = np.random.default_rng(5)
+    for s, t in [(1, 2), (10, 11), (20, 33), (40, 41), (50, 60)]:
+        x[t, 1:] = x[s, :-1] + 0.2 * rng.random(T - 1).astype(np.float32)
+    names = np.array([f"g{i}" for i in range(n)])
+    res = tnx.TransferEntropyEngine(x, names).compute(accelerator="gpu")
+    nw = tnx.NetWeaver(res, fdr=0.05)
+    plain = nw.infer("surrogate_test", n_surrogates=30)
+    lr = nw.infer("surrogate_test", n_surrogates=30, learned="mlp",
+                  learned_kwargs=dict(train_frac=0.1, min_train=1000, steps=300))
+    info = lr.learned_info
+    assert info["plain_fraction"] < 0.6
+    a = set(map(tuple, plain.grn.pairs.tolist()))
+    b = set(map(tuple, lr.grn.pairs.tolist()))
+    jac = len(a & b) / max(1, len(a | b))
+    assert jac >= 0.9, (jac, len(a), len(b))
