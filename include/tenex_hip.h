@@ -46,7 +46,7 @@ extern "C" {
 #define TNX_API
 #endif
 
-#define TNX_ABI_VERSION 1
+#define TNX_ABI_VERSION 2
 
 /* Status codes. */
 #define TNX_OK 0
@@ -249,18 +249,40 @@ typedef struct {
 TNX_API int tnx_mlp_forward(const tnx_mlp_t *m, const float *X, int64_t n_rows, float *Y,
                     void *stream);
 
+/* Bytes of device workspace tnx_mlp_backward / tnx_mlp_fit need for
+ * n_rows training rows: one partial gradient per workgroup (at most 256
+ * workgroups of 64-row chunks), summed in a fixed order. */
+TNX_API int64_t tnx_mlp_workspace_bytes(int32_t D, int32_t O, int64_t n_rows);
+
 /* Loss 0.5 * w_o * (y - tgt)^2 averaged over rows and outputs; gradients
- * (fp32, params layout W1, b1, W2, b2, W3, b3) accumulated into grad
- * (caller zeroes); loss_out (device float, accumulated). */
+ * (fp32, params layout W1, b1, W2, b2, W3, b3) written to grad and the loss
+ * to loss_out (device float; may be NULL).  Deterministic: per-workgroup
+ * partials in `workspace` summed in workgroup order. */
 TNX_API int tnx_mlp_backward(const tnx_mlp_t *m, const float *X, const float *tgt,
-                     const float *w_out, int64_t n_rows, float *grad, float *loss_out,
-                     void *stream);
+                             const float *w_out, int64_t n_rows, float *grad, float *loss_out,
+                             void *workspace, int64_t ws_bytes, void *stream);
 
 /* Adam step on fp32 master weights `master` (params layout), refreshing the
  * bf16 copies in m; moments m1, m2 fp32; step t >= 1. */
 TNX_API int tnx_mlp_adam(const tnx_mlp_t *m, float *master, const float *grad, float *m1,
                  float *m2, int64_t n_params, float lr, float beta1, float beta2,
                  float eps, int32_t t, void *stream);
+
+/* Full-batch Adam training schedule for tnx_mlp_fit: step s in [0, steps)
+ * uses lr_min + (lr - lr_min) * 0.5 * (1 + cos(pi * s / steps)) and Adam
+ * step number t0 + s + 1. */
+typedef struct {
+    float lr, lr_min, beta1, beta2, eps;
+    int32_t steps, t0;
+} tnx_adam_t;
+
+/* `opt->steps` full-batch steps (backward + Adam) launched from the host
+ * loop with no host synchronisation; loss_hist (device float[steps], may be
+ * NULL) receives each step's loss.  grad: device float[n_params] scratch. */
+TNX_API int tnx_mlp_fit(const tnx_mlp_t *m, float *master, float *m1, float *m2,
+                        int64_t n_params, const float *X, const float *tgt, int64_t n_rows,
+                        const tnx_adam_t *opt, float *grad, void *workspace, int64_t ws_bytes,
+                        float *loss_hist, void *stream);
 
 /* ── Hardware ceilings (roofline peaks) ────────────────────────────────── */
 

@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libtenex_hip.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 TNX_OUT_PAIRFREE = 0
 TNX_OUT_MATRIX = 1
@@ -65,6 +65,11 @@ class Mlp(C.Structure):
                 ("D", i32), ("H", i32), ("O", i32)]
 
 
+class Adam(C.Structure):
+    _fields_ = [("lr", C.c_float), ("lr_min", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("steps", i32), ("t0", i32)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "tnx_abi_version": (C.c_int, []),
@@ -84,7 +89,10 @@ SIGNATURES = {
     "tnx_te_gemm_b2": (C.c_int, [p_, i64, i64, i64, i32, p_, p_, p_, p_, i64, p_, i64, p_, p_]),
     "tnx_surrogate_run": (C.c_int, [C.POINTER(SurrArgs), p_]),
     "tnx_mlp_forward": (C.c_int, [C.POINTER(Mlp), p_, i64, p_, p_]),
-    "tnx_mlp_backward": (C.c_int, [C.POINTER(Mlp), p_, p_, p_, i64, p_, p_, p_]),
+    "tnx_mlp_workspace_bytes": (i64, [C.c_int32, C.c_int32, i64]),
+    "tnx_mlp_backward": (C.c_int, [C.POINTER(Mlp), p_, p_, p_, i64, p_, p_, p_, i64, p_]),
+    "tnx_mlp_fit": (C.c_int, [C.POINTER(Mlp), p_, p_, p_, i64, p_, p_, i64, C.POINTER(Adam), p_,
+                              p_, i64, p_, p_]),
     "tnx_mlp_adam": (C.c_int, [C.POINTER(Mlp), p_, p_, p_, p_, i64, C.c_float, C.c_float,
                                C.c_float, C.c_float, i32, p_]),
     "tnx_bench_lds_atomics": (C.c_int, [i32, i64, i32, p_, p_]),

@@ -85,118 +85,281 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(tnx_mlp_t m, const flo
         if (o < O) Y[r * O + o] = y[o];
 }
 
-// One wavefront = 64 rows.  LDS per wave: x, h1, h2, d1, d2, dy per row.
-struct MlpRows {
-    float x[64][kMlpMaxD + 1];
-    float h1[64][kMlpH + 1];
-    float h2[64][kMlpH + 1];
-    float d1[64][kMlpH + 1];
-    float d2[64][kMlpH + 1];
+// Backward: a 256-thread workgroup walks 64-row chunks.  Thread t works on
+// row r = t % 64 and on hidden units [16q, 16q + 16), q = t / 64, for the
+// forward and delta passes (weights read as broadcasts from LDS, activations
+// and deltas exchanged through LDS).  For the weight gradients thread t owns
+// a 4 x 4 block: output units i0 = 4 (t % 16) .. +3 and inputs k0 = 4 (t / 16)
+// .. +3 of W2 and W1, accumulated in registers over every chunk the
+// workgroup visits; the threads of column block 0..3 also own b2, b1, W3 and
+// b3.  Each workgroup writes one partial gradient row; a second kernel sums
+// the rows in workgroup order, so the result does not depend on timing.
+constexpr int kBwdThreads = 256;
+constexpr int kBwdMaxWG = 256;
+constexpr int kLd = kMlpH + 1;
+
+struct BwdShared {
+    float W1T[kMlpMaxD * kMlpH];  // [k][j]
+    float W2[kMlpH * kMlpH];      // [j][k]
+    float W2T[kMlpH * kMlpH];     // [k][j]
+    float W3[kMlpMaxO * kMlpH];
+    float b1[kMlpH], b2[kMlpH], b3[kMlpMaxO];
+    float x[64][kLd];
+    float h1[64][kLd];
+    float h2[64][kLd];
+    float d1[64][kLd];
+    float d2[64][kLd];
+    float yp[4][64][kMlpMaxO];
     float dy[64][kMlpMaxO];
 };
 
-__global__ __launch_bounds__(64) void mlp_backward_kernel(tnx_mlp_t m, const float *__restrict__ X,
-                                                          const float *__restrict__ tgt,
-                                                          const float *__restrict__ w_out,
-                                                          int64_t n_rows, float *__restrict__ grad,
-                                                          float *__restrict__ loss_out) {
-    extern __shared__ float dyn[];
-    MlpShared &s = *reinterpret_cast<MlpShared *>(dyn);
-    MlpRows &R = *reinterpret_cast<MlpRows *>(dyn + sizeof(MlpShared) / 4);
-    load_weights(s, m);
-    __syncthreads();
-    const int lane = threadIdx.x;
-    const int D = m.D, O = m.O;
-    const int64_t r0 = (int64_t)blockIdx.x * 64;
-    const int64_t r = r0 + lane;
-    const bool valid = r < n_rows;
-    const float inv = 1.0f / (float)(n_rows * O);
-    float loss = 0.f;
-    // forward, keep activations; dy
-    for (int k = 0; k < D; k++) R.x[lane][k] = valid ? X[r * D + k] : 0.f;
-    for (int i = 0; i < kMlpH; i++) {
-        float z = s.b1[i];
-        for (int k = 0; k < D; k++) z += s.W1[i * D + k] * R.x[lane][k];
-        R.h1[lane][i] = z > 0.f ? z : 0.f;
+struct ParamLayout {
+    int W1, b1, W2, b2, W3, b3, P;
+    __host__ __device__ ParamLayout(int D, int O) {
+        W1 = 0;
+        b1 = kMlpH * D;
+        W2 = b1 + kMlpH;
+        b2 = W2 + kMlpH * kMlpH;
+        W3 = b2 + kMlpH;
+        b3 = W3 + O * kMlpH;
+        P = b3 + O;
     }
-    float y[kMlpMaxO];
-#pragma unroll
-    for (int o = 0; o < kMlpMaxO; o++) y[o] = o < O ? s.b3[o] : 0.f;
-    for (int i = 0; i < kMlpH; i++) {
-        float z = s.b2[i];
-        for (int k = 0; k < kMlpH; k++) z += s.W2[i * kMlpH + k] * R.h1[lane][k];
-        const float h2 = z > 0.f ? z : 0.f;
-        R.h2[lane][i] = h2;
-#pragma unroll
-        for (int o = 0; o < kMlpMaxO; o++)
-            if (o < O) y[o] += s.W3[o * kMlpH + i] * h2;
-    }
-#pragma unroll
-    for (int o = 0; o < kMlpMaxO; o++) {
-        if (o >= O) continue;
-        const float w = w_out ? w_out[o] : 1.0f;
-        const float e = valid ? (y[o] - tgt[r * O + o]) : 0.f;
-        loss += 0.5f * w * e * e * inv;
-        R.dy[lane][o] = w * e * inv;
-    }
-    // backward deltas
-    for (int k = 0; k < kMlpH; k++) {
-        float g = 0.f;
-        for (int o = 0; o < O; o++) g += s.W3[o * kMlpH + k] * R.dy[lane][o];
-        R.d2[lane][k] = R.h2[lane][k] > 0.f ? g : 0.f;
-    }
-    for (int k = 0; k < kMlpH; k++) {
-        float g = 0.f;
-        for (int i = 0; i < kMlpH; i++) g += s.W2[i * kMlpH + k] * R.d2[lane][i];
-        R.d1[lane][k] = R.h1[lane][k] > 0.f ? g : 0.f;
-    }
-    __syncthreads();
-    // weight gradients over the wave's 64 rows: lane i owns row i of each
-    // weight matrix (and bias entry i)
-    float *gW1 = grad, *gb1 = gW1 + kMlpH * D, *gW2 = gb1 + kMlpH, *gb2 = gW2 + kMlpH * kMlpH,
-          *gW3 = gb2 + kMlpH, *gb3 = gW3 + O * kMlpH;
-    const int i = lane;
-    {
-        float gb = 0.f;
-        for (int rr = 0; rr < 64; rr++) gb += R.d2[rr][i];
-        atomicAdd(gb2 + i, gb);
-        for (int k = 0; k < kMlpH; k++) {
-            float g = 0.f;
-            for (int rr = 0; rr < 64; rr++) g += R.d2[rr][i] * R.h1[rr][k];
-            atomicAdd(gW2 + i * kMlpH + k, g);
-        }
-        float gb1v = 0.f;
-        for (int rr = 0; rr < 64; rr++) gb1v += R.d1[rr][i];
-        atomicAdd(gb1 + i, gb1v);
-        for (int k = 0; k < D; k++) {
-            float g = 0.f;
-            for (int rr = 0; rr < 64; rr++) g += R.d1[rr][i] * R.x[rr][k];
-            atomicAdd(gW1 + i * D + k, g);
-        }
-        for (int o = 0; o < O; o++) {
-            float g = 0.f;
-            for (int rr = 0; rr < 64; rr++) g += R.dy[rr][o] * R.h2[rr][i];
-            atomicAdd(gW3 + o * kMlpH + i, g);
-        }
-        if (i < O) {
-            float g = 0.f;
-            for (int rr = 0; rr < 64; rr++) g += R.dy[rr][i];
-            atomicAdd(gb3 + i, g);
-        }
-    }
-    for (int off = 32; off >= 1; off >>= 1) loss += __shfl_xor(loss, off, 64);
-    if (lane == 0 && loss_out) atomicAdd(loss_out, loss);
+};
+
+__host__ __device__ inline int64_t bwd_groups(int64_t n_rows) {
+    const int64_t chunks = (n_rows + 63) / 64;
+    return chunks < kBwdMaxWG ? chunks : kBwdMaxWG;
 }
 
-__global__ void mlp_adam_kernel(float *__restrict__ master, const float *__restrict__ grad,
-                                float *__restrict__ m1, float *__restrict__ m2, int64_t n,
-                                float lr, float b1, float b2, float eps, float c1, float c2,
-                                uint16_t *__restrict__ W1, uint16_t *__restrict__ W2,
-                                uint16_t *__restrict__ W3, float *__restrict__ bb1,
-                                float *__restrict__ bb2, float *__restrict__ bb3, int D, int O) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float g = grad[i];
+__global__ __launch_bounds__(kBwdThreads) void mlp_backward_kernel(
+    tnx_mlp_t m, const float *__restrict__ X, const float *__restrict__ tgt,
+    const float *__restrict__ w_out, int64_t n_rows, float *__restrict__ part) {
+    extern __shared__ float dyn[];
+    BwdShared &S = *reinterpret_cast<BwdShared *>(dyn);
+    const int D = m.D, O = m.O;
+    const int t = threadIdx.x, r = t & 63, q = t >> 6;
+    const ParamLayout L(D, O);
+    {
+        // every global load is issued before the first LDS write
+        uint16_t w1v[16], w2v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int i = t + kBwdThreads * u;
+            w1v[u] = i < kMlpH * D ? m.W1[i] : (uint16_t)0;
+            w2v[u] = m.W2[i];
+        }
+        const uint16_t w3v = t < O * kMlpH ? m.W3[t] : (uint16_t)0;
+        const float b1v = t < kMlpH ? m.b1[t] : 0.f;
+        const float b2v = t < kMlpH ? m.b2[t] : 0.f;
+        const float b3v = t < O ? m.b3[t] : 0.f;
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int i = t + kBwdThreads * u;
+            if (i < kMlpH * D) {
+                const int j = i / D, k = i - j * D;
+                S.W1T[k * kMlpH + j] = bf16_to_f32(w1v[u]);
+            }
+            const float v = bf16_to_f32(w2v[u]);
+            S.W2[i] = v;
+            S.W2T[(i & 63) * kMlpH + (i >> 6)] = v;
+        }
+        if (t < O * kMlpH) S.W3[t] = bf16_to_f32(w3v);
+        if (t < kMlpH) {
+            S.b1[t] = b1v;
+            S.b2[t] = b2v;
+        }
+        if (t < O) S.b3[t] = b3v;
+    }
+    __syncthreads();
+
+    const float inv = 1.0f / (float)(n_rows * O);
+    const int ib = t & 15, kb = t >> 4;
+    const int i0 = 4 * ib, k0 = 4 * kb;
+    float gw2[4][4], gw1[4][4], gw3[kMlpMaxO][4], gb[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        gb[u] = 0.f;
+#pragma unroll
+        for (int v = 0; v < 4; v++) gw2[u][v] = gw1[u][v] = gw3[u][v] = 0.f;
+    }
+    float loss = 0.f;
+    const int64_t n_chunks = (n_rows + 63) / 64;
+    for (int64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        const int64_t base = c * 64;
+        const int rows = (int)(n_rows - base < 64 ? n_rows - base : 64);
+        float xv[16], tv[kMlpMaxO];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = t + kBwdThreads * u;
+            xv[u] = e < rows * D ? X[base * D + e] : 0.f;
+        }
+#pragma unroll
+        for (int o = 0; o < kMlpMaxO; o++)
+            tv[o] = (q == 0 && o < O && r < rows) ? tgt[(base + r) * O + o] : 0.f;
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = t + kBwdThreads * u;
+            if (e < 64 * D) {
+                const int rr = e / D, k = e - rr * D;
+                S.x[rr][k] = xv[u];
+            }
+        }
+        __syncthreads();
+        // layer 1: units 16q .. 16q+15 of row r
+        float h1r[16];
+#pragma unroll
+        for (int jj = 0; jj < 16; jj++) h1r[jj] = S.b1[16 * q + jj];
+        for (int k = 0; k < D; k++) {
+            const float xv = S.x[r][k];
+            const float *w = &S.W1T[k * kMlpH + 16 * q];
+#pragma unroll
+            for (int jj = 0; jj < 16; jj++) h1r[jj] += w[jj] * xv;
+        }
+#pragma unroll
+        for (int jj = 0; jj < 16; jj++) {
+            h1r[jj] = h1r[jj] > 0.f ? h1r[jj] : 0.f;
+            S.h1[r][16 * q + jj] = h1r[jj];
+        }
+        __syncthreads();
+        // layer 2 and this quarter's share of the outputs
+        float h2r[16];
+#pragma unroll
+        for (int jj = 0; jj < 16; jj++) h2r[jj] = S.b2[16 * q + jj];
+        for (int k = 0; k < kMlpH; k++) {
+            const float hv = S.h1[r][k];
+            const float *w = &S.W2T[k * kMlpH + 16 * q];
+#pragma unroll
+            for (int jj = 0; jj < 16; jj++) h2r[jj] += w[jj] * hv;
+        }
+#pragma unroll
+        for (int jj = 0; jj < 16; jj++) {
+            h2r[jj] = h2r[jj] > 0.f ? h2r[jj] : 0.f;
+            S.h2[r][16 * q + jj] = h2r[jj];
+        }
+        for (int o = 0; o < O; o++) {
+            float yv = 0.f;
+#pragma unroll
+            for (int jj = 0; jj < 16; jj++) yv += S.W3[o * kMlpH + 16 * q + jj] * h2r[jj];
+            S.yp[q][r][o] = yv;
+        }
+        __syncthreads();
+        if (q == 0) {
+            const bool valid = r < rows;
+#pragma unroll
+            for (int o = 0; o < kMlpMaxO; o++) {
+                if (o >= O) continue;
+                const float y = S.b3[o] + S.yp[0][r][o] + S.yp[1][r][o] + S.yp[2][r][o] +
+                                S.yp[3][r][o];
+                const float w = w_out ? w_out[o] : 1.0f;
+                const float e = valid ? (y - tv[o]) : 0.f;
+                loss += 0.5f * w * e * e * inv;
+                S.dy[r][o] = w * e * inv;
+            }
+        }
+        __syncthreads();
+        // output deltas of layer 2 (units 16q ..)
+        {
+            float dyr[kMlpMaxO];
+#pragma unroll
+            for (int o = 0; o < kMlpMaxO; o++) dyr[o] = o < O ? S.dy[r][o] : 0.f;
+#pragma unroll
+            for (int jj = 0; jj < 16; jj++) {
+                float g = 0.f;
+#pragma unroll
+                for (int o = 0; o < kMlpMaxO; o++)
+                    if (o < O) g += S.W3[o * kMlpH + 16 * q + jj] * dyr[o];
+                S.d2[r][16 * q + jj] = h2r[jj] > 0.f ? g : 0.f;
+            }
+        }
+        __syncthreads();
+        // deltas of layer 1 (units 16q ..): d1[k] = sum_j W2[j][k] d2[j]
+        {
+            float d1r[16];
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) d1r[kk] = 0.f;
+            for (int j = 0; j < kMlpH; j++) {
+                const float dv = S.d2[r][j];
+                const float *w = &S.W2[j * kMlpH + 16 * q];
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++) d1r[kk] += w[kk] * dv;
+            }
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) S.d1[r][16 * q + kk] = h1r[kk] > 0.f ? d1r[kk] : 0.f;
+        }
+        __syncthreads();
+        // weight gradients of the chunk (padding rows carry zero deltas)
+        for (int rr = 0; rr < 64; rr++) {
+            float a2[4], a1[4], hh[4], xx[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                a2[u] = S.d2[rr][i0 + u];
+                a1[u] = S.d1[rr][i0 + u];
+                hh[u] = S.h1[rr][k0 + u];
+                xx[u] = (k0 + u < D) ? S.x[rr][k0 + u] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    gw2[u][v] += a2[u] * hh[v];
+                    gw1[u][v] += a1[u] * xx[v];
+                }
+        }
+        if (kb == 0) {
+            for (int rr = 0; rr < 64; rr++)
+#pragma unroll
+                for (int u = 0; u < 4; u++) gb[u] += S.d2[rr][i0 + u];
+        } else if (kb == 1) {
+            for (int rr = 0; rr < 64; rr++)
+#pragma unroll
+                for (int u = 0; u < 4; u++) gb[u] += S.d1[rr][i0 + u];
+        } else if (kb == 2) {
+            for (int rr = 0; rr < 64; rr++) {
+#pragma unroll
+                for (int o = 0; o < kMlpMaxO; o++) {
+                    if (o >= O) continue;
+                    const float dv = S.dy[rr][o];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) gw3[o][u] += dv * S.h2[rr][i0 + u];
+                }
+            }
+        } else if (kb == 3 && ib == 0) {
+            for (int rr = 0; rr < 64; rr++)
+#pragma unroll
+                for (int o = 0; o < kMlpMaxO; o++)
+                    if (o < O) gb[o] += S.dy[rr][o];
+        }
+        __syncthreads();  // the next chunk overwrites x, h1, h2, d1, d2, dy
+    }
+    float *pr = part + (int64_t)blockIdx.x * (L.P + 1);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = i0 + u, k = k0 + v;
+            pr[L.W2 + i * kMlpH + k] = gw2[u][v];
+            if (k < D) pr[L.W1 + i * D + k] = gw1[u][v];
+        }
+    if (kb == 0) {
+        for (int u = 0; u < 4; u++) pr[L.b2 + i0 + u] = gb[u];
+    } else if (kb == 1) {
+        for (int u = 0; u < 4; u++) pr[L.b1 + i0 + u] = gb[u];
+    } else if (kb == 2) {
+        for (int o = 0; o < O; o++)
+            for (int u = 0; u < 4; u++) pr[L.W3 + o * kMlpH + i0 + u] = gw3[o][u];
+    } else if (kb == 3 && ib == 0) {
+        for (int o = 0; o < O; o++) pr[L.b3 + o] = gb[o];
+    }
+    if (q == 0) {
+        for (int off = 32; off >= 1; off >>= 1) loss += __shfl_xor(loss, off, 64);
+        if (r == 0) pr[L.P] = loss;
+    }
+}
+
+__device__ __forceinline__ void adam_update(int i, float g, float *__restrict__ master,
+                                            float *__restrict__ m1, float *__restrict__ m2,
+                                            float lr, float b1, float b2, float eps, float c1,
+                                            float c2, const tnx_mlp_t &m) {
     const float a = b1 * m1[i] + (1.f - b1) * g;
     const float v = b2 * m2[i] + (1.f - b2) * g * g;
     m1[i] = a;
@@ -204,14 +367,55 @@ __global__ void mlp_adam_kernel(float *__restrict__ master, const float *__restr
     const float w = master[i] - lr * (a / c1) / (sqrtf(v / c2) + eps);
     master[i] = w;
     // refresh the bf16 / bias copies (params layout W1, b1, W2, b2, W3, b3)
-    const int64_t n1 = (int64_t)kMlpH * D, n2 = n1 + kMlpH, n3 = n2 + kMlpH * kMlpH,
-                  n4 = n3 + kMlpH, n5 = n4 + (int64_t)O * kMlpH;
-    if (i < n1) W1[i] = f32_to_bf16(w);
-    else if (i < n2) bb1[i - n1] = w;
-    else if (i < n3) W2[i - n2] = f32_to_bf16(w);
-    else if (i < n4) bb2[i - n3] = w;
-    else if (i < n5) W3[i - n4] = f32_to_bf16(w);
-    else bb3[i - n5] = w;
+    const ParamLayout L(m.D, m.O);
+    if (i < L.b1) const_cast<uint16_t *>(m.W1)[i] = f32_to_bf16(w);
+    else if (i < L.W2) const_cast<float *>(m.b1)[i - L.b1] = w;
+    else if (i < L.b2) const_cast<uint16_t *>(m.W2)[i - L.W2] = f32_to_bf16(w);
+    else if (i < L.W3) const_cast<float *>(m.b2)[i - L.b2] = w;
+    else if (i < L.b3) const_cast<uint16_t *>(m.W3)[i - L.W3] = f32_to_bf16(w);
+    else const_cast<float *>(m.b3)[i - L.b3] = w;
+}
+
+// Sum of the per-workgroup partial rows: a 256-thread block covers 32
+// parameters, eight threads per parameter each summing every eighth row,
+// then the eight sums are added in a fixed order.  Entry P is the loss.
+// With do_adam the Adam step follows for each parameter.
+constexpr int kRedParams = 32;
+constexpr int kRedSlices = 8;
+
+__global__ __launch_bounds__(256) void mlp_reduce_adam_kernel(
+    const float *__restrict__ part, int G, float *__restrict__ grad, float *__restrict__ loss,
+    int do_adam, tnx_mlp_t m, float *__restrict__ master, float *__restrict__ m1,
+    float *__restrict__ m2, float lr, float b1, float b2, float eps, float c1, float c2) {
+    __shared__ float red[kRedSlices][kRedParams];
+    const ParamLayout L(m.D, m.O);
+    const int pl = threadIdx.x % kRedParams, gs = threadIdx.x / kRedParams;
+    const int p = blockIdx.x * kRedParams + pl;
+    float acc = 0.f;
+    if (p <= L.P) {
+#pragma unroll 4
+        for (int g = gs; g < G; g += kRedSlices) acc += part[(int64_t)g * (L.P + 1) + p];
+    }
+    red[gs][pl] = acc;
+    __syncthreads();
+    if (gs != 0 || p > L.P) return;
+    float sum = red[0][pl];
+#pragma unroll
+    for (int k = 1; k < kRedSlices; k++) sum += red[k][pl];
+    if (p == L.P) {
+        if (loss) *loss = sum;
+        return;
+    }
+    grad[p] = sum;
+    if (do_adam) adam_update(p, sum, master, m1, m2, lr, b1, b2, eps, c1, c2, m);
+}
+
+__global__ void mlp_adam_kernel(const float *__restrict__ grad, tnx_mlp_t m,
+                                float *__restrict__ master, float *__restrict__ m1,
+                                float *__restrict__ m2, int n, float lr, float b1, float b2,
+                                float eps, float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) adam_update(i, grad[i], master, m1, m2, lr, b1, b2, eps, c1, c2, m);
 }
 
 static int check_mlp(const tnx_mlp_t *m) {
@@ -245,19 +449,52 @@ int tnx_mlp_forward(const tnx_mlp_t *m, const float *X, int64_t n_rows, float *Y
     return TNX_OK;
 }
 
-int tnx_mlp_backward(const tnx_mlp_t *m, const float *X, const float *tgt, const float *w_out,
-                     int64_t n_rows, float *grad, float *loss_out, void *stream) {
-    clear_error();
-    int rc = check_mlp(m);
-    if (rc) return rc;
-    TNX_CHECK_ARG(X && tgt && grad && n_rows >= 0, "tnx_mlp_backward: bad arguments");
-    if (n_rows == 0) return TNX_OK;
-    const size_t shm = sizeof(MlpShared) + sizeof(MlpRows);
+int64_t tnx_mlp_workspace_bytes(int32_t D, int32_t O, int64_t n_rows) {
+    if (D < 1 || D > kMlpMaxD || O < 1 || O > kMlpMaxO || n_rows < 0) return -1;
+    const ParamLayout L(D, O);
+    return bwd_groups(n_rows) * (int64_t)(L.P + 1) * (int64_t)sizeof(float);
+}
+
+static int launch_backward(const tnx_mlp_t *m, const float *X, const float *tgt,
+                           const float *w_out, int64_t n_rows, float *part, hipStream_t st) {
+    const size_t shm = sizeof(BwdShared);
+    static_assert(sizeof(BwdShared) <= 160 * 1024, "backward LDS exceeds 160 KiB");
     TNX_CHECK_HIP(hipFuncSetAttribute((const void *)mlp_backward_kernel,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    hipLaunchKernelGGL(mlp_backward_kernel, dim3((unsigned)((n_rows + 63) / 64)), dim3(64), shm,
-                       as_stream(stream), *m, X, tgt, w_out, n_rows, grad, loss_out);
+    hipLaunchKernelGGL(mlp_backward_kernel, dim3((unsigned)bwd_groups(n_rows)),
+                       dim3(kBwdThreads), shm, st, *m, X, tgt, w_out, n_rows, part);
     TNX_CHECK_LAUNCH("mlp_backward_kernel");
+    return TNX_OK;
+}
+
+static int check_train_args(const tnx_mlp_t *m, int64_t n_rows, void *workspace,
+                            int64_t ws_bytes, const char *who) {
+    int rc = check_mlp(m);
+    if (rc) return rc;
+    TNX_CHECK_ARG(n_rows >= 1 && n_rows <= ((int64_t)1 << 31), "%s: n_rows must be in [1, 2^31]",
+                  who);
+    const int64_t need = tnx_mlp_workspace_bytes(m->D, m->O, n_rows);
+    TNX_CHECK_ARG(workspace && ws_bytes >= need, "%s: workspace of %lld bytes needed (got %lld)",
+                  who, (long long)need, (long long)ws_bytes);
+    return TNX_OK;
+}
+
+int tnx_mlp_backward(const tnx_mlp_t *m, const float *X, const float *tgt, const float *w_out,
+                     int64_t n_rows, float *grad, float *loss_out, void *workspace,
+                     int64_t ws_bytes, void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(X && tgt && grad, "tnx_mlp_backward: null pointer");
+    int rc = check_train_args(m, n_rows, workspace, ws_bytes, "tnx_mlp_backward");
+    if (rc) return rc;
+    hipStream_t st = as_stream(stream);
+    float *part = static_cast<float *>(workspace);
+    rc = launch_backward(m, X, tgt, w_out, n_rows, part, st);
+    if (rc) return rc;
+    const ParamLayout L(m->D, m->O);
+    hipLaunchKernelGGL(mlp_reduce_adam_kernel, dim3((unsigned)((L.P + kRedParams) / kRedParams)),
+                       dim3(256), 0, st, part, (int)bwd_groups(n_rows), grad, loss_out, 0, *m,
+                       nullptr, nullptr, nullptr, 0.f, 0.f, 0.f, 0.f, 1.f, 1.f);
+    TNX_CHECK_LAUNCH("mlp_reduce_adam_kernel");
     return TNX_OK;
 }
 
@@ -267,18 +504,47 @@ int tnx_mlp_adam(const tnx_mlp_t *m, float *master, const float *grad, float *m1
     clear_error();
     int rc = check_mlp(m);
     if (rc) return rc;
-    const int64_t need = (int64_t)kMlpH * m->D + kMlpH + kMlpH * kMlpH + kMlpH +
-                         (int64_t)m->O * kMlpH + m->O;
-    TNX_CHECK_ARG(master && grad && m1 && m2 && n_params == need && t >= 1,
-                  "tnx_mlp_adam: bad arguments (n_params %lld, expected %lld)",
-                  (long long)n_params, (long long)need);
+    const ParamLayout L(m->D, m->O);
+    TNX_CHECK_ARG(master && grad && m1 && m2 && n_params == L.P && t >= 1,
+                  "tnx_mlp_adam: bad arguments (n_params %lld, expected %d)",
+                  (long long)n_params, L.P);
     const float c1 = 1.f - powf(beta1, (float)t), c2 = 1.f - powf(beta2, (float)t);
-    hipLaunchKernelGGL(mlp_adam_kernel, dim3((unsigned)((n_params + 255) / 256)), dim3(256), 0,
-                       as_stream(stream), master, grad, m1, m2, n_params, lr, beta1, beta2, eps,
-                       c1, c2, const_cast<uint16_t *>(m->W1), const_cast<uint16_t *>(m->W2),
-                       const_cast<uint16_t *>(m->W3), const_cast<float *>(m->b1),
-                       const_cast<float *>(m->b2), const_cast<float *>(m->b3), m->D, m->O);
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3((unsigned)((L.P + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), grad, *m, master, m1, m2, L.P, lr, beta1, beta2, eps,
+                       c1, c2);
     TNX_CHECK_LAUNCH("mlp_adam_kernel");
+    return TNX_OK;
+}
+
+int tnx_mlp_fit(const tnx_mlp_t *m, float *master, float *m1, float *m2, int64_t n_params,
+                const float *X, const float *tgt, int64_t n_rows, const tnx_adam_t *opt,
+                float *grad, void *workspace, int64_t ws_bytes, float *loss_hist, void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(master && m1 && m2 && X && tgt && opt && grad, "tnx_mlp_fit: null pointer");
+    int rc = check_train_args(m, n_rows, workspace, ws_bytes, "tnx_mlp_fit");
+    if (rc) return rc;
+    const ParamLayout L(m->D, m->O);
+    TNX_CHECK_ARG(n_params == L.P, "tnx_mlp_fit: n_params %lld, expected %d",
+                  (long long)n_params, L.P);
+    TNX_CHECK_ARG(opt->steps >= 0 && opt->t0 >= 0 && opt->lr > 0.f && opt->lr_min >= 0.f,
+                  "tnx_mlp_fit: bad schedule (steps %d, t0 %d)", opt->steps, opt->t0);
+    hipStream_t st = as_stream(stream);
+    float *part = static_cast<float *>(workspace);
+    const int G = (int)bwd_groups(n_rows);
+    const double pi = 3.14159265358979323846;
+    for (int s = 0; s < opt->steps; s++) {
+        const float lr = (float)(opt->lr_min + (opt->lr - opt->lr_min) * 0.5 *
+                                                   (1.0 + cos(pi * (double)s / opt->steps)));
+        rc = launch_backward(m, X, tgt, nullptr, n_rows, part, st);
+        if (rc) return rc;
+        const int t = opt->t0 + s + 1;
+        const float c1 = 1.f - powf(opt->beta1, (float)t), c2 = 1.f - powf(opt->beta2, (float)t);
+        hipLaunchKernelGGL(mlp_reduce_adam_kernel,
+                           dim3((unsigned)((L.P + kRedParams) / kRedParams)), dim3(256), 0, st,
+                           part, G, grad, loss_hist ? loss_hist + s : nullptr, 1, *m, master, m1,
+                           m2, lr, opt->beta1, opt->beta2, opt->eps, c1, c2);
+        TNX_CHECK_LAUNCH("mlp_reduce_adam_kernel");
+    }
     return TNX_OK;
 }
 

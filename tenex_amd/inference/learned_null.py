@@ -94,14 +94,21 @@ class NullMLP:
         call("tnx_mlp_forward", C.byref(m), ptr(X), X.shape[0], ptr(Y), self._s())
         return Y
 
+    def workspace(self, n_rows: int) -> torch.Tensor:
+        nb = int(_lib.load().tnx_mlp_workspace_bytes(self.D, self.O, int(n_rows)))
+        if nb < 0:
+            raise ValueError(f"no MLP workspace for D={self.D}, O={self.O}, rows={n_rows}")
+        return torch.empty(max(1, nb // 4), dtype=torch.float32, device=self.dev)
+
     def loss_grad(self, X, T, w=None):
         X = X.contiguous().to(torch.float32)
         T = T.contiguous().to(torch.float32)
-        grad = torch.zeros(self.n_params, dtype=torch.float32, device=self.dev)
-        loss = torch.zeros(1, dtype=torch.float32, device=self.dev)
+        grad = torch.empty(self.n_params, dtype=torch.float32, device=self.dev)
+        loss = torch.empty(1, dtype=torch.float32, device=self.dev)
+        ws = self.workspace(X.shape[0])
         m = self.struct()
         call("tnx_mlp_backward", C.byref(m), ptr(X), ptr(T), ptr(w), X.shape[0], ptr(grad),
-             ptr(loss), self._s())
+             ptr(loss), ptr(ws), ws.numel() * 4, self._s())
         return loss, grad
 
     def adam_step(self, grad, lr=3e-3, b1=0.9, b2=0.999, eps=1e-8):
@@ -110,29 +117,46 @@ class NullMLP:
         call("tnx_mlp_adam", C.byref(m), ptr(self.master), ptr(grad), ptr(self.m1), ptr(self.m2),
              self.n_params, float(lr), float(b1), float(b2), float(eps), self.t, self._s())
 
-    def fit(self, X, T, steps=400, lr=3e-3, batch=8192, seed=0):
+    def fit(self, X, T, steps=300, lr=3e-3, lr_min=1e-5, batch=65536, seed=0):
+        """Adam with a cosine learning-rate schedule from lr down to lr_min.
+        Up to `batch` rows train full batch in one native call
+        (tnx_mlp_fit: no host round trip per step); larger sets draw a
+        device-side minibatch per step.  Returns a few of the step losses."""
         n = X.shape[0]
-        g = torch.Generator(device="cpu").manual_seed(seed)
-        losses = []
-        for s in range(steps):
-            if n > batch:
-                idx = torch.randint(0, n, (batch,), generator=g).to(self.dev)
-                xb, tb = X[idx], T[idx]
-            else:
-                xb, tb = X, T
-            loss, grad = self.loss_grad(xb, tb)
-            self.adam_step(grad, lr=lr * (0.5 * (1 + math.cos(math.pi * s / steps))) + 1e-5)
-            if s % 50 == 0 or s == steps - 1:
-                losses.append(float(loss.item()))
-        return losses
+        X = X.contiguous().to(torch.float32)
+        T = T.contiguous().to(torch.float32)
+        grad = torch.empty(self.n_params, dtype=torch.float32, device=self.dev)
+        hist = torch.zeros(max(1, steps), dtype=torch.float32, device=self.dev)
+        m = self.struct()
+        st = self._s()
+        if n <= batch:
+            ws = self.workspace(n)
+            opt = _lib.Adam(lr=lr, lr_min=lr_min, beta1=0.9, beta2=0.999, eps=1e-8, steps=steps,
+                            t0=self.t)
+            call("tnx_mlp_fit", C.byref(m), ptr(self.master), ptr(self.m1), ptr(self.m2),
+                 self.n_params, ptr(X), ptr(T), n, C.byref(opt), ptr(grad), ptr(ws),
+                 ws.numel() * 4, ptr(hist), st)
+            self.t += steps
+        else:
+            ws = self.workspace(batch)
+            g = torch.Generator(device=self.dev).manual_seed(seed)
+            for s in range(steps):
+                idx = torch.randint(0, n, (batch,), generator=g, device=self.dev)
+                xb, tb = X[idx].contiguous(), T[idx].contiguous()
+                call("tnx_mlp_backward", C.byref(m), ptr(xb), ptr(tb), None, batch, ptr(grad),
+                     ptr(hist[s:s + 1]), ptr(ws), ws.numel() * 4, st)
+                self.adam_step(grad, lr=lr_min + (lr - lr_min) * 0.5 *
+                               (1 + math.cos(math.pi * s / steps)))
+        h = hist.cpu().numpy()
+        return [float(v) for v in h[:: max(1, steps // 8)]] + [float(h[-1])]
 
 
 def pair_features(prep: core.Prepared, tgt: np.ndarray, src: np.ndarray) -> torch.Tensor:
     """Features of pairs: per-gene marginals of source and target plus the
     log of the plug-in bias scale b_t (b_t - 1)(b_s - 1) / (2 L)."""
     dev = prep.device
-    t = torch.from_numpy(np.asarray(tgt, np.int64)).to(dev)
-    s = torch.from_numpy(np.asarray(src, np.int64)).to(dev)
+    t = torch.as_tensor(tgt, dtype=torch.int64).to(dev)
+    s = torch.as_tensor(src, dtype=torch.int64).to(dev)
     f = prep.feat
     bt = f[t, 0]
     bs = f[s, 0]
@@ -149,7 +173,8 @@ def _norm_stats(X: torch.Tensor):
 def plain_stats_pairs(prep, pairs, obs_pub, n_surr, shuffle_method, block_length, seed,
                       rng_device, perm_batch=32):
     """Plain surrogate (sum, sumsq, count) of an explicit [target, source]
-    pair list, as (P,) arrays (float64, float64, int32)."""
+    pair list (host int array (P, 2)), as device (P,) tensors (float64,
+    float64, int32)."""
     n, T = prep.n_vars, prep.T
     dev = prep.device
     cap = core.surrogate_cap(prep.npv_host, pairs[:, 0], pairs[:, 1])
@@ -162,106 +187,104 @@ def plain_stats_pairs(prep, pairs, obs_pub, n_surr, shuffle_method, block_length
         perm, eff, nb = core.block_perm_device(n, T, shuffle_method, block_length, k0, k1, seed,
                                                rng_device, dev)
         core.surrogate_run(prep, plan, perm, eff, obs_pub, s, s2, cg)
-    slot_s = torch.from_numpy(pairs[:, 1]).to(dev)
-    slot_t = torch.from_numpy(pairs[:, 0]).to(dev)
-    return (s[slot_s, slot_t].cpu().numpy(), s2[slot_s, slot_t].cpu().numpy(),
-            cg[slot_s, slot_t].cpu().numpy())
+    pd = torch.from_numpy(np.ascontiguousarray(pairs, dtype=np.int64)).to(dev)
+    return s[pd[:, 1], pd[:, 0]], s2[pd[:, 1], pd[:, 0]], cg[pd[:, 1], pd[:, 0]]
+
+
+def _moments(s, s2, n_surr):
+    mean = s / n_surr
+    return mean, torch.sqrt(torch.clamp(s2 / n_surr - mean * mean, min=0.0))
 
 
 def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogates,
                            shuffle_method, block_length, seed, rng_device, p_method, fdr, mask,
                            src_idx, dev, train_frac: float = 0.05, band: float = 1.0,
-                           steps: int = 400, lr: float = 3e-3, min_train: int = 2000,
-                           model_seed: int = 0, band_sigmas: float = 3.0):
+                           steps: int = 300, lr: float = 3e-3, min_train: int = 2000,
+                           model_seed: int = 0, band_sigmas: float = 4.0):
     """Learned-null surrogate test; returns a SurrogateTestResult with an
-    extra `learned_info` dict (training size, re-run size, timings, losses)."""
-    from scipy.special import erfc
-
+    extra `learned_info` dict (training size, re-run size, timings, losses).
+    `mask` is the (n, n) bool tensor of tested [source, target] pairs."""
     from . import bh_fdr_gpu
-    from .surrogate_test import SurrogateTestResult
+    from .surrogate_test import SurrogateTestResult, surrogate_decision
     t0 = time.perf_counter()
     n = prep.n_vars
-    pub = np.argwhere(mask)                  # [source, target] of tested pairs
+    mask = torch.as_tensor(mask, dtype=torch.bool, device=dev)
+    pub_d = torch.nonzero(mask)                           # [source, target] of tested pairs
     if src_idx is not None:
-        pub = pub[np.isin(pub[:, 0], src_idx)]
+        pub_d = pub_d[torch.isin(pub_d[:, 0], torch.as_tensor(src_idx, device=dev).long())]
+    pub = pub_d.cpu().numpy()
     pairs = np.stack([pub[:, 1], pub[:, 0]], 1).astype(np.int64)   # [target, source]
     P = len(pairs)
     rng = np.random.default_rng(seed)
     n_train = min(P, max(min_train, int(train_frac * P)))
     tr = rng.choice(P, size=n_train, replace=False)
-    # 1. plain surrogates on the training subset
-    s, s2, cg = plain_stats_pairs(prep, pairs[tr], obs_d, n_surrogates, shuffle_method,
-                                  block_length, seed, rng_device)
-    mean_tr = s / n_surrogates
-    std_tr = np.sqrt(np.maximum(s2 / n_surrogates - mean_tr ** 2, 0.0))
-    t_train_plain = time.perf_counter() - t0
-    # 2. train the MLP on (mean, log std)
-    X_all = pair_features(prep, pairs[:, 0], pairs[:, 1])
-    mu, sd = _norm_stats(X_all)
-    Xn = (X_all - mu) / sd
-    Y = np.stack([mean_tr, np.log(np.maximum(std_tr, 1e-9))], 1)
-    ymu, ysd = Y.mean(0), np.maximum(Y.std(0), 1e-9)
-    Yn = torch.from_numpy(((Y - ymu) / ysd).astype(np.float32)).to(dev)
-    # hold out a fifth of the plain pairs to measure the model's z error
-    n_fit = max(1, int(0.8 * n_train))
-    fit_rows = torch.from_numpy(np.arange(n_fit)).to(dev)
     trd = torch.from_numpy(tr).to(dev)
+    # 1. plain surrogates on the training subset
+    s, s2, _ = plain_stats_pairs(prep, pairs[tr], obs_d, n_surrogates, shuffle_method,
+                                 block_length, seed, rng_device)
+    mean_tr, std_tr = _moments(s, s2, n_surrogates)
+    torch.cuda.synchronize(dev)
+    t_train_plain = time.perf_counter() - t0
+    # 2. train the MLP on (mean, log std); a fifth of the plain pairs is held
+    #    out to measure the model's z error
+    X_all = pair_features(prep, pub_d[:, 1], pub_d[:, 0])
+    mu, sd = _norm_stats(X_all)
+    Xn = ((X_all - mu) / sd).contiguous()
+    Y = torch.stack([mean_tr, torch.log(torch.clamp(std_tr, min=1e-9))], 1)
+    ymu = Y.mean(0)
+    ysd = torch.clamp(Y.std(0, unbiased=False), min=1e-9)
+    Yn = ((Y - ymu) / ysd).to(torch.float32)
+    n_fit = max(1, int(0.8 * n_train))
     model = NullMLP(Xn.shape[1], 2, dev, seed=model_seed)
-    losses = model.fit(Xn[trd][fit_rows], Yn[fit_rows], steps=steps, lr=lr, seed=seed)
+    losses = model.fit(Xn[trd[:n_fit]], Yn[:n_fit], steps=steps, lr=lr, seed=seed)
     t_fit = time.perf_counter() - t0 - t_train_plain
-    # 3. predict for every pair
-    Yp = model.forward(Xn).cpu().numpy().astype(np.float64) * ysd + ymu
-    mean_p = Yp[:, 0]
-    std_p = np.exp(Yp[:, 1])
-    obs = observed_te[pub[:, 0], pub[:, 1]].astype(np.float64)
-    z = (obs - mean_p) / np.maximum(std_p, 1e-12)
-    ho = tr[n_fit:]
-    if len(ho):
-        z_true = (obs[ho] - mean_tr[n_fit:]) / np.maximum(std_tr[n_fit:], 1e-12)
-        z_err = float(np.sqrt(np.mean((z[ho] - z_true) ** 2)))
+    # 3. predicted null for every pair, plain statistics where measured
+    Yp = model.forward(Xn).to(torch.float64) * ysd + ymu
+    mean_p = Yp[:, 0].contiguous()
+    std_p = torch.exp(Yp[:, 1])
+    obs = obs_d[pub_d[:, 0], pub_d[:, 1]].to(torch.float64)
+    zf = lambda m, sdv: (obs - m) / torch.clamp(sdv, min=1e-12)
+    z = zf(mean_p, std_p)
+    if n_train > n_fit:
+        ho = trd[n_fit:]
+        z_true = (obs[ho] - mean_tr[n_fit:]) / torch.clamp(std_tr[n_fit:], min=1e-12)
+        z_err = float(torch.sqrt(torch.mean((z[ho] - z_true) ** 2)))
     else:
         z_err = 0.0
     band = max(band, band_sigmas * z_err)
-    mean_p[tr] = mean_tr
-    std_p[tr] = std_tr
-    z = (obs - mean_p) / np.maximum(std_p, 1e-12)
-    p_pred = (0.5 * erfc(z / math.sqrt(2.0))).astype(np.float32)
-    adj = bh_fdr_gpu(torch.from_numpy(p_pred).to(dev), alpha=fdr).cpu().numpy()
+    mean_p[trd] = mean_tr
+    std_p[trd] = std_tr
+    z = zf(mean_p, std_p)
+    p_pred = (0.5 * torch.special.erfc(z / math.sqrt(2.0))).to(torch.float32)
+    adj = bh_fdr_gpu(p_pred, alpha=fdr)
     sig = (adj < fdr) & (obs - mean_p > 0)
-    if sig.any():
+    if bool(sig.any()):
         z_thr = float(z[sig].min())
     else:
-        z_thr = float(-np.log(fdr / max(P, 1)))  # no predicted edge: a high bar
-    border = np.abs(z - z_thr) < band
-    border[tr] = False
-    rr = np.nonzero(border)[0]
+        z_thr = float(-math.log(fdr / max(P, 1)))  # no predicted edge: a high bar
+    border = torch.abs(z - z_thr) < band
+    border[trd] = False
+    rr_d = torch.nonzero(border).flatten()
+    rr = rr_d.cpu().numpy()
     # 4. plain surrogates for the borderline pairs
     if len(rr):
-        s_b, s2_b, cg_b = plain_stats_pairs(prep, pairs[rr], obs_d, n_surrogates,
-                                            shuffle_method, block_length, seed, rng_device)
-        mb = s_b / n_surrogates
-        mean_p[rr] = mb
-        std_p[rr] = np.sqrt(np.maximum(s2_b / n_surrogates - mb ** 2, 0.0))
-    t_total = time.perf_counter() - t0
+        s_b, s2_b, _ = plain_stats_pairs(prep, pairs[rr], obs_d, n_surrogates, shuffle_method,
+                                         block_length, seed, rng_device)
+        mean_p[rr_d], std_p[rr_d] = _moments(s_b, s2_b, n_surrogates)
     # 5. decision with the reference arithmetic on the merged null
-    mean_m = np.zeros((n, n), np.float32)
-    std_m = np.zeros((n, n), np.float32)
-    mean_m[pub[:, 0], pub[:, 1]] = mean_p
-    std_m[pub[:, 0], pub[:, 1]] = std_p
-    eff = (observed_te - mean_m).astype(np.float32)
-    eps = np.float32(1e-12)
-    zz = (observed_te - mean_m) / np.maximum(std_m, eps)
-    pv = (0.5 * erfc(zz / np.sqrt(2.0))).astype(np.float32)
-    np.fill_diagonal(pv, 1.0)
-    adj_off = bh_fdr_gpu(torch.from_numpy(pv[mask]).to(dev), alpha=fdr).cpu().numpy()
-    sigm = np.zeros((n, n), bool)
-    sigm[mask] = adj_off < fdr
-    sigm &= eff > 0
-    edges = np.argwhere(sigm).astype(np.int64)
-    grn = make_grn(np.asarray(variable_names), edges, eff[edges[:, 0], edges[:, 1]])
+    S = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    S2 = torch.zeros_like(S)
+    S[pub_d[:, 0], pub_d[:, 1]] = mean_p * n_surrogates
+    S2[pub_d[:, 0], pub_d[:, 1]] = (std_p * std_p + mean_p * mean_p) * n_surrogates
+    mean_m, std_m, eff, pv, sigm = surrogate_decision(obs_d, S, S2, None, n_surrogates,
+                                                      "parametric", fdr, mask, dev, host=False)
+    edges = torch.nonzero(sigm)
+    grn = make_grn(np.asarray(variable_names), edges.cpu().numpy().astype(np.int64),
+                   eff[edges[:, 0], edges[:, 1]].cpu().numpy())
+    t_total = time.perf_counter() - t0
     vprint(f"[TENEX] learned null: trained on {n_train} pairs, re-ran {len(rr)} of {P}; "
            f"{len(grn)} edges")
-    res = SurrogateTestResult(observed_te=observed_te, mean_surrogate_te=mean_m,
+    res = SurrogateTestResult(observed_te=obs_d, mean_surrogate_te=mean_m,
                               std_surrogate_te=std_m, effective_te=eff, p_values=pv, grn=grn,
                               n_surrogates=n_surrogates, shuffle_method=shuffle_method,
                               block_length=block_length if shuffle_method == "block" else None,

@@ -36,12 +36,19 @@ from ..kernels import get_kernel
 from . import GRN, InferenceMethod, bh_fdr_gpu, build_pairs, make_grn
 
 
+_MATRIX_FIELDS = frozenset(("observed_te", "mean_surrogate_te", "std_surrogate_te",
+                            "effective_te", "p_values"))
+
+
 @dataclass
 class SurrogateTestResult:
     """observed_te, mean_surrogate_te, std_surrogate_te, effective_te,
     p_values: (n, n) float32 in the public layout [i, j] = TE(i -> j);
     grn; n_surrogates; shuffle_method; block_length; p_method; fdr
-    (tenex/inference/surrogate_test.py:74-104)."""
+    (tenex/inference/surrogate_test.py:74-104).
+
+    The matrices may be handed over as device tensors: they stay in device
+    memory until first read, which copies them to numpy once."""
     observed_te: np.ndarray
     mean_surrogate_te: np.ndarray
     std_surrogate_te: np.ndarray
@@ -53,6 +60,13 @@ class SurrogateTestResult:
     block_length: int | None
     p_method: str
     fdr: float
+
+    def __getattribute__(self, name):
+        v = object.__getattribute__(self, name)
+        if name in _MATRIX_FIELDS and isinstance(v, torch.Tensor):
+            v = v.cpu().numpy()
+            object.__setattr__(self, name, v)
+        return v
 
 
 def shuffle_bins(bins: torch.Tensor, T: int, perm_k: torch.Tensor, eff_bl: int) -> torch.Tensor:
@@ -115,28 +129,37 @@ def surrogate_accumulate(prep: core.Prepared, observed_pub: torch.Tensor, n_surr
 
 
 def surrogate_decision(observed_te, sum_s, sumsq_s, count_ge, n_surr, p_method, fdr, mask,
-                       device):
-    """Statistics -> p-values -> BH -> edges, with the reference's arithmetic
+                       device, host: bool = True):
+    """Statistics -> p-values -> BH -> edges with the reference's arithmetic
     (surrogate_test.py:890-929): float32 mean/std from FP64 sums, z in
-    float32, erfc in float64, BH in float32 on the device."""
-    mean = (sum_s / n_surr).astype(np.float32)
-    var = sumsq_s / n_surr - (sum_s / n_surr) ** 2
-    std = np.sqrt(np.maximum(var, 0.0)).astype(np.float32)
-    eff = (observed_te - mean).astype(np.float32)
+    float32, erfc in float64, BH in float32.  Inputs may be numpy arrays or
+    tensors; the arithmetic runs on `device`.  Returns (mean, std, eff, p,
+    sig) as numpy arrays, or as tensors on `device` when host=False."""
+    dev = torch.device(device)
+    t = lambda a, dt: (a if isinstance(a, torch.Tensor) else torch.from_numpy(np.asarray(a))).to(
+        device=dev, dtype=dt)
+    obs = t(observed_te, torch.float32)
+    S = t(sum_s, torch.float64)
+    S2 = t(sumsq_s, torch.float64)
+    mean64 = S / n_surr
+    mean = mean64.to(torch.float32)
+    var = S2 / n_surr - mean64 * mean64
+    std = torch.sqrt(torch.clamp(var, min=0.0)).to(torch.float32)
+    eff = obs - mean
     if p_method == "parametric":
-        from scipy.special import erfc
-        eps = np.float32(1e-12)
-        z = (observed_te - mean) / np.maximum(std, eps)
-        p = (0.5 * erfc(z / np.sqrt(2.0))).astype(np.float32)
+        z = (obs - mean) / torch.clamp(std, min=1e-12)
+        p = (0.5 * torch.special.erfc(z.to(torch.float64) / np.sqrt(2.0))).to(torch.float32)
     else:
-        p = ((1.0 + count_ge) / (1.0 + n_surr)).astype(np.float32)
-    np.fill_diagonal(p, 1.0)
-    p_off = p[mask]
-    adj_off = bh_fdr_gpu(torch.from_numpy(p_off).to(device), alpha=fdr).cpu().numpy()
-    sig = np.zeros(p.shape, dtype=bool)
-    sig[mask] = adj_off < fdr
+        cg = t(count_ge, torch.float64)
+        p = ((1.0 + cg) / (1.0 + n_surr)).to(torch.float32)
+    p.fill_diagonal_(1.0)
+    m = t(mask, torch.bool)
+    adj = bh_fdr_gpu(p[m], alpha=fdr)
+    sig = torch.zeros_like(m)
+    sig[m] = adj < fdr
     sig &= eff > 0
-    return mean, std, eff, p, sig
+    out = (mean, std, eff, p, sig)
+    return tuple(x.cpu().numpy() for x in out) if host else out
 
 
 class SurrogateTestMethod(InferenceMethod):
@@ -151,7 +174,8 @@ class SurrogateTestMethod(InferenceMethod):
         n_surrogates=100, shuffle_method='block', block_length=None,
         p_method='parametric', fdr=0.01, seed=42, fused=None, sources) plus
         rng_device (generator device, default: the compute device),
-        pairs_scope ('all' | 'sources': compute only source pairs) and
+        targets (restrict the tested pairs to these target names),
+        pairs_scope ('all' | 'sources': compute only the tested pairs) and
         learned (None | 'mlp', with learned_kwargs)."""
         bin_data = kwargs.get("bin_data")
         n_per_var = kwargs.get("n_per_var")
@@ -193,13 +217,25 @@ class SurrogateTestMethod(InferenceMethod):
                                else None)
         rng_device = kwargs.get("rng_device") or dev
         sources = kwargs.get("sources")
+        targets = kwargs.get("targets")
         scope = kwargs.get("pairs_scope", "all")
+        names_arr = np.asarray(variable_names)
+        mask = torch.zeros((n_vars, n_vars), dtype=torch.bool, device=dev)
         if sources is not None:
-            pairs_bh = build_pairs(np.asarray(variable_names), sources)
-            mask = np.zeros((n_vars, n_vars), dtype=bool)
-            mask[pairs_bh[:, 0], pairs_bh[:, 1]] = True
+            # rows = the sources, every other variable a target (build_pairs)
+            _, s_rows, _ = np.intersect1d(names_arr, sources, return_indices=True)
+            mask[torch.from_numpy(s_rows.astype(np.int64)).to(dev)] = True
         else:
-            mask = ~np.eye(n_vars, dtype=bool)
+            mask.fill_(True)
+        mask.fill_diagonal_(False)
+        tgt_idx = None
+        if targets is not None:
+            # extension: restrict the tested pairs to these target names too
+            _, tgt_idx, _ = np.intersect1d(names_arr, targets, return_indices=True)
+            tmask = torch.zeros(n_vars, dtype=torch.bool, device=dev)
+            tmask[torch.from_numpy(tgt_idx.astype(np.int64)).to(dev)] = True
+            mask &= tmask[None, :]
+            tgt_idx = tgt_idx.astype(np.int32)
         src_idx = None
         if scope == "sources" and sources is not None:
             _, src_idx, _ = np.intersect1d(np.asarray(variable_names), sources,
@@ -222,21 +258,20 @@ class SurrogateTestMethod(InferenceMethod):
                 block_length, seed, rng_device, p_method, fdr, mask, src_idx, dev,
                 **(kwargs.get("learned_kwargs") or {}))
         fused = fused_kw is not False
+        tg_plan = tgt_idx if (scope == "sources" and tgt_idx is not None) else None
         s, s2, cg = surrogate_accumulate(prep, obs_d, n_surrogates, shuffle_method, block_length,
-                                         seed, rng_device, fused=fused, sources=src_idx)
-        sum_s = s.cpu().numpy()
-        sumsq_s = s2.cpu().numpy()
-        count_ge = cg.cpu().numpy()
+                                         seed, rng_device, fused=fused, targets=tg_plan,
+                                         sources=src_idx)
         if fused:
-            d = np.arange(n_vars)
-            count_ge[d, d] += n_surrogates  # the loop path's diagonal (te_k = obs = 0)
-        mean, std, eff, p, sig = surrogate_decision(observed_te, sum_s, sumsq_s, count_ge,
-                                                    n_surrogates, p_method, fdr, mask, dev)
-        pairs = np.argwhere(sig).astype(np.int64)
-        grn = make_grn(np.asarray(variable_names), pairs, eff[pairs[:, 0], pairs[:, 1]])
+            cg.diagonal().add_(n_surrogates)  # the loop path's diagonal (te_k = obs = 0)
+        mean, std, eff, p, sig = surrogate_decision(obs_d, s, s2, cg, n_surrogates, p_method,
+                                                    fdr, mask, dev, host=False)
+        edges = torch.nonzero(sig)
+        grn = make_grn(names_arr, edges.cpu().numpy().astype(np.int64),
+                       eff[edges[:, 0], edges[:, 1]].cpu().numpy())
         vprint(f"[TENEX] surrogate_test: {len(grn)} significant edges (FDR<{fdr})")
         return SurrogateTestResult(
-            observed_te=observed_te, mean_surrogate_te=mean, std_surrogate_te=std,
+            observed_te=obs_d, mean_surrogate_te=mean, std_surrogate_te=std,
             effective_te=eff, p_values=p, grn=grn, n_surrogates=n_surrogates,
             shuffle_method=shuffle_method,
             block_length=block_length if shuffle_method == "block" else None,

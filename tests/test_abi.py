@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_error_reporting():
     lib = _lib.load()
-    assert lib.tnx_abi_version() == _lib.ABI_VERSION == 1
+    assert lib.tnx_abi_version() == _lib.ABI_VERSION == 2
     info = _lib.PlanInfo()
     rc = lib.tnx_te_plan(None, 3, 10, None, 0, None, 0, 0, 0, None, None, None, 0, C.byref(info))
     assert rc == -1
@@ -42,9 +42,9 @@ def test_struct_layouts_match_header(tmp_path):
 #include <stddef.h>
 #include "{HEADER}"
 int main(void) {{
-  printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(tnx_device_props_t), sizeof(tnx_work_t),
+  printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(tnx_device_props_t), sizeof(tnx_work_t),
          sizeof(tnx_plan_info_t), sizeof(tnx_te_args_t), sizeof(tnx_surr_args_t),
-         sizeof(tnx_mlp_t));
+         sizeof(tnx_mlp_t), sizeof(tnx_adam_t));
   printf("%zu %zu %zu\\n", offsetof(tnx_te_args_t, out), offsetof(tnx_surr_args_t, sum_te),
          offsetof(tnx_plan_info_t, est_cost_wave));
   return 0;
@@ -54,5 +54,5 @@ int main(void) {{
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
     got = [C.sizeof(_lib.DeviceProps), C.sizeof(_lib.Work), C.sizeof(_lib.PlanInfo),
            C.sizeof(_lib.TeArgs), C.sizeof(_lib.SurrArgs), C.sizeof(_lib.Mlp),
-           _lib.TeArgs.out.offset, _lib.SurrArgs.sum_te.offset, _lib.PlanInfo.est_cost_wave.offset]
+           C.sizeof(_lib.Adam), _lib.TeArgs.out.offset, _lib.SurrArgs.sum_te.offset, _lib.PlanInfo.est_cost_wave.offset]
     assert [int(v) for v in out] == got

@@ -26,14 +26,18 @@ def test_golden_c0_bins_and_te_oracle():
     assert str(g["kernel"]) == "scatter_add"
 
 
-def test_golden_c0_scatter_add_restatement_bit_exact():
+def test_golden_c0_scatter_add_restatement():
+    """The torch-CPU restatement reproduces the reference's matrix.  torch's
+    CPU float32 sums split work by thread count and SIMD width, so the
+    comparison is bit-exact only on the host that wrote the fixture; across
+    hosts it holds to float32 rounding."""
     g = load("te_c0_100x500.npz")
     d = g["bins"].astype(np.int32)
     pairs = O.build_pairs_engine(100)
     te = S.te_pairs(d, int(g["b_max"]), pairs)
     M = np.zeros((100, 100), np.float32)
     M[pairs[:, 0], pairs[:, 1]] = te
-    assert np.array_equal(M.T, g["matrix"])
+    assert np.abs(M.T - g["matrix"]).max() < 1e-6
 
 
 def test_golden_surrogate_oracle():

@@ -21,8 +21,11 @@ def _params_as_used(model):
     return np.concatenate([bf(w1), b1, bf(w2), b2, bf(w3), b3])
 
 
-def test_mlp_forward_and_gradient_vs_oracle(dev):
-    D, Oo, N = 17, 2, 300
+@pytest.mark.parametrize("N", [1, 300, 20000])
+def test_mlp_forward_and_gradient_vs_oracle(dev, N):
+    """N = 20000 spans more 64-row chunks than workgroups, so workgroups
+    accumulate several chunks before their partial rows are summed."""
+    D, Oo = 17, 2
     m = NullMLP(D, Oo, dev, seed=3)
     rng = np.random.default_rng(0)
     X = rng.standard_normal((N, D)).astype(np.float32)
@@ -38,6 +41,42 @@ def test_mlp_forward_and_gradient_vs_oracle(dev):
     g = g.cpu().numpy()
     scale = np.abs(go).max()
     assert np.abs(g - go).max() < 1e-4 * scale
+
+
+def test_mlp_gradient_deterministic_and_wide_input(dev):
+    D, Oo, N = 64, 4, 5000
+    m = NullMLP(D, Oo, dev, seed=7)
+    rng = np.random.default_rng(1)
+    X = torch.from_numpy(rng.standard_normal((N, D)).astype(np.float32)).to(dev)
+    T = torch.from_numpy(rng.standard_normal((N, Oo)).astype(np.float32)).to(dev)
+    l1, g1 = m.loss_grad(X, T)
+    l2, g2 = m.loss_grad(X, T)
+    assert torch.equal(g1, g2) and torch.equal(l1, l2)
+    p = _params_as_used(m)
+    lo, go = O.mlp_loss_grad(p, D, 64, Oo, X.cpu().double().numpy(), T.cpu().double().numpy())
+    assert np.abs(g1.cpu().numpy() - go).max() < 1e-4 * np.abs(go).max()
+
+
+def test_native_fit_matches_stepwise_schedule(dev):
+    """tnx_mlp_fit (host loop in the library) against backward + Adam driven
+    step by step from Python with the same cosine schedule; learning rates
+    are powers of two so both sides form identical float32 rates."""
+    import math
+    rng = np.random.default_rng(3)
+    X = torch.from_numpy(rng.standard_normal((3000, 9)).astype(np.float32)).to(dev)
+    T = torch.from_numpy(rng.standard_normal((3000, 2)).astype(np.float32)).to(dev)
+    lr, lr_min, steps = 2.0 ** -9, 2.0 ** -17, 20
+    a = NullMLP(9, 2, dev, seed=5)
+    la = a.fit(X, T, steps=steps, lr=lr, lr_min=lr_min)
+    b = NullMLP(9, 2, dev, seed=5)
+    lb = []
+    for s in range(steps):
+        loss, g = b.loss_grad(X, T)
+        lb.append(float(loss.item()))
+        rate = lr_min + (lr - lr_min) * 0.5 * (1 + math.cos(math.pi * s / steps))
+        b.adam_step(g, lr=rate)
+    assert la[0] == lb[0] and la[-1] == lb[-1]
+    assert torch.equal(a.master, b.master) and torch.equal(a.W2, b.W2)
 
 
 def test_mlp_adam_and_bf16_copies(dev):
