@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void surrogate_wave_kernel(tnx_surr_args_t a, 
                     const uint32_t s4[4] = {sw.x & 0xFFFFu, sw.x >> 16, sw.y & 0xFFFFu, sw.y >> 16};
 #pragma unroll
                     for (int q = 0; q < 4; q++)
-                        if (t0 + q < L) lds_inc16(hist, s4[q] * (uint32_t)by + ((yw >> (8 * q)) & 0xFFu));
+                        if (t0 + q < L) lds_inc16(hist, __umul24(s4[q], (uint32_t)by) + ((yw >> (8 * q)) & 0xFFu));
                 }
                 wave_lds_fence();
                 // reduce

@@ -16,27 +16,40 @@ __device__ __forceinline__ void lds_inc(uint32_t *p) {
     __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Cell of key state * by + y as a byte offset: states < 2^16 and by <= 127,
+// so the product fits the full-rate 24-bit multiplier (v_mul_u32_u24, which
+// also takes the 16-bit half of a packed state register directly); by4 is
+// 4 * by.  Three VALU operations per sample: multiply, shift of the source
+// byte, three-way add with the copy base.
+__device__ __forceinline__ void lds_inc_key(uint32_t *hist, uint32_t sv, uint32_t by4,
+                                            uint32_t yv) {
+    const uint32_t off = __umul24(sv, by4) + (yv << 2);
+    lds_inc(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) + off));
+}
+
 // 16 samples of one lane: 8 packed 16-bit states, 4 source dwords.
 __device__ __forceinline__ void acc16(uint32_t *hist, const uint32_t *sreg, const uint4 y4,
                                       uint32_t by) {
+    const uint32_t by4 = by << 2;
     const uint32_t yw[4] = {y4.x, y4.y, y4.z, y4.w};
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const uint32_t sv = (sreg[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
         const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-        lds_inc(hist + sv * by + yv);
+        lds_inc_key(hist, sv, by4, yv);
     }
 }
 
 __device__ __forceinline__ void acc16_tail(uint32_t *hist, const uint32_t *sreg, const uint4 y4,
                                            uint32_t by, int n_valid) {
+    const uint32_t by4 = by << 2;
     const uint32_t yw[4] = {y4.x, y4.y, y4.z, y4.w};
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         if (q < n_valid) {
             const uint32_t sv = (sreg[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
             const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-            lds_inc(hist + sv * by + yv);
+            lds_inc_key(hist, sv, by4, yv);
         }
     }
 }
@@ -108,6 +121,7 @@ __device__ __forceinline__ void hist_accumulate_regs(uint32_t *hist, TargetRegs<
 #pragma unroll
     for (int k = 0; k < NI * 8; k++) asm volatile("" : "+v"(r.s[k]));
     const uint32_t *sreg = r.s;
+    const uint32_t by4 = by << 2;
 #pragma unroll
     for (int i = 0; i < NI; i++) {
         const int t = 16 * lane + 1024 * i;
@@ -119,7 +133,7 @@ __device__ __forceinline__ void hist_accumulate_regs(uint32_t *hist, TargetRegs<
             for (int q = 0; q < 16; q++) {
                 const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
                 const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                lds_inc(hist + sv * by + yv);
+                lds_inc_key(hist, sv, by4, yv);
             }
         } else {
 #pragma unroll
@@ -127,7 +141,7 @@ __device__ __forceinline__ void hist_accumulate_regs(uint32_t *hist, TargetRegs<
                 if (t + q < L) {
                     const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
                     const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                    lds_inc(hist + sv * by + yv);
+                    lds_inc_key(hist, sv, by4, yv);
                 }
             }
         }
@@ -168,17 +182,62 @@ __device__ __forceinline__ uint32_t take_cell(uint32_t *hist, int cell, const Wa
     return v;
 }
 
-// Phases 2-4 with copies: as hist_reduce_wave.
+// Read an LDS cell and leave zero behind (one ds_wrxchg_rtn_b32).
+__device__ __forceinline__ uint32_t lds_take(uint32_t *p) {
+    return __hip_atomic_exchange(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Fold the R copies of a pair's histogram onto copy 0 with every lane busy.
+// Small histograms (S < 64) split the lanes into G = min(64 / S, R) groups:
+// lane (k, g) takes cell k of copies g, g + G, ...; the G partial sums then
+// meet in copies 0 .. G-1 and group 0 adds them up.  Larger ones walk the
+// cells 64 at a time.  On exit copy 0 holds the totals of cells [0, cells)
+// and every other copy is zero.  One wavefront; wave_lds_fence between
+// dependent LDS phases.
+__device__ __forceinline__ void fold_wave_copies(uint32_t *hist, int cells, const WaveCopies &cp,
+                                                 int lane) {
+    const int S = cp.S, R = cp.R;
+    if (R <= 1) return;
+    if (S < kWave) {
+        int G = kWave / S;
+        if (G > R) G = R;
+        const int k = lane % S, g = lane / S;
+        const bool act = g < G && k < cells;
+        uint32_t v = 0;
+        if (act)
+            for (int r = g; r < R; r += G) v += lds_take(hist + r * S + k);
+        wave_lds_fence();
+        if (act) hist[g * S + k] = v;
+        wave_lds_fence();
+        if (act && g == 0) {
+            uint32_t t = v;
+            for (int gg = 1; gg < G; gg++) t += lds_take(hist + gg * S + k);
+            hist[k] = t;
+        }
+    } else {
+        for (int k = lane; k < cells; k += kWave) {
+            uint32_t t = hist[k];
+            for (int r = 1; r < R; r++) t += lds_take(hist + r * S + k);
+            hist[k] = t;
+        }
+    }
+    wave_lds_fence();
+}
+
+// Phases 2-4 with copies: fold the copies, then per (x_t, y_t) column the
+// b_x cells of x_{t+tau}: S3 from every cell, S2b from the column sums.
+// Leaves the histogram zero.
 __device__ __forceinline__ void hist_reduce_wave_copies(uint32_t *hist, int bx, int by,
                                                         const WaveCopies &cp,
                                                         const float2 *__restrict__ tbl, int lane,
                                                         DF &s3, DF &s2b) {
     const int ncol = bx * by;
+    fold_wave_copies(hist, bx * ncol, cp, lane);
     DF a3 = {0.f, 0.f}, a2 = {0.f, 0.f};
     for (int col = lane; col < ncol; col += kWave) {
         uint32_t c2 = 0;
         for (int aa = 0; aa < bx; aa++) {
-            const uint32_t v = take_cell(hist, aa * ncol + col, cp);
+            const uint32_t v = lds_take(hist + aa * ncol + col);
             c2 += v;
             xlogx_add(a3, tbl, v);
         }
@@ -227,6 +286,7 @@ __device__ __forceinline__ void hist_pair_block(uint32_t *hist, uint32_t *c2b, i
                                                 DF &s3, DF &s2b) {
     const int tid = threadIdx.x, nth = blockDim.x;
     const int ncol = bx * by;
+    const uint32_t by4 = (uint32_t)by << 2;
     for (int i = tid; i < ncol; i += nth) c2b[i] = 0;
     int apass = cap / ncol;
     if (apass < 1) apass = 1;
@@ -248,7 +308,7 @@ __device__ __forceinline__ void hist_pair_block(uint32_t *hist, uint32_t *c2b, i
                 for (int q = 0; q < 16; q++) {
                     const uint32_t sv = (regs.s[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
                     const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                    if (t + q < L && sv >= lo && sv < hi) lds_inc(hc + sv * by + yv);
+                    if (t + q < L && sv >= lo && sv < hi) lds_inc_key(hc, sv, by4, yv);
                 }
             }
         }

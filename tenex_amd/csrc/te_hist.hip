@@ -39,6 +39,9 @@ constexpr int kBlockThreads = 512;
 #endif
 constexpr int kDefaultCapWave = TNX_DEFAULT_CAP_WAVE;
 constexpr int kDefaultCapBlock = 16384;  // 64 KiB + the b_t b_s marginal
+// LDS the planner aims for per block-kernel workgroup, so that two
+// workgroups fit one CU's 160 KiB: histogram + marginal + reduction scratch.
+constexpr int kBlockLdsTarget = 80 * 1024 - 512;
 constexpr int kWaveItemPairs = 256;  // pairs per wave-kernel work item
 constexpr int kBlockItemPairs = 16;  // pairs per block-kernel work item
 
@@ -250,7 +253,6 @@ int tnx_te_plan(const int32_t *npv, int64_t n_vars, int64_t L, const int32_t *tg
     TNX_CHECK_ARG(npv && info, "tnx_te_plan: null pointer");
     TNX_CHECK_ARG(n_vars >= 0 && n_vars < (1LL << 31), "tnx_te_plan: bad n_vars");
     if (cap_wave <= 0) cap_wave = kDefaultCapWave;
-    if (cap_block <= 0) cap_block = kDefaultCapBlock;
     std::vector<int32_t> tg, sr;
     if (tgt_list) tg.assign(tgt_list, tgt_list + n_tgt);
     else { tg.resize(n_vars); std::iota(tg.begin(), tg.end(), 0); }
@@ -261,7 +263,6 @@ int tnx_te_plan(const int32_t *npv, int64_t n_vars, int64_t L, const int32_t *tg
     for (int64_t v = 0; v < n_vars; v++)
         TNX_CHECK_ARG(npv[v] >= 1 && npv[v] <= 127, "tnx_te_plan: n_per_var[%lld]=%d not in [1,127]",
                       (long long)v, npv[v]);
-    TNX_CHECK_ARG(cap_block >= 127 * 127, "tnx_te_plan: cap_block must be >= 127*127 cells");
     // Sources ascending by level count, targets descending (heavy first).
     std::stable_sort(sr.begin(), sr.end(), [&](int32_t a, int32_t b) { return npv[a] < npv[b]; });
     std::stable_sort(tg.begin(), tg.end(), [&](int32_t a, int32_t b) { return npv[a] > npv[b]; });
@@ -276,9 +277,20 @@ int tnx_te_plan(const int32_t *npv, int64_t n_vars, int64_t L, const int32_t *tg
         kw[i] = std::upper_bound(sb.begin(), sb.end(), (int32_t)std::min<int64_t>(lim, 1 << 30)) -
                 sb.begin();
     }
+    // Widest (x_t, y_t) marginal among block-kernel pairs: a pass of the
+    // block kernel holds at least one value of x_{t+tau}, i.e. b_x * b_y cells.
+    int32_t maxcols = 0;
+    for (int64_t i = 0; i < nt; i++)
+        if (kw[i] < ns) maxcols = std::max<int32_t>(maxcols, npv[tg[i]] * sb[ns - 1]);
+    if (cap_block <= 0) {
+        cap_block = std::min(kDefaultCapBlock, kBlockLdsTarget / 4 - maxcols);
+        cap_block = std::max(cap_block, maxcols);
+    }
+    TNX_CHECK_ARG(cap_block >= maxcols && cap_block >= 1,
+                  "tnx_te_plan: cap_block %d below the widest block pair's b_t*b_s = %d",
+                  cap_block, maxcols);
     std::vector<tnx_work_t> wv, bk;
     double cw = 0, cb = 0;
-    int32_t maxcols = 0;
     for (int64_t g0 = 0; g0 < nt; g0 += kWavesPerWG) {
         int64_t g1 = std::min<int64_t>(nt, g0 + kWavesPerWG);
         int32_t cnt = (int32_t)(g1 - g0);
@@ -298,7 +310,6 @@ int tnx_te_plan(const int32_t *npv, int64_t n_vars, int64_t L, const int32_t *tg
                 int64_t cols = bx * sb[j];
                 int64_t passes = (bx * cols + cap_block - 1) / cap_block;
                 cb += (double)L * (double)passes + (double)(bx * cols);
-                maxcols = std::max<int32_t>(maxcols, (int32_t)cols);
             }
         }
     }
@@ -347,7 +358,6 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
                   "tnx_te_run: bad out_mode");
     TNX_CHECK_ARG(a.cap_wave >= 8 && a.cap_wave <= 8192, "tnx_te_run: cap_wave out of range");
     TNX_CHECK_ARG((size_t)kWavesPerWG * a.cap_wave * 4 <= 160 * 1024, "tnx_te_run: cap_wave too large");
-    TNX_CHECK_ARG(a.cap_block >= 127 * 127, "tnx_te_run: cap_block too small");
     hipStream_t s = as_stream(stream);
     if (a.n_wave_items > 0) {
         TNX_CHECK_ARG(a.work, "tnx_te_run: null work");
@@ -378,6 +388,9 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
         TNX_CHECK_ARG(a.work, "tnx_te_run: null work");
         TNX_CHECK_ARG(a.max_block_cols >= 1 && a.max_block_cols <= 127 * 127,
                       "tnx_te_run: bad max_block_cols");
+        TNX_CHECK_ARG(a.cap_block >= a.max_block_cols,
+                      "tnx_te_run: cap_block %d below max_block_cols %d", a.cap_block,
+                      a.max_block_cols);
         size_t shm = (size_t)(a.cap_block + a.max_block_cols) * 4;
         TNX_CHECK_ARG(shm <= 160 * 1024, "tnx_te_run: block kernel LDS %zu exceeds 160 KiB", shm);
         unsigned grid = (unsigned)std::min<int64_t>(a.n_block_items, 1 << 20);
