@@ -6,9 +6,6 @@
 #ifndef TNX_WAVE_COPY_BUDGET
 #define TNX_WAVE_COPY_BUDGET 2048
 #endif
-#ifndef TNX_BLOCK_COPY_BUDGET
-#define TNX_BLOCK_COPY_BUDGET 1
-#endif
 
 namespace tnx {
 
@@ -16,40 +13,40 @@ __device__ __forceinline__ void lds_inc(uint32_t *p) {
     __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Cell of key state * by + y as a byte offset: states < 2^16 and by <= 127,
-// so the product fits the full-rate 24-bit multiplier (v_mul_u32_u24, which
-// also takes the 16-bit half of a packed state register directly); by4 is
-// 4 * by.  Three VALU operations per sample: multiply, shift of the source
-// byte, three-way add with the copy base.
-__device__ __forceinline__ void lds_inc_key(uint32_t *hist, uint32_t sv, uint32_t by4,
-                                            uint32_t yv) {
-    const uint32_t off = __umul24(sv, by4) + (yv << 2);
+// Byte offset of key state * by + y in a histogram whose cells are `G`
+// words apart (cell-major privatised copies, below): kmul = 4 * G * by and
+// ysh = 2 + log2 G.  States < 2^16 and by <= 127, so the product fits the
+// full-rate 24-bit multiplier (v_mul_u32_u24, which also takes the 16-bit
+// half of a packed state register directly).  Three VALU operations per
+// sample: multiply, shift of the source byte, three-way add with the copy
+// base.
+__device__ __forceinline__ void lds_inc_key(uint32_t *hist, uint32_t sv, uint32_t kmul,
+                                            uint32_t yv, uint32_t ysh) {
+    const uint32_t off = __umul24(sv, kmul) + (yv << ysh);
     lds_inc(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) + off));
 }
 
 // 16 samples of one lane: 8 packed 16-bit states, 4 source dwords.
 __device__ __forceinline__ void acc16(uint32_t *hist, const uint32_t *sreg, const uint4 y4,
-                                      uint32_t by) {
-    const uint32_t by4 = by << 2;
+                                      uint32_t kmul, uint32_t ysh) {
     const uint32_t yw[4] = {y4.x, y4.y, y4.z, y4.w};
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const uint32_t sv = (sreg[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
         const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-        lds_inc_key(hist, sv, by4, yv);
+        lds_inc_key(hist, sv, kmul, yv, ysh);
     }
 }
 
 __device__ __forceinline__ void acc16_tail(uint32_t *hist, const uint32_t *sreg, const uint4 y4,
-                                           uint32_t by, int n_valid) {
-    const uint32_t by4 = by << 2;
+                                           uint32_t kmul, uint32_t ysh, int n_valid) {
     const uint32_t yw[4] = {y4.x, y4.y, y4.z, y4.w};
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         if (q < n_valid) {
             const uint32_t sv = (sreg[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
             const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-            lds_inc_key(hist, sv, by4, yv);
+            lds_inc_key(hist, sv, kmul, yv, ysh);
         }
     }
 }
@@ -59,8 +56,8 @@ __device__ __forceinline__ void acc16_tail(uint32_t *hist, const uint32_t *sreg,
 // per step (one dwordx4 of source bytes, two of states), two steps in
 // flight.  Rows must be padded to a multiple of 64 elements.
 __device__ __forceinline__ void hist_accumulate_mem(uint32_t *hist, const uint16_t *__restrict__ st,
-                                                    const int8_t *__restrict__ yr, uint32_t by,
-                                                    int L, int lane) {
+                                                    const int8_t *__restrict__ yr, uint32_t kmul,
+                                                    uint32_t ysh, int L, int lane) {
     int t = 16 * lane;
     for (; t + 1024 + 16 <= L; t += 2048) {
         const uint4 ya = *reinterpret_cast<const uint4 *>(yr + t);
@@ -71,15 +68,15 @@ __device__ __forceinline__ void hist_accumulate_mem(uint32_t *hist, const uint16
         const uint4 sb1 = *reinterpret_cast<const uint4 *>(st + t + 1032);
         const uint32_t sa[8] = {sa0.x, sa0.y, sa0.z, sa0.w, sa1.x, sa1.y, sa1.z, sa1.w};
         const uint32_t sb[8] = {sb0.x, sb0.y, sb0.z, sb0.w, sb1.x, sb1.y, sb1.z, sb1.w};
-        acc16(hist, sa, ya, by);
-        acc16(hist, sb, yb, by);
+        acc16(hist, sa, ya, kmul, ysh);
+        acc16(hist, sb, yb, kmul, ysh);
     }
     for (; t < L; t += 1024) {
         const uint4 ya = *reinterpret_cast<const uint4 *>(yr + t);
         const uint4 sa0 = *reinterpret_cast<const uint4 *>(st + t);
         const uint4 sa1 = *reinterpret_cast<const uint4 *>(st + t + 8);
         const uint32_t sa[8] = {sa0.x, sa0.y, sa0.z, sa0.w, sa1.x, sa1.y, sa1.z, sa1.w};
-        acc16_tail(hist, sa, ya, by, L - t);
+        acc16_tail(hist, sa, ya, kmul, ysh, L - t);
     }
 }
 
@@ -113,73 +110,76 @@ __device__ __forceinline__ void load_target_regs(TargetRegs<NI> &r,
 // (one dwordx4 load), 16 LDS increments.
 template <int NI>
 __device__ __forceinline__ void hist_accumulate_regs(uint32_t *hist, TargetRegs<NI> &r,
-                                                     const int8_t *__restrict__ yr, uint32_t by,
-                                                     int L, int lane) {
+                                                     const int8_t *__restrict__ yr, uint32_t kmul,
+                                                     uint32_t ysh, int L, int lane) {
     // Keep the packed states opaque per call: otherwise the compiler hoists
     // the per-sample 16-bit extractions out of the caller's source loop and
     // spends a register on every sample.
 #pragma unroll
     for (int k = 0; k < NI * 8; k++) asm volatile("" : "+v"(r.s[k]));
     const uint32_t *sreg = r.s;
-    const uint32_t by4 = by << 2;
+    // the source bytes of PF chunks are requested before the first of them is
+    // counted: one memory latency per PF chunks instead of one per chunk
+    // (PF = NI up to 10 chunks, half of NI above, to stay within 3 waves'
+    // worth of registers)
+    constexpr int PF = NI <= 10 ? NI : (NI + 1) / 2;
 #pragma unroll
-    for (int i = 0; i < NI; i++) {
-        const int t = 16 * lane + 1024 * i;
-        if (t >= L) break;
-        const uint4 y4 = *reinterpret_cast<const uint4 *>(yr + t);
-        const uint32_t yw[4] = {y4.x, y4.y, y4.z, y4.w};
-        if (t + 16 <= L) {
+    for (int g0 = 0; g0 < NI; g0 += PF) {
+        uint4 ys[PF];
 #pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
-                const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                lds_inc_key(hist, sv, by4, yv);
-            }
-        } else {
+        for (int u = 0; u < PF; u++) {
+            const int t = 16 * lane + 1024 * (g0 + u);
+            ys[u] = (g0 + u < NI && t < L) ? *reinterpret_cast<const uint4 *>(yr + t)
+                                           : make_uint4(0u, 0u, 0u, 0u);
+        }
 #pragma unroll
-            for (int q = 0; q < 16; q++) {
-                if (t + q < L) {
+        for (int u = 0; u < PF; u++) {
+            const int i = g0 + u;
+            const int t = 16 * lane + 1024 * i;
+            if (i >= NI || t >= L) break;
+            const uint32_t yw[4] = {ys[u].x, ys[u].y, ys[u].z, ys[u].w};
+            if (t + 16 <= L) {
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
                     const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
                     const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                    lds_inc_key(hist, sv, by4, yv);
+                    lds_inc_key(hist, sv, kmul, yv, ysh);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    if (t + q < L) {
+                        const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
+                        const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                        lds_inc_key(hist, sv, kmul, yv, ysh);
+                    }
                 }
             }
         }
     }
 }
 
-// Privatised histograms: a pair with `cells` cells gets R copies at an odd
-// stride S >= cells inside the wave's cap-cell LDS slice; lane l counts into
-// copy l % R.  Lanes whose samples share a key then mostly hit different
-// copies, and the odd stride puts one key's copies in different LDS banks,
-// which removes the same-address serialisation that dominates small
-// histograms (a pair with 8 cells would otherwise see ~20 lanes of a wave
-// on one address).
+// Privatised histograms, cell-major: a pair with `cells` cells gets G copies
+// (G a power of two <= 32); cell k of copy r is word k * G + r and lane l
+// counts into copy l % G.  LDS has 32 banks of one word, so with G = 32 a
+// lane always hits bank l % 32 whatever its key (no bank conflicts, and
+// lanes l and l + 32, the only ones sharing a copy, are served in different
+// halves of the wave).  Smaller G leaves 32 / G banks per copy and a 1 / cells
+// chance of a same-address pair.  The copies take G * cells words and the
+// folded totals `cells` more, within the wave's cap-word slice; budget
+// bounds G * cells because the reduction reads every copy.
 struct WaveCopies {
-    int S, R;
+    int G;    // copies
+    int gsh;  // log2 G
 };
 
-// budget bounds R * S: the reduction reads and zeroes R * S cells per pair,
-// so copies are only worth it for small histograms.
 __device__ __forceinline__ WaveCopies wave_copies(int cells, int cap, int budget) {
-    WaveCopies c;
-    c.S = cells | 1;
-    int lim = cap < budget ? cap : budget;
-    c.R = lim / c.S;
-    if (c.R > 32) c.R = 32;
-    if (c.R < 1) c.R = 1;
-    return c;
-}
-
-// Sum of the R copies of a cell; zeroes them.
-__device__ __forceinline__ uint32_t take_cell(uint32_t *hist, int cell, const WaveCopies &cp) {
-    uint32_t v = 0;
-    uint32_t *p = hist + cell;
-    for (int r = 0; r < cp.R; r++, p += cp.S) {
-        v += *p;
-        *p = 0;
+    WaveCopies c{1, 0};
+    while (c.G < 32 && 2 * c.G * cells <= budget && (2 * c.G + 1) * cells <= cap) {
+        c.G *= 2;
+        c.gsh += 1;
     }
-    return v;
+    return c;
 }
 
 // Read an LDS cell and leave zero behind (one ds_wrxchg_rtn_b32).
@@ -187,61 +187,47 @@ __device__ __forceinline__ uint32_t lds_take(uint32_t *p) {
     return __hip_atomic_exchange(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Fold the R copies of a pair's histogram onto copy 0 with every lane busy.
-// Small histograms (S < 64) split the lanes into G = min(64 / S, R) groups:
-// lane (k, g) takes cell k of copies g, g + G, ...; the G partial sums then
-// meet in copies 0 .. G-1 and group 0 adds them up.  Larger ones walk the
-// cells 64 at a time.  On exit copy 0 holds the totals of cells [0, cells)
-// and every other copy is zero.  One wavefront; wave_lds_fence between
-// dependent LDS phases.
-__device__ __forceinline__ void fold_wave_copies(uint32_t *hist, int cells, const WaveCopies &cp,
-                                                 int lane) {
-    const int S = cp.S, R = cp.R;
-    if (R <= 1) return;
-    if (S < kWave) {
-        int G = kWave / S;
-        if (G > R) G = R;
-        const int k = lane % S, g = lane / S;
-        const bool act = g < G && k < cells;
-        uint32_t v = 0;
-        if (act)
-            for (int r = g; r < R; r += G) v += lds_take(hist + r * S + k);
-        wave_lds_fence();
-        if (act) hist[g * S + k] = v;
-        wave_lds_fence();
-        if (act && g == 0) {
-            uint32_t t = v;
-            for (int gg = 1; gg < G; gg++) t += lds_take(hist + gg * S + k);
-            hist[k] = t;
-        }
-    } else {
-        for (int k = lane; k < cells; k += kWave) {
-            uint32_t t = hist[k];
-            for (int r = 1; r < R; r++) t += lds_take(hist + r * S + k);
-            hist[k] = t;
-        }
-    }
-    wave_lds_fence();
-}
-
-// Phases 2-4 with copies: fold the copies, then per (x_t, y_t) column the
-// b_x cells of x_{t+tau}: S3 from every cell, S2b from the column sums.
-// Leaves the histogram zero.
+// Phases 2-4 for one wavefront.  With copies, lane l first folds cells
+// k = l, l + 64, ...: the G copies of cell k are read from rotation
+// (k / (32 / G)) on, so the 32 lanes of a half-wave hit 32 distinct banks;
+// the totals go to the words after the copies and give S3.  Then per
+// (x_t, y_t) column the b_x totals of x_{t+tau} give S2b (without copies,
+// this column pass gives S3 too).  Leaves the slice zero.
 __device__ __forceinline__ void hist_reduce_wave_copies(uint32_t *hist, int bx, int by,
                                                         const WaveCopies &cp,
                                                         const float2 *__restrict__ tbl, int lane,
                                                         DF &s3, DF &s2b) {
     const int ncol = bx * by;
-    fold_wave_copies(hist, bx * ncol, cp, lane);
+    const int cells = bx * ncol;
     DF a3 = {0.f, 0.f}, a2 = {0.f, 0.f};
-    for (int col = lane; col < ncol; col += kWave) {
-        uint32_t c2 = 0;
-        for (int aa = 0; aa < bx; aa++) {
-            const uint32_t v = lds_take(hist + aa * ncol + col);
-            c2 += v;
-            xlogx_add(a3, tbl, v);
+    if (cp.G > 1) {
+        const int G = cp.G;
+        uint32_t *tot = hist + G * cells;
+        const int per_sh = 5 - cp.gsh;  // log2(32 / G)
+        for (int k = lane; k < cells; k += kWave) {
+            const int f = k >> per_sh;
+            uint32_t *row = hist + (k << cp.gsh);
+            uint32_t t = 0;
+            for (int j = 0; j < G; j++) t += lds_take(row + ((j + f) & (G - 1)));
+            tot[k] = t;
+            xlogx_add(a3, tbl, t);
         }
-        xlogx_add(a2, tbl, c2);
+        wave_lds_fence();
+        for (int col = lane; col < ncol; col += kWave) {
+            uint32_t c2 = 0;
+            for (int aa = 0; aa < bx; aa++) c2 += lds_take(tot + aa * ncol + col);
+            xlogx_add(a2, tbl, c2);
+        }
+    } else {
+        for (int col = lane; col < ncol; col += kWave) {
+            uint32_t c2 = 0;
+            for (int aa = 0; aa < bx; aa++) {
+                const uint32_t v = lds_take(hist + aa * ncol + col);
+                c2 += v;
+                xlogx_add(a3, tbl, v);
+            }
+            xlogx_add(a2, tbl, c2);
+        }
     }
     s3 = df_wave_sum(a3);
     s2b = df_wave_sum(a2);
@@ -276,7 +262,7 @@ __device__ __forceinline__ void df_block_sum2(DF &u, DF &v, float *red) {
 // ranges [a0, a1) of x_{t+tau} so any b_x^2 b_y fits; c2b (b_x b_y cells)
 // accumulates the (x_t, y_t) marginal across passes.  The target's states
 // are in registers (thread tid owns samples 16 tid + 16 nth i + q, NB chunks,
-// L <= 16 nth NB); a pass with few cells counts into privatised copies.
+// L <= 16 nth NB).
 // S3, S2b valid in thread 0.  hist must be zero on entry and is zero on exit.
 template <int NB>
 __device__ __forceinline__ void hist_pair_block(uint32_t *hist, uint32_t *c2b, int cap,
@@ -295,20 +281,29 @@ __device__ __forceinline__ void hist_pair_block(uint32_t *hist, uint32_t *c2b, i
         const int a1 = (a0 + apass < bx) ? (a0 + apass) : bx;
         const uint32_t lo = (uint32_t)(a0 * bx), hi = (uint32_t)(a1 * bx);
         const int cells = (a1 - a0) * ncol;
-        const WaveCopies cp = wave_copies(cells, cap, TNX_BLOCK_COPY_BUDGET);
-        uint32_t *hc = hist + (tid % cp.R) * cp.S - a0 * ncol;  // cell index of the full pair
+        uint32_t *hc = hist - a0 * ncol;  // cell index of the full pair
         __syncthreads();
+        constexpr int PF = NB < 4 ? NB : 4;  // source chunks requested together
 #pragma unroll
-        for (int i = 0; i < NB; i++) {
-            const int t = 16 * tid + 16 * nth * i;
-            if (t < L) {
-                const uint4 y4 = *reinterpret_cast<const uint4 *>(yr + t);
-                const uint32_t yw[4] = {y4.x, y4.y, y4.z, y4.w};
+        for (int g0 = 0; g0 < NB; g0 += PF) {
+            uint4 ys[PF];
 #pragma unroll
-                for (int q = 0; q < 16; q++) {
-                    const uint32_t sv = (regs.s[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
-                    const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                    if (t + q < L && sv >= lo && sv < hi) lds_inc_key(hc, sv, by4, yv);
+            for (int u = 0; u < PF; u++) {
+                const int t = 16 * tid + 16 * nth * (g0 + u);
+                ys[u] = t < L ? *reinterpret_cast<const uint4 *>(yr + t) : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int u = 0; u < PF; u++) {
+                const int i = g0 + u;
+                const int t = 16 * tid + 16 * nth * i;
+                if (t < L) {
+                    const uint32_t yw[4] = {ys[u].x, ys[u].y, ys[u].z, ys[u].w};
+#pragma unroll
+                    for (int q = 0; q < 16; q++) {
+                        const uint32_t sv = (regs.s[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
+                        const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                        if (t + q < L && sv >= lo && sv < hi) lds_inc_key(hc, sv, by4, yv, 2u);
+                    }
                 }
             }
         }
@@ -318,7 +313,7 @@ __device__ __forceinline__ void hist_pair_block(uint32_t *hist, uint32_t *c2b, i
         int col = tid % ncol;
 #pragma unroll 4
         for (int i = tid; i < cells; i += nth) {
-            const uint32_t v = take_cell(hist, i, cp);
+            const uint32_t v = lds_take(hist + i);
             const float2 tv = tbl[v];  // tbl[0] = tbl[1] = 0
             df_add(a3, tv.x, tv.y);
             if (v) __hip_atomic_fetch_add(c2b + col, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);

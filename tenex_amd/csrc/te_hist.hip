@@ -92,16 +92,17 @@ __device__ __forceinline__ void te_wave_body(const tnx_te_args_t &a) {
             if (want && !fits && lane == 0) a.out[slot] = __builtin_nanf("");  // plan mismatch
             if (!(want && fits)) continue;
             const WaveCopies cp = wave_copies(bx * bx * by, a.cap_wave, TNX_WAVE_COPY_BUDGET);
-            uint32_t *hc = hist + (lane % cp.R) * cp.S;
+            uint32_t *hc = hist + (lane & (cp.G - 1));
+            const uint32_t kmul = (uint32_t)by << (2 + cp.gsh), ysh = 2 + cp.gsh;
             const int8_t *yr = a.bins + (int64_t)y * a.ld;
             if constexpr (NI > 0) {
                 if (x != cur_x) {
                     load_target_regs<NI>(regs, a.states + (int64_t)x * a.lds, L, lane);
                     cur_x = x;
                 }
-                hist_accumulate_regs<NI>(hc, regs, yr, (uint32_t)by, L, lane);
+                hist_accumulate_regs<NI>(hc, regs, yr, kmul, ysh, L, lane);
             } else {
-                hist_accumulate_mem(hc, a.states + (int64_t)x * a.lds, yr, (uint32_t)by, L, lane);
+                hist_accumulate_mem(hc, a.states + (int64_t)x * a.lds, yr, kmul, ysh, L, lane);
             }
             wave_lds_fence();
             DF s3, s2b;
@@ -208,19 +209,20 @@ __global__ __launch_bounds__(256) void te_counts_kernel(
     }
     const int cells = bx * bx * by;
     const WaveCopies cp = wave_copies(cells, cap, TNX_WAVE_COPY_BUDGET);
-    uint32_t *hc = hist + (lane % cp.R) * cp.S;
+    uint32_t *hc = hist + (lane & (cp.G - 1));
+    const uint32_t kmul = (uint32_t)by << (2 + cp.gsh), ysh = 2 + cp.gsh;
     if constexpr (NI > 0) {
         TargetRegs<NI> regs;
         load_target_regs<NI>(regs, states + (int64_t)x * lds, L, lane);
-        hist_accumulate_regs<NI>(hc, regs, bins + (int64_t)y * ld, (uint32_t)by, L, lane);
+        hist_accumulate_regs<NI>(hc, regs, bins + (int64_t)y * ld, kmul, ysh, L, lane);
     } else {
-        hist_accumulate_mem(hc, states + (int64_t)x * lds, bins + (int64_t)y * ld, (uint32_t)by, L,
+        hist_accumulate_mem(hc, states + (int64_t)x * lds, bins + (int64_t)y * ld, kmul, ysh, L,
                             lane);
     }
     wave_lds_fence();
     for (int i = lane; i < cells; i += kWave) {
         uint32_t v = 0;
-        for (int r = 0; r < cp.R; r++) v += hist[r * cp.S + i];
+        for (int r = 0; r < cp.G; r++) v += hist[(i << cp.gsh) + r];
         counts[p * cap + i] = (int32_t)v;
     }
     wave_lds_fence();
