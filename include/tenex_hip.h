@@ -143,8 +143,8 @@ typedef struct {
  * cap_wave <= 0 selects the gfx950 default.  A pair (x, y) is counted by
  * the wave kernel iff b_x^2 * b_y <= cap_wave.  cap_block <= 0 sizes the
  * block kernel's histogram so that two workgroups share a CU's LDS; an
- * explicit cap_block must be >= the widest block pair's b_t * b_s
- * (reported as max_block_cols).
+ * explicit cap_block must be a multiple of 4 and >= the widest block
+ * pair's b_t * b_s (reported as max_block_cols).
  * This is the scheduling half of te_(adaptive_)smem_launch_pairfree
  * (tenex/csrc/full_smem.cu:350-381, tenex/csrc/adaptive_smem.cu:351-376). */
 TNX_API int tnx_te_plan(const int32_t *n_per_var, int64_t n_vars, int64_t L,

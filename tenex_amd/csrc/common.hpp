@@ -67,15 +67,29 @@ __device__ __forceinline__ void df_add(DF &acc, float xh, float xl) {
     acc.hi = s2;
 }
 
+// v from lane (l - N) mod 16 of the same row of 16 lanes (DPP row_ror:N).
+template <int N>
+__device__ __forceinline__ float row_ror(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + N, 0xF, 0xF,
+                                                      false));
+}
+
 // Sum of a DF over the 64 lanes of a wavefront; result valid in every lane.
+// Rotate-and-add inside each row of 16 lanes (DPP, no LDS traffic), then
+// the four row totals through readlane.
 __device__ __forceinline__ DF df_wave_sum(DF v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        float h = __shfl_xor(v.hi, off, kWave);
-        float l = __shfl_xor(v.lo, off, kWave);
-        df_add(v, h, l);
-    }
-    return v;
+    df_add(v, row_ror<1>(v.hi), row_ror<1>(v.lo));
+    df_add(v, row_ror<2>(v.hi), row_ror<2>(v.lo));
+    df_add(v, row_ror<4>(v.hi), row_ror<4>(v.lo));
+    df_add(v, row_ror<8>(v.hi), row_ror<8>(v.lo));
+    auto lane_f = [](float f, int l) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(f), l));
+    };
+    DF t = {lane_f(v.hi, 0), lane_f(v.lo, 0)};
+    df_add(t, lane_f(v.hi, 16), lane_f(v.lo, 16));
+    df_add(t, lane_f(v.hi, 32), lane_f(v.lo, 32));
+    df_add(t, lane_f(v.hi, 48), lane_f(v.lo, 48));
+    return t;
 }
 
 __device__ __forceinline__ double df_to_double(DF v) { return (double)v.hi + (double)v.lo; }

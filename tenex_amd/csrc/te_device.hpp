@@ -6,6 +6,14 @@
 #ifndef TNX_WAVE_COPY_BUDGET
 #define TNX_WAVE_COPY_BUDGET 2048
 #endif
+// Most copies of a wave histogram (a power of two <= 2^TNX_LDS_BANK_SHIFT).
+#ifndef TNX_MAX_COPIES
+#define TNX_MAX_COPIES 32
+#endif
+// log2 of the LDS banks the fold's rotation spreads a half-wave over.
+#ifndef TNX_LDS_BANK_SHIFT
+#define TNX_LDS_BANK_SHIFT 5
+#endif
 
 namespace tnx {
 
@@ -175,7 +183,7 @@ struct WaveCopies {
 
 __device__ __forceinline__ WaveCopies wave_copies(int cells, int cap, int budget) {
     WaveCopies c{1, 0};
-    while (c.G < 32 && 2 * c.G * cells <= budget && (2 * c.G + 1) * cells <= cap) {
+    while (c.G < TNX_MAX_COPIES && 2 * c.G * cells <= budget && (2 * c.G + 1) * cells <= cap) {
         c.G *= 2;
         c.gsh += 1;
     }
@@ -187,23 +195,23 @@ __device__ __forceinline__ uint32_t lds_take(uint32_t *p) {
     return __hip_atomic_exchange(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Phases 2-4 for one wavefront.  With copies, lane l first folds cells
-// k = l, l + 64, ...: the G copies of cell k are read from rotation
-// (k / (32 / G)) on, so the 32 lanes of a half-wave hit 32 distinct banks;
-// the totals go to the words after the copies and give S3.  Then per
-// (x_t, y_t) column the b_x totals of x_{t+tau} give S2b (without copies,
-// this column pass gives S3 too).  Leaves the slice zero.
-__device__ __forceinline__ void hist_reduce_wave_copies(uint32_t *hist, int bx, int by,
-                                                        const WaveCopies &cp,
-                                                        const float2 *__restrict__ tbl, int lane,
-                                                        DF &s3, DF &s2b) {
+// Phases 2-4 for one wavefront: returns S3 - S2b (valid in every lane).
+// With copies, lane l first folds cells k = l, l + 64, ...: the G copies of
+// cell k are read from rotation (k / (32 / G)) on, so the 32 lanes of a
+// half-wave hit 32 distinct banks; the totals go to the words after the
+// copies and give S3.  Then per (x_t, y_t) column the b_x totals of
+// x_{t+tau} give S2b (without copies, this column pass gives S3 too).
+// Leaves the slice zero.
+__device__ __forceinline__ DF hist_reduce_wave_copies(uint32_t *hist, int bx, int by,
+                                                      const WaveCopies &cp,
+                                                      const float2 *__restrict__ tbl, int lane) {
     const int ncol = bx * by;
     const int cells = bx * ncol;
     DF a3 = {0.f, 0.f}, a2 = {0.f, 0.f};
     if (cp.G > 1) {
         const int G = cp.G;
         uint32_t *tot = hist + G * cells;
-        const int per_sh = 5 - cp.gsh;  // log2(32 / G)
+        const int per_sh = TNX_LDS_BANK_SHIFT - cp.gsh;  // log2(banks / G)
         for (int k = lane; k < cells; k += kWave) {
             const int f = k >> per_sh;
             uint32_t *row = hist + (k << cp.gsh);
@@ -229,103 +237,117 @@ __device__ __forceinline__ void hist_reduce_wave_copies(uint32_t *hist, int bx, 
             xlogx_add(a2, tbl, c2);
         }
     }
-    s3 = df_wave_sum(a3);
-    s2b = df_wave_sum(a2);
+    df_add(a3, -a2.hi, -a2.lo);
+    return df_wave_sum(a3);
 }
 
-// Workgroup reduction of two DF values; result valid in thread 0.
-// red: shared float[4 * n_waves].
-__device__ __forceinline__ void df_block_sum2(DF &u, DF &v, float *red) {
+// Workgroup sum of a DF; result valid in thread 0.  red: shared
+// float[2 * n_waves].  The partials in red are read by thread 0 only, before
+// it reaches the caller's next barrier, so no trailing barrier is needed.
+__device__ __forceinline__ DF df_block_sum(DF u, float *red) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     u = df_wave_sum(u);
-    v = df_wave_sum(v);
     if (lane == 0) {
-        red[4 * w + 0] = u.hi;
-        red[4 * w + 1] = u.lo;
-        red[4 * w + 2] = v.hi;
-        red[4 * w + 3] = v.lo;
+        red[2 * w + 0] = u.hi;
+        red[2 * w + 1] = u.lo;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        DF su = {0.f, 0.f}, sv = {0.f, 0.f};
-        for (int i = 0; i < nw; i++) {
-            df_add(su, red[4 * i + 0], red[4 * i + 1]);
-            df_add(sv, red[4 * i + 2], red[4 * i + 3]);
-        }
-        u = su;
-        v = sv;
-    }
-    __syncthreads();
+    DF su = {0.f, 0.f};
+    if (threadIdx.x == 0)
+        for (int i = 0; i < nw; i++) df_add(su, red[2 * i + 0], red[2 * i + 1]);
+    return su;
 }
 
 // Whole workgroup on one pair with a histogram of cap cells: passes over
 // ranges [a0, a1) of x_{t+tau} so any b_x^2 b_y fits; c2b (b_x b_y cells)
 // accumulates the (x_t, y_t) marginal across passes.  The target's states
 // are in registers (thread tid owns samples 16 tid + 16 nth i + q, NB chunks,
-// L <= 16 nth NB).
-// S3, S2b valid in thread 0.  hist must be zero on entry and is zero on exit.
+// L <= 16 nth NB) and so is the source row (ys, same samples).  Returns
+// S3 - S2b, valid in thread 0.  hist and c2b must be zero on entry and are
+// zero on exit; the caller's next pair may start right away.
 template <int NB>
-__device__ __forceinline__ void hist_pair_block(uint32_t *hist, uint32_t *c2b, int cap,
-                                                const TargetRegs<NB> &regs,
-                                                const int8_t *__restrict__ yr, int bx, int by,
-                                                int L, const float2 *__restrict__ tbl, float *red,
-                                                DF &s3, DF &s2b) {
+__device__ __forceinline__ DF hist_pair_block(uint32_t *hist, uint32_t *c2b, int cap,
+                                              const TargetRegs<NB> &regs, const uint4 *ys,
+                                              int bx, int by, int L,
+                                              const float2 *__restrict__ tbl, float *red) {
     const int tid = threadIdx.x, nth = blockDim.x;
     const int ncol = bx * by;
     const uint32_t by4 = (uint32_t)by << 2;
-    for (int i = tid; i < ncol; i += nth) c2b[i] = 0;
     int apass = cap / ncol;
     if (apass < 1) apass = 1;
-    DF a3 = {0.f, 0.f}, a2 = {0.f, 0.f};
+    DF a3 = {0.f, 0.f};
     for (int a0 = 0; a0 < bx; a0 += apass) {
         const int a1 = (a0 + apass < bx) ? (a0 + apass) : bx;
-        const uint32_t lo = (uint32_t)(a0 * bx), hi = (uint32_t)(a1 * bx);
+        const uint32_t lo = (uint32_t)(a0 * bx), span = (uint32_t)((a1 - a0) * bx);
         const int cells = (a1 - a0) * ncol;
         uint32_t *hc = hist - a0 * ncol;  // cell index of the full pair
-        __syncthreads();
-        constexpr int PF = NB < 4 ? NB : 4;  // source chunks requested together
+        const bool single = a1 - a0 == bx;  // one pass: no range test
 #pragma unroll
-        for (int g0 = 0; g0 < NB; g0 += PF) {
-            uint4 ys[PF];
+        for (int i = 0; i < NB; i++) {
+            const int t = 16 * tid + 16 * nth * i;
+            if (t >= L) continue;
+            const uint32_t yw[4] = {ys[i].x, ys[i].y, ys[i].z, ys[i].w};
+            if (single && t + 16 <= L) {
 #pragma unroll
-            for (int u = 0; u < PF; u++) {
-                const int t = 16 * tid + 16 * nth * (g0 + u);
-                ys[u] = t < L ? *reinterpret_cast<const uint4 *>(yr + t) : make_uint4(0u, 0u, 0u, 0u);
-            }
+                for (int q = 0; q < 16; q++) {
+                    const uint32_t sv = (regs.s[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
+                    const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                    lds_inc_key(hc, sv, by4, yv, 2u);
+                }
+            } else if (t + 16 <= L) {
 #pragma unroll
-            for (int u = 0; u < PF; u++) {
-                const int i = g0 + u;
-                const int t = 16 * tid + 16 * nth * i;
-                if (t < L) {
-                    const uint32_t yw[4] = {ys[u].x, ys[u].y, ys[u].z, ys[u].w};
+                for (int q = 0; q < 16; q++) {
+                    const uint32_t sv = (regs.s[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
+                    const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                    if (sv - lo < span) lds_inc_key(hc, sv, by4, yv, 2u);
+                }
+            } else {
 #pragma unroll
-                    for (int q = 0; q < 16; q++) {
-                        const uint32_t sv = (regs.s[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
-                        const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                        if (t + q < L && sv >= lo && sv < hi) lds_inc_key(hc, sv, by4, yv, 2u);
-                    }
+                for (int q = 0; q < 16; q++) {
+                    const uint32_t sv = (regs.s[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
+                    const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                    if (t + q < L && sv - lo < span) lds_inc_key(hc, sv, by4, yv, 2u);
                 }
             }
         }
         __syncthreads();
-        // flattened scan over the pass's cells: coalesced, independent lookups
-        const int step = nth % ncol;
-        int col = tid % ncol;
-#pragma unroll 4
-        for (int i = tid; i < cells; i += nth) {
-            const uint32_t v = lds_take(hist + i);
-            const float2 tv = tbl[v];  // tbl[0] = tbl[1] = 0
-            df_add(a3, tv.x, tv.y);
-            if (v) __hip_atomic_fetch_add(c2b + col, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        // flattened scan over the pass's cells, four at a time (cap is a
+        // multiple of 4 and cells past the pass are zero): groups of four
+        // empty cells cost one read; only nonzero groups are cleared, and
+        // empty or single cells need no table lookup
+        const int ng = (cells + 3) >> 2;
+        const int step = (4 * nth) % ncol;
+        int col = (4 * tid) % ncol;
+        for (int g = tid; g < ng; g += nth) {
+            uint4 *p4 = reinterpret_cast<uint4 *>(hist) + g;
+            const uint4 v4 = *p4;
+            if (v4.x | v4.y | v4.z | v4.w) {
+                *p4 = make_uint4(0u, 0u, 0u, 0u);
+                const uint32_t vv[4] = {v4.x, v4.y, v4.z, v4.w};
+                int c = col;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t v = vv[k];
+                    if (v) {
+                        if (v > 1) {
+                            const float2 tv = tbl[v];
+                            df_add(a3, tv.x, tv.y);
+                        }
+                        __hip_atomic_fetch_add(c2b + c, v, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                    if (++c == ncol) c = 0;
+                }
+            }
             col += step;
             if (col >= ncol) col -= ncol;
         }
+        __syncthreads();
     }
-    __syncthreads();
-    for (int col = tid; col < ncol; col += nth) xlogx_add(a2, tbl, c2b[col]);
-    df_block_sum2(a3, a2, red);
-    s3 = a3;
-    s2b = a2;
+    DF a2 = {0.f, 0.f};
+    for (int c = tid; c < ncol; c += nth) xlogx_add(a2, tbl, lds_take(c2b + c));
+    df_add(a3, -a2.hi, -a2.lo);
+    return df_block_sum(a3, red);
 }
 
 }  // namespace tnx

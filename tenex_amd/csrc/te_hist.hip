@@ -76,18 +76,37 @@ __device__ __forceinline__ void te_wave_body(const tnx_te_args_t &a) {
     const float2 *tbl = reinterpret_cast<const float2 *>(a.xlogx);
     const double invN = 1.0 / (double)L;
     TargetRegs<(NI > 0 ? NI : 1)> regs;
+    int cur_x = -1;
+    double s12 = 0.0;  // S1[x] - S2a[x] of the target in registers
     for (int64_t w = blockIdx.x; w < a.n_wave_items; w += gridDim.x) {
         const tnx_work_t it = a.work[w];
-        const int nsrc = it.src_end - it.src_beg;
-        const int npair = it.tgt_cnt * nsrc;
-        int cur_x = -1;
+        const int cnt = it.tgt_cnt;
+        const int npair = cnt * (it.src_end - it.src_beg);
+        // pair q = sj * cnt + ti; this wave takes q = wave, wave + 4, ...
+        // (ti, sj) advance without division, and the next pair's indices
+        // and level counts are loaded one pair ahead.
+        int ti = 0, sj = 0;
+        for (int u = 0; u < wave; u++)
+            if (++ti == cnt) { ti = 0; ++sj; }
+        int xn = 0, yn = 0, bxn = 0, byn = 0;
+        if (wave < npair) {
+            xn = a.tgt_order[it.tgt_beg + ti];
+            yn = a.src_order[it.src_beg + sj];
+            bxn = a.n_per_var[xn];
+            byn = a.n_per_var[yn];
+        }
         for (int q = wave; q < npair; q += kWavesPerWG) {
-            const int ti = q % it.tgt_cnt, sj = q / it.tgt_cnt;
-            const int x = a.tgt_order[it.tgt_beg + ti];
-            const int y = a.src_order[it.src_beg + sj];
+            const int x = xn, y = yn, bx = bxn, by = byn;
+            for (int u = 0; u < kWavesPerWG; u++)
+                if (++ti == cnt) { ti = 0; ++sj; }
+            if (q + kWavesPerWG < npair) {
+                xn = a.tgt_order[it.tgt_beg + ti];
+                yn = a.src_order[it.src_beg + sj];
+                bxn = a.n_per_var[xn];
+                byn = a.n_per_var[yn];
+            }
             int64_t slot;
             const bool want = (x != y) && out_slot(a, x, y, slot);
-            const int bx = a.n_per_var[x], by = a.n_per_var[y];
             const bool fits = bx * bx * by <= a.cap_wave;
             if (want && !fits && lane == 0) a.out[slot] = __builtin_nanf("");  // plan mismatch
             if (!(want && fits)) continue;
@@ -95,23 +114,20 @@ __device__ __forceinline__ void te_wave_body(const tnx_te_args_t &a) {
             uint32_t *hc = hist + (lane & (cp.G - 1));
             const uint32_t kmul = (uint32_t)by << (2 + cp.gsh), ysh = 2 + cp.gsh;
             const int8_t *yr = a.bins + (int64_t)y * a.ld;
+            if (x != cur_x) {
+                if constexpr (NI > 0) load_target_regs<NI>(regs, a.states + (int64_t)x * a.lds, L, lane);
+                s12 = a.S1[x] - a.S2a[x];
+                cur_x = x;
+            }
             if constexpr (NI > 0) {
-                if (x != cur_x) {
-                    load_target_regs<NI>(regs, a.states + (int64_t)x * a.lds, L, lane);
-                    cur_x = x;
-                }
                 hist_accumulate_regs<NI>(hc, regs, yr, kmul, ysh, L, lane);
             } else {
                 hist_accumulate_mem(hc, a.states + (int64_t)x * a.lds, yr, kmul, ysh, L, lane);
             }
             wave_lds_fence();
-            DF s3, s2b;
-            hist_reduce_wave_copies(hist, bx, by, cp, tbl, lane, s3, s2b);
+            const DF d = hist_reduce_wave_copies(hist, bx, by, cp, tbl, lane);
             wave_lds_fence();
-            if (lane == 0) {
-                double te = (df_to_double(s3) + a.S1[x] - a.S2a[x] - df_to_double(s2b)) * invN;
-                a.out[slot] = (float)te;
-            }
+            if (lane == 0) a.out[slot] = (float)((df_to_double(d) + s12) * invN);
         }
     }
 }
@@ -127,26 +143,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 // Workgroup-per-pair kernel for pairs above cap_wave cells (multi-pass, exact).
 // The target's states are loaded into registers once per target and reused
 // for every source and pass.  Requires L <= 16 * kBlockThreads * NB.
+// Loads this thread's NB chunks of 16 source bytes (zero past L).
+template <int NB>
+__device__ __forceinline__ void load_block_row(uint4 *ys, const int8_t *__restrict__ yr, int L) {
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        const int t = 16 * threadIdx.x + 16 * kBlockThreads * i;
+        ys[i] = t < L ? *reinterpret_cast<const uint4 *>(yr + t) : make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
 template <int NB>
 __global__ __launch_bounds__(kBlockThreads) void te_block_kernel(tnx_te_args_t a) {
     extern __shared__ uint32_t lds_u32[];
-    __shared__ float red[4 * (kBlockThreads / kWave)];
+    __shared__ float red[2 * (kBlockThreads / kWave)];
     uint32_t *hist = lds_u32;
     uint32_t *c2b = lds_u32 + a.cap_block;
     const int tid = threadIdx.x;
-    for (int i = tid; i < a.cap_block; i += kBlockThreads) hist[i] = 0;
+    for (int i = tid; i < a.cap_block + a.max_block_cols; i += kBlockThreads) lds_u32[i] = 0;
     __syncthreads();
     const int L = (int)(a.T - a.tau);
     const float2 *tbl = reinterpret_cast<const float2 *>(a.xlogx);
     const double invN = 1.0 / (double)L;
     const tnx_work_t *work = a.work + a.n_wave_items;
     TargetRegs<NB> regs;
+    // long series (NB > 4) skip the row prefetch to stay within registers
+    constexpr bool kPipe = NB <= 4;
+    uint4 ycur[NB], ynext[kPipe ? NB : 1];
+    int cur_x = -1;
+    double s12 = 0.0;
     for (int64_t w = blockIdx.x; w < a.n_block_items; w += gridDim.x) {
         const tnx_work_t it = work[w];
         for (int ti = 0; ti < it.tgt_cnt; ti++) {
             const int x = a.tgt_order[it.tgt_beg + ti];
             const int bx = a.n_per_var[x];
-            {
+            if (x != cur_x) {
                 const uint16_t *st = a.states + (int64_t)x * a.lds;
 #pragma unroll
                 for (int i = 0; i < NB; i++) {
@@ -160,22 +191,36 @@ __global__ __launch_bounds__(kBlockThreads) void te_block_kernel(tnx_te_args_t a
                         regs.s[8 * i + 6] = v.z; regs.s[8 * i + 7] = v.w;
                     }
                 }
+                s12 = a.S1[x] - a.S2a[x];
+                cur_x = x;
             }
+            // sources are software-pipelined: the next source's index, level
+            // count and row are requested while the current pair is counted
+            int y = a.src_order[it.src_beg];
+            int by = a.n_per_var[y];
+            if constexpr (kPipe) load_block_row<NB>(ycur, a.bins + (int64_t)y * a.ld, L);
             for (int sj = it.src_beg; sj < it.src_end; sj++) {
-                const int y = a.src_order[sj];
+                int yn = 0, byn = 0;
+                if (sj + 1 < it.src_end) {
+                    yn = a.src_order[sj + 1];
+                    byn = a.n_per_var[yn];
+                    if constexpr (kPipe) load_block_row<NB>(ynext, a.bins + (int64_t)yn * a.ld, L);
+                }
+                if constexpr (!kPipe) load_block_row<NB>(ycur, a.bins + (int64_t)y * a.ld, L);
                 int64_t slot;
                 const bool want = (x != y) && out_slot(a, x, y, slot);
-                const int by = a.n_per_var[y];
                 const bool fits = bx * by <= a.max_block_cols;
                 if (want && !fits && tid == 0) a.out[slot] = __builtin_nanf("");  // plan mismatch
-                if (!(want && fits)) continue;
-                DF s3, s2b;
-                hist_pair_block<NB>(hist, c2b, a.cap_block, regs, a.bins + (int64_t)y * a.ld, bx,
-                                    by, L, tbl, red, s3, s2b);
-                if (tid == 0) {
-                    double te =
-                        (df_to_double(s3) + a.S1[x] - a.S2a[x] - df_to_double(s2b)) * invN;
-                    a.out[slot] = (float)te;
+                if (want && fits) {
+                    const DF d = hist_pair_block<NB>(hist, c2b, a.cap_block, regs, ycur, bx, by, L,
+                                                     tbl, red);
+                    if (tid == 0) a.out[slot] = (float)((df_to_double(d) + s12) * invN);
+                }
+                y = yn;
+                by = byn;
+                if constexpr (kPipe) {
+#pragma unroll
+                    for (int i = 0; i < NB; i++) ycur[i] = ynext[kPipe ? i : 0];
                 }
             }
         }
@@ -226,10 +271,8 @@ __global__ __launch_bounds__(256) void te_counts_kernel(
         counts[p * cap + i] = (int32_t)v;
     }
     wave_lds_fence();
-    DF s3, s2b;
-    hist_reduce_wave_copies(hist, bx, by, cp, tbl, lane, s3, s2b);
-    if (lane == 0)
-        te_out[p] = (float)((df_to_double(s3) + S1[x] - S2a[x] - df_to_double(s2b)) / (double)L);
+    const DF d = hist_reduce_wave_copies(hist, bx, by, cp, tbl, lane);
+    if (lane == 0) te_out[p] = (float)((df_to_double(d) + (S1[x] - S2a[x])) / (double)L);
 }
 
 // Register-variant choice for a series of L samples: the smallest
@@ -285,12 +328,12 @@ int tnx_te_plan(const int32_t *npv, int64_t n_vars, int64_t L, const int32_t *tg
     for (int64_t i = 0; i < nt; i++)
         if (kw[i] < ns) maxcols = std::max<int32_t>(maxcols, npv[tg[i]] * sb[ns - 1]);
     if (cap_block <= 0) {
-        cap_block = std::min(kDefaultCapBlock, kBlockLdsTarget / 4 - maxcols);
-        cap_block = std::max(cap_block, maxcols);
+        cap_block = std::min(kDefaultCapBlock, kBlockLdsTarget / 4 - maxcols) & ~3;
+        cap_block = std::max(cap_block, (maxcols + 3) & ~3);
     }
-    TNX_CHECK_ARG(cap_block >= maxcols && cap_block >= 1,
-                  "tnx_te_plan: cap_block %d below the widest block pair's b_t*b_s = %d",
-                  cap_block, maxcols);
+    TNX_CHECK_ARG(cap_block >= maxcols && cap_block >= 4 && cap_block % 4 == 0,
+                  "tnx_te_plan: cap_block %d must be a multiple of 4 and >= the widest block "
+                  "pair's b_t*b_s = %d", cap_block, maxcols);
     std::vector<tnx_work_t> wv, bk;
     double cw = 0, cb = 0;
     for (int64_t g0 = 0; g0 < nt; g0 += kWavesPerWG) {
@@ -390,9 +433,9 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
         TNX_CHECK_ARG(a.work, "tnx_te_run: null work");
         TNX_CHECK_ARG(a.max_block_cols >= 1 && a.max_block_cols <= 127 * 127,
                       "tnx_te_run: bad max_block_cols");
-        TNX_CHECK_ARG(a.cap_block >= a.max_block_cols,
-                      "tnx_te_run: cap_block %d below max_block_cols %d", a.cap_block,
-                      a.max_block_cols);
+        TNX_CHECK_ARG(a.cap_block >= a.max_block_cols && a.cap_block % 4 == 0,
+                      "tnx_te_run: cap_block %d must be a multiple of 4 and >= max_block_cols %d",
+                      a.cap_block, a.max_block_cols);
         size_t shm = (size_t)(a.cap_block + a.max_block_cols) * 4;
         TNX_CHECK_ARG(shm <= 160 * 1024, "tnx_te_run: block kernel LDS %zu exceeds 160 KiB", shm);
         unsigned grid = (unsigned)std::min<int64_t>(a.n_block_items, 1 << 20);
