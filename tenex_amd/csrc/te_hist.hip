@@ -43,6 +43,17 @@ constexpr int kDefaultCapBlock = 16384;  // 64 KiB + the b_t b_s marginal
 // workgroups fit one CU's 160 KiB: histogram + marginal + reduction scratch.
 constexpr int kBlockLdsTarget = 80 * 1024 - 512;
 constexpr int kWaveItemPairs = 256;  // pairs per wave-kernel work item
+constexpr int kXcds = 8;             // gfx950: 8 XCDs, workgroups dealt round-robin
+constexpr int kChunkSrc = 64;        // sources per locality chunk
+
+// Item of workgroup slot w in [0, n): workgroup w runs on XCD w % 8, and XCD
+// x takes the x-th contiguous slice of the items, in order (a bijection of
+// [0, n)).
+__device__ __forceinline__ int64_t xcd_item(int64_t w, int64_t n) {
+    const int64_t x = w % kXcds, i = w / kXcds;
+    const int64_t q = n / kXcds, r = n % kXcds;
+    return x * q + (x < r ? x : r) + i;
+}
 constexpr int kBlockItemPairs = 16;  // pairs per block-kernel work item
 
 // Output slot of pair (x, y); returns false if the pair is filtered out.
@@ -79,7 +90,7 @@ __device__ __forceinline__ void te_wave_body(const tnx_te_args_t &a) {
     int cur_x = -1;
     double s12 = 0.0;  // S1[x] - S2a[x] of the target in registers
     for (int64_t w = blockIdx.x; w < a.n_wave_items; w += gridDim.x) {
-        const tnx_work_t it = a.work[w];
+        const tnx_work_t it = a.work[xcd_item(w, a.n_wave_items)];
         const int cnt = it.tgt_cnt;
         const int npair = cnt * (it.src_end - it.src_beg);
         // pair q = sj * cnt + ti; this wave takes q = wave, wave + 4, ...
@@ -358,6 +369,19 @@ int tnx_te_plan(const int32_t *npv, int64_t n_vars, int64_t L, const int32_t *tg
             }
         }
     }
+    // L2 locality: the wave kernel hands item ranges to the 8 XCDs in
+    // contiguous slices (xcd_item), so order its items by source chunk,
+    // chunks dealt to slices round-robin: each XCD's L2 then holds the
+    // source rows its items share, and every slice mixes cheap and dear
+    // sources.  Block items keep the heavy-target-first order, which
+    // balances their far more uneven costs across the chip.
+    auto by_chunk = [](const tnx_work_t &u, const tnx_work_t &v) {
+        const int cu = u.src_beg / kChunkSrc, cv = v.src_beg / kChunkSrc;
+        if (cu % kXcds != cv % kXcds) return cu % kXcds < cv % kXcds;
+        if (cu != cv) return cu < cv;
+        return u.tgt_beg < v.tgt_beg;
+    };
+    std::stable_sort(wv.begin(), wv.end(), by_chunk);
     // pairs (excluding x == y)
     std::vector<char> in_src(n_vars, 0);
     for (int32_t v : sr) in_src[v] = 1;
