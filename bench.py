@@ -116,6 +116,42 @@ def cpu_baseline_pairs(bins_np, npv, pairs, gpu_te_internal=None):
     }, err
 
 
+TRAFFIC_FILE = "profiles/te_wave_c2_4000x10000_traffic.md"
+
+
+def measured_traffic():
+    """HBM bytes per launch of the C2 wave kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None if
+    absent."""
+    import re
+    path = os.path.join(HERE, TRAFFIC_FILE)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", open(path).read())
+    return int(m.group(1)) if m else None
+
+
+def run_traffic_probe(args):
+    """One C2 TE matrix, then a 256 MiB device copy: the program the
+    FETCH_SIZE / WRITE_SIZE counter passes run (the copy calibrates the
+    counters)."""
+    import torch
+    from tenex_amd import _lib, core, synthetic
+    dev = core.require_gpu("cuda:0")
+    n, T = 4000, 10000
+    bins, npv, b_max, T = core.upload_bins(synthetic.mixed_cardinality_bins(n, T, seed=args.seed),
+                                           dev)
+    prep = core.prepare(bins, npv, T, 1)
+    plan = core.get_plan(prep)
+    out = torch.zeros((n, n), dtype=torch.float32, device=dev)
+    core.te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n)
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log("traffic probe done: one C2 matrix, one 256 MiB copy")
+
+
 def run_c2(args):
     import torch
     from tenex_amd import _lib, core, synthetic
@@ -178,12 +214,14 @@ def run_c2(args):
                    "arith": "int32 counts, fp32 compensated entropy terms, fp64 final combine"},
         "roofline": {"bound": "hbm", "kernel": "te_wave_reg_kernel", "achieved": hbm_achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_achieved / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": measured_traffic(),
+                     "traffic_source": TRAFFIC_FILE,
                      "note": "histogram path is LDS-atomic bound; see lds_roofline"},
         "lds_roofline": {"bound": "lds_atomic", "kernel": "te_wave_reg_kernel",
                          "achieved": upd_rate / 1e9, "peak": lds_peak / 1e9,
                          "peak_random_keys": lds_rand / 1e9, "unit": "Gupdates/s",
-                         "frac": upd_rate / lds_peak, "updates_per_launch": wave_pairs * L},
+                         "frac": upd_rate / lds_peak, "updates_per_launch": wave_pairs * L,
+                         "step_frac": n_pairs * L / dt / lds_peak},
         "kernels_ms": {"te_wave_kernel": t_wave * 1e3, "te_block_kernel": t_block * 1e3,
                        "gene_prep_kernel": t_prep * 1e3},
         "wave_pairs": wave_pairs, "block_pairs": n_pairs - wave_pairs,
@@ -361,11 +399,16 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=20480)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run one C2 matrix and a 256 MiB copy (for rocprofv3 --pmc), no JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs one MI355X per process and has no "
               "multi-GPU path (--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+    if args.traffic_probe:
+        run_traffic_probe(args)
+        return
     fn = {"c0": run_c0, "c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4}[args.workload]
     res = fn(args)
     print(json.dumps(res), flush=True)
