@@ -46,7 +46,7 @@ extern "C" {
 #define TNX_API
 #endif
 
-#define TNX_ABI_VERSION 2
+#define TNX_ABI_VERSION 3
 
 /* Status codes. */
 #define TNX_OK 0
@@ -225,6 +225,9 @@ typedef struct {
     double *sum_te, *sumsq_te; /* (n_vars, n_vars) public layout, accumulated (+=) */
     int32_t *count_ge;         /* (n_vars, n_vars) public layout, accumulated (+=) */
     float *te_k_dbg;           /* optional (n_vars*n_vars, n_surr): te_k per slot */
+    int32_t max_levels;        /* max n_per_var over the plan's targets if known, else 0;
+                                * <= 16 lets the kernel keep shuffled states as bytes */
+    int32_t max_tgt_cnt;       /* max tgt_cnt over the work items (0 = unknown: 4) */
 } tnx_surr_args_t;
 
 /* Block-shuffle surrogate statistics for every pair of the plan: for each

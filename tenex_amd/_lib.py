@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libtenex_hip.so")
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 TNX_OUT_PAIRFREE = 0
 TNX_OUT_MATRIX = 1
@@ -57,7 +57,8 @@ class SurrArgs(C.Structure):
                 ("tau", i32), ("n_surr", i32), ("n_blocks", i32), ("block_length", i32),
                 ("n_wave_items", i64), ("n_block_items", i64),
                 ("cap_wave", i32), ("cap_block", i32), ("max_block_cols", i32),
-                ("sum_te", p_), ("sumsq_te", p_), ("count_ge", p_), ("te_k_dbg", p_)]
+                ("sum_te", p_), ("sumsq_te", p_), ("count_ge", p_), ("te_k_dbg", p_),
+                ("max_levels", i32), ("max_tgt_cnt", i32)]
 
 
 class Mlp(C.Structure):

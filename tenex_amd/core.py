@@ -365,7 +365,9 @@ def surrogate_run(prep: Prepared, plan: Plan, perm: torch.Tensor, block_length: 
         n_blocks=nb, block_length=int(block_length), n_wave_items=plan.info.n_wave_items,
         n_block_items=0, cap_wave=plan.info.cap_wave, cap_block=plan.info.cap_block,
         max_block_cols=max(plan.info.max_block_cols, 1), sum_te=ptr(sum_te),
-        sumsq_te=ptr(sumsq_te), count_ge=ptr(count_ge), te_k_dbg=ptr(te_k_dbg))
+        sumsq_te=ptr(sumsq_te), count_ge=ptr(count_ge), te_k_dbg=ptr(te_k_dbg),
+        max_levels=int(prep.npv_host[plan.tgt_host].max(initial=0)),
+        max_tgt_cnt=int(plan.work_host[:, 1].max(initial=0)))
     call("tnx_surrogate_run", C.byref(args), _stream(prep.device))
 
 

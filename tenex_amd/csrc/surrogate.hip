@@ -28,7 +28,7 @@
 namespace tnx {
 
 constexpr int kSurrWaves = 4;
-constexpr int kSurrMaxSrcPerWave = 64;
+constexpr int kSmallTbl = 256;    // c log2 c of counts below this is read from LDS
 
 struct BlockMap {
     const int32_t *prow;  // perm[k, v, :]
@@ -58,18 +58,205 @@ __device__ __forceinline__ uint32_t get16(const uint32_t *words, uint32_t cell) 
     return (words[cell >> 1] >> (16 * (cell & 1))) & 0xFFFFu;
 }
 
+// y'_k[t0 .. t0 + 4) of the block-shuffled row yr as one packed word, in
+// two steps so the loads overlap other work: shuffle_issue requests the
+// bytes, shuffle_finish assembles them.  Four consecutive positions inside
+// one block are four consecutive bytes of the original row: two aligned
+// dword loads and a byte funnel shift.  Groups that straddle a block
+// boundary, reach the tail or pass L take the per-byte path at finish.
+// Rows are padded to a multiple of 16 bytes, so the second dword of the
+// fast path stays inside the row.
+struct ShufWord {
+    uint32_t lo, hi;
+    int32_t sh;  // byte shift, or -1: per-byte path
+};
+
+__device__ __forceinline__ ShufWord shuffle_issue(const int8_t *__restrict__ yr, const BlockMap &m,
+                                                  int32_t t0, int32_t L) {
+    ShufWord w{0u, 0u, -1};
+    const int32_t b = (int32_t)__umulhi((uint32_t)t0, m.magic);
+    int32_t bb = b, r = t0 - b * m.bl;
+    if (r >= m.bl) { bb += 1; r -= m.bl; }
+    if (r + 3 < m.bl && t0 + 3 < m.head && t0 + 3 < L) {
+        uint32_t pv = (uint32_t)m.prow[bb];
+        if (pv >= (uint32_t)m.nb) pv = 0;
+        const int32_t src = (int32_t)pv * m.bl + r;
+        const uint32_t *p = reinterpret_cast<const uint32_t *>(yr + (src & ~3));
+        w.sh = src & 3;
+        w.lo = p[0];
+        w.hi = w.sh ? p[1] : 0u;
+    }
+    return w;
+}
+
+// The rare groups off the fast path, one byte at a time (kept out of line
+// so the common path stays small).
+__device__ __noinline__ uint32_t shuffle_bytes(const int8_t *__restrict__ yr, BlockMap m,
+                                               int32_t t0, int32_t L) {
+    uint32_t wv = 0;
+    for (int q = 0; q < 4; q++) {
+        const int32_t t = t0 + q;
+        const uint32_t v = t < L ? (uint32_t)(uint8_t)yr[bmap(m, t)] : 0u;
+        wv |= v << (8 * q);
+    }
+    return wv;
+}
+
+__device__ __forceinline__ uint32_t shuffle_finish(const ShufWord &w, const int8_t *__restrict__ yr,
+                                                   const BlockMap &m, int32_t t0, int32_t L) {
+    if (w.sh >= 0) return __builtin_amdgcn_alignbyte(w.hi, w.lo, (uint32_t)w.sh);  // byte shift
+    return shuffle_bytes(yr, m, t0, L);
+}
+
+// Stage y'_k into LDS: the words t0 = first, first + step, ... < L of this
+// thread (at most NW), requested by stage_issue and written by
+// stage_finish.
+template <int NW>
+struct Stage {
+    ShufWord w[NW];
+};
+
+template <int NW>
+__device__ __forceinline__ void stage_issue(Stage<NW> &st, const int8_t *__restrict__ yr,
+                                            const BlockMap &m, int first, int step, int L) {
+#pragma unroll
+    for (int i = 0; i < NW; i++) {
+        const int t0 = first + i * step;
+        st.w[i] = t0 < L ? shuffle_issue(yr, m, t0, L) : ShufWord{0u, 0u, 0};
+    }
+}
+
+template <int NW>
+__device__ __forceinline__ void stage_finish(const Stage<NW> &st, int8_t *yb,
+                                             const int8_t *__restrict__ yr, const BlockMap &m,
+                                             int first, int step, int L) {
+#pragma unroll
+    for (int i = 0; i < NW; i++) {
+        const int t0 = first + i * step;
+        if (t0 < L) *reinterpret_cast<uint32_t *>(yb + t0) = shuffle_finish(st.w[i], yr, m, t0, L);
+    }
+}
+
+// Count key = states'[t] * b_y + y'[t], t < L, into a u16-packed histogram,
+// except key 0 (all three values at level 0): with zero-inflated data a
+// quarter of the samples can share that key, and their atomics would
+// serialise on one address.  Its count is L minus the others'
+// (reduce_pair16).
+// Four consecutive states from LDS (ST = uint8_t when every target has at
+// most 16 levels, uint16_t otherwise).
+__device__ __forceinline__ void load_states4(const uint8_t *stp, int t0, uint32_t *s4) {
+    const uint32_t w = *reinterpret_cast<const uint32_t *>(stp + t0);
+    s4[0] = w & 0xFFu;
+    s4[1] = (w >> 8) & 0xFFu;
+    s4[2] = (w >> 16) & 0xFFu;
+    s4[3] = w >> 24;
+}
+
+__device__ __forceinline__ void load_states4(const uint16_t *stp, int t0, uint32_t *s4) {
+    const uint2 sw = *reinterpret_cast<const uint2 *>(stp + t0);
+    s4[0] = sw.x & 0xFFFFu;
+    s4[1] = sw.x >> 16;
+    s4[2] = sw.y & 0xFFFFu;
+    s4[3] = sw.y >> 16;
+}
+
+template <typename ST>
+__device__ __forceinline__ void count_pair16(uint32_t *hist, const ST *stp,
+                                             const int8_t *yb, uint32_t by, int L, int lane) {
+#pragma unroll 4
+    for (int t0 = lane * 4; t0 < L; t0 += 256) {
+        const uint32_t yw = *reinterpret_cast<const uint32_t *>(yb + t0);
+        uint32_t s4[4];
+        load_states4(stp, t0, s4);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t cell = __umul24(s4[q], by) + ((yw >> (8 * q)) & 0xFFu);
+            if (t0 + q < L && cell != 0u) lds_inc16(hist, cell);
+        }
+    }
+}
+
+// acc += c log2 c: counts below kSmallTbl from the workgroup's LDS copy of
+// the table, larger ones from memory.
+__device__ __forceinline__ void xlogx_add_l(DF &acc, const float2 *ltbl,
+                                            const float2 *__restrict__ tbl, uint32_t c) {
+    if (c > 1) {
+        const float2 t = c < (uint32_t)kSmallTbl ? ltbl[c] : tbl[c];
+        df_add(acc, t.x, t.y);
+    }
+}
+
+// Integer sum over the 64 lanes (every lane gets it).
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+    v += __builtin_amdgcn_update_dpp(0, (int)v, 0x121, 0xF, 0xF, false);  // row_ror:1
+    v += __builtin_amdgcn_update_dpp(0, (int)v, 0x122, 0xF, 0xF, false);  // row_ror:2
+    v += __builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false);  // row_ror:4
+    v += __builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);  // row_ror:8
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 0) +
+           (uint32_t)__builtin_amdgcn_readlane((int)v, 16) +
+           (uint32_t)__builtin_amdgcn_readlane((int)v, 32) +
+           (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+}
+
+// S3 - S2b of a u16-packed pair histogram (cells (a * b_x + b) * b_y + c)
+// counted by count_pair16 from L samples, summed over the wave; zeroes the
+// histogram.  Cell 0 (not counted) is L minus the rest; it belongs to
+// column 0, which lane 0 finishes once that count is known.
+__device__ __forceinline__ DF reduce_pair16(uint32_t *hist, int bx, int by, int L,
+                                            const float2 *ltbl, const float2 *__restrict__ tbl,
+                                            int lane) {
+    const int ncol = bx * by;
+    DF a3 = {0.f, 0.f}, a2 = {0.f, 0.f};
+    uint32_t mine = 0, c2_col0 = 0;
+    for (int col = lane; col < ncol; col += kWave) {
+        uint32_t c2 = 0;
+        for (int aa = 0; aa < bx; aa++) {
+            const uint32_t v = get16(hist, (uint32_t)(aa * ncol + col));
+            c2 += v;
+            xlogx_add_l(a3, ltbl, tbl, v);
+        }
+        mine += c2;
+        if (col == 0) c2_col0 = c2;
+        else xlogx_add_l(a2, ltbl, tbl, c2);
+    }
+    const uint32_t c0 = (uint32_t)L - wave_sum_u32(mine);
+    if (lane == 0) {
+        xlogx_add_l(a3, ltbl, tbl, c0);
+        xlogx_add_l(a2, ltbl, tbl, c2_col0 + c0);
+    }
+    df_add(a3, -a2.hi, -a2.lo);
+    const DF d = df_wave_sum(a3);
+    wave_lds_fence();
+    for (int i = lane; i < (bx * ncol + 1) / 2; i += kWave) hist[i] = 0;
+    wave_lds_fence();
+    return d;
+}
+
+// One workgroup per plan item.  With tg > 1 targets of the item handled
+// together (tg_max from the host, by LDS), wave w owns target w: per
+// surrogate k every target's shuffled states and S1'_k, S2a'_k are built
+// once, then per source the shuffled row y'_k is staged once by the whole
+// workgroup (double buffer, one barrier per source) and every wave counts
+// its pair from it.  With one target, the four waves take every fourth
+// source and stage their own rows.  Per-pair FP64 sums, sums of squares and
+// counts of TE_k >= observed stay in registers (lane = source index / step)
+// and are added to the public slots once per item.
+template <int NW, typename ST>
 __global__ __launch_bounds__(256) void surrogate_wave_kernel(tnx_surr_args_t a, int32_t lds_len,
-                                                             int32_t ystage_len) {
+                                                             int32_t tg_max, int32_t nbatch) {
     extern __shared__ uint32_t lds_u32[];
-    __shared__ double sk[2];
+    __shared__ double s12_sh;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int tid = threadIdx.x;
     const int cap_words = a.cap_wave / 2;
-    // LDS: [hist words x 4 waves][states' u16 lds_len][y' bytes x 4 waves]
+    // LDS: [hist words x 4 waves][states' ST lds_len x tg_max]
+    //      [y' bytes lds_len x (nbatch, or 4 with one target)][small c log2 c table]
     uint32_t *hist = lds_u32 + wave * cap_words;
-    uint16_t *stp = reinterpret_cast<uint16_t *>(lds_u32 + kSurrWaves * cap_words);
-    int8_t *ystage = reinterpret_cast<int8_t *>(stp + lds_len) + wave * ystage_len;
+    ST *stp0 = reinterpret_cast<ST *>(lds_u32 + kSurrWaves * cap_words);
+    int8_t *ybuf = reinterpret_cast<int8_t *>(stp0 + (int64_t)lds_len * tg_max);
+    float2 *ltbl = reinterpret_cast<float2 *>(ybuf +
+                                              (int64_t)(tg_max > 1 ? nbatch : kSurrWaves) * lds_len);
     const int L = (int)(a.T - a.tau);
     const int n = (int)a.n_vars;
     const float2 *tbl = reinterpret_cast<const float2 *>(a.xlogx);
@@ -77,116 +264,141 @@ __global__ __launch_bounds__(256) void surrogate_wave_kernel(tnx_surr_args_t a, 
     const uint32_t magic = 0xFFFFFFFFu / (uint32_t)a.block_length;
     const int32_t head = a.n_blocks * a.block_length;
     for (int i = tid; i < kSurrWaves * cap_words; i += 256) lds_u32[i] = 0;
+    for (int i = tid; i < kSmallTbl; i += 256)
+        ltbl[i] = i <= L ? tbl[i] : make_float2(0.f, 0.f);
     __syncthreads();
+    Stage<NW> stg;
     for (int64_t w = blockIdx.x; w < a.n_wave_items; w += gridDim.x) {
         const tnx_work_t it = a.work[w];
-        for (int ti = 0; ti < it.tgt_cnt; ti++)
-        for (int s0 = it.src_beg; s0 < it.src_end; s0 += kSurrWaves * kSurrMaxSrcPerWave) {
-        const int x = a.tgt_order[it.tgt_beg + ti];
-        const int bx = a.n_per_var[x];
-        if (bx * bx > a.cap_wave) continue;  // every pair of x exceeds the histogram
-        const int8_t *xr = a.bins + (int64_t)x * a.ld;
-        const int nsrc = min(it.src_end - s0, kSurrWaves * kSurrMaxSrcPerWave);
-        // per-source accumulators: lane s of wave w holds source w + 4 s
+        for (int ti0 = 0; ti0 < it.tgt_cnt; ti0 += tg_max) {
+        const int tg = min(tg_max, it.tgt_cnt - ti0);
+        // batched staging whenever the launch has several-target items (a
+        // leftover item with one target then counts on wave 0 only)
+        const bool shared = tg_max > 1;
+        const int chunk = shared ? kWave : kSurrWaves * kWave;  // sources per pass
+        // this wave's target (shared mode) or the item's target
+        const int my_t = shared ? (wave < tg ? wave : -1) : 0;
+        const int x = my_t >= 0 ? a.tgt_order[it.tgt_beg + ti0 + my_t] : -1;
+        const int bx = x >= 0 ? a.n_per_var[x] : 0;
+        const ST *my_stp = stp0 + (int64_t)(my_t > 0 ? my_t : 0) * lds_len;
+        for (int s0 = it.src_beg; s0 < it.src_end; s0 += chunk) {
+        const int nsrc = min(it.src_end - s0, chunk);
         double acc_s = 0.0, acc_q = 0.0;
         int32_t acc_c = 0;
         for (int k = 0; k < a.n_surr; k++) {
-            // ── 1. shuffled target states and S1'_k, S2a'_k ────────────────
-            BlockMap mx{a.perm + ((int64_t)k * n + x) * a.n_blocks, a.block_length, head, a.n_blocks,
-                        magic};
-            uint32_t *h0 = lds_u32;  // wave 0's histogram, bx^2 cells
-            for (int t = tid; t < L; t += 256) {
-                const int32_t c = xr[bmap(mx, t)];
-                const int32_t aa = xr[bmap(mx, t + a.tau)];
-                const uint32_t s = (uint32_t)(aa * bx + c);
-                stp[t] = (uint16_t)s;
-                lds_inc16(h0, s);
+            // ── 1. shuffled states of the tg targets, S1'_k and S2a'_k ─────
+            for (int j = 0; j < tg; j++) {
+                const int xj = a.tgt_order[it.tgt_beg + ti0 + j];
+                const int bxj = a.n_per_var[xj];
+                const int8_t *xr = a.bins + (int64_t)xj * a.ld;
+                BlockMap mx{a.perm + ((int64_t)k * n + xj) * a.n_blocks, a.block_length, head,
+                            a.n_blocks, magic};
+                ST *stp = stp0 + (int64_t)j * lds_len;
+                uint32_t *hj = lds_u32 + j * cap_words;  // wave j's histogram
+                for (int t = tid; t < L; t += 256) {
+                    const int32_t c = xr[bmap(mx, t)];
+                    const int32_t aa = xr[bmap(mx, t + a.tau)];
+                    const uint32_t st = (uint32_t)(aa * bxj + c);
+                    stp[t] = (ST)st;
+                    if (bxj * bxj <= a.cap_wave) lds_inc16(hj, st);
+                }
             }
             __syncthreads();
-            if (wave == 0) {
+            double S12k = 0.0;  // S1'_k - S2a'_k of this wave's target
+            if (x >= 0 && (shared || wave == 0)) {
                 DF s2a = {0.f, 0.f}, s1 = {0.f, 0.f};
                 for (int b = lane; b < bx; b += kWave) {
                     uint32_t c1 = 0;
                     for (int aa = 0; aa < bx; aa++) {
-                        const uint32_t v = get16(h0, (uint32_t)(aa * bx + b));
+                        const uint32_t v = get16(hist, (uint32_t)(aa * bx + b));
                         c1 += v;
-                        xlogx_add(s2a, tbl, v);
+                        xlogx_add_l(s2a, ltbl, tbl, v);
                     }
-                    xlogx_add(s1, tbl, c1);
+                    xlogx_add_l(s1, ltbl, tbl, c1);
                 }
-                s2a = df_wave_sum(s2a);
-                s1 = df_wave_sum(s1);
+                df_add(s1, -s2a.hi, -s2a.lo);
+                S12k = df_to_double(df_wave_sum(s1));
                 wave_lds_fence();
-                for (int i = lane; i < (bx * bx + 1) / 2; i += kWave) h0[i] = 0;
-                if (lane == 0) {
-                    sk[0] = df_to_double(s1);
-                    sk[1] = df_to_double(s2a);
+                for (int i = lane; i < (bx * bx + 1) / 2; i += kWave) hist[i] = 0;
+                wave_lds_fence();
+            }
+            // ── 2. sources ─────────────────────────────────────────────────
+            if (shared) {
+                // sources in batches of nbatch: the workgroup stages the
+                // batch's shuffled rows, one barrier, then every wave counts
+                // its target against each of them
+                for (int sb0 = 0; sb0 < nsrc; sb0 += nbatch) {
+                    const int nbt = min(nbatch, nsrc - sb0);
+                    for (int j = 0; j < nbt; j++) {
+                        const int y = a.src_order[s0 + sb0 + j];
+                        const int8_t *yr = a.bins + (int64_t)y * a.ld;
+                        const BlockMap my{a.perm + ((int64_t)k * n + y) * a.n_blocks,
+                                          a.block_length, head, a.n_blocks, magic};
+                        stage_issue<NW>(stg, yr, my, tid * 4, 1024, L);
+                        stage_finish<NW>(stg, ybuf + (int64_t)j * lds_len, yr, my, tid * 4, 1024,
+                                         L);
+                    }
+                    __syncthreads();
+                    for (int j = 0; j < nbt; j++) {
+                        const int sj = sb0 + j;
+                        const int y = a.src_order[s0 + sj];
+                        const int by = a.n_per_var[y];
+                        if (x < 0 || y == x || bx * bx * by > a.cap_wave) continue;
+                        count_pair16(hist, my_stp, ybuf + (int64_t)j * lds_len, (uint32_t)by, L,
+                                     lane);
+                        wave_lds_fence();
+                        const DF d = reduce_pair16(hist, bx, by, L, ltbl, tbl, lane);
+                        const double te = (df_to_double(d) + S12k) * invN;
+                        const int64_t slot = (int64_t)y * n + x;
+                        if (lane == sj) {
+                            acc_s += te;
+                            acc_q += te * te;
+                            acc_c += ((float)te >= a.observed[slot]) ? 1 : 0;
+                        }
+                        if (a.te_k_dbg && lane == 0) a.te_k_dbg[slot * a.n_surr + k] = (float)te;
+                    }
+                    __syncthreads();  // the batch's rows are replaced next
+                }
+            } else {
+                // one target: wave 0 holds S1'_k - S2a'_k; hand it to all waves
+                if (wave == 0 && lane == 0) s12_sh = S12k;
+                __syncthreads();
+                S12k = s12_sh;
+                // each wave its own sources and its own staging buffer
+                int8_t *yb = ybuf + (int64_t)wave * lds_len;
+                for (int sj = wave; sj < nsrc; sj += kSurrWaves) {
+                    const int y = a.src_order[s0 + sj];
+                    const int by = a.n_per_var[y];
+                    if (y == x || bx * bx * by > a.cap_wave) continue;
+                    const int8_t *yr = a.bins + (int64_t)y * a.ld;
+                    const BlockMap my{a.perm + ((int64_t)k * n + y) * a.n_blocks, a.block_length,
+                                      head, a.n_blocks, magic};
+                    Stage<4> sb;
+                    for (int f = lane * 4; f < L; f += 4 * 256) {
+                        stage_issue<4>(sb, yr, my, f, 256, L);
+                        stage_finish<4>(sb, yb, yr, my, f, 256, L);
+                    }
+                    wave_lds_fence();  // y' of sj written by this wave
+                    {
+                        count_pair16(hist, my_stp, yb, (uint32_t)by, L, lane);
+                        wave_lds_fence();
+                        const DF d = reduce_pair16(hist, bx, by, L, ltbl, tbl, lane);
+                        const double te = (df_to_double(d) + S12k) * invN;
+                        const int64_t slot = (int64_t)y * n + x;
+                        if (lane == (sj / kSurrWaves)) {
+                            acc_s += te;
+                            acc_q += te * te;
+                            acc_c += ((float)te >= a.observed[slot]) ? 1 : 0;
+                        }
+                        if (a.te_k_dbg && lane == 0) a.te_k_dbg[slot * a.n_surr + k] = (float)te;
+                    }
                 }
             }
-            __syncthreads();
-            const double S1k = sk[0], S2ak = sk[1];
-            // ── 2. this wave's sources ────────────────────────────────────
-            for (int sj = wave; sj < nsrc; sj += kSurrWaves) {
-                const int y = a.src_order[s0 + sj];
-                if (y == x) continue;
-                const int by = a.n_per_var[y];
-                if (bx * bx * by > a.cap_wave) continue;  // excluded by the host
-                const int8_t *yr = a.bins + (int64_t)y * a.ld;
-                BlockMap my{a.perm + ((int64_t)k * n + y) * a.n_blocks, a.block_length, head,
-                            a.n_blocks, magic};
-                // stage y'_k[t], t < L, four bytes per lane step
-                for (int t0 = lane * 4; t0 < L; t0 += 256) {
-                    uint32_t wv = 0;
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const int t = t0 + q;
-                        const uint32_t v = t < L ? (uint32_t)(uint8_t)yr[bmap(my, t)] : 0u;
-                        wv |= v << (8 * q);
-                    }
-                    *reinterpret_cast<uint32_t *>(ystage + t0) = wv;
-                }
-                wave_lds_fence();
-                // count
-                for (int t0 = lane * 4; t0 < L; t0 += 256) {
-                    const uint32_t yw = *reinterpret_cast<const uint32_t *>(ystage + t0);
-                    const uint2 sw = *reinterpret_cast<const uint2 *>(stp + t0);
-                    const uint32_t s4[4] = {sw.x & 0xFFFFu, sw.x >> 16, sw.y & 0xFFFFu, sw.y >> 16};
-#pragma unroll
-                    for (int q = 0; q < 4; q++)
-                        if (t0 + q < L) lds_inc16(hist, __umul24(s4[q], (uint32_t)by) + ((yw >> (8 * q)) & 0xFFu));
-                }
-                wave_lds_fence();
-                // reduce
-                const int ncol = bx * by;
-                DF a3 = {0.f, 0.f}, a2 = {0.f, 0.f};
-                for (int col = lane; col < ncol; col += kWave) {
-                    uint32_t c2 = 0;
-                    for (int aa = 0; aa < bx; aa++) {
-                        const uint32_t v = get16(hist, (uint32_t)(aa * ncol + col));
-                        c2 += v;
-                        xlogx_add(a3, tbl, v);
-                    }
-                    xlogx_add(a2, tbl, c2);
-                }
-                a3 = df_wave_sum(a3);
-                a2 = df_wave_sum(a2);
-                wave_lds_fence();
-                for (int i = lane; i < (bx * ncol + 1) / 2; i += kWave) hist[i] = 0;
-                wave_lds_fence();
-                const double te = (df_to_double(a3) + S1k - S2ak - df_to_double(a2)) * invN;
-                const int64_t slot = (int64_t)y * n + x;
-                if (lane == (sj / kSurrWaves)) {
-                    acc_s += te;
-                    acc_q += te * te;
-                    acc_c += ((float)te >= a.observed[slot]) ? 1 : 0;
-                }
-                if (a.te_k_dbg && lane == 0) a.te_k_dbg[slot * a.n_surr + k] = (float)te;
-            }
-            __syncthreads();  // states' is rebuilt for the next k
+            __syncthreads();  // states' and y' are rebuilt for the next k
         }
-        // write the per-source accumulators
-        const int sj = wave + kSurrWaves * lane;
-        if (lane < kSurrMaxSrcPerWave && sj < nsrc) {
+        // the per-pair accumulators of this pass
+        const int sj = shared ? lane : wave + kSurrWaves * lane;
+        if (x >= 0 && sj < nsrc) {
             const int y = a.src_order[s0 + sj];
             const int by = a.n_per_var[y];
             if (y != x && bx * bx * by <= a.cap_wave) {
@@ -197,6 +409,7 @@ __global__ __launch_bounds__(256) void surrogate_wave_kernel(tnx_surr_args_t a, 
             }
         }
         __syncthreads();
+        }
         }
     }
 }
@@ -227,16 +440,49 @@ extern "C" int tnx_surrogate_run(const tnx_surr_args_t *args, void *stream) {
     if (a.n_wave_items == 0) return TNX_OK;
     const int64_t L = a.T - a.tau;
     const int32_t lds_len = (int32_t)((L + 63) / 64 * 64);
-    const int32_t ystage_len = lds_len;
-    const size_t shm = (size_t)kSurrWaves * (a.cap_wave / 2) * 4 + (size_t)lds_len * 2 +
-                       (size_t)kSurrWaves * ystage_len;
+    // several targets per workgroup share each staged source row when their
+    // shuffled states fit beside the histograms; else one target at a time.
+    // States are bytes when every target has at most 16 levels, else 16 bit.
+    const bool st8 = a.max_levels >= 1 && a.max_levels <= 16;
+    const int st_bytes = st8 ? 1 : 2;
+    // staged rows: nb in several-target mode, one per wave with one target
+    auto lds_for = [&](int tg, int nb) {
+        const int rows = tg > 1 ? nb : kSurrWaves;
+        return (size_t)kSurrWaves * (a.cap_wave / 2) * 4 + (size_t)lds_len * st_bytes * tg +
+               (size_t)rows * lds_len + (size_t)kSmallTbl * sizeof(float2);
+    };
+    const int items_tg = a.max_tgt_cnt > 0 ? std::min(a.max_tgt_cnt, kSurrWaves) : kSurrWaves;
+    const int tg_max = (items_tg > 1 && lds_for(kSurrWaves, 2) <= 160 * 1024) ? kSurrWaves : 1;
+    // the largest batch of staged rows (2..6) that keeps three workgroups per
+    // CU, else the largest that fits
+    int nbatch = 2;
+    for (int nb = 6; nb >= 2; nb--)
+        if (3 * lds_for(tg_max, nb) <= 160 * 1024) { nbatch = nb; break; }
+    if (3 * lds_for(tg_max, nbatch) > 160 * 1024)
+        for (int nb = 6; nb >= 2; nb--)
+            if (lds_for(tg_max, nb) <= 160 * 1024) { nbatch = nb; break; }
+    const size_t shm = lds_for(tg_max, nbatch);
     TNX_CHECK_ARG(shm <= 160 * 1024, "tnx_surrogate_run: LDS %zu bytes exceeds 160 KiB "
                   "(reduce cap_wave or T)", shm);
-    TNX_CHECK_HIP(hipFuncSetAttribute((const void *)surrogate_wave_kernel,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    // words staged per thread and source with several targets (L / 1024);
+    // the one-target mode stages in groups of four words
+    const int64_t words = tg_max > 1 ? (L + 1023) / 1024 : 1;
     const unsigned grid = (unsigned)std::min<int64_t>(a.n_wave_items, 1 << 20);
-    hipLaunchKernelGGL(surrogate_wave_kernel, dim3(grid), dim3(256), shm, as_stream(stream), a,
-                       lds_len, ystage_len);
+    const void *kfn = nullptr;
+#define TNX_SURR(NWV, STT)                                                                   \
+    if (!kfn && words <= NWV && (sizeof(STT) == 1) == st8) {                                \
+        kfn = (const void *)surrogate_wave_kernel<NWV, STT>;                                  \
+        TNX_CHECK_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize,    \
+                                          (int)shm));                                         \
+        hipLaunchKernelGGL((surrogate_wave_kernel<NWV, STT>), dim3(grid), dim3(256), shm,     \
+                           as_stream(stream), a, lds_len, tg_max, nbatch);                    \
+    }
+    TNX_SURR(1, uint8_t) TNX_SURR(2, uint8_t) TNX_SURR(4, uint8_t) TNX_SURR(8, uint8_t)
+    TNX_SURR(16, uint8_t)
+    TNX_SURR(1, uint16_t) TNX_SURR(2, uint16_t) TNX_SURR(4, uint16_t) TNX_SURR(8, uint16_t)
+    TNX_SURR(16, uint16_t)
+#undef TNX_SURR
+    TNX_CHECK_ARG(kfn != nullptr, "tnx_surrogate_run: L too long");
     TNX_CHECK_LAUNCH("surrogate_wave_kernel");
     return TNX_OK;
 }

@@ -54,6 +54,39 @@ def test_fused_te_k_vs_oracle(dev, bl, method, tau):
     assert np.array_equal(cgp[~near], ocg[~near])
 
 
+def test_pairlist_plan_single_target_items(dev):
+    """Explicit pair lists give one target per work item: the kernel's
+    one-target mode (waves split the sources) against the oracle."""
+    d, npv = _small(n=25, T=350, seed=9)
+    n, T = d.shape
+    prep = _prep(d, npv, dev)
+    obs_int = core.te_matrix_internal(prep).cpu().numpy()
+    rng = np.random.default_rng(1)
+    allp = O.pairfree_pairs(n)
+    pairs = allp[rng.choice(len(allp), 150, replace=False)]  # [target, source]
+    cap = core.surrogate_cap(prep.npv_host, pairs[:, 0], pairs[:, 1])
+    plan = core.pairlist_plan(prep.npv_host, pairs, dev, cap)
+    assert np.all(plan.work_host[:, 1] == 1)
+    n_surr = 5
+    perm, eff_bl, nb = core.block_perm_device(n, T, "block", 30, 0, n_surr, 7, dev, dev)
+    s = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    s2 = torch.zeros_like(s)
+    cg = torch.zeros((n, n), dtype=torch.int32, device=dev)
+    tk = torch.zeros((n * n, n_surr), dtype=torch.float32, device=dev)
+    obs_pub = torch.from_numpy(obs_int.T.copy()).to(dev)
+    core.surrogate_run(prep, plan, perm, eff_bl, obs_pub, s, s2, cg, tk)
+    obs_pairs = obs_int[pairs[:, 0], pairs[:, 1]]
+    os_, os2, ocg, otk = O.surrogate_stats(d, npv, perm.cpu().numpy(), eff_bl, pairs, obs_pairs,
+                                           return_te_k=True)
+    slot = pairs[:, 1] * n + pairs[:, 0]
+    assert np.abs(tk.cpu().numpy()[slot] - otk).max() < TOL
+    assert np.abs(s.cpu().numpy().ravel()[slot] - os_).max() < n_surr * TOL
+    # slots outside the list stay untouched
+    mask = np.ones(n * n, bool)
+    mask[slot] = False
+    assert np.all(s.cpu().numpy().ravel()[mask] == 0)
+
+
 def test_identity_permutation_gives_observed(dev):
     d, npv = _small(n=20, T=300, seed=3)
     n, T = d.shape
