@@ -46,7 +46,7 @@ extern "C" {
 #define TNX_API
 #endif
 
-#define TNX_ABI_VERSION 3
+#define TNX_ABI_VERSION 4
 
 /* Status codes. */
 #define TNX_OK 0
@@ -156,6 +156,13 @@ TNX_API int tnx_te_plan(const int32_t *n_per_var, int64_t n_vars, int64_t L,
 /* Output modes for tnx_te_run. */
 #define TNX_OUT_PAIRFREE 0 /* out[id - pair_offset], id in [pair_offset, pair_offset+n_pairs_local) */
 #define TNX_OUT_MATRIX 1   /* out[target * ld_out + source] (internal layout) */
+#define TNX_OUT_STATS 2    /* surrogate statistics at the public slot source * ld_out + target:
+                            * te = (float)TE; sum_te += te; sumsq_te += te^2 (FP64);
+                            * count_ge += (te >= observed[slot]) */
+#define TNX_OUT_STAGE 3    /* K surrogate copies in one launch: variable v = k * m + j
+                            * (m = stage_vars) is copy k of variable j; out[k * stage_stride
+                            * + (source % m) * ld_out + (target % m)] = (float)TE, for
+                            * tnx_stats_accumulate to fold in copy order */
 
 typedef struct {
     const int8_t *bins;       /* (n_vars, ld) */
@@ -171,6 +178,15 @@ typedef struct {
     int32_t cap_wave, cap_block, max_block_cols;
     float *out;
     int64_t ld_out, pair_offset, n_pairs_local;
+    double *sum_te, *sumsq_te;  /* TNX_OUT_STATS only (accumulated) */
+    int32_t *count_ge;
+    const float *observed;
+    int32_t hot_zero;  /* nonzero: key 0 (state 0, source level 0) is frequent
+                        * (zero-inflated data); the kernels count it in
+                        * per-lane words instead of one histogram cell.
+                        * Results are identical either way; only speed differs. */
+    int32_t stage_vars;    /* TNX_OUT_STAGE only: variables per copy (m) */
+    int64_t stage_stride;  /* TNX_OUT_STAGE only: floats per copy (>= m * ld_out) */
 } tnx_te_args_t;
 
 /* Pairwise TE for every pair of the plan: FP32 entropy terms with
@@ -180,6 +196,18 @@ typedef struct {
  * (tenex/csrc/full_smem.cu:319-381, adaptive_smem.cu:307-376,
  * gmem.cu:190-230). */
 TNX_API int tnx_te_run(const tnx_te_args_t *args, void *stream);
+
+/* Surrogate statistics from a TNX_OUT_STAGE run: for each public slot
+ * slots[i] (ascending, < stage_stride) and copies k = 0..K-1 in order,
+ * te = stage[k * stage_stride + slot]; sum_te[slot] += te, sumsq_te[slot] +=
+ * te^2 (FP64), count_ge[slot] += (te >= observed[slot]): the same additions
+ * in the same order as K TNX_OUT_STATS runs.  No reference counterpart: it
+ * batches the per-surrogate accumulation of tenex/inference/
+ * surrogate_test.py:890-929 (sums over the surrogate TE matrices). */
+TNX_API int tnx_stats_accumulate(const float *stage, int32_t K, int64_t stage_stride,
+                                 const int64_t *slots, int64_t n_slots, const float *observed,
+                                 double *sum_te, double *sumsq_te, int32_t *count_ge,
+                                 void *stream);
 
 /* Debug: raw joint counts of explicit pairs (host-side count n_pairs,
  * pairs device int32 [target, source]) through the same accumulation code
@@ -209,6 +237,15 @@ TNX_API int tnx_te_gemm_b2(const int8_t *bins, int64_t n, int64_t T, int64_t ld,
                    int32_t *counts_dbg, void *stream);
 
 /* ── Surrogate test ──────────────────────────────────────────────────── */
+
+/* One block-shuffled copy of the bins: out[v, t] = bins[v, perm[v, t / bl] * bl
+ * + t % bl] for t < n_blocks * bl, the tail [n_blocks * bl, T) and the row
+ * padding copied as they are.  perm: int32 (n_vars, n_blocks), one surrogate.
+ * Replaces _block_shuffle_gpu / _random_shuffle_gpu
+ * (tenex/inference/surrogate_test.py:107-162). */
+TNX_API int tnx_block_shuffle(const int8_t *bins, int64_t n_vars, int64_t T, int64_t ld,
+                              const int32_t *perm, int32_t n_blocks, int32_t block_length,
+                              int8_t *out, void *stream);
 
 typedef struct {
     const int8_t *bins;       /* (n_vars, ld) */
@@ -280,11 +317,15 @@ TNX_API int tnx_mlp_adam(const tnx_mlp_t *m, float *master, const float *grad, f
 typedef struct {
     float lr, lr_min, beta1, beta2, eps;
     int32_t steps, t0;
+    int32_t batch;  /* rows per step: 0 (or >= n_rows) full batch, else step s
+                     * takes rows [(s mod nb) * batch, +batch), nb = n_rows / batch,
+                     * of X and tgt, whose rows the caller has shuffled */
 } tnx_adam_t;
 
-/* `opt->steps` full-batch steps (backward + Adam) launched from the host
- * loop with no host synchronisation; loss_hist (device float[steps], may be
- * NULL) receives each step's loss.  grad: device float[n_params] scratch. */
+/* `opt->steps` steps (backward + Adam) launched from the host loop with no
+ * host synchronisation; loss_hist (device float[steps], may be NULL)
+ * receives each step's loss.  grad: device float[n_params] scratch; the
+ * workspace is sized for the rows of one step (tnx_mlp_workspace_bytes). */
 TNX_API int tnx_mlp_fit(const tnx_mlp_t *m, float *master, float *m1, float *m2,
                         int64_t n_params, const float *X, const float *tgt, int64_t n_rows,
                         const tnx_adam_t *opt, float *grad, void *workspace, int64_t ws_bytes,

@@ -11,10 +11,12 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libtenex_hip.so")
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 TNX_OUT_PAIRFREE = 0
 TNX_OUT_MATRIX = 1
+TNX_OUT_STATS = 2
+TNX_OUT_STAGE = 3
 
 _lock = threading.Lock()
 _lib = None
@@ -47,7 +49,9 @@ class TeArgs(C.Structure):
                 ("tau", i32), ("out_mode", i32),
                 ("n_wave_items", i64), ("n_block_items", i64),
                 ("cap_wave", i32), ("cap_block", i32), ("max_block_cols", i32),
-                ("out", p_), ("ld_out", i64), ("pair_offset", i64), ("n_pairs_local", i64)]
+                ("out", p_), ("ld_out", i64), ("pair_offset", i64), ("n_pairs_local", i64),
+                ("sum_te", p_), ("sumsq_te", p_), ("count_ge", p_), ("observed", p_),
+                ("hot_zero", C.c_int32), ("stage_vars", C.c_int32), ("stage_stride", C.c_int64)]
 
 
 class SurrArgs(C.Structure):
@@ -68,7 +72,8 @@ class Mlp(C.Structure):
 
 class Adam(C.Structure):
     _fields_ = [("lr", C.c_float), ("lr_min", C.c_float), ("beta1", C.c_float),
-                ("beta2", C.c_float), ("eps", C.c_float), ("steps", i32), ("t0", i32)]
+                ("beta2", C.c_float), ("eps", C.c_float), ("steps", i32), ("t0", i32),
+                ("batch", i32)]
 
 
 # name -> (restype, argtypes)
@@ -89,6 +94,8 @@ SIGNATURES = {
     "tnx_gemm_b2_workspace": (i64, [i64, i64, i32]),
     "tnx_te_gemm_b2": (C.c_int, [p_, i64, i64, i64, i32, p_, p_, p_, p_, i64, p_, i64, p_, p_]),
     "tnx_surrogate_run": (C.c_int, [C.POINTER(SurrArgs), p_]),
+    "tnx_block_shuffle": (C.c_int, [p_, i64, i64, i64, p_, i32, i32, p_, p_]),
+    "tnx_stats_accumulate": (C.c_int, [p_, i32, i64, p_, i64, p_, p_, p_, p_, p_]),
     "tnx_mlp_forward": (C.c_int, [C.POINTER(Mlp), p_, i64, p_, p_]),
     "tnx_mlp_workspace_bytes": (i64, [C.c_int32, C.c_int32, i64]),
     "tnx_mlp_backward": (C.c_int, [C.POINTER(Mlp), p_, p_, p_, i64, p_, p_, p_, i64, p_]),

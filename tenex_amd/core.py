@@ -137,6 +137,7 @@ class Prepared:
     feat: torch.Tensor       # float32 (n, 8)
     xlogx: torch.Tensor
     plans: dict = field(default_factory=dict)
+    hot_zero: bool = False   # key 0 frequent enough to divert (see key0_fraction)
 
     @property
     def n_vars(self) -> int:
@@ -174,7 +175,26 @@ def prepare(bins: torch.Tensor, npv: torch.Tensor, T: int, tau: int) -> Prepared
          ptr(S2a), ptr(feat), _stream(dev))
     return Prepared(bins=bins, npv=npv, npv_host=npv.cpu().numpy().astype(np.int32), T=T,
                     tau=tau, states=states, S1=S1, S2a=S2a, feat=feat,
-                    xlogx=xlogx_table(L, dev))
+                    xlogx=xlogx_table(L, dev),
+                    hot_zero=key0_fraction(states, bins, L) >= HOT_ZERO_FRACTION)
+
+
+# A histogram key held by a fraction p of the samples puts about 64 p lanes
+# of every wavefront atomic on one LDS address; at p >= 1/16 (four lanes)
+# diverting it pays for the two extra VALU operations per sample.
+HOT_ZERO_FRACTION = 1.0 / 16.0
+
+
+def key0_fraction(states: torch.Tensor, bins: torch.Tensor, L: int) -> float:
+    """Estimated share of samples with histogram key 0 (target state 0 and
+    source level 0) over all ordered pairs: mean P(state 0) * mean P(level 0),
+    two device reductions.  Zero-inflated expression data reach 0.2-0.3;
+    near-uniform levels stay below 0.03."""
+    if states.shape[0] == 0 or L <= 0:
+        return 0.0
+    ps = (states[:, :L] == 0).float().mean()
+    pb = (bins[:, :L] == 0).float().mean()
+    return float((ps * pb).item())
 
 
 # ── plan ─────────────────────────────────────────────────────────────────
@@ -188,6 +208,71 @@ class Plan:
     work_host: np.ndarray
     tgt_host: np.ndarray
     src_host: np.ndarray
+    cache: dict = field(default_factory=dict)   # derived data (pair slots, copies)
+
+
+def h2d(a, device) -> torch.Tensor:
+    """Host array -> device tensor without a host wait: staged through a new
+    pinned buffer (torch's pinned-memory cache keeps it until the copy is
+    done) and copied asynchronously on the current stream."""
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if torch.device(device).type != "cuda":
+        return t.to(device)
+    return t.pin_memory().to(device, non_blocking=True)
+
+
+def _ranges(beg: np.ndarray, end: np.ndarray) -> np.ndarray:
+    """Concatenation of arange(beg[i], end[i])."""
+    ln = np.maximum(end - beg, 0)
+    tot = int(ln.sum())
+    if tot == 0:
+        return np.zeros(0, np.int64)
+    start = np.repeat(beg - np.concatenate([[0], np.cumsum(ln)[:-1]]), ln)
+    return start + np.arange(tot, dtype=np.int64)
+
+
+def plan_pairs(plan: Plan) -> tuple[np.ndarray, np.ndarray]:
+    """(target, source) variable indices of every pair of the plan's items
+    (target != source), item order."""
+    w = plan.work_host.astype(np.int64).reshape(-1, 4)
+    xs, ys = [], []
+    for j in range(int(w[:, 1].max(initial=0))):
+        it = w[w[:, 1] > j]
+        src_pos = _ranges(it[:, 2], it[:, 3])
+        tgt_pos = np.repeat(it[:, 0] + j, it[:, 3] - it[:, 2])
+        xs.append(plan.tgt_host[tgt_pos].astype(np.int64))
+        ys.append(plan.src_host[src_pos].astype(np.int64))
+    x = np.concatenate(xs) if xs else np.zeros(0, np.int64)
+    y = np.concatenate(ys) if ys else np.zeros(0, np.int64)
+    keep = x != y
+    return x[keep], y[keep]
+
+
+def replicate_plan(plan: Plan, K: int, m: int) -> Plan:
+    """The plan's pairs for K copies of an m-variable problem stacked as
+    variables k * m + j: copy k's items read the order arrays at offsets
+    k * n_tgt and k * n_src.  Wave items of every copy come first."""
+    i = plan.info
+    nw = int(i.n_wave_items)
+    nt, ns = len(plan.tgt_host), len(plan.src_host)
+    w = plan.work_host.astype(np.int64).reshape(-1, 4)
+    reps = []
+    for part in (w[:nw], w[nw:]):
+        for k in range(K):
+            q = part.copy()
+            q[:, 0] += k * nt
+            q[:, 2:] += k * ns
+            reps.append(q)
+    work = np.concatenate(reps).astype(np.int32) if reps else np.zeros((0, 4), np.int32)
+    tgt = np.concatenate([plan.tgt_host.astype(np.int64) + k * m for k in range(K)]).astype(np.int32)
+    src = np.concatenate([plan.src_host.astype(np.int64) + k * m for k in range(K)]).astype(np.int32)
+    info = _lib.PlanInfo.from_buffer_copy(i)
+    info.n_wave_items, info.n_block_items = nw * K, (len(w) - nw) * K
+    info.n_pairs, info.n_tgt, info.n_src = i.n_pairs * K, len(tgt), len(src)
+    dev = plan.work.device
+    wd = work if len(work) else np.zeros((1, 4), np.int32)
+    return Plan(info=info, tgt_order=h2d(tgt, dev), src_order=h2d(src, dev), work=h2d(wd, dev),
+                work_host=work, tgt_host=tgt, src_host=src)
 
 
 def make_plan(npv_host: np.ndarray, L: int, device, targets=None, sources=None,
@@ -205,9 +290,8 @@ def make_plan(npv_host: np.ndarray, L: int, device, targets=None, sources=None,
     src = np.empty(info.n_src, np.int32)
     work = np.empty((max(n_items, 1), 4), np.int32)
     call("tnx_te_plan", *args, ptr(tgt), ptr(src), ptr(work), n_items, C.byref(info))
-    return Plan(info=info, tgt_order=torch.from_numpy(tgt).to(device),
-                src_order=torch.from_numpy(src).to(device),
-                work=torch.from_numpy(work).to(device), work_host=work[:n_items],
+    return Plan(info=info, tgt_order=h2d(tgt, device), src_order=h2d(src, device),
+                work=h2d(work, device), work_host=work[:n_items],
                 tgt_host=tgt, src_host=src)
 
 
@@ -224,9 +308,13 @@ def get_plan(prep: Prepared, targets=None, sources=None, cap_wave: int = 0,
 
 # ── TE run ────────────────────────────────────────────────────────────────
 
-def te_args(prep: Prepared, plan: Plan, out: torch.Tensor, out_mode: int, ld_out: int = 0,
-            pair_offset: int = 0, n_pairs_local: int = 0) -> _lib.TeArgs:
+def te_args(prep: Prepared, plan: Plan, out: torch.Tensor | None, out_mode: int, ld_out: int = 0,
+            pair_offset: int = 0, n_pairs_local: int = 0, stats=None, stage_vars: int = 0,
+            stage_stride: int = 0) -> _lib.TeArgs:
+    """stats: (sum, sumsq, count_ge, observed) public-layout tensors for
+    TNX_OUT_STATS; stage_vars / stage_stride for TNX_OUT_STAGE."""
     i = plan.info
+    st = stats if stats is not None else (None, None, None, None)
     return _lib.TeArgs(
         bins=ptr(prep.bins), states=ptr(prep.states), n_per_var=ptr(prep.npv), S1=ptr(prep.S1),
         S2a=ptr(prep.S2a), xlogx=ptr(prep.xlogx), tgt_order=ptr(plan.tgt_order),
@@ -234,7 +322,9 @@ def te_args(prep: Prepared, plan: Plan, out: torch.Tensor, out_mode: int, ld_out
         ld=prep.ld, lds=prep.lds, tau=prep.tau, out_mode=out_mode,
         n_wave_items=i.n_wave_items, n_block_items=i.n_block_items, cap_wave=i.cap_wave,
         cap_block=i.cap_block, max_block_cols=max(i.max_block_cols, 1), out=ptr(out),
-        ld_out=ld_out, pair_offset=pair_offset, n_pairs_local=n_pairs_local)
+        ld_out=ld_out, pair_offset=pair_offset, n_pairs_local=n_pairs_local,
+        sum_te=ptr(st[0]), sumsq_te=ptr(st[1]), count_ge=ptr(st[2]), observed=ptr(st[3]),
+        hot_zero=int(prep.hot_zero), stage_vars=int(stage_vars), stage_stride=int(stage_stride))
 
 
 def te_run(prep: Prepared, plan: Plan, out: torch.Tensor, out_mode: int, ld_out: int = 0,
@@ -332,19 +422,41 @@ def surrogate_cap(npv_host: np.ndarray, targets=None, sources=None) -> int:
 
 
 def block_perm_device(n_vars: int, T: int, shuffle_method: str, block_length: int, k_start: int,
-                      k_end: int, seed: int, rng_device, out_device):
+                      k_end: int, seed: int, rng_device, out_device, rows=None):
     """Block orders for surrogates [k_start, k_end): perm[k] =
     argsort(rand(n_vars, n_blocks)) after manual_seed(seed + k) on a torch
     Generator on rng_device (tenex/inference/surrogate_test.py:165-207).
-    Returns (perm int32 (k, n_vars, n_blocks) on out_device, eff_bl, n_blocks)."""
+    rows: only these variables' orders (the same draws, the other rows
+    discarded).  The sort is stable, so tied draws order by block index
+    whatever the number of rows sorted.  Returns (perm int32
+    (k, n_vars or len(rows), n_blocks) on out_device, eff_bl, n_blocks)."""
     rnd_all = shuffle_method == "random" or block_length <= 1 or block_length >= T
     eff, nb = (1, T) if rnd_all else (int(block_length), T // int(block_length))
-    out = torch.empty((k_end - k_start, n_vars, nb), dtype=torch.int32, device=out_device)
+    if rows is None or (isinstance(rows, torch.Tensor) and rows.device == torch.device(rng_device)):
+        rows_t = rows
+    else:
+        rows_t = h2d(np.asarray(rows.cpu() if isinstance(rows, torch.Tensor) else rows,
+                                dtype=np.int64), rng_device)
+    m = n_vars if rows is None else int(rows_t.numel())
+    nk = k_end - k_start
     gen = torch.Generator(device=rng_device)
+    if nk * n_vars * nb <= PERM_DRAW_FLOATS:
+        # the batch's draws side by side, then one gather and one sort
+        r = torch.empty((nk, n_vars, nb), dtype=torch.float32, device=rng_device)
+        for ki, k in enumerate(range(k_start, k_end)):
+            gen.manual_seed(seed + k)
+            torch.rand(n_vars, nb, device=rng_device, generator=gen, out=r[ki])
+        if rows_t is not None:
+            r = r.index_select(1, rows_t)
+        return torch.argsort(r, dim=2, stable=True).to(device=out_device,
+                                                       dtype=torch.int32), eff, nb
+    out = torch.empty((nk, m, nb), dtype=torch.int32, device=out_device)
     for ki, k in enumerate(range(k_start, k_end)):
         gen.manual_seed(seed + k)
         r = torch.rand(n_vars, nb, device=rng_device, generator=gen)
-        out[ki].copy_(torch.argsort(r, dim=1).to(torch.int32))
+        if rows_t is not None:
+            r = r.index_select(0, rows_t)
+        out[ki].copy_(torch.argsort(r, dim=1, stable=True).to(torch.int32))
     return out, eff, nb
 
 
@@ -371,6 +483,152 @@ def surrogate_run(prep: Prepared, plan: Plan, perm: torch.Tensor, block_length: 
     call("tnx_surrogate_run", C.byref(args), _stream(prep.device))
 
 
+def block_shuffle(bins: torch.Tensor, T: int, perm_k: torch.Tensor, block_length: int,
+                  out: torch.Tensor):
+    """out = one block-shuffled copy of bins (perm_k int32 (n_vars, n_blocks))."""
+    n, ld = bins.shape
+    perm_k = perm_k.contiguous()
+    call("tnx_block_shuffle", ptr(bins), n, T, ld, ptr(perm_k), perm_k.shape[1],
+         int(block_length), ptr(out), _stream(bins.device))
+
+
+STAGE_FLOATS = 64 << 20   # staging for batched surrogate copies: 256 MiB of float32
+PERM_DRAW_FLOATS = 32 << 20   # uniform draws of one batch of block orders held at once
+
+
+def surrogate_stats_materialized(prep: Prepared, plan: Plan, perm: torch.Tensor,
+                                 block_length: int, observed: torch.Tensor,
+                                 sum_te: torch.Tensor, sumsq_te: torch.Tensor,
+                                 count_ge: torch.Tensor, work: dict | None = None):
+    """Surrogate statistics through the TE kernels: for each surrogate k of
+    perm (k, n_vars, n_blocks), a block-shuffled copy of the bins, its
+    per-variable states and entropies (tnx_gene_prep), then the plan's TE,
+    added in surrogate order into the public-layout sum / sumsq (FP64) and
+    count_ge (int32).  Up to STAGE_FLOATS / n_vars^2 surrogates go through
+    one launch of each kernel: their copies are stacked as variables
+    k * n_vars + j, the TE kernels write each copy's TE to a staging slab
+    (TNX_OUT_STAGE) and tnx_stats_accumulate folds the slab into the
+    statistics copy by copy; otherwise one TNX_OUT_STATS run per surrogate.
+    Both give the same additions in the same order.  `work` caches the
+    buffers across calls."""
+    n = prep.n_vars
+    if work is None:
+        work = {}
+    n_k = perm.shape[0]
+    kb = min(n_k, STAGE_FLOATS // max(1, n * n))
+    if kb >= 2:
+        slots = plan.cache.get("slots")
+        if slots is None:
+            x, y = plan_pairs(plan)
+            slots = h2d(np.unique(y * n + x), prep.device)
+            plan.cache["slots"] = slots
+        for k0 in range(0, n_k, kb):
+            kk = min(kb, n_k - k0)
+            bw = work.get(("batch", kk))
+            if bw is None:
+                bw = _stacked_copies(prep, plan, kk)
+                work[("batch", kk)] = bw
+            src, sp, args, stage, _ = bw
+            block_shuffle(src, prep.T, perm[k0:k0 + kk].reshape(kk * n, -1), block_length,
+                          sp.bins)
+            gene_prep_into(sp)
+            te_run(sp, None, None, _lib.TNX_OUT_STAGE, args=args)
+            call("tnx_stats_accumulate", ptr(stage), kk, n * n, ptr(slots), slots.numel(),
+                 ptr(observed), ptr(sum_te), ptr(sumsq_te), ptr(count_ge), _stream(prep.device))
+        return
+    sp = work.get("prep")
+    if sp is None:
+        sb = torch.empty_like(prep.bins)
+        sp = Prepared(bins=sb, npv=prep.npv, npv_host=prep.npv_host, T=prep.T, tau=prep.tau,
+                      states=torch.empty_like(prep.states), S1=torch.empty_like(prep.S1),
+                      S2a=torch.empty_like(prep.S2a), feat=torch.empty_like(prep.feat),
+                      xlogx=prep.xlogx, hot_zero=prep.hot_zero)  # block shuffles keep levels
+        work["prep"] = sp
+    args = te_args(sp, plan, None, _lib.TNX_OUT_STATS, ld_out=n,
+                   stats=(sum_te, sumsq_te, count_ge, observed))
+    for kk in range(n_k):
+        block_shuffle(prep.bins, prep.T, perm[kk], block_length, sp.bins)
+        gene_prep_into(sp)
+        te_run(sp, plan, None, _lib.TNX_OUT_STATS, args=args)
+
+
+def _stacked_copies(prep: Prepared, plan: Plan, K: int):
+    """Buffers for K stacked surrogate copies of prep's variables: the bins
+    repeated K times (the shuffle's input), the copies' Prepared, TE args
+    for the K-fold plan writing to a (K, n, n) staging slab, and the slab."""
+    n = prep.n_vars
+    src = prep.bins.repeat(K, 1).contiguous()
+    sp = Prepared(bins=torch.empty_like(src), npv=prep.npv.repeat(K).contiguous(),
+                  npv_host=np.tile(prep.npv_host, K), T=prep.T, tau=prep.tau,
+                  states=torch.empty((K * n, prep.lds), dtype=prep.states.dtype,
+                                     device=prep.device),
+                  S1=torch.empty(K * n, dtype=torch.float64, device=prep.device),
+                  S2a=torch.empty(K * n, dtype=torch.float64, device=prep.device),
+                  feat=torch.empty((K * n, prep.feat.shape[1]), dtype=torch.float32,
+                                   device=prep.device),
+                  xlogx=prep.xlogx, hot_zero=prep.hot_zero)
+    planK = plan.cache.get(("copies", K))
+    if planK is None:
+        planK = replicate_plan(plan, K, n)
+        plan.cache[("copies", K)] = planK
+    stage = torch.empty(K * n * n, dtype=torch.float32, device=prep.device)
+    args = te_args(sp, planK, stage, _lib.TNX_OUT_STAGE, ld_out=n, stage_vars=n,
+                   stage_stride=n * n)
+    return src, sp, args, stage, planK
+
+
+def var_subset(prep: Prepared, vars_host: np.ndarray) -> Prepared:
+    """The rows `vars_host` (sorted variable indices) of prep's bins as a
+    source for surrogate_stats_materialized: shuffling and re-preparing only
+    the variables a plan touches.  Cached on prep."""
+    vars_host = np.ascontiguousarray(vars_host, dtype=np.int64)
+    key = ("subset", hash(vars_host.tobytes()), len(vars_host))
+    sub = prep.plans.get(key)
+    if sub is None:
+        vd = h2d(vars_host, prep.device)
+        m = len(vars_host)
+        sub = Prepared(bins=prep.bins.index_select(0, vd).contiguous(),
+                       npv=prep.npv.index_select(0, vd).contiguous(),
+                       npv_host=np.ascontiguousarray(prep.npv_host[vars_host]), T=prep.T,
+                       tau=prep.tau, states=torch.empty((m, prep.lds), dtype=prep.states.dtype,
+                                                        device=prep.device),
+                       S1=torch.empty(m, dtype=torch.float64, device=prep.device),
+                       S2a=torch.empty(m, dtype=torch.float64, device=prep.device),
+                       feat=torch.empty((m, prep.feat.shape[1]), dtype=torch.float32,
+                                        device=prep.device),
+                       xlogx=prep.xlogx, hot_zero=prep.hot_zero)
+        sub.plans["rows"] = vd    # the subset's variable indices on the device
+        prep.plans[key] = sub
+    return sub
+
+
+def surrogate_stats_subset(prep: Prepared, vars_host: np.ndarray, plan: Plan,
+                           observed_pub: torch.Tensor, n_surr: int, shuffle_method: str,
+                           block_length: int, seed: int, rng_device, k_start: int = 0,
+                           perm_batch: int = 32):
+    """Surrogate statistics of the pairs of `plan`, a plan over local indices
+    of the variable subset vars_host (sorted): the subset's bins are
+    block-shuffled with the same orders as in the full data (rows of
+    block_perm_device), so every pair's statistics equal those of the full
+    computation.  Returns public-layout (m, m) sum, sumsq (float64) and
+    count_ge (int32) over the subset."""
+    sub = var_subset(prep, vars_host)
+    m = len(vars_host)
+    vd = sub.plans["rows"]
+    obs = observed_pub.index_select(0, vd).index_select(1, vd).contiguous()
+    s = torch.zeros((m, m), dtype=torch.float64, device=prep.device)
+    s2 = torch.zeros_like(s)
+    cg = torch.zeros((m, m), dtype=torch.int32, device=prep.device)
+    work = {}
+    k_end = k_start + n_surr
+    for k0 in range(k_start, k_end, perm_batch):
+        k1 = min(k0 + perm_batch, k_end)
+        perm, eff, nb = block_perm_device(prep.n_vars, prep.T, shuffle_method, block_length, k0,
+                                          k1, seed, rng_device, prep.device, rows=vd)
+        surrogate_stats_materialized(sub, plan, perm, eff, obs, s, s2, cg, work)
+    return s, s2, cg
+
+
 def surrogate_plan(prep: Prepared, targets=None, sources=None) -> Plan:
     cap = surrogate_cap(prep.npv_host, targets, sources)
     if cap > SURR_MAX_CELLS:
@@ -378,6 +636,62 @@ def surrogate_plan(prep: Prepared, targets=None, sources=None) -> Plan:
             f"surrogate test needs b_t^2*b_s <= {SURR_MAX_CELLS} histogram cells (got {cap}); "
             f"use coarsening=True or a smaller kp")
     return get_plan(prep, targets, sources, cap_wave=cap, cap_block=0)
+
+
+WAVE_CAP = 3200          # cells per wave histogram (te_hist.hip's default)
+FILL_WAVE_ITEMS = 256 * 3 * 4    # items that fill the chip: 4 per resident workgroup
+FILL_BLOCK_ITEMS = 256 * 2 * 4
+BLOCK_LDS = 80 * 1024 - 512
+
+
+def te_pairlist_plan(npv_host: np.ndarray, pairs: np.ndarray, device,
+                     cap_wave: int = WAVE_CAP) -> Plan:
+    """TE plan for an explicit [target, source] pair list: pairs grouped by
+    target, each target's sources sorted by level count; the wave-kernel
+    pairs (b_t^2 b_s <= cap_wave) in items of up to 256 sources, the rest in
+    block-kernel items of up to 16, with the block histogram sized like
+    tnx_te_plan's (two workgroups per CU)."""
+    pairs = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    info = _lib.PlanInfo()
+    bt = npv_host[pairs[:, 0]].astype(np.int64)
+    bs = npv_host[pairs[:, 1]].astype(np.int64)
+    is_wave = bt * bt * bs <= cap_wave
+    order = np.lexsort((bs, ~is_wave, pairs[:, 0]))   # target, wave first, levels ascending
+    p = pairs[order]
+    w = is_wave[order]
+    tg, start, cnt = np.unique(p[:, 0], return_index=True, return_counts=True)
+    # pairs per item: up to 256 (wave) / 16 (block), fewer for small lists so
+    # that the items fill the chip (as tnx_te_plan does)
+    nwp = int(w.sum())
+    per_w = int(np.clip(-(-nwp // FILL_WAVE_ITEMS), 1, 256))
+    per_b = int(np.clip(-(-(len(p) - nwp) // FILL_BLOCK_ITEMS), 1, 16))
+    nw = np.add.reduceat(w.astype(np.int64), start) if len(start) else np.zeros(0, np.int64)
+    gi = np.arange(len(tg), dtype=np.int64)
+    wave = _chunk_items(gi, start, start + nw, per_w)
+    block = _chunk_items(gi, start + nw, start + cnt, per_b)
+    blk = ~w
+    maxcols = int((npv_host[p[blk, 0]].astype(np.int64) * npv_host[p[blk, 1]]).max(initial=0))
+    cap_block = min(16384, BLOCK_LDS // 4 - maxcols) & ~3
+    cap_block = max(cap_block, (maxcols + 3) & ~3, 4)
+    work = np.concatenate([wave, block]).astype(np.int32).reshape(-1, 4)
+    info.n_wave_items, info.n_block_items = len(wave), len(block)
+    info.n_pairs = len(p)
+    info.cap_wave, info.cap_block, info.max_block_cols = int(cap_wave), int(cap_block), maxcols
+    info.n_tgt, info.n_src = len(tg), len(p)
+    wd = work if len(work) else np.zeros((1, 4), np.int32)
+    return Plan(info=info, tgt_order=h2d(tg.astype(np.int32), device),
+                src_order=h2d(p[:, 1].astype(np.int32), device), work=h2d(wd, device),
+                work_host=work, tgt_host=tg.astype(np.int32), src_host=p[:, 1].astype(np.int32))
+
+
+def _chunk_items(group: np.ndarray, beg: np.ndarray, end: np.ndarray, per: int) -> np.ndarray:
+    """Work items (group, 1, b, e) splitting each [beg, end) into runs of at
+    most `per`, groups in order."""
+    n = np.maximum(-(-(end - beg) // per), 0)
+    b = np.repeat(beg, n) + _ranges(np.zeros_like(n), n) * per
+    e = np.minimum(b + per, np.repeat(end, n))
+    return np.stack([np.repeat(group, n), np.ones_like(b), b, e], 1)
 
 
 def pairlist_plan(npv_host: np.ndarray, pairs: np.ndarray, device, cap_wave: int,
@@ -389,7 +703,7 @@ def pairlist_plan(npv_host: np.ndarray, pairs: np.ndarray, device, cap_wave: int
     pairs = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
     info = _lib.PlanInfo()
     if len(pairs) == 0:
-        info.cap_wave, info.cap_block = int(cap_wave), 127 * 127
+        info.cap_wave, info.cap_block = int(cap_wave), 16128
         z = torch.zeros(1, dtype=torch.int32, device=device)
         return Plan(info=info, tgt_order=z, src_order=z, work=torch.zeros((1, 4), dtype=torch.int32,
                     device=device), work_host=np.zeros((0, 4), np.int32),
@@ -407,7 +721,7 @@ def pairlist_plan(npv_host: np.ndarray, pairs: np.ndarray, device, cap_wave: int
     work = np.asarray(items, np.int32)
     info.n_wave_items, info.n_block_items = len(work), 0
     info.n_pairs = int((p[:, 0] != p[:, 1]).sum())
-    info.cap_wave, info.cap_block, info.max_block_cols = int(cap_wave), 127 * 127, 1
+    info.cap_wave, info.cap_block, info.max_block_cols = int(cap_wave), 16128, 1
     info.n_tgt, info.n_src = len(tg), len(p)
     return Plan(info=info, tgt_order=torch.from_numpy(tg.astype(np.int32)).to(device),
                 src_order=torch.from_numpy(p[:, 1].astype(np.int32)).to(device),

@@ -94,6 +94,18 @@ __device__ __forceinline__ DF df_wave_sum(DF v) {
 
 __device__ __forceinline__ double df_to_double(DF v) { return (double)v.hi + (double)v.lo; }
 
+// Integer sum over the 64 lanes (every lane gets it).
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+    v += __builtin_amdgcn_update_dpp(0, (int)v, 0x121, 0xF, 0xF, false);  // row_ror:1
+    v += __builtin_amdgcn_update_dpp(0, (int)v, 0x122, 0xF, 0xF, false);  // row_ror:2
+    v += __builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false);  // row_ror:4
+    v += __builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);  // row_ror:8
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 0) +
+           (uint32_t)__builtin_amdgcn_readlane((int)v, 16) +
+           (uint32_t)__builtin_amdgcn_readlane((int)v, 32) +
+           (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+}
+
 // c*log2(c) lookup from the hi/lo table.
 __device__ __forceinline__ void xlogx_add(DF &acc, const float2 *__restrict__ tbl, uint32_t c) {
     if (c > 1) {

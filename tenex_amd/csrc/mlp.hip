@@ -85,18 +85,23 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(tnx_mlp_t m, const flo
         if (o < O) Y[r * O + o] = y[o];
 }
 
-// Backward: a 256-thread workgroup walks 64-row chunks.  Thread t works on
-// row r = t % 64 and on hidden units [16q, 16q + 16), q = t / 64, for the
-// forward and delta passes (weights read as broadcasts from LDS, activations
-// and deltas exchanged through LDS).  For the weight gradients thread t owns
-// a 4 x 4 block: output units i0 = 4 (t % 16) .. +3 and inputs k0 = 4 (t / 16)
-// .. +3 of W2 and W1, accumulated in registers over every chunk the
-// workgroup visits; the threads of column block 0..3 also own b2, b1, W3 and
-// b3.  Each workgroup writes one partial gradient row; a second kernel sums
-// the rows in workgroup order, so the result does not depend on timing.
+// Backward: a 256-thread workgroup walks chunks of kBwdRows = 16 rows.
+// Thread t works on row r = t % 16 and on hidden units [4q, 4q + 4),
+// q = t / 16, for the forward and delta passes (weights read from LDS as
+// 16-byte vectors, activations and deltas exchanged through LDS).  For the
+// weight gradients thread t owns a 4 x 4 block: output units i0 = 4 (t % 16)
+// .. +3 and inputs k0 = 4 (t / 16) .. +3 of W2 and W1, accumulated in
+// registers over every chunk the workgroup visits; the threads of column
+// block 0..3 also own b2, b1, W3 and b3.  Short chunks keep a step of a few
+// thousand rows spread over every CU.  Each workgroup writes one partial
+// gradient row; a second kernel sums the rows in workgroup order, so the
+// result does not depend on timing.
 constexpr int kBwdThreads = 256;
 constexpr int kBwdMaxWG = 256;
-constexpr int kLd = kMlpH + 1;
+constexpr int kBwdRows = 16;                        // rows per chunk
+constexpr int kBwdQ = kBwdThreads / kBwdRows;       // unit groups per row (16)
+constexpr int kBwdU = kMlpH / kBwdQ;                // units per thread (4)
+constexpr int kLd = kMlpH + 4;                      // row stride: 16-byte aligned rows
 
 struct BwdShared {
     float W1T[kMlpMaxD * kMlpH];  // [k][j]
@@ -104,13 +109,13 @@ struct BwdShared {
     float W2T[kMlpH * kMlpH];     // [k][j]
     float W3[kMlpMaxO * kMlpH];
     float b1[kMlpH], b2[kMlpH], b3[kMlpMaxO];
-    float x[64][kLd];
-    float h1[64][kLd];
-    float h2[64][kLd];
-    float d1[64][kLd];
-    float d2[64][kLd];
-    float yp[4][64][kMlpMaxO];
-    float dy[64][kMlpMaxO];
+    float x[kBwdRows][kLd];
+    float h1[kBwdRows][kLd];
+    float h2[kBwdRows][kLd];
+    float d1[kBwdRows][kLd];
+    float d2[kBwdRows][kLd];
+    float yp[kBwdQ][kBwdRows][kMlpMaxO];
+    float dy[kBwdRows][kMlpMaxO];
 };
 
 struct ParamLayout {
@@ -127,8 +132,12 @@ struct ParamLayout {
 };
 
 __host__ __device__ inline int64_t bwd_groups(int64_t n_rows) {
-    const int64_t chunks = (n_rows + 63) / 64;
+    const int64_t chunks = (n_rows + kBwdRows - 1) / kBwdRows;
     return chunks < kBwdMaxWG ? chunks : kBwdMaxWG;
+}
+
+__device__ __forceinline__ float4 lds4(const float *p) {
+    return *reinterpret_cast<const float4 *>(p);
 }
 
 __global__ __launch_bounds__(kBwdThreads) void mlp_backward_kernel(
@@ -137,7 +146,8 @@ __global__ __launch_bounds__(kBwdThreads) void mlp_backward_kernel(
     extern __shared__ float dyn[];
     BwdShared &S = *reinterpret_cast<BwdShared *>(dyn);
     const int D = m.D, O = m.O;
-    const int t = threadIdx.x, r = t & 63, q = t >> 6;
+    const int t = threadIdx.x, r = t % kBwdRows, q = t / kBwdRows;
+    const int u0 = kBwdU * q;
     const ParamLayout L(D, O);
     {
         // every global load is issued before the first LDS write
@@ -183,13 +193,14 @@ __global__ __launch_bounds__(kBwdThreads) void mlp_backward_kernel(
         for (int v = 0; v < 4; v++) gw2[u][v] = gw1[u][v] = gw3[u][v] = 0.f;
     }
     float loss = 0.f;
-    const int64_t n_chunks = (n_rows + 63) / 64;
+    constexpr int kXLoads = (kBwdRows * kMlpMaxD + kBwdThreads - 1) / kBwdThreads;
+    const int64_t n_chunks = (n_rows + kBwdRows - 1) / kBwdRows;
     for (int64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
-        const int64_t base = c * 64;
-        const int rows = (int)(n_rows - base < 64 ? n_rows - base : 64);
-        float xv[16], tv[kMlpMaxO];
+        const int64_t base = c * kBwdRows;
+        const int rows = (int)(n_rows - base < kBwdRows ? n_rows - base : kBwdRows);
+        float xv[kXLoads], tv[kMlpMaxO];
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
+        for (int u = 0; u < kXLoads; u++) {
             const int e = t + kBwdThreads * u;
             xv[u] = e < rows * D ? X[base * D + e] : 0.f;
         }
@@ -197,50 +208,38 @@ __global__ __launch_bounds__(kBwdThreads) void mlp_backward_kernel(
         for (int o = 0; o < kMlpMaxO; o++)
             tv[o] = (q == 0 && o < O && r < rows) ? tgt[(base + r) * O + o] : 0.f;
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
+        for (int u = 0; u < kXLoads; u++) {
             const int e = t + kBwdThreads * u;
-            if (e < 64 * D) {
+            if (e < kBwdRows * D) {
                 const int rr = e / D, k = e - rr * D;
                 S.x[rr][k] = xv[u];
             }
         }
         __syncthreads();
-        // layer 1: units 16q .. 16q+15 of row r
-        float h1r[16];
-#pragma unroll
-        for (int jj = 0; jj < 16; jj++) h1r[jj] = S.b1[16 * q + jj];
+        // layer 1: units u0 .. u0+3 of row r
+        float4 h1r = lds4(&S.b1[u0]);
         for (int k = 0; k < D; k++) {
-            const float xv = S.x[r][k];
-            const float *w = &S.W1T[k * kMlpH + 16 * q];
-#pragma unroll
-            for (int jj = 0; jj < 16; jj++) h1r[jj] += w[jj] * xv;
+            const float xk = S.x[r][k];
+            const float4 w = lds4(&S.W1T[k * kMlpH + u0]);
+            h1r.x += w.x * xk; h1r.y += w.y * xk; h1r.z += w.z * xk; h1r.w += w.w * xk;
         }
-#pragma unroll
-        for (int jj = 0; jj < 16; jj++) {
-            h1r[jj] = h1r[jj] > 0.f ? h1r[jj] : 0.f;
-            S.h1[r][16 * q + jj] = h1r[jj];
-        }
+        h1r.x = fmaxf(h1r.x, 0.f); h1r.y = fmaxf(h1r.y, 0.f);
+        h1r.z = fmaxf(h1r.z, 0.f); h1r.w = fmaxf(h1r.w, 0.f);
+        *reinterpret_cast<float4 *>(&S.h1[r][u0]) = h1r;
         __syncthreads();
-        // layer 2 and this quarter's share of the outputs
-        float h2r[16];
-#pragma unroll
-        for (int jj = 0; jj < 16; jj++) h2r[jj] = S.b2[16 * q + jj];
+        // layer 2 and this group's share of the outputs
+        float4 h2r = lds4(&S.b2[u0]);
         for (int k = 0; k < kMlpH; k++) {
-            const float hv = S.h1[r][k];
-            const float *w = &S.W2T[k * kMlpH + 16 * q];
-#pragma unroll
-            for (int jj = 0; jj < 16; jj++) h2r[jj] += w[jj] * hv;
+            const float hk = S.h1[r][k];
+            const float4 w = lds4(&S.W2T[k * kMlpH + u0]);
+            h2r.x += w.x * hk; h2r.y += w.y * hk; h2r.z += w.z * hk; h2r.w += w.w * hk;
         }
-#pragma unroll
-        for (int jj = 0; jj < 16; jj++) {
-            h2r[jj] = h2r[jj] > 0.f ? h2r[jj] : 0.f;
-            S.h2[r][16 * q + jj] = h2r[jj];
-        }
+        h2r.x = fmaxf(h2r.x, 0.f); h2r.y = fmaxf(h2r.y, 0.f);
+        h2r.z = fmaxf(h2r.z, 0.f); h2r.w = fmaxf(h2r.w, 0.f);
+        *reinterpret_cast<float4 *>(&S.h2[r][u0]) = h2r;
         for (int o = 0; o < O; o++) {
-            float yv = 0.f;
-#pragma unroll
-            for (int jj = 0; jj < 16; jj++) yv += S.W3[o * kMlpH + 16 * q + jj] * h2r[jj];
-            S.yp[q][r][o] = yv;
+            const float4 w = lds4(&S.W3[o * kMlpH + u0]);
+            S.yp[q][r][o] = w.x * h2r.x + w.y * h2r.y + w.z * h2r.z + w.w * h2r.w;
         }
         __syncthreads();
         if (q == 0) {
@@ -248,8 +247,9 @@ __global__ __launch_bounds__(kBwdThreads) void mlp_backward_kernel(
 #pragma unroll
             for (int o = 0; o < kMlpMaxO; o++) {
                 if (o >= O) continue;
-                const float y = S.b3[o] + S.yp[0][r][o] + S.yp[1][r][o] + S.yp[2][r][o] +
-                                S.yp[3][r][o];
+                float y = S.b3[o];
+#pragma unroll
+                for (int qq = 0; qq < kBwdQ; qq++) y += S.yp[qq][r][o];
                 const float w = w_out ? w_out[o] : 1.0f;
                 const float e = valid ? (y - tv[o]) : 0.f;
                 loss += 0.5f * w * e * e * inv;
@@ -257,74 +257,73 @@ __global__ __launch_bounds__(kBwdThreads) void mlp_backward_kernel(
             }
         }
         __syncthreads();
-        // output deltas of layer 2 (units 16q ..)
+        // deltas of layer 2 (units u0 ..)
         {
-            float dyr[kMlpMaxO];
-#pragma unroll
-            for (int o = 0; o < kMlpMaxO; o++) dyr[o] = o < O ? S.dy[r][o] : 0.f;
-#pragma unroll
-            for (int jj = 0; jj < 16; jj++) {
-                float g = 0.f;
-#pragma unroll
-                for (int o = 0; o < kMlpMaxO; o++)
-                    if (o < O) g += S.W3[o * kMlpH + 16 * q + jj] * dyr[o];
-                S.d2[r][16 * q + jj] = h2r[jj] > 0.f ? g : 0.f;
+            float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int o = 0; o < O; o++) {
+                const float dv = S.dy[r][o];
+                const float4 w = lds4(&S.W3[o * kMlpH + u0]);
+                g.x += w.x * dv; g.y += w.y * dv; g.z += w.z * dv; g.w += w.w * dv;
             }
+            g.x = h2r.x > 0.f ? g.x : 0.f; g.y = h2r.y > 0.f ? g.y : 0.f;
+            g.z = h2r.z > 0.f ? g.z : 0.f; g.w = h2r.w > 0.f ? g.w : 0.f;
+            *reinterpret_cast<float4 *>(&S.d2[r][u0]) = g;
         }
         __syncthreads();
-        // deltas of layer 1 (units 16q ..): d1[k] = sum_j W2[j][k] d2[j]
+        // deltas of layer 1 (units u0 ..): d1[k] = sum_j W2[j][k] d2[j]
         {
-            float d1r[16];
-#pragma unroll
-            for (int kk = 0; kk < 16; kk++) d1r[kk] = 0.f;
+            float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
             for (int j = 0; j < kMlpH; j++) {
                 const float dv = S.d2[r][j];
-                const float *w = &S.W2[j * kMlpH + 16 * q];
-#pragma unroll
-                for (int kk = 0; kk < 16; kk++) d1r[kk] += w[kk] * dv;
+                const float4 w = lds4(&S.W2[j * kMlpH + u0]);
+                d.x += w.x * dv; d.y += w.y * dv; d.z += w.z * dv; d.w += w.w * dv;
             }
-#pragma unroll
-            for (int kk = 0; kk < 16; kk++) S.d1[r][16 * q + kk] = h1r[kk] > 0.f ? d1r[kk] : 0.f;
+            d.x = h1r.x > 0.f ? d.x : 0.f; d.y = h1r.y > 0.f ? d.y : 0.f;
+            d.z = h1r.z > 0.f ? d.z : 0.f; d.w = h1r.w > 0.f ? d.w : 0.f;
+            *reinterpret_cast<float4 *>(&S.d1[r][u0]) = d;
         }
         __syncthreads();
         // weight gradients of the chunk (padding rows carry zero deltas)
-        for (int rr = 0; rr < 64; rr++) {
-            float a2[4], a1[4], hh[4], xx[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                a2[u] = S.d2[rr][i0 + u];
-                a1[u] = S.d1[rr][i0 + u];
-                hh[u] = S.h1[rr][k0 + u];
-                xx[u] = (k0 + u < D) ? S.x[rr][k0 + u] : 0.f;
-            }
+        for (int rr = 0; rr < kBwdRows; rr++) {
+            const float4 a2 = lds4(&S.d2[rr][i0]);
+            const float4 a1 = lds4(&S.d1[rr][i0]);
+            const float4 hh = lds4(&S.h1[rr][k0]);
+            const float4 x4 = lds4(&S.x[rr][k0]);
+            const float av2[4] = {a2.x, a2.y, a2.z, a2.w}, av1[4] = {a1.x, a1.y, a1.z, a1.w};
+            const float hv[4] = {hh.x, hh.y, hh.z, hh.w};
+            const float xx[4] = {k0 + 0 < D ? x4.x : 0.f, k0 + 1 < D ? x4.y : 0.f,
+                                 k0 + 2 < D ? x4.z : 0.f, k0 + 3 < D ? x4.w : 0.f};
 #pragma unroll
             for (int u = 0; u < 4; u++)
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    gw2[u][v] += a2[u] * hh[v];
-                    gw1[u][v] += a1[u] * xx[v];
+                    gw2[u][v] += av2[u] * hv[v];
+                    gw1[u][v] += av1[u] * xx[v];
                 }
         }
         if (kb == 0) {
-            for (int rr = 0; rr < 64; rr++)
-#pragma unroll
-                for (int u = 0; u < 4; u++) gb[u] += S.d2[rr][i0 + u];
+            for (int rr = 0; rr < kBwdRows; rr++) {
+                const float4 a = lds4(&S.d2[rr][i0]);
+                gb[0] += a.x; gb[1] += a.y; gb[2] += a.z; gb[3] += a.w;
+            }
         } else if (kb == 1) {
-            for (int rr = 0; rr < 64; rr++)
-#pragma unroll
-                for (int u = 0; u < 4; u++) gb[u] += S.d1[rr][i0 + u];
+            for (int rr = 0; rr < kBwdRows; rr++) {
+                const float4 a = lds4(&S.d1[rr][i0]);
+                gb[0] += a.x; gb[1] += a.y; gb[2] += a.z; gb[3] += a.w;
+            }
         } else if (kb == 2) {
-            for (int rr = 0; rr < 64; rr++) {
+            for (int rr = 0; rr < kBwdRows; rr++) {
+                const float4 h = lds4(&S.h2[rr][i0]);
 #pragma unroll
                 for (int o = 0; o < kMlpMaxO; o++) {
                     if (o >= O) continue;
                     const float dv = S.dy[rr][o];
-#pragma unroll
-                    for (int u = 0; u < 4; u++) gw3[o][u] += dv * S.h2[rr][i0 + u];
+                    gw3[o][0] += dv * h.x; gw3[o][1] += dv * h.y;
+                    gw3[o][2] += dv * h.z; gw3[o][3] += dv * h.w;
                 }
             }
         } else if (kb == 3 && ib == 0) {
-            for (int rr = 0; rr < 64; rr++)
+            for (int rr = 0; rr < kBwdRows; rr++)
 #pragma unroll
                 for (int o = 0; o < kMlpMaxO; o++)
                     if (o < O) gb[o] += S.dy[rr][o];
@@ -350,9 +349,9 @@ __global__ __launch_bounds__(kBwdThreads) void mlp_backward_kernel(
     } else if (kb == 3 && ib == 0) {
         for (int o = 0; o < O; o++) pr[L.b3 + o] = gb[o];
     }
-    if (q == 0) {
+    if (t < kWave) {  // the loss lives in threads q == 0 (t < kBwdRows) of wave 0
         for (int off = 32; off >= 1; off >>= 1) loss += __shfl_xor(loss, off, 64);
-        if (r == 0) pr[L.P] = loss;
+        if (t == 0) pr[L.P] = loss;
     }
 }
 
@@ -521,7 +520,10 @@ int tnx_mlp_fit(const tnx_mlp_t *m, float *master, float *m1, float *m2, int64_t
                 float *grad, void *workspace, int64_t ws_bytes, float *loss_hist, void *stream) {
     clear_error();
     TNX_CHECK_ARG(master && m1 && m2 && X && tgt && opt && grad, "tnx_mlp_fit: null pointer");
-    int rc = check_train_args(m, n_rows, workspace, ws_bytes, "tnx_mlp_fit");
+    TNX_CHECK_ARG(opt->batch >= 0, "tnx_mlp_fit: batch %d < 0", opt->batch);
+    const int64_t rows = (opt->batch > 0 && opt->batch < n_rows) ? opt->batch : n_rows;
+    const int64_t n_batches = rows > 0 ? n_rows / rows : 1;
+    int rc = check_train_args(m, rows, workspace, ws_bytes, "tnx_mlp_fit");
     if (rc) return rc;
     const ParamLayout L(m->D, m->O);
     TNX_CHECK_ARG(n_params == L.P, "tnx_mlp_fit: n_params %lld, expected %d",
@@ -530,12 +532,13 @@ int tnx_mlp_fit(const tnx_mlp_t *m, float *master, float *m1, float *m2, int64_t
                   "tnx_mlp_fit: bad schedule (steps %d, t0 %d)", opt->steps, opt->t0);
     hipStream_t st = as_stream(stream);
     float *part = static_cast<float *>(workspace);
-    const int G = (int)bwd_groups(n_rows);
+    const int G = (int)bwd_groups(rows);
     const double pi = 3.14159265358979323846;
     for (int s = 0; s < opt->steps; s++) {
         const float lr = (float)(opt->lr_min + (opt->lr - opt->lr_min) * 0.5 *
                                                    (1.0 + cos(pi * (double)s / opt->steps)));
-        rc = launch_backward(m, X, tgt, nullptr, n_rows, part, st);
+        const int64_t r0 = (s % n_batches) * rows;
+        rc = launch_backward(m, X + r0 * m->D, tgt + r0 * m->O, nullptr, rows, part, st);
         if (rc) return rc;
         const int t = opt->t0 + s + 1;
         const float c1 = 1.f - powf(opt->beta1, (float)t), c2 = 1.f - powf(opt->beta2, (float)t);

@@ -186,17 +186,6 @@ __device__ __forceinline__ void xlogx_add_l(DF &acc, const float2 *ltbl,
     }
 }
 
-// Integer sum over the 64 lanes (every lane gets it).
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
-    v += __builtin_amdgcn_update_dpp(0, (int)v, 0x121, 0xF, 0xF, false);  // row_ror:1
-    v += __builtin_amdgcn_update_dpp(0, (int)v, 0x122, 0xF, 0xF, false);  // row_ror:2
-    v += __builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false);  // row_ror:4
-    v += __builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);  // row_ror:8
-    return (uint32_t)__builtin_amdgcn_readlane((int)v, 0) +
-           (uint32_t)__builtin_amdgcn_readlane((int)v, 16) +
-           (uint32_t)__builtin_amdgcn_readlane((int)v, 32) +
-           (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
-}
 
 // S3 - S2b of a u16-packed pair histogram (cells (a * b_x + b) * b_y + c)
 // counted by count_pair16 from L samples, summed over the wave; zeroes the
@@ -243,7 +232,8 @@ __device__ __forceinline__ DF reduce_pair16(uint32_t *hist, int bx, int by, int 
 // and are added to the public slots once per item.
 template <int NW, typename ST>
 __global__ __launch_bounds__(256) void surrogate_wave_kernel(tnx_surr_args_t a, int32_t lds_len,
-                                                             int32_t tg_max, int32_t nbatch) {
+                                                             int32_t tg_max, int32_t nbatch,
+                                                             int32_t ksplit) {
     extern __shared__ uint32_t lds_u32[];
     __shared__ double s12_sh;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -268,7 +258,15 @@ __global__ __launch_bounds__(256) void surrogate_wave_kernel(tnx_surr_args_t a, 
         ltbl[i] = i <= L ? tbl[i] : make_float2(0.f, 0.f);
     __syncthreads();
     Stage<NW> stg;
-    for (int64_t w = blockIdx.x; w < a.n_wave_items; w += gridDim.x) {
+    // work unit g = (item g % n_items, surrogate slice g / n_items): slices
+    // of the k range give small plans enough workgroups; the statistics of
+    // different slices meet in the public slots through atomic adds
+    const int64_t n_units = a.n_wave_items * ksplit;
+    for (int64_t g = blockIdx.x; g < n_units; g += gridDim.x) {
+        const int64_t w = g % a.n_wave_items;
+        const int kc = (int)(g / a.n_wave_items);
+        const int k_beg = (int)((int64_t)a.n_surr * kc / ksplit);
+        const int k_end = (int)((int64_t)a.n_surr * (kc + 1) / ksplit);
         const tnx_work_t it = a.work[w];
         for (int ti0 = 0; ti0 < it.tgt_cnt; ti0 += tg_max) {
         const int tg = min(tg_max, it.tgt_cnt - ti0);
@@ -285,7 +283,7 @@ __global__ __launch_bounds__(256) void surrogate_wave_kernel(tnx_surr_args_t a, 
         const int nsrc = min(it.src_end - s0, chunk);
         double acc_s = 0.0, acc_q = 0.0;
         int32_t acc_c = 0;
-        for (int k = 0; k < a.n_surr; k++) {
+        for (int k = k_beg; k < k_end; k++) {
             // ── 1. shuffled states of the tg targets, S1'_k and S2a'_k ─────
             for (int j = 0; j < tg; j++) {
                 const int xj = a.tgt_order[it.tgt_beg + ti0 + j];
@@ -403,9 +401,9 @@ __global__ __launch_bounds__(256) void surrogate_wave_kernel(tnx_surr_args_t a, 
             const int by = a.n_per_var[y];
             if (y != x && bx * bx * by <= a.cap_wave) {
                 const int64_t slot = (int64_t)y * n + x;
-                a.sum_te[slot] += acc_s;
-                a.sumsq_te[slot] += acc_q;
-                a.count_ge[slot] += acc_c;
+                atomicAdd(a.sum_te + slot, acc_s);
+                atomicAdd(a.sumsq_te + slot, acc_q);
+                atomicAdd(a.count_ge + slot, acc_c);
             }
         }
         __syncthreads();
@@ -414,9 +412,58 @@ __global__ __launch_bounds__(256) void surrogate_wave_kernel(tnx_surr_args_t a, 
     }
 }
 
+// One shuffled copy of every row: thread i of the grid writes 16 bytes of
+// one row (four words, zero-padded past T).
+__global__ __launch_bounds__(256) void block_shuffle_kernel(const int8_t *__restrict__ bins,
+                                                            int64_t n_vars, int32_t T, int64_t ld,
+                                                            const int32_t *__restrict__ perm,
+                                                            int32_t n_blocks, int32_t bl,
+                                                            int8_t *__restrict__ out) {
+    const int64_t per_row = ld / 16;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_vars * per_row) return;
+    const int64_t v = i / per_row;
+    const int32_t t16 = (int32_t)(i - v * per_row) * 16;
+    const int8_t *yr = bins + v * ld;
+    const BlockMap m{perm + v * n_blocks, bl, n_blocks * bl, n_blocks, 0xFFFFFFFFu / (uint32_t)bl};
+    uint32_t w[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int32_t t0 = t16 + 4 * j;
+        if (t0 >= m.head && t0 + 3 < T) {
+            w[j] = *reinterpret_cast<const uint32_t *>(yr + t0);  // tail: unshuffled
+        } else if (t0 < T) {
+            const ShufWord sw = shuffle_issue(yr, m, t0, T);
+            w[j] = shuffle_finish(sw, yr, m, t0, T);
+        } else {
+            w[j] = 0u;
+        }
+    }
+    *reinterpret_cast<uint4 *>(out + v * ld + t16) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 }  // namespace tnx
 
 using namespace tnx;
+
+extern "C" int tnx_block_shuffle(const int8_t *bins, int64_t n_vars, int64_t T, int64_t ld,
+                                 const int32_t *perm, int32_t n_blocks, int32_t block_length,
+                                 int8_t *out, void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(bins && perm && out && n_vars >= 0 && T >= 1, "tnx_block_shuffle: bad arguments");
+    TNX_CHECK_ARG(ld >= T && ld % 16 == 0, "tnx_block_shuffle: ld must be >= T, multiple of 16");
+    TNX_CHECK_ARG(block_length >= 1 && n_blocks >= 1 && (int64_t)n_blocks * block_length <= T,
+                  "tnx_block_shuffle: need n_blocks * block_length <= T");
+    TNX_CHECK_ARG(T < (1LL << 31) / 2, "tnx_block_shuffle: T too large");
+    TNX_CHECK_ARG(bins != out, "tnx_block_shuffle: out must not alias bins");
+    if (n_vars == 0) return TNX_OK;
+    const int64_t threads = n_vars * (ld / 16);
+    hipLaunchKernelGGL(block_shuffle_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), bins, n_vars, (int32_t)T, ld, perm, n_blocks,
+                       block_length, out);
+    TNX_CHECK_LAUNCH("block_shuffle_kernel");
+    return TNX_OK;
+}
 
 extern "C" int tnx_surrogate_run(const tnx_surr_args_t *args, void *stream) {
     clear_error();
@@ -467,7 +514,11 @@ extern "C" int tnx_surrogate_run(const tnx_surr_args_t *args, void *stream) {
     // words staged per thread and source with several targets (L / 1024);
     // the one-target mode stages in groups of four words
     const int64_t words = tg_max > 1 ? (L + 1023) / 1024 : 1;
-    const unsigned grid = (unsigned)std::min<int64_t>(a.n_wave_items, 1 << 20);
+    // enough work units to fill the chip: slice the surrogates of small plans
+    const int64_t want_units = 3 * 256;
+    const int ksplit = (int)std::max<int64_t>(
+        1, std::min<int64_t>(a.n_surr, (want_units + a.n_wave_items - 1) / a.n_wave_items));
+    const unsigned grid = (unsigned)std::min<int64_t>(a.n_wave_items * ksplit, 1 << 20);
     const void *kfn = nullptr;
 #define TNX_SURR(NWV, STT)                                                                   \
     if (!kfn && words <= NWV && (sizeof(STT) == 1) == st8) {                                \
@@ -475,7 +526,7 @@ extern "C" int tnx_surrogate_run(const tnx_surr_args_t *args, void *stream) {
         TNX_CHECK_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize,    \
                                           (int)shm));                                         \
         hipLaunchKernelGGL((surrogate_wave_kernel<NWV, STT>), dim3(grid), dim3(256), shm,     \
-                           as_stream(stream), a, lds_len, tg_max, nbatch);                    \
+                           as_stream(stream), a, lds_len, tg_max, nbatch, ksplit);            \
     }
     TNX_SURR(1, uint8_t) TNX_SURR(2, uint8_t) TNX_SURR(4, uint8_t) TNX_SURR(8, uint8_t)
     TNX_SURR(16, uint8_t)

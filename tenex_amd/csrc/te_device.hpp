@@ -21,6 +21,11 @@ __device__ __forceinline__ void lds_inc(uint32_t *p) {
     __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Read an LDS word and leave zero behind (one ds_wrxchg_rtn_b32).
+__device__ __forceinline__ uint32_t lds_take_w(uint32_t *p) {
+    return __hip_atomic_exchange(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Byte offset of key state * by + y in a histogram whose cells are `G`
 // words apart (cell-major privatised copies, below): kmul = 4 * G * by and
 // ysh = 2 + log2 G.  States < 2^16 and by <= 127, so the product fits the
@@ -34,27 +39,59 @@ __device__ __forceinline__ void lds_inc_key(uint32_t *hist, uint32_t sv, uint32_
     lds_inc(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) + off));
 }
 
+// With HZ (hot zero key), as lds_inc_key but key 0 (state 0 and source
+// level 0) goes to `zero`, a word private to the lane.  In zero-inflated
+// count data level 0 is each variable's most frequent level and key 0 can
+// hold a quarter of all samples; counted in one cell it would serialise the
+// wave's atomics on one address.  fold_zero_key adds the lanes' words back
+// into cell 0.  The diversion costs two VALU operations per sample (compare,
+// select), so data without a hot key 0 run the plain form (HZ false).
+template <bool HZ>
+__device__ __forceinline__ void lds_inc_key0(uint32_t *hist, uint32_t sv, uint32_t kmul,
+                                             uint32_t yv, uint32_t ysh, uint32_t *zero) {
+    const uint32_t off = __umul24(sv, kmul) + (yv << ysh);
+    uint32_t *cell = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) + off);
+    if constexpr (HZ) {
+        lds_inc(off ? cell : zero);
+    } else {
+        lds_inc(cell);
+    }
+}
+
+// Cell 0 (copy 0, the first word of the slice) += the 64 lanes' key-0 words
+// zero_row[0..63], which are cleared.  One wavefront; nothing without HZ.
+template <bool HZ>
+__device__ __forceinline__ void fold_zero_key(uint32_t *hist, uint32_t *zero_row, int lane) {
+    if constexpr (!HZ) return;
+    const uint32_t c0 = wave_sum_u32(lds_take_w(zero_row + lane));
+    if (lane == 0 && c0) hist[0] += c0;
+    wave_lds_fence();
+}
+
 // 16 samples of one lane: 8 packed 16-bit states, 4 source dwords.
+template <bool HZ>
 __device__ __forceinline__ void acc16(uint32_t *hist, const uint32_t *sreg, const uint4 y4,
-                                      uint32_t kmul, uint32_t ysh) {
+                                      uint32_t kmul, uint32_t ysh, uint32_t *zero) {
     const uint32_t yw[4] = {y4.x, y4.y, y4.z, y4.w};
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const uint32_t sv = (sreg[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
         const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-        lds_inc_key(hist, sv, kmul, yv, ysh);
+        lds_inc_key0<HZ>(hist, sv, kmul, yv, ysh, zero);
     }
 }
 
+template <bool HZ>
 __device__ __forceinline__ void acc16_tail(uint32_t *hist, const uint32_t *sreg, const uint4 y4,
-                                           uint32_t kmul, uint32_t ysh, int n_valid) {
+                                           uint32_t kmul, uint32_t ysh, uint32_t *zero,
+                                           int n_valid) {
     const uint32_t yw[4] = {y4.x, y4.y, y4.z, y4.w};
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         if (q < n_valid) {
             const uint32_t sv = (sreg[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
             const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-            lds_inc_key(hist, sv, kmul, yv, ysh);
+            lds_inc_key0<HZ>(hist, sv, kmul, yv, ysh, zero);
         }
     }
 }
@@ -63,9 +100,11 @@ __device__ __forceinline__ void acc16_tail(uint32_t *hist, const uint32_t *sreg,
 // state[t] * by + y[t] for t < L.  Lane `lane` takes 16 consecutive samples
 // per step (one dwordx4 of source bytes, two of states), two steps in
 // flight.  Rows must be padded to a multiple of 64 elements.
+template <bool HZ>
 __device__ __forceinline__ void hist_accumulate_mem(uint32_t *hist, const uint16_t *__restrict__ st,
                                                     const int8_t *__restrict__ yr, uint32_t kmul,
-                                                    uint32_t ysh, int L, int lane) {
+                                                    uint32_t ysh, uint32_t *zero, int L,
+                                                    int lane) {
     int t = 16 * lane;
     for (; t + 1024 + 16 <= L; t += 2048) {
         const uint4 ya = *reinterpret_cast<const uint4 *>(yr + t);
@@ -76,15 +115,15 @@ __device__ __forceinline__ void hist_accumulate_mem(uint32_t *hist, const uint16
         const uint4 sb1 = *reinterpret_cast<const uint4 *>(st + t + 1032);
         const uint32_t sa[8] = {sa0.x, sa0.y, sa0.z, sa0.w, sa1.x, sa1.y, sa1.z, sa1.w};
         const uint32_t sb[8] = {sb0.x, sb0.y, sb0.z, sb0.w, sb1.x, sb1.y, sb1.z, sb1.w};
-        acc16(hist, sa, ya, kmul, ysh);
-        acc16(hist, sb, yb, kmul, ysh);
+        acc16<HZ>(hist, sa, ya, kmul, ysh, zero);
+        acc16<HZ>(hist, sb, yb, kmul, ysh, zero);
     }
     for (; t < L; t += 1024) {
         const uint4 ya = *reinterpret_cast<const uint4 *>(yr + t);
         const uint4 sa0 = *reinterpret_cast<const uint4 *>(st + t);
         const uint4 sa1 = *reinterpret_cast<const uint4 *>(st + t + 8);
         const uint32_t sa[8] = {sa0.x, sa0.y, sa0.z, sa0.w, sa1.x, sa1.y, sa1.z, sa1.w};
-        acc16_tail(hist, sa, ya, kmul, ysh, L - t);
+        acc16_tail<HZ>(hist, sa, ya, kmul, ysh, zero, L - t);
     }
 }
 
@@ -116,10 +155,11 @@ __device__ __forceinline__ void load_target_regs(TargetRegs<NI> &r,
 
 // Phase 1 with the target in registers: 16 source bytes per lane per chunk
 // (one dwordx4 load), 16 LDS increments.
-template <int NI>
+template <int NI, bool HZ>
 __device__ __forceinline__ void hist_accumulate_regs(uint32_t *hist, TargetRegs<NI> &r,
                                                      const int8_t *__restrict__ yr, uint32_t kmul,
-                                                     uint32_t ysh, int L, int lane) {
+                                                     uint32_t ysh, uint32_t *zero, int L,
+                                                     int lane) {
     // Keep the packed states opaque per call: otherwise the compiler hoists
     // the per-sample 16-bit extractions out of the caller's source loop and
     // spends a register on every sample.
@@ -151,7 +191,7 @@ __device__ __forceinline__ void hist_accumulate_regs(uint32_t *hist, TargetRegs<
                 for (int q = 0; q < 16; q++) {
                     const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
                     const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                    lds_inc_key(hist, sv, kmul, yv, ysh);
+                    lds_inc_key0<HZ>(hist, sv, kmul, yv, ysh, zero);
                 }
             } else {
 #pragma unroll
@@ -159,7 +199,7 @@ __device__ __forceinline__ void hist_accumulate_regs(uint32_t *hist, TargetRegs<
                     if (t + q < L) {
                         const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
                         const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                        lds_inc_key(hist, sv, kmul, yv, ysh);
+                        lds_inc_key0<HZ>(hist, sv, kmul, yv, ysh, zero);
                     }
                 }
             }
@@ -265,12 +305,13 @@ __device__ __forceinline__ DF df_block_sum(DF u, float *red) {
 // L <= 16 nth NB) and so is the source row (ys, same samples).  Returns
 // S3 - S2b, valid in thread 0.  hist and c2b must be zero on entry and are
 // zero on exit; the caller's next pair may start right away.
-template <int NB>
-__device__ __forceinline__ DF hist_pair_block(uint32_t *hist, uint32_t *c2b, int cap,
-                                              const TargetRegs<NB> &regs, const uint4 *ys,
-                                              int bx, int by, int L,
+template <int NB, bool HZ>
+__device__ __forceinline__ DF hist_pair_block(uint32_t *hist, uint32_t *c2b, uint32_t *zero_row,
+                                              int cap, const TargetRegs<NB> &regs,
+                                              const uint4 *ys, int bx, int by, int L,
                                               const float2 *__restrict__ tbl, float *red) {
     const int tid = threadIdx.x, nth = blockDim.x;
+    uint32_t *zero = zero_row + (tid & 63);
     const int ncol = bx * by;
     const uint32_t by4 = (uint32_t)by << 2;
     int apass = cap / ncol;
@@ -292,25 +333,29 @@ __device__ __forceinline__ DF hist_pair_block(uint32_t *hist, uint32_t *c2b, int
                 for (int q = 0; q < 16; q++) {
                     const uint32_t sv = (regs.s[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
                     const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                    lds_inc_key(hc, sv, by4, yv, 2u);
+                    lds_inc_key0<HZ>(hc, sv, by4, yv, 2u, zero);
                 }
             } else if (t + 16 <= L) {
 #pragma unroll
                 for (int q = 0; q < 16; q++) {
                     const uint32_t sv = (regs.s[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
                     const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                    if (sv - lo < span) lds_inc_key(hc, sv, by4, yv, 2u);
+                    if (sv - lo < span) lds_inc_key0<HZ>(hc, sv, by4, yv, 2u, zero);
                 }
             } else {
 #pragma unroll
                 for (int q = 0; q < 16; q++) {
                     const uint32_t sv = (regs.s[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
                     const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                    if (t + q < L && sv - lo < span) lds_inc_key(hc, sv, by4, yv, 2u);
+                    if (t + q < L && sv - lo < span) lds_inc_key0<HZ>(hc, sv, by4, yv, 2u, zero);
                 }
             }
         }
         __syncthreads();
+        if (HZ && a0 == 0) {  // key 0 only occurs in the first pass
+            if (tid < kWave) fold_zero_key<HZ>(hist, zero_row, tid);
+            __syncthreads();
+        }
         // flattened scan over the pass's cells, four at a time (cap is a
         // multiple of 4 and cells past the pass are zero): groups of four
         // empty cells cost one read; only nonzero groups are cleared, and

@@ -49,17 +49,48 @@ constexpr int kChunkSrc = 64;        // sources per locality chunk
 // Item of workgroup slot w in [0, n): workgroup w runs on XCD w % 8, and XCD
 // x takes the x-th contiguous slice of the items, in order (a bijection of
 // [0, n)).
+// Writes TE of the pair at slot: the value, or (surrogate statistics) its
+// float32 rounding added to the FP64 sum and sum of squares and the count of
+// values >= the observed TE (float32 compare), as the reference accumulates
+// its per-surrogate float32 TE matrices.
+__device__ __forceinline__ void emit_te(const tnx_te_args_t &a, int64_t slot, double te) {
+    if (a.out_mode == TNX_OUT_STATS) {
+        const float tf = (float)te;
+        const double td = (double)tf;
+        a.sum_te[slot] += td;
+        a.sumsq_te[slot] += td * td;
+        a.count_ge[slot] += (tf >= a.observed[slot]) ? 1 : 0;
+    } else {
+        a.out[slot] = (float)te;
+    }
+}
+
 __device__ __forceinline__ int64_t xcd_item(int64_t w, int64_t n) {
     const int64_t x = w % kXcds, i = w / kXcds;
     const int64_t q = n / kXcds, r = n % kXcds;
     return x * q + (x < r ? x : r) + i;
 }
-constexpr int kBlockItemPairs = 16;  // pairs per block-kernel work item
+constexpr int kBlockItemPairs = 16;  // pairs per block-kernel work item (at most)
+// Small plans split their pairs into more, shorter items so that every wave
+// slot gets work: about four items per resident workgroup (256 CUs x 3
+// wave-kernel or 2 block-kernel workgroups).
+constexpr int64_t kFillWaveItems = 256 * 3 * 4;
+constexpr int64_t kFillBlockItems = 256 * 2 * 4;
 
 // Output slot of pair (x, y); returns false if the pair is filtered out.
 __device__ __forceinline__ bool out_slot(const tnx_te_args_t &a, int x, int y, int64_t &slot) {
     if (a.out_mode == TNX_OUT_MATRIX) {
         slot = (int64_t)x * a.ld_out + y;
+        return true;
+    }
+    if (a.out_mode == TNX_OUT_STATS) {
+        slot = (int64_t)y * a.ld_out + x;  // public layout [source, target]
+        return true;
+    }
+    if (a.out_mode == TNX_OUT_STAGE) {  // x, y in the same copy x / stage_vars
+        const int k = x / a.stage_vars;
+        slot = (int64_t)k * a.stage_stride + (int64_t)(y - k * a.stage_vars) * a.ld_out +
+               (x - k * a.stage_vars);
         return true;
     }
     int64_t id = (int64_t)x * (a.n_vars - 1) + (y - (y > x ? 1 : 0));
@@ -75,13 +106,15 @@ __device__ __forceinline__ bool out_slot(const tnx_te_args_t &a, int x, int y, i
 // of its histogram (wave_copies) inside the wave's cap_wave-cell LDS slice.
 // The register variants are limited to 3 waves per SIMD so the state
 // registers do not spill; the LDS slice is sized for that occupancy.
-template <int NI>
+template <int NI, bool HZ>
 __device__ __forceinline__ void te_wave_body(const tnx_te_args_t &a) {
     extern __shared__ uint32_t lds_u32[];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    uint32_t *hist = lds_u32 + wave * a.cap_wave;
-    for (int i = lane; i < a.cap_wave; i += kWave) hist[i] = 0;
+    // slice: cap_wave histogram words, then the lanes' key-0 words
+    uint32_t *hist = lds_u32 + wave * (a.cap_wave + kWave);
+    uint32_t *zero_row = hist + a.cap_wave;
+    for (int i = lane; i < a.cap_wave + kWave; i += kWave) hist[i] = 0;
     wave_lds_fence();
     const int L = (int)(a.T - a.tau);
     const float2 *tbl = reinterpret_cast<const float2 *>(a.xlogx);
@@ -119,7 +152,7 @@ __device__ __forceinline__ void te_wave_body(const tnx_te_args_t &a) {
             int64_t slot;
             const bool want = (x != y) && out_slot(a, x, y, slot);
             const bool fits = bx * bx * by <= a.cap_wave;
-            if (want && !fits && lane == 0) a.out[slot] = __builtin_nanf("");  // plan mismatch
+            if (want && !fits && lane == 0) emit_te(a, slot, __builtin_nan(""));  // plan mismatch
             if (!(want && fits)) continue;
             const WaveCopies cp = wave_copies(bx * bx * by, a.cap_wave, TNX_WAVE_COPY_BUDGET);
             uint32_t *hc = hist + (lane & (cp.G - 1));
@@ -131,24 +164,27 @@ __device__ __forceinline__ void te_wave_body(const tnx_te_args_t &a) {
                 cur_x = x;
             }
             if constexpr (NI > 0) {
-                hist_accumulate_regs<NI>(hc, regs, yr, kmul, ysh, L, lane);
+                hist_accumulate_regs<NI, HZ>(hc, regs, yr, kmul, ysh, zero_row + lane, L, lane);
             } else {
-                hist_accumulate_mem(hc, a.states + (int64_t)x * a.lds, yr, kmul, ysh, L, lane);
+                hist_accumulate_mem<HZ>(hc, a.states + (int64_t)x * a.lds, yr, kmul, ysh,
+                                        zero_row + lane, L, lane);
             }
             wave_lds_fence();
+            fold_zero_key<HZ>(hist, zero_row, lane);
             const DF d = hist_reduce_wave_copies(hist, bx, by, cp, tbl, lane);
             wave_lds_fence();
-            if (lane == 0) a.out[slot] = (float)((df_to_double(d) + s12) * invN);
+            if (lane == 0) emit_te(a, slot, (df_to_double(d) + s12) * invN);
         }
     }
 }
 
-__global__ __launch_bounds__(256) void te_wave_kernel(tnx_te_args_t a) { te_wave_body<0>(a); }
+template <bool HZ>
+__global__ __launch_bounds__(256) void te_wave_kernel(tnx_te_args_t a) { te_wave_body<0, HZ>(a); }
 
-template <int NI>
+template <int NI, bool HZ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void te_wave_reg_kernel(
     tnx_te_args_t a) {
-    te_wave_body<NI>(a);
+    te_wave_body<NI, HZ>(a);
 }
 
 // Workgroup-per-pair kernel for pairs above cap_wave cells (multi-pass, exact).
@@ -164,14 +200,16 @@ __device__ __forceinline__ void load_block_row(uint4 *ys, const int8_t *__restri
     }
 }
 
-template <int NB>
+template <int NB, bool HZ>
 __global__ __launch_bounds__(kBlockThreads) void te_block_kernel(tnx_te_args_t a) {
     extern __shared__ uint32_t lds_u32[];
     __shared__ float red[2 * (kBlockThreads / kWave)];
     uint32_t *hist = lds_u32;
     uint32_t *c2b = lds_u32 + a.cap_block;
+    uint32_t *zero_row = c2b + a.max_block_cols;
     const int tid = threadIdx.x;
-    for (int i = tid; i < a.cap_block + a.max_block_cols; i += kBlockThreads) lds_u32[i] = 0;
+    for (int i = tid; i < a.cap_block + a.max_block_cols + kWave; i += kBlockThreads)
+        lds_u32[i] = 0;
     __syncthreads();
     const int L = (int)(a.T - a.tau);
     const float2 *tbl = reinterpret_cast<const float2 *>(a.xlogx);
@@ -221,11 +259,11 @@ __global__ __launch_bounds__(kBlockThreads) void te_block_kernel(tnx_te_args_t a
                 int64_t slot;
                 const bool want = (x != y) && out_slot(a, x, y, slot);
                 const bool fits = bx * by <= a.max_block_cols;
-                if (want && !fits && tid == 0) a.out[slot] = __builtin_nanf("");  // plan mismatch
+                if (want && !fits && tid == 0) emit_te(a, slot, __builtin_nan(""));  // plan mismatch
                 if (want && fits) {
-                    const DF d = hist_pair_block<NB>(hist, c2b, a.cap_block, regs, ycur, bx, by, L,
-                                                     tbl, red);
-                    if (tid == 0) a.out[slot] = (float)((df_to_double(d) + s12) * invN);
+                    const DF d = hist_pair_block<NB, HZ>(hist, c2b, zero_row, a.cap_block, regs, ycur,
+                                                     bx, by, L, tbl, red);
+                    if (tid == 0) emit_te(a, slot, (df_to_double(d) + s12) * invN);
                 }
                 y = yn;
                 by = byn;
@@ -250,8 +288,9 @@ __global__ __launch_bounds__(256) void te_counts_kernel(
     extern __shared__ uint32_t lds_u32[];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    uint32_t *hist = lds_u32 + wave * cap;
-    for (int i = lane; i < cap; i += kWave) hist[i] = 0;
+    uint32_t *hist = lds_u32 + wave * (cap + kWave);
+    uint32_t *zero_row = hist + cap;
+    for (int i = lane; i < cap + kWave; i += kWave) hist[i] = 0;
     wave_lds_fence();
     const int L = (int)(T - tau);
     const float2 *tbl = reinterpret_cast<const float2 *>(xlogx);
@@ -270,12 +309,14 @@ __global__ __launch_bounds__(256) void te_counts_kernel(
     if constexpr (NI > 0) {
         TargetRegs<NI> regs;
         load_target_regs<NI>(regs, states + (int64_t)x * lds, L, lane);
-        hist_accumulate_regs<NI>(hc, regs, bins + (int64_t)y * ld, kmul, ysh, L, lane);
+        hist_accumulate_regs<NI, true>(hc, regs, bins + (int64_t)y * ld, kmul, ysh, zero_row + lane,
+                                       L, lane);
     } else {
-        hist_accumulate_mem(hc, states + (int64_t)x * lds, bins + (int64_t)y * ld, kmul, ysh, L,
-                            lane);
+        hist_accumulate_mem<true>(hc, states + (int64_t)x * lds, bins + (int64_t)y * ld, kmul,
+                                  ysh, zero_row + lane, L, lane);
     }
     wave_lds_fence();
+    fold_zero_key<true>(hist, zero_row, lane);
     for (int i = lane; i < cells; i += kWave) {
         uint32_t v = 0;
         for (int r = 0; r < cp.G; r++) v += hist[(i << cp.gsh) + r];
@@ -298,6 +339,69 @@ static int wave_variant(int64_t L) {
 }  // namespace tnx
 
 using namespace tnx;
+
+// One thread per slot, copies in order: the additions of K statistics runs.
+__global__ __launch_bounds__(256) void stats_accumulate_kernel(
+    const float *__restrict__ stage, int K, int64_t stride, const int64_t *__restrict__ slots,
+    int64_t n_slots, const float *__restrict__ observed, double *__restrict__ sum,
+    double *__restrict__ sumsq, int32_t *__restrict__ count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_slots) return;
+    const int64_t sl = slots[i];
+    const float ob = observed[sl];
+    double s = sum[sl], s2 = sumsq[sl];
+    int32_t c = count[sl];
+    for (int k = 0; k < K; k++) {
+        const float tf = stage[(int64_t)k * stride + sl];
+        const double td = (double)tf;
+        s += td;
+        s2 += td * td;
+        c += (tf >= ob) ? 1 : 0;
+    }
+    sum[sl] = s;
+    sumsq[sl] = s2;
+    count[sl] = c;
+}
+
+template <bool HZ>
+static int launch_wave(int ni, unsigned grid, size_t shm, hipStream_t s, const tnx_te_args_t &a) {
+    const void *kfn = (const void *)te_wave_kernel<HZ>;
+    switch (ni) {
+#define TNX_WAVE_FN(NI) \
+    case NI: kfn = (const void *)te_wave_reg_kernel<NI, HZ>; break;
+        TNX_WAVE_FN(1) TNX_WAVE_FN(2) TNX_WAVE_FN(3) TNX_WAVE_FN(4) TNX_WAVE_FN(6)
+        TNX_WAVE_FN(8) TNX_WAVE_FN(10) TNX_WAVE_FN(12)
+#undef TNX_WAVE_FN
+        default: break;
+    }
+    TNX_CHECK_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    switch (ni) {
+#define TNX_WAVE_LAUNCH(NI) \
+    case NI: hipLaunchKernelGGL((te_wave_reg_kernel<NI, HZ>), dim3(grid), dim3(256), shm, s, a); break;
+        TNX_WAVE_LAUNCH(1) TNX_WAVE_LAUNCH(2) TNX_WAVE_LAUNCH(3) TNX_WAVE_LAUNCH(4)
+        TNX_WAVE_LAUNCH(6) TNX_WAVE_LAUNCH(8) TNX_WAVE_LAUNCH(10) TNX_WAVE_LAUNCH(12)
+#undef TNX_WAVE_LAUNCH
+        default: hipLaunchKernelGGL(te_wave_kernel<HZ>, dim3(grid), dim3(256), shm, s, a); break;
+    }
+    return TNX_OK;
+}
+
+template <bool HZ>
+static int launch_block(int nbi, unsigned grid, size_t shm, hipStream_t s,
+                         const tnx_te_args_t &a) {
+    switch (nbi) {
+#define TNX_BLOCK(NB)                                                                          \
+    case NB:                                                                                   \
+        TNX_CHECK_HIP(hipFuncSetAttribute((const void *)te_block_kernel<NB, HZ>,               \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm)); \
+        hipLaunchKernelGGL((te_block_kernel<NB, HZ>), dim3(grid), dim3(kBlockThreads), shm, s, a); \
+        break;
+        TNX_BLOCK(1) TNX_BLOCK(2) TNX_BLOCK(4) TNX_BLOCK(8) TNX_BLOCK(16)
+#undef TNX_BLOCK
+        default: break;
+    }
+    return TNX_OK;
+}
 
 extern "C" {
 
@@ -347,21 +451,30 @@ int tnx_te_plan(const int32_t *npv, int64_t n_vars, int64_t L, const int32_t *tg
                   "pair's b_t*b_s = %d", cap_block, maxcols);
     std::vector<tnx_work_t> wv, bk;
     double cw = 0, cb = 0;
+    int64_t pw = 0;
+    for (int64_t i = 0; i < nt; i++) pw += kw[i];
+    const int64_t pb = nt * ns - pw;
+    // pairs per item: as many as kWaveItemPairs / kBlockItemPairs, fewer when
+    // the plan is too small to fill the chip otherwise
+    const int64_t wave_pairs = std::clamp<int64_t>((pw + kFillWaveItems - 1) / kFillWaveItems,
+                                                   kWavesPerWG, kWaveItemPairs);
+    const int64_t block_pairs = std::clamp<int64_t>((pb + kFillBlockItems - 1) / kFillBlockItems,
+                                                    1, kBlockItemPairs);
     for (int64_t g0 = 0; g0 < nt; g0 += kWavesPerWG) {
         int64_t g1 = std::min<int64_t>(nt, g0 + kWavesPerWG);
         int32_t cnt = (int32_t)(g1 - g0);
         int64_t kmin = ns;
         for (int64_t i = g0; i < g1; i++) kmin = std::min(kmin, kw[i]);
-        int64_t ch = kWaveItemPairs / cnt;
+        int64_t ch = std::max<int64_t>(1, wave_pairs / cnt);
         for (int64_t j = 0; j < kmin; j += ch)
             wv.push_back({(int32_t)g0, cnt, (int32_t)j, (int32_t)std::min(kmin, j + ch)});
         for (int64_t i = g0; i < g1; i++) {
             int64_t bx = npv[tg[i]];
-            for (int64_t j = kmin; j < kw[i]; j += kWaveItemPairs)
-                wv.push_back({(int32_t)i, 1, (int32_t)j, (int32_t)std::min(kw[i], j + kWaveItemPairs)});
+            for (int64_t j = kmin; j < kw[i]; j += wave_pairs)
+                wv.push_back({(int32_t)i, 1, (int32_t)j, (int32_t)std::min(kw[i], j + wave_pairs)});
             for (int64_t j = 0; j < kw[i]; j++) cw += (double)L + (double)(bx * bx * sb[j]);
-            for (int64_t j = kw[i]; j < ns; j += kBlockItemPairs)
-                bk.push_back({(int32_t)i, 1, (int32_t)j, (int32_t)std::min<int64_t>(ns, j + kBlockItemPairs)});
+            for (int64_t j = kw[i]; j < ns; j += block_pairs)
+                bk.push_back({(int32_t)i, 1, (int32_t)j, (int32_t)std::min<int64_t>(ns, j + block_pairs)});
             for (int64_t j = kw[i]; j < ns; j++) {
                 int64_t cols = bx * sb[j];
                 int64_t passes = (bx * cols + cap_block - 1) / cap_block;
@@ -416,41 +529,37 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
     TNX_CHECK_ARG(args != nullptr, "tnx_te_run: args is NULL");
     const tnx_te_args_t &a = *args;
     TNX_CHECK_ARG(a.bins && a.states && a.n_per_var && a.S1 && a.S2a && a.xlogx && a.tgt_order &&
-                      a.src_order && a.out,
+                      a.src_order,
                   "tnx_te_run: null pointer");
+    TNX_CHECK_ARG(a.out_mode == TNX_OUT_STATS
+                      ? (a.sum_te && a.sumsq_te && a.count_ge && a.observed && a.ld_out >= a.n_vars)
+                      : (a.out != nullptr),
+                  "tnx_te_run: null output pointer for out_mode %d", a.out_mode);
     TNX_CHECK_ARG(a.tau >= 1 && a.tau < a.T, "tnx_te_run: need 1 <= tau < T");
     TNX_CHECK_ARG(a.ld >= a.T && a.ld % 16 == 0, "tnx_te_run: ld must be >= T and a multiple of 16");
     TNX_CHECK_ARG(a.lds >= a.T - a.tau && a.lds % 8 == 0,
                   "tnx_te_run: lds must be >= T - tau and a multiple of 8");
     TNX_CHECK_ARG(a.T - a.tau < (1LL << 31), "tnx_te_run: L too large");
-    TNX_CHECK_ARG(a.out_mode == TNX_OUT_MATRIX || a.out_mode == TNX_OUT_PAIRFREE,
+    TNX_CHECK_ARG(a.out_mode == TNX_OUT_MATRIX || a.out_mode == TNX_OUT_PAIRFREE ||
+                      a.out_mode == TNX_OUT_STATS || a.out_mode == TNX_OUT_STAGE,
                   "tnx_te_run: bad out_mode");
+    TNX_CHECK_ARG(a.out_mode != TNX_OUT_STAGE ||
+                      (a.stage_vars >= 1 && a.ld_out >= a.stage_vars &&
+                       a.stage_stride >= (int64_t)a.stage_vars * a.ld_out),
+                  "tnx_te_run: TNX_OUT_STAGE needs stage_vars >= 1, ld_out >= stage_vars and "
+                  "stage_stride >= stage_vars * ld_out");
     TNX_CHECK_ARG(a.cap_wave >= 8 && a.cap_wave <= 8192, "tnx_te_run: cap_wave out of range");
-    TNX_CHECK_ARG((size_t)kWavesPerWG * a.cap_wave * 4 <= 160 * 1024, "tnx_te_run: cap_wave too large");
+    TNX_CHECK_ARG((size_t)kWavesPerWG * (a.cap_wave + kWave) * 4 <= 160 * 1024,
+                  "tnx_te_run: cap_wave too large");
     hipStream_t s = as_stream(stream);
     if (a.n_wave_items > 0) {
         TNX_CHECK_ARG(a.work, "tnx_te_run: null work");
-        size_t shm = (size_t)kWavesPerWG * a.cap_wave * 4;
+        size_t shm = (size_t)kWavesPerWG * (a.cap_wave + kWave) * 4;
         unsigned grid = (unsigned)std::min<int64_t>(a.n_wave_items, 1 << 20);
         const int ni = wave_variant(a.T - a.tau);
-        const void *kfn = (const void *)te_wave_kernel;
-        switch (ni) {
-#define TNX_WAVE_FN(NI) \
-    case NI: kfn = (const void *)te_wave_reg_kernel<NI>; break;
-            TNX_WAVE_FN(1) TNX_WAVE_FN(2) TNX_WAVE_FN(3) TNX_WAVE_FN(4) TNX_WAVE_FN(6)
-            TNX_WAVE_FN(8) TNX_WAVE_FN(10) TNX_WAVE_FN(12)
-#undef TNX_WAVE_FN
-            default: break;
-        }
-        TNX_CHECK_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        switch (ni) {
-#define TNX_WAVE_LAUNCH(NI) \
-    case NI: hipLaunchKernelGGL(te_wave_reg_kernel<NI>, dim3(grid), dim3(256), shm, s, a); break;
-            TNX_WAVE_LAUNCH(1) TNX_WAVE_LAUNCH(2) TNX_WAVE_LAUNCH(3) TNX_WAVE_LAUNCH(4)
-            TNX_WAVE_LAUNCH(6) TNX_WAVE_LAUNCH(8) TNX_WAVE_LAUNCH(10) TNX_WAVE_LAUNCH(12)
-#undef TNX_WAVE_LAUNCH
-            default: hipLaunchKernelGGL(te_wave_kernel, dim3(grid), dim3(256), shm, s, a); break;
-        }
+        const int rc = a.hot_zero ? launch_wave<true>(ni, grid, shm, s, a)
+                                  : launch_wave<false>(ni, grid, shm, s, a);
+        if (rc != TNX_OK) return rc;
         TNX_CHECK_LAUNCH("te_wave_kernel");
     }
     if (a.n_block_items > 0) {
@@ -460,27 +569,17 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
         TNX_CHECK_ARG(a.cap_block >= a.max_block_cols && a.cap_block % 4 == 0,
                       "tnx_te_run: cap_block %d must be a multiple of 4 and >= max_block_cols %d",
                       a.cap_block, a.max_block_cols);
-        size_t shm = (size_t)(a.cap_block + a.max_block_cols) * 4;
+        size_t shm = (size_t)(a.cap_block + a.max_block_cols + kWave) * 4;
         TNX_CHECK_ARG(shm <= 160 * 1024, "tnx_te_run: block kernel LDS %zu exceeds 160 KiB", shm);
         unsigned grid = (unsigned)std::min<int64_t>(a.n_block_items, 1 << 20);
         const int64_t L = a.T - a.tau;
         const int nb = (int)((L + 16 * kBlockThreads - 1) / (16 * kBlockThreads));
         TNX_CHECK_ARG(nb <= 16, "tnx_te_run: L = %lld too long for the block kernel (max %d)",
                       (long long)L, 16 * 16 * kBlockThreads);
-        const void *kfn = nullptr;
         const int nbi = nb <= 1 ? 1 : nb <= 2 ? 2 : nb <= 4 ? 4 : nb <= 8 ? 8 : 16;
-        switch (nbi) {
-#define TNX_BLOCK(NB)                                                                     \
-    case NB:                                                                              \
-        kfn = (const void *)te_block_kernel<NB>;                                          \
-        TNX_CHECK_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                          (int)shm));                                     \
-        hipLaunchKernelGGL(te_block_kernel<NB>, dim3(grid), dim3(kBlockThreads), shm, s, a); \
-        break;
-            TNX_BLOCK(1) TNX_BLOCK(2) TNX_BLOCK(4) TNX_BLOCK(8) TNX_BLOCK(16)
-#undef TNX_BLOCK
-            default: break;
-        }
+        const int rc = a.hot_zero ? launch_block<true>(nbi, grid, shm, s, a)
+                                  : launch_block<false>(nbi, grid, shm, s, a);
+        if (rc != TNX_OK) return rc;
         TNX_CHECK_LAUNCH("te_block_kernel");
     }
     return TNX_OK;
@@ -498,7 +597,7 @@ int tnx_te_counts_debug(const int8_t *bins, const uint16_t *states, const int32_
     TNX_CHECK_ARG(ld % 16 == 0 && lds % 8 == 0 && tau >= 1 && tau < T,
                   "tnx_te_counts_debug: bad strides / tau");
     if (n_pairs == 0) return TNX_OK;
-    size_t shm = (size_t)kWavesPerWG * max_cells * 4;
+    size_t shm = (size_t)kWavesPerWG * (max_cells + kWave) * 4;
     unsigned grid = (unsigned)((n_pairs + kWavesPerWG - 1) / kWavesPerWG);
     const int ni = wave_variant(T - tau);
     switch (ni) {
@@ -516,6 +615,22 @@ int tnx_te_counts_debug(const int8_t *bins, const uint16_t *states, const int32_
         default: break;
     }
     TNX_CHECK_LAUNCH("te_counts_kernel");
+    return TNX_OK;
+}
+
+int tnx_stats_accumulate(const float *stage, int32_t K, int64_t stage_stride,
+                         const int64_t *slots, int64_t n_slots, const float *observed,
+                         double *sum_te, double *sumsq_te, int32_t *count_ge, void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(K >= 0 && n_slots >= 0 && stage_stride >= 1, "tnx_stats_accumulate: bad sizes");
+    if (K == 0 || n_slots == 0) return TNX_OK;
+    TNX_CHECK_ARG(stage && slots && observed && sum_te && sumsq_te && count_ge,
+                  "tnx_stats_accumulate: null pointer");
+    const unsigned grid = (unsigned)((n_slots + 255) / 256);
+    hipLaunchKernelGGL(stats_accumulate_kernel, dim3(grid), dim3(256), 0, as_stream(stream), stage,
+                       (int)K, stage_stride, slots, n_slots, observed, sum_te, sumsq_te,
+                       count_ge);
+    TNX_CHECK_LAUNCH("stats_accumulate_kernel");
     return TNX_OK;
 }
 

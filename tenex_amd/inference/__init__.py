@@ -86,6 +86,24 @@ def bh_fdr_gpu(pval: torch.Tensor, alpha: float) -> torch.Tensor:
     return result
 
 
+def bh_reject_gpu(pval: torch.Tensor, alpha: float) -> torch.Tensor:
+    """bh_fdr_gpu(pval, alpha) < alpha without the reverse running minimum:
+    with q_(i) = min(p_(i) m / i, 1) in the p-value dtype (the same
+    arithmetic), adj_(i) = min_{j >= i} q_(j) < alpha exactly when some
+    j >= i has q_(j) < alpha, i.e. i <= k, the last rank with q < alpha."""
+    m = pval.shape[0]
+    if m == 0:
+        return torch.zeros(0, dtype=torch.bool, device=pval.device)
+    sorted_pval, sort_idx = pval.sort()
+    rank = torch.arange(1, m + 1, device=pval.device, dtype=pval.dtype)
+    q = (sorted_pval * m / rank).clamp(max=1.0)
+    pos = torch.arange(1, m + 1, device=pval.device)
+    k = torch.where(q < alpha, pos, torch.zeros_like(pos)).max()
+    result = torch.empty(m, dtype=torch.bool, device=pval.device)
+    result[sort_idx] = pos <= k
+    return result
+
+
 def bh_adjust_np(p: np.ndarray) -> np.ndarray:
     """BH adjusted p-values in float64 (the statsmodels fdr_bh arithmetic the
     reference's CPU path calls, tenex/inference/__init__.py:150)."""

@@ -5,11 +5,15 @@ each pair's surrogate-null mean and standard deviation of TE from per-gene
 marginal features (level count, H(X_t), H(X_t+1 | X_t), zero and modal
 fractions, ...; tnx_gene_prep).  It is trained on the GPU by the project's
 own forward / backward / Adam kernels on plain surrogate statistics of a
-random subset of pairs.  Pairs whose predicted z lies within `band` of the
-BH threshold are re-run with the plain surrogate test, which stays both the
-fallback and the judge; every other pair keeps its predicted null.  The band
-is max(band, band_sigmas * RMS z error on held-out plain pairs): 1.0 when the
-model is accurate, wider when it is not.
+random grid of targets x sources (only the grid's variables are shuffled
+and re-prepared).  Pairs whose predicted z lies within `band` of the BH
+threshold are re-run with the plain surrogate test, which stays both the
+fallback and the judge; every other pair keeps its predicted null.  The
+band is max(band, band_sigmas * RMS z error on the grid's held-out
+targets): 1.0 when the model is accurate, wider when it is not.  The
+prediction errors have heavier tails than a Gaussian (a few pairs in 10^4
+miss by more than 4 RMS), hence five RMS by default; one missed edge near
+the threshold also shifts the BH cut for the edges next to it.
 
 Because surrogate k of a variable depends only on (seed, k, variable), the
 plain statistics of a re-run pair equal those of the full plain test.
@@ -118,35 +122,30 @@ class NullMLP:
              self.n_params, float(lr), float(b1), float(b2), float(eps), self.t, self._s())
 
     def fit(self, X, T, steps=300, lr=3e-3, lr_min=1e-5, batch=65536, seed=0):
-        """Adam with a cosine learning-rate schedule from lr down to lr_min.
-        Up to `batch` rows train full batch in one native call
-        (tnx_mlp_fit: no host round trip per step); larger sets draw a
-        device-side minibatch per step.  Returns a few of the step losses."""
+        """Adam with a cosine learning-rate schedule from lr down to lr_min,
+        in one native call (tnx_mlp_fit: no host round trip per step).  Up
+        to `batch` rows train full batch; larger sets are shuffled once
+        (seeded) and each step takes the next `batch` rows of the shuffled
+        order.  Returns a few of the step losses."""
         n = X.shape[0]
         X = X.contiguous().to(torch.float32)
         T = T.contiguous().to(torch.float32)
+        rows = n
+        if n > batch:
+            g = torch.Generator(device=self.dev).manual_seed(seed)
+            order = torch.randperm(n, generator=g, device=self.dev)
+            X, T = X[order].contiguous(), T[order].contiguous()
+            rows = batch
         grad = torch.empty(self.n_params, dtype=torch.float32, device=self.dev)
         hist = torch.zeros(max(1, steps), dtype=torch.float32, device=self.dev)
         m = self.struct()
-        st = self._s()
-        if n <= batch:
-            ws = self.workspace(n)
-            opt = _lib.Adam(lr=lr, lr_min=lr_min, beta1=0.9, beta2=0.999, eps=1e-8, steps=steps,
-                            t0=self.t)
-            call("tnx_mlp_fit", C.byref(m), ptr(self.master), ptr(self.m1), ptr(self.m2),
-                 self.n_params, ptr(X), ptr(T), n, C.byref(opt), ptr(grad), ptr(ws),
-                 ws.numel() * 4, ptr(hist), st)
-            self.t += steps
-        else:
-            ws = self.workspace(batch)
-            g = torch.Generator(device=self.dev).manual_seed(seed)
-            for s in range(steps):
-                idx = torch.randint(0, n, (batch,), generator=g, device=self.dev)
-                xb, tb = X[idx].contiguous(), T[idx].contiguous()
-                call("tnx_mlp_backward", C.byref(m), ptr(xb), ptr(tb), None, batch, ptr(grad),
-                     ptr(hist[s:s + 1]), ptr(ws), ws.numel() * 4, st)
-                self.adam_step(grad, lr=lr_min + (lr - lr_min) * 0.5 *
-                               (1 + math.cos(math.pi * s / steps)))
+        ws = self.workspace(rows)
+        opt = _lib.Adam(lr=lr, lr_min=lr_min, beta1=0.9, beta2=0.999, eps=1e-8, steps=steps,
+                        t0=self.t, batch=rows if rows < n else 0)
+        call("tnx_mlp_fit", C.byref(m), ptr(self.master), ptr(self.m1), ptr(self.m2),
+             self.n_params, ptr(X), ptr(T), n, C.byref(opt), ptr(grad), ptr(ws),
+             ws.numel() * 4, ptr(hist), self._s())
+        self.t += steps
         h = hist.cpu().numpy()
         return [float(v) for v in h[:: max(1, steps // 8)]] + [float(h[-1])]
 
@@ -155,8 +154,8 @@ def pair_features(prep: core.Prepared, tgt: np.ndarray, src: np.ndarray) -> torc
     """Features of pairs: per-gene marginals of source and target plus the
     log of the plug-in bias scale b_t (b_t - 1)(b_s - 1) / (2 L)."""
     dev = prep.device
-    t = torch.as_tensor(tgt, dtype=torch.int64).to(dev)
-    s = torch.as_tensor(src, dtype=torch.int64).to(dev)
+    t = tgt.long() if isinstance(tgt, torch.Tensor) else core.h2d(np.asarray(tgt, np.int64), dev)
+    s = src.long() if isinstance(src, torch.Tensor) else core.h2d(np.asarray(src, np.int64), dev)
     f = prep.feat
     bt = f[t, 0]
     bs = f[s, 0]
@@ -165,8 +164,11 @@ def pair_features(prep: core.Prepared, tgt: np.ndarray, src: np.ndarray) -> torc
 
 
 def _norm_stats(X: torch.Tensor):
-    mu = X.mean(0)
-    sd = X.std(0).clamp(min=1e-6)
+    """Column mean and standard deviation; the columns are made rows first
+    (a column reduction of a narrow matrix runs on few lanes)."""
+    Xt = X.t().contiguous()
+    mu = Xt.mean(1)
+    sd = Xt.std(1).clamp(min=1e-6)
     return mu, sd
 
 
@@ -174,21 +176,55 @@ def plain_stats_pairs(prep, pairs, obs_pub, n_surr, shuffle_method, block_length
                       rng_device, perm_batch=32):
     """Plain surrogate (sum, sumsq, count) of an explicit [target, source]
     pair list (host int array (P, 2)), as device (P,) tensors (float64,
-    float64, int32)."""
-    n, T = prep.n_vars, prep.T
+    float64, int32): only the variables the pairs touch are shuffled."""
+    pairs = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
+    involved = np.unique(pairs)
+    loc = np.searchsorted(involved, pairs)
+    sub = core.var_subset(prep, involved)
+    plan = core.te_pairlist_plan(sub.npv_host, loc, prep.device)
+    s, s2, cg = core.surrogate_stats_subset(prep, involved, plan, obs_pub, n_surr,
+                                            shuffle_method, block_length, seed, rng_device,
+                                            perm_batch=perm_batch)
+    ld = core.h2d(loc, prep.device)
+    return s[ld[:, 1], ld[:, 0]], s2[ld[:, 1], ld[:, 0]], cg[ld[:, 1], ld[:, 0]]
+
+
+def plain_stats_grid(prep, targets, sources, obs_pub, n_surr, shuffle_method, block_length, seed,
+                     rng_device, perm_batch=32):
+    """Plain surrogate (sum, sumsq) of every pair of targets x sources, as
+    device (n, n) public-layout tensors (other slots zero), through the TE
+    kernels on block-shuffled copies of the grid's variables."""
+    n = prep.n_vars
     dev = prep.device
-    cap = core.surrogate_cap(prep.npv_host, pairs[:, 0], pairs[:, 1])
-    plan = core.pairlist_plan(prep.npv_host, pairs, dev, cap)
+    tg = np.asarray(targets, dtype=np.int64)
+    sr = np.asarray(sources, dtype=np.int64)
+    involved = np.union1d(tg, sr)
+    sub = core.var_subset(prep, involved)
+    plan = core.get_plan(sub, np.searchsorted(involved, tg).astype(np.int32),
+                         np.searchsorted(involved, sr).astype(np.int32))
+    ss, ss2, _ = core.surrogate_stats_subset(prep, involved, plan, obs_pub, n_surr,
+                                             shuffle_method, block_length, seed, rng_device,
+                                             perm_batch=perm_batch)
     s = torch.zeros((n, n), dtype=torch.float64, device=dev)
     s2 = torch.zeros_like(s)
-    cg = torch.zeros((n, n), dtype=torch.int32, device=dev)
-    for k0 in range(0, n_surr, perm_batch):
-        k1 = min(k0 + perm_batch, n_surr)
-        perm, eff, nb = core.block_perm_device(n, T, shuffle_method, block_length, k0, k1, seed,
-                                               rng_device, dev)
-        core.surrogate_run(prep, plan, perm, eff, obs_pub, s, s2, cg)
-    pd = torch.from_numpy(np.ascontiguousarray(pairs, dtype=np.int64)).to(dev)
-    return s[pd[:, 1], pd[:, 0]], s2[pd[:, 1], pd[:, 0]], cg[pd[:, 1], pd[:, 0]]
+    vd = sub.plans["rows"]
+    s[vd[:, None], vd[None, :]] = ss
+    s2[vd[:, None], vd[None, :]] = ss2
+    return s, s2
+
+
+def training_grid(pub: np.ndarray, n_train: int, rng):
+    """Targets and sources of the training grid for `pub` ([source, target]
+    tested pairs): all sources when they are few (<= 2 sqrt(n_train)), else
+    sqrt(n_train) of them, and enough targets for n_train pairs."""
+    srcs = np.unique(pub[:, 0])
+    tgts = np.unique(pub[:, 1])
+    root = max(1, int(math.sqrt(n_train)))
+    ns = len(srcs) if len(srcs) <= 2 * root else root
+    nt = min(len(tgts), max(2, -(-n_train // ns)))
+    gs = np.sort(rng.choice(srcs, size=ns, replace=False))
+    gt = rng.choice(tgts, size=nt, replace=False)
+    return gt, gs
 
 
 def _moments(s, s2, n_surr):
@@ -200,43 +236,59 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
                            shuffle_method, block_length, seed, rng_device, p_method, fdr, mask,
                            src_idx, dev, train_frac: float = 0.05, band: float = 1.0,
                            steps: int = 300, lr: float = 3e-3, min_train: int = 2000,
-                           model_seed: int = 0, band_sigmas: float = 4.0):
+                           model_seed: int = 0, band_sigmas: float = 5.0,
+                           fit_batch: int = 4096):
     """Learned-null surrogate test; returns a SurrogateTestResult with an
     extra `learned_info` dict (training size, re-run size, timings, losses).
     `mask` is the (n, n) bool tensor of tested [source, target] pairs."""
-    from . import bh_fdr_gpu
+    from . import bh_reject_gpu
     from .surrogate_test import SurrogateTestResult, surrogate_decision
     t0 = time.perf_counter()
     n = prep.n_vars
     mask = torch.as_tensor(mask, dtype=torch.bool, device=dev)
     pub_d = torch.nonzero(mask)                           # [source, target] of tested pairs
     if src_idx is not None:
-        pub_d = pub_d[torch.isin(pub_d[:, 0], torch.as_tensor(src_idx, device=dev).long())]
+        pub_d = pub_d[torch.isin(pub_d[:, 0], core.h2d(np.asarray(src_idx, np.int64), dev))]
     pub = pub_d.cpu().numpy()
     pairs = np.stack([pub[:, 1], pub[:, 0]], 1).astype(np.int64)   # [target, source]
     P = len(pairs)
     rng = np.random.default_rng(seed)
-    n_train = min(P, max(min_train, int(train_frac * P)))
-    tr = rng.choice(P, size=n_train, replace=False)
-    trd = torch.from_numpy(tr).to(dev)
-    # 1. plain surrogates on the training subset
-    s, s2, _ = plain_stats_pairs(prep, pairs[tr], obs_d, n_surrogates, shuffle_method,
+    n_want = min(P, max(min_train, int(train_frac * P)))
+    # 1. plain surrogates on a grid of targets x sources; a fifth of the grid's
+    #    targets is held out whole to measure how the model carries to
+    #    unseen targets
+    gt, gs = training_grid(pub, n_want, rng)
+    S_g, S2_g = plain_stats_grid(prep, gt, gs, obs_d, n_surrogates, shuffle_method,
                                  block_length, seed, rng_device)
-    mean_tr, std_tr = _moments(s, s2, n_surrogates)
+    n_ho = max(1, len(gt) // 5) if len(gt) > 1 else 0
+    ho_t = gt[:n_ho]
+    in_gs = torch.zeros(n, dtype=torch.bool, device=dev)
+    in_gs[core.h2d(gs.astype(np.int64), dev)] = True
+    in_gt = torch.zeros(n, dtype=torch.bool, device=dev)
+    in_gt[core.h2d(gt.astype(np.int64), dev)] = True
+    in_ho = torch.zeros(n, dtype=torch.bool, device=dev)
+    in_ho[core.h2d(ho_t.astype(np.int64), dev)] = True
+    grid_rows = in_gs[pub_d[:, 0]] & in_gt[pub_d[:, 1]]       # tested pairs on the grid
+    trd = torch.nonzero(grid_rows).flatten()
+    n_train = int(trd.numel())
+    fit_rows = trd[~in_ho[pub_d[trd, 1]]]
+    ho_rows = trd[in_ho[pub_d[trd, 1]]]
+    mean_g, std_g = _moments(S_g, S2_g, n_surrogates)
+    mean_tr, std_tr = mean_g[pub_d[trd, 0], pub_d[trd, 1]], std_g[pub_d[trd, 0], pub_d[trd, 1]]
     torch.cuda.synchronize(dev)
     t_train_plain = time.perf_counter() - t0
-    # 2. train the MLP on (mean, log std); a fifth of the plain pairs is held
-    #    out to measure the model's z error
+    # 2. train the MLP on (mean, log std)
     X_all = pair_features(prep, pub_d[:, 1], pub_d[:, 0])
     mu, sd = _norm_stats(X_all)
     Xn = ((X_all - mu) / sd).contiguous()
     Y = torch.stack([mean_tr, torch.log(torch.clamp(std_tr, min=1e-9))], 1)
     ymu = Y.mean(0)
     ysd = torch.clamp(Y.std(0, unbiased=False), min=1e-9)
-    Yn = ((Y - ymu) / ysd).to(torch.float32)
-    n_fit = max(1, int(0.8 * n_train))
+    Yg = torch.zeros((P, 2), dtype=torch.float64, device=dev)
+    Yg[trd] = (Y - ymu) / ysd
     model = NullMLP(Xn.shape[1], 2, dev, seed=model_seed)
-    losses = model.fit(Xn[trd[:n_fit]], Yn[:n_fit], steps=steps, lr=lr, seed=seed)
+    losses = model.fit(Xn[fit_rows], Yg[fit_rows].to(torch.float32), steps=steps, lr=lr,
+                       batch=fit_batch, seed=seed)
     t_fit = time.perf_counter() - t0 - t_train_plain
     # 3. predicted null for every pair, plain statistics where measured
     Yp = model.forward(Xn).to(torch.float64) * ysd + ymu
@@ -245,10 +297,10 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     obs = obs_d[pub_d[:, 0], pub_d[:, 1]].to(torch.float64)
     zf = lambda m, sdv: (obs - m) / torch.clamp(sdv, min=1e-12)
     z = zf(mean_p, std_p)
-    if n_train > n_fit:
-        ho = trd[n_fit:]
-        z_true = (obs[ho] - mean_tr[n_fit:]) / torch.clamp(std_tr[n_fit:], min=1e-12)
-        z_err = float(torch.sqrt(torch.mean((z[ho] - z_true) ** 2)))
+    if ho_rows.numel():
+        hs, ht = pub_d[ho_rows, 0], pub_d[ho_rows, 1]
+        z_true = (obs[ho_rows] - mean_g[hs, ht]) / torch.clamp(std_g[hs, ht], min=1e-12)
+        z_err = float(torch.sqrt(torch.mean((z[ho_rows] - z_true) ** 2)))
     else:
         z_err = 0.0
     band = max(band, band_sigmas * z_err)
@@ -256,8 +308,7 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     std_p[trd] = std_tr
     z = zf(mean_p, std_p)
     p_pred = (0.5 * torch.special.erfc(z / math.sqrt(2.0))).to(torch.float32)
-    adj = bh_fdr_gpu(p_pred, alpha=fdr)
-    sig = (adj < fdr) & (obs - mean_p > 0)
+    sig = bh_reject_gpu(p_pred, fdr) & (obs - mean_p > 0)
     if bool(sig.any()):
         z_thr = float(z[sig].min())
     else:
@@ -266,11 +317,14 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     border[trd] = False
     rr_d = torch.nonzero(border).flatten()
     rr = rr_d.cpu().numpy()
+    t_predict = time.perf_counter() - t0 - t_train_plain - t_fit
     # 4. plain surrogates for the borderline pairs
     if len(rr):
         s_b, s2_b, _ = plain_stats_pairs(prep, pairs[rr], obs_d, n_surrogates, shuffle_method,
                                          block_length, seed, rng_device)
         mean_p[rr_d], std_p[rr_d] = _moments(s_b, s2_b, n_surrogates)
+    torch.cuda.synchronize(dev)
+    t_rerun = time.perf_counter() - t0 - t_train_plain - t_fit - t_predict
     # 5. decision with the reference arithmetic on the merged null
     S = torch.zeros((n, n), dtype=torch.float64, device=dev)
     S2 = torch.zeros_like(S)
@@ -292,6 +346,6 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     res.learned_info = dict(n_pairs=P, n_train=n_train, n_rerun=int(len(rr)), band=band,
                             z_err_holdout=z_err,
                             plain_fraction=(n_train + len(rr)) / max(P, 1), z_threshold=z_thr,
-                            t_train_plain=t_train_plain, t_fit=t_fit, t_total=t_total,
-                            losses=losses)
+                            t_train_plain=t_train_plain, t_fit=t_fit, t_predict=t_predict,
+                            t_rerun=t_rerun, t_total=t_total, losses=losses)
     return res

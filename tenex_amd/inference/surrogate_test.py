@@ -6,10 +6,18 @@ per-pair Gaussian z-test (parametric) or the Monte Carlo rank (mc) gives
 p-values; BH-FDR over the tested pairs and effective TE > 0 give the GRN.
 
 Paths:
-  fused (default)  the gfx950 kernel surrogate_wave_kernel computes every
-                   (pair, surrogate) histogram and accumulates FP64
-                   statistics on the device (the role of the reference's
-                   fused Full/Adaptive-SMEM surrogate kernels).
+  default          per surrogate, tnx_block_shuffle writes a block-shuffled
+                   copy of the bins to HBM, tnx_gene_prep rebuilds its
+                   states and entropies, and the TE kernels (tnx_te_run)
+                   add each pair's float32 TE_k to FP64 statistics in their
+                   epilogue (TNX_OUT_STATS).  When the tested pairs touch
+                   only some variables (targets / sources given), only those
+                   rows are shuffled and re-prepared, with the same block
+                   orders (rows of the full draw).
+  fused=True       the fused kernel surrogate_wave_kernel shuffles on the fly
+                   and counts every (pair, surrogate) histogram itself (the
+                   role of the reference's fused Full/Adaptive-SMEM
+                   surrogate kernels); it can also return every TE_k.
   fused=False      loop path: per surrogate, shuffle the bins on the device
                    (torch gather) and run the TE kernels on the shuffled data,
                    accumulating in FP64 (the reference's loop path,
@@ -33,7 +41,7 @@ import torch
 from .. import core
 from .._log import vprint
 from ..kernels import get_kernel
-from . import GRN, InferenceMethod, bh_fdr_gpu, build_pairs, make_grn
+from . import GRN, InferenceMethod, bh_fdr_gpu, bh_reject_gpu, build_pairs, make_grn
 
 
 _MATRIX_FIELDS = frozenset(("observed_te", "mean_surrogate_te", "std_surrogate_te",
@@ -87,17 +95,52 @@ def shuffle_bins(bins: torch.Tensor, T: int, perm_k: torch.Tensor, eff_bl: int) 
 
 def surrogate_accumulate(prep: core.Prepared, observed_pub: torch.Tensor, n_surrogates: int,
                          shuffle_method: str, block_length: int, seed: int, rng_device,
-                         fused: bool = True, targets=None, sources=None, perm_batch: int = 32,
-                         k_start: int = 0, te_k_dbg: torch.Tensor | None = None):
+                         fused: bool | None = None, targets=None, sources=None,
+                         perm_batch: int = 32, k_start: int = 0,
+                         te_k_dbg: torch.Tensor | None = None):
     """Device (n, n) public-layout sum, sumsq (float64) and count_ge (int32)
-    over surrogates [k_start, k_start + n_surrogates)."""
+    over surrogates [k_start, k_start + n_surrogates).
+
+    fused=None (default): per surrogate a block-shuffled copy of the bins in
+    HBM through the TE kernels with their statistics epilogue.  fused=True:
+    the fused surrogate kernel (shuffles on the fly; te_k_dbg available).
+    fused=False: the reference's loop (torch gather, TE matrix, torch sums).
+    The first two leave the diagonal untouched."""
     n, T = prep.n_vars, prep.T
     dev = prep.device
     s = torch.zeros((n, n), dtype=torch.float64, device=dev)
     s2 = torch.zeros((n, n), dtype=torch.float64, device=dev)
     cg = torch.zeros((n, n), dtype=torch.int32, device=dev)
     k_end = k_start + n_surrogates
-    if fused:
+    if fused is None and te_k_dbg is None:
+        every = np.arange(n, dtype=np.int64)
+        tg = every if targets is None else np.asarray(targets, dtype=np.int64)
+        sr = every if sources is None else np.asarray(sources, dtype=np.int64)
+        involved = np.union1d(tg, sr)
+        if len(involved) < n:
+            # shuffle and re-prepare only the variables the pairs touch
+            plan = core.get_plan(core.var_subset(prep, involved),
+                                 np.searchsorted(involved, tg).astype(np.int32),
+                                 np.searchsorted(involved, sr).astype(np.int32))
+            ss, ss2, scg = core.surrogate_stats_subset(prep, involved, plan, observed_pub,
+                                                       n_surrogates, shuffle_method,
+                                                       block_length, seed, rng_device, k_start,
+                                                       perm_batch)
+            vd = core.var_subset(prep, involved).plans["rows"]
+            s[vd[:, None], vd[None, :]] = ss
+            s2[vd[:, None], vd[None, :]] = ss2
+            cg[vd[:, None], vd[None, :]] = scg
+            return s, s2, cg
+        plan = core.get_plan(prep, targets, sources)
+        work = {}
+        for k0 in range(k_start, k_end, perm_batch):
+            k1 = min(k0 + perm_batch, k_end)
+            perm, eff_bl, nb = core.block_perm_device(n, T, shuffle_method, block_length, k0, k1,
+                                                      seed, rng_device, dev)
+            core.surrogate_stats_materialized(prep, plan, perm, eff_bl, observed_pub, s, s2, cg,
+                                              work)
+        return s, s2, cg
+    if fused is None or fused:
         plan = core.surrogate_plan(prep, targets, sources)
         for k0 in range(k_start, k_end, perm_batch):
             k1 = min(k0 + perm_batch, k_end)
@@ -154,9 +197,8 @@ def surrogate_decision(observed_te, sum_s, sumsq_s, count_ge, n_surr, p_method, 
         p = ((1.0 + cg) / (1.0 + n_surr)).to(torch.float32)
     p.fill_diagonal_(1.0)
     m = t(mask, torch.bool)
-    adj = bh_fdr_gpu(p[m], alpha=fdr)
     sig = torch.zeros_like(m)
-    sig[m] = adj < fdr
+    sig[m] = bh_reject_gpu(p[m], fdr)
     sig &= eff > 0
     out = (mean, std, eff, p, sig)
     return tuple(x.cpu().numpy() for x in out) if host else out
@@ -224,7 +266,7 @@ class SurrogateTestMethod(InferenceMethod):
         if sources is not None:
             # rows = the sources, every other variable a target (build_pairs)
             _, s_rows, _ = np.intersect1d(names_arr, sources, return_indices=True)
-            mask[torch.from_numpy(s_rows.astype(np.int64)).to(dev)] = True
+            mask[core.h2d(s_rows.astype(np.int64), dev)] = True
         else:
             mask.fill_(True)
         mask.fill_diagonal_(False)
@@ -233,7 +275,7 @@ class SurrogateTestMethod(InferenceMethod):
             # extension: restrict the tested pairs to these target names too
             _, tgt_idx, _ = np.intersect1d(names_arr, targets, return_indices=True)
             tmask = torch.zeros(n_vars, dtype=torch.bool, device=dev)
-            tmask[torch.from_numpy(tgt_idx.astype(np.int64)).to(dev)] = True
+            tmask[core.h2d(tgt_idx.astype(np.int64), dev)] = True
             mask &= tmask[None, :]
             tgt_idx = tgt_idx.astype(np.int32)
         src_idx = None
@@ -245,11 +287,11 @@ class SurrogateTestMethod(InferenceMethod):
                f"block_length={block_length}, kernel={kernel_name}, p={p_method}, "
                f"device={dev})")
         bins, npv_d, _, T_ = core.upload_bins(bin_data, dev)
-        npv_in = torch.as_tensor(np.asarray(n_per_var), dtype=torch.int32).to(dev)
+        npv_in = core.h2d(np.asarray(n_per_var, dtype=np.int32), dev)
         if npv_in.shape[0] == n_vars and bool((npv_in >= npv_d).all()):
             npv_d = npv_in.contiguous()
         prep = core.prepare(bins, npv_d, T, tau)
-        obs_d = torch.from_numpy(observed_te).to(dev)
+        obs_d = core.h2d(observed_te, dev)
         learned = kwargs.get("learned")
         if learned is not None:
             from .learned_null import learned_surrogate_test
@@ -257,12 +299,11 @@ class SurrogateTestMethod(InferenceMethod):
                 prep, observed_te, obs_d, variable_names, n_surrogates, shuffle_method,
                 block_length, seed, rng_device, p_method, fdr, mask, src_idx, dev,
                 **(kwargs.get("learned_kwargs") or {}))
-        fused = fused_kw is not False
         tg_plan = tgt_idx if (scope == "sources" and tgt_idx is not None) else None
         s, s2, cg = surrogate_accumulate(prep, obs_d, n_surrogates, shuffle_method, block_length,
-                                         seed, rng_device, fused=fused, targets=tg_plan,
+                                         seed, rng_device, fused=fused_kw, targets=tg_plan,
                                          sources=src_idx)
-        if fused:
+        if fused_kw is not False:
             cg.diagonal().add_(n_surrogates)  # the loop path's diagonal (te_k = obs = 0)
         mean, std, eff, p, sig = surrogate_decision(obs_d, s, s2, cg, n_surrogates, p_method,
                                                     fdr, mask, dev, host=False)

@@ -79,6 +79,32 @@ def test_native_fit_matches_stepwise_schedule(dev):
     assert torch.equal(a.master, b.master) and torch.equal(a.W2, b.W2)
 
 
+def test_native_minibatch_fit_matches_stepwise(dev):
+    """Minibatch steps of tnx_mlp_fit: rows shuffled once, step s takes the
+    (s mod nb)-th run of `batch` rows (the tail past nb * batch unused);
+    the same steps driven from Python give identical weights."""
+    import math
+    rng = np.random.default_rng(4)
+    X = torch.from_numpy(rng.standard_normal((2600, 9)).astype(np.float32)).to(dev)
+    T = torch.from_numpy(rng.standard_normal((2600, 2)).astype(np.float32)).to(dev)
+    lr, lr_min, steps, batch = 2.0 ** -9, 2.0 ** -17, 7, 1000
+    a = NullMLP(9, 2, dev, seed=6)
+    la = a.fit(X, T, steps=steps, lr=lr, lr_min=lr_min, batch=batch, seed=9)
+    g = torch.Generator(device=dev).manual_seed(9)
+    order = torch.randperm(2600, generator=g, device=dev)
+    Xs, Ts = X[order].contiguous(), T[order].contiguous()
+    b = NullMLP(9, 2, dev, seed=6)
+    lb = []
+    for s in range(steps):
+        r0 = (s % 2) * batch
+        loss, gr = b.loss_grad(Xs[r0:r0 + batch].contiguous(), Ts[r0:r0 + batch].contiguous())
+        lb.append(float(loss.item()))
+        rate = lr_min + (lr - lr_min) * 0.5 * (1 + math.cos(math.pi * s / steps))
+        b.adam_step(gr, lr=rate)
+    assert la[0] == lb[0] and la[-1] == lb[-1]
+    assert torch.equal(a.master, b.master)
+
+
 def test_mlp_adam_and_bf16_copies(dev):
     m = NullMLP(8, 2, dev, seed=1)
     g = torch.ones(m.n_params, device=dev) * 0.5

@@ -87,6 +87,104 @@ def test_pairlist_plan_single_target_items(dev):
     assert np.all(s.cpu().numpy().ravel()[mask] == 0)
 
 
+def test_block_shuffle_bit_exact(dev):
+    """tnx_block_shuffle against the shuffle schedule restated in numpy:
+    blocks reordered by perm, tail and padding unchanged."""
+    rng = np.random.default_rng(4)
+    for n, T, bl in [(7, 103, 10), (5, 64, 16), (9, 250, 1), (4, 99, 99), (6, 333, 7)]:
+        raw = rng.integers(0, 9, size=(n, T)).astype(np.int32)
+        bins, _, _, T_ = core.upload_bins(raw, dev)
+        nb = T // bl
+        perm = torch.stack([torch.randperm(nb, generator=torch.Generator().manual_seed(i))
+                            for i in range(n)]).to(torch.int32).to(dev)
+        out = torch.full_like(bins, -1)
+        core.block_shuffle(bins, T, perm, bl, out)
+        got = out.cpu().numpy()[:, :T]
+        p = perm.cpu().numpy()
+        exp = raw.copy()
+        head = nb * bl
+        for v in range(n):
+            exp[v, :head] = raw[v, :head].reshape(nb, bl)[p[v]].ravel()
+        assert np.array_equal(got, exp.astype(np.int8)), (n, T, bl)
+        assert np.all(out.cpu().numpy()[:, T:] == 0)
+
+
+@pytest.mark.parametrize("bl,tau", [(20, 1), (33, 2)])
+def test_materialized_stats_vs_oracle(dev, bl, tau):
+    """Default path: shuffled copies through the TE kernels' statistics
+    epilogue, against the oracle; a pair list with block-kernel pairs too."""
+    d, npv = _small(n=30, T=420, seed=2)
+    n, T = d.shape
+    prep = _prep(d, npv, dev, tau)
+    obs_int = core.te_matrix_internal(prep).cpu().numpy()
+    obs_pub = torch.from_numpy(obs_int.T.copy()).to(dev)
+    n_surr = 5
+    s, s2, cg = surrogate_accumulate(prep, obs_pub, n_surr, "block", bl, 11, dev)
+    perm, eff_bl, nb = core.block_perm_device(n, T, "block", bl, 0, n_surr, 11, dev, dev)
+    pairs = O.pairfree_pairs(n)
+    obs_pairs = obs_int[pairs[:, 0], pairs[:, 1]]
+    os_, os2, ocg = O.surrogate_stats(d, npv, perm.cpu().numpy(), eff_bl, pairs, obs_pairs,
+                                      tau=tau)
+    slot = pairs[:, 1] * n + pairs[:, 0]
+    assert np.abs(s.cpu().numpy().ravel()[slot] - os_).max() < n_surr * TOL
+    assert np.abs(s2.cpu().numpy().ravel()[slot] - os2).max() < n_surr * 2 * TOL
+    assert np.all(np.diag(s.cpu().numpy()) == 0)
+    # a pair list whose large pairs take the block kernel (tiny wave cap)
+    rng = np.random.default_rng(3)
+    sub = pairs[rng.choice(len(pairs), 120, replace=False)]
+    cells = npv[sub[:, 0]].astype(np.int64) ** 2 * npv[sub[:, 1]]
+    plan = core.te_pairlist_plan(prep.npv_host, sub, dev, cap_wave=int(np.median(cells)))
+    assert plan.info.n_block_items > 0 and plan.info.n_wave_items > 0
+    S = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    S2 = torch.zeros_like(S)
+    CG = torch.zeros((n, n), dtype=torch.int32, device=dev)
+    core.surrogate_stats_materialized(prep, plan, perm, eff_bl, obs_pub, S, S2, CG)
+    sl = sub[:, 1] * n + sub[:, 0]
+    ref = dict(zip(slot.tolist(), os_.tolist()))
+    assert max(abs(S.cpu().numpy().ravel()[k] - ref[k]) for k in sl.tolist()) < n_surr * TOL
+    mask = np.ones(n * n, bool)
+    mask[sl] = False
+    assert np.all(S.cpu().numpy().ravel()[mask] == 0)
+
+
+def test_subset_stats_equal_full(dev):
+    """Targets x sources touching a subset of the variables: only the subset
+    is shuffled (the same block orders as rows of the full draw), and every
+    pair's statistics equal the full computation's bit for bit."""
+    from tenex_amd.inference.learned_null import plain_stats_pairs
+    d, npv = _small(n=40, T=450, seed=6)
+    n, T = d.shape
+    prep = _prep(d, npv, dev)
+    obs = torch.from_numpy(core.te_matrix_internal(prep).cpu().numpy().T.copy()).to(dev)
+    full = surrogate_accumulate(prep, obs, 7, "block", 25, 5, dev)
+    tg = np.array([3, 9, 17, 30, 31], np.int64)
+    sr = np.array([0, 9, 22, 39], np.int64)
+    part = surrogate_accumulate(prep, obs, 7, "block", 25, 5, dev, targets=tg, sources=sr)
+    sel = np.zeros((n, n), bool)
+    sel[np.ix_(sr, tg)] = True           # public layout [source, target]
+    sel[np.eye(n, dtype=bool)] = False
+    for a, b in zip(full, part):
+        a, b = a.cpu().numpy(), b.cpu().numpy()
+        assert np.array_equal(a[sel], b[sel])
+        assert np.all(b[~sel] == 0)
+    pairs = np.array([[30, 0], [3, 22], [17, 39], [31, 9]], np.int64)   # [target, source]
+    s, s2, cg = plain_stats_pairs(prep, pairs, obs, 7, "block", 25, 5, dev)
+    for got, ref in zip((s, s2, cg), full):
+        assert np.array_equal(got.cpu().numpy(), ref.cpu().numpy()[pairs[:, 1], pairs[:, 0]])
+
+
+def test_materialized_matches_fused_and_loop(dev):
+    d, npv = _small(n=24, T=500, seed=5)
+    prep = _prep(d, npv, dev)
+    obs = torch.from_numpy(core.te_matrix_internal(prep).cpu().numpy().T.copy()).to(dev)
+    m = surrogate_accumulate(prep, obs, 8, "block", 22, 42, dev)
+    f = surrogate_accumulate(prep, obs, 8, "block", 22, 42, dev, fused=True)
+    off = ~np.eye(24, dtype=bool)
+    assert (np.abs(m[0].cpu().numpy() - f[0].cpu().numpy())[off] / 8).max() < 1e-8
+    near = np.abs(m[2].cpu().numpy() - f[2].cpu().numpy())[off]
+    assert near.max() <= 1  # float32 rounding of one te_k against observed at most
+
+
 def test_identity_permutation_gives_observed(dev):
     d, npv = _small(n=20, T=300, seed=3)
     n, T = d.shape

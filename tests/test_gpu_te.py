@@ -94,6 +94,34 @@ def test_te_mixed_cardinality_vs_oracle(dev, cap_wave):
     assert err.max() < TOL, err.max()
 
 
+@pytest.mark.parametrize("T,cap_wave", [(500, 0), (500, 64), (4100, 0), (4100, 512), (13000, 0), (13000, 512)])
+def test_hot_zero_key_diversion(dev, T, cap_wave):
+    """Zero-inflated data (key 0 holds about a quarter of the samples): the
+    kernels with key 0 diverted to per-lane words and the plain kernels give
+    the same TE bit for bit, within 1e-6 of the FP64 oracle, in the wave
+    (register and memory target) and the multi-pass block kernels."""
+    x = synthetic.zero_inflated_lognormal(60, T, seed=11)
+    d, nb, npv, bm = O.discretize(x)
+    prep = _prep_from_dense(d, dev)
+    assert prep.hot_zero, core.key0_fraction(prep.states, prep.bins, prep.L)
+    if cap_wave:
+        assert core.get_plan(prep, cap_wave=cap_wave).info.n_block_items > 0
+    m_hot = core.te_matrix_internal(prep, cap_wave=cap_wave).cpu().numpy()
+    prep.hot_zero = False
+    m_plain = core.te_matrix_internal(prep, cap_wave=cap_wave).cpu().numpy()
+    assert np.array_equal(m_hot, m_plain)
+    pairs = O.pairfree_pairs(60)
+    te64 = O.te_pairs(d.astype(np.int32), prep.npv_host, pairs)
+    err = np.abs(m_hot[pairs[:, 0], pairs[:, 1]].astype(np.float64) - te64)
+    assert err.max() < TOL, err.max()
+
+
+def test_hot_zero_flag_uniform_levels(dev):
+    """Near-uniform levels keep key 0 rare: the plain kernels are chosen."""
+    prep = _prep_from_dense(synthetic.mixed_cardinality_bins(50, 3000, seed=2), dev)
+    assert not prep.hot_zero
+
+
 def test_pairfree_ranges(dev):
     b = synthetic.mixed_cardinality_bins(40, 900, seed=9)
     prep = _prep_from_dense(b, dev)

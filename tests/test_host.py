@@ -72,6 +72,46 @@ def test_planner_sources_and_targets():
     assert {p[0] for p in wave + block} == {0, 1, 2}
 
 
+def test_planner_small_plans_split_items():
+    """Small plans get short items (enough to fill the chip); large ones keep
+    up to 256 pairs per wave item and 16 per block item."""
+    npv = synthetic_levels(300)
+    small, _, _, w_small = _plan_host(npv, 999, tg=np.arange(8, dtype=np.int32),
+                                      sr=np.arange(100, 160, dtype=np.int32))
+    pairs = lambda w: (w[:, 1] * (w[:, 3] - w[:, 2]))
+    assert pairs(w_small[:small.n_wave_items]).max() <= 4
+    big, _, _, w_big = _plan_host(npv, 999)
+    assert pairs(w_big[:big.n_wave_items]).max() > 4
+
+
+def test_plan_pairs_and_copies():
+    """plan_pairs enumerates the items' pairs; replicate_plan's K copies of
+    an m-variable plan cover the same pairs offset by k * m."""
+    npv = synthetic_levels(40)
+    src = np.array([1, 5, 9, 30], np.int32)
+    plan = core.make_plan(npv, 500, "cpu", sources=src, cap_wave=256)
+    assert plan.info.n_block_items > 0
+    x, y = core.plan_pairs(plan)
+    exp = {(t, s) for t in range(40) for s in src if s != t}
+    assert set(zip(x.tolist(), y.tolist())) == exp and len(x) == len(exp)
+    rep = core.replicate_plan(plan, 3, 40)
+    xr, yr = core.plan_pairs(rep)
+    assert rep.info.n_wave_items == 3 * plan.info.n_wave_items
+    assert rep.info.n_block_items == 3 * plan.info.n_block_items
+    got = set(zip(xr.tolist(), yr.tolist()))
+    assert got == {(t + 40 * k, s + 40 * k) for k in range(3) for t, s in exp}
+    # items of the copies stay wave-first, then block
+    w = rep.work_host
+    nw = rep.info.n_wave_items
+    cells = lambda it: [int(npv[rep.tgt_host[t] % 40]) ** 2 * int(npv[rep.src_host[s] % 40])
+                        for t in range(it[0], it[0] + it[1]) for s in range(it[2], it[3])]
+    assert all(max(cells(it)) <= 256 for it in w[:nw])
+    assert all(min(cells(it)) > 256 for it in w[nw:])
+    pl = core.te_pairlist_plan(npv, np.array([[3, 7], [3, 1], [12, 30], [0, 5]]), "cpu")
+    xp, yp = core.plan_pairs(pl)
+    assert set(zip(xp.tolist(), yp.tolist())) == {(3, 7), (3, 1), (12, 30), (0, 5)}
+
+
 def test_planner_rejects_bad_levels():
     with pytest.raises(ValueError):
         _plan_host(np.array([2, 0, 3], np.int32), 10)
@@ -229,3 +269,20 @@ def test_multi_lag_combine():
         exp = np.minimum(1.0, pv.min(0) * 3).astype(np.float32)
         np.fill_diagonal(exp, 1)
         assert np.array_equal(r.p_values, exp)
+
+
+def test_bh_reject_matches_adjusted_pvalues():
+    """bh_reject_gpu(p, a) is bh_fdr_gpu(p, a) < a (same float32 arithmetic),
+    ties, values at the threshold and p = 1 included."""
+    from tenex_amd.inference import bh_reject_gpu
+    rng = np.random.default_rng(7)
+    for m in (1, 2, 17, 1000, 40000):
+        p = rng.random(m).astype(np.float32) ** 3
+        p[rng.random(m) < 0.2] = np.float32(1.0)
+        if m > 10:
+            p[:5] = p[5]                      # ties
+            p[6] = np.float32(0.05 * 3 / m)   # near the step-up line
+        t = torch.from_numpy(p)
+        for a in (0.01, 0.05, 0.2):
+            assert torch.equal(bh_reject_gpu(t, a), bh_fdr_gpu(t, a) < a), (m, a)
+    assert bh_reject_gpu(torch.zeros(0), 0.05).numel() == 0
