@@ -12,6 +12,7 @@ Other workloads (their own JSON line, same fields):
   c1  2,000 x 5,000 binary (200 planted lagged pairs): GEMM-B2 on INT8 MFMA
   c3  1,000 x 3,000 plain surrogate test, 100 block shuffles (block 50)
   c4  6,000 x 3,000, 200 TF x 2,000 target pairs: learned vs plain null
+  dpi 4,000-gene FDR network (1% of ordered pairs): DPI trimming
 
 Prints one JSON line with the throughput, the roofline of the dominant
 kernel and the CPU baseline (the torch-CPU restatement of the reference's
@@ -390,12 +391,71 @@ def run_c4(args):
             "learned_info": info}
 
 
+# fp32 VALU issue: 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-instructions/s
+# (157.3 TFLOP/s counts a packed FMA as four flops).  One max-min update
+# (acc = max(acc, min(a, b))) takes 1.5 instructions: v_min plus half a v_max3.
+VALU_LANE_OPS = 256 * 64 * 2.4e9
+MAXMIN_PEAK = VALU_LANE_OPS / 1.5
+
+
+def run_dpi(args):
+    """DPI trimming of a 4,000-gene network holding 1% of the ordered pairs
+    (uniform weights): every row is a mediator, so the max-min product is
+    4,000 x 4,000 x 4,000."""
+    import torch
+    from tenex_amd import core
+    from tenex_amd.inference.fdr import _maxmin_keep, trim_dpi
+    dev = core.require_gpu("cuda:0")
+    n = 4000
+    rng = np.random.default_rng(args.seed)
+    A = np.where(rng.random((n, n)) < 0.01, rng.random((n, n)), 0).astype(np.float32)
+    np.fill_diagonal(A, 0)
+    p = np.argwhere(A != 0)
+    names = np.array([f"g{i}" for i in range(n)])
+    log(f"workload dpi: {n} genes, {len(p)} edges")
+
+    def step():
+        return trim_dpi(A, names, p, A[p[:, 0], p[:, 1]], dev)
+
+    dt = timed_steps(step, args.steps, args.warmup)
+    kept = step()
+    Ad = torch.from_numpy(A).to(dev)
+    out_probe = _maxmin_keep(Ad, Ad, Ad, 0.0, dev)  # noqa: F841 (warm)
+    t_k = time_launches(lambda: _maxmin_keep(Ad, Ad, Ad, 0.0, dev), max(3, args.steps))
+    upd = float(n) ** 3    # max-min updates per launch (every row a mediator)
+    result = {
+        "metric": "DPI trim seconds, 4000-gene FDR network (1% of pairs)",
+        "value": dt, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic random network (seeded)",
+        "config": {"workload": "DPI trim of the FDR network", "n_genes": n,
+                   "n_edges": int(len(p)), "kept": int(len(kept)), "seed": args.seed},
+        "roofline": {"bound": "valu", "kernel": "dpi_maxmin_kernel",
+                     "achieved": upd / t_k / 1e12, "peak": MAXMIN_PEAK / 1e12,
+                     "unit": "T max-min updates/s", "frac": upd / t_k / MAXMIN_PEAK,
+                     "traffic": None, "updates_per_launch": upd},
+        "kernels_ms": {"dpi_maxmin_kernel": t_k * 1e3},
+    }
+    if not args.no_cpu_baseline:
+        rows = 200
+        Ac = torch.from_numpy(A)
+        t0 = time.perf_counter()
+        _maxmin_keep(Ac[:rows].contiguous(), Ac, Ac[:rows].contiguous(), 0.0,
+                     torch.device("cpu"))
+        tc = (time.perf_counter() - t0) * n / rows
+        result["cpu_baseline"] = {"value": tc, "unit": "s", "cores": torch.get_num_threads(),
+                                  "kind": "port",
+                                  "sample": f"max-min product of the first {rows} rows on the "
+                                            f"torch CPU path, extrapolated x{n // rows}"}
+    return result
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3", "c4", "dpi"])
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=20480)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -409,7 +469,8 @@ def main():
     if args.traffic_probe:
         run_traffic_probe(args)
         return
-    fn = {"c0": run_c0, "c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4}[args.workload]
+    fn = {"c0": run_c0, "c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4,
+          "dpi": run_dpi}[args.workload]
     res = fn(args)
     print(json.dumps(res), flush=True)
 

@@ -277,6 +277,20 @@ typedef struct {
  * tenex/csrc/adaptive_smem_surrogate_test.cu:306-400). */
 TNX_API int tnx_surrogate_run(const tnx_surr_args_t *args, void *stream);
 
+/* ── FDR network: DPI trimming ───────────────────────────────────────── */
+
+/* Max-min product for the data-processing-inequality trim of the FDR
+ * network (replaces the mediator loop of _trim, tenex/inference/fdr.py:61-133,
+ * :104-110): with A_ik = A[rows, mediators] (m x k), B_kn = A[mediators, :]
+ * (k x n) and E = A[rows, :] (m x n), all float32 row-major,
+ *     out[i,j] = E[i,j] if E[i,j] >= max(0, max_l min(A_ik[i,l], B_kn[l,j]))
+ *                           + trim_threshold, else 0.
+ * Exact (comparisons only). */
+TNX_API int tnx_dpi_maxmin(const float *A_ik, int64_t m, int64_t k, int64_t lda,
+                           const float *B_kn, int64_t n, int64_t ldb, const float *E,
+                           int64_t lde, float trim_threshold, float *out, int64_t ldo,
+                           void *stream);
+
 /* ── Learned null: 3-layer MLP (no reference counterpart) ────────────── */
 
 /* Network: h1 = relu(W1 x + b1), h2 = relu(W2 h1 + b2), y = W3 h2 + b3.

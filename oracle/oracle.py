@@ -330,6 +330,49 @@ def bh_fdr(p: np.ndarray) -> np.ndarray:
     return out
 
 
+# ── DPI trimming ───────────────────────────────────────────────────────────
+
+def trim_dpi(A: np.ndarray, batch_size: int = 512, trim_threshold: float = 0.0):
+    """_trim (tenex/inference/fdr.py:61-133) restated in numpy float32 on the
+    thresholded network A (n x n): mediators are rows with positive sum
+    (:69-71); no mediator returns the input edges unchanged (:74-75); per
+    batch of rows, the active mediators are the batch's nonzero columns
+    that are mediators (:92-95); a batch without one keeps every nonzero
+    entry (:97-103); otherwise indirect = max(0, max over active k of
+    min(A[i,k], A[k,j])) (:106-114), plus trim_threshold when nonzero
+    (:116-117), and entries >= indirect survive (:119).  Returns
+    (pairs (m, 2) int64 row-major, values float32), or None for the
+    unchanged-input case."""
+    A = np.asarray(A, np.float32)
+    n = A.shape[0]
+    mediator = A.sum(axis=1, dtype=np.float32) > 0
+    if not mediator.any():
+        return None
+    thr = np.float32(trim_threshold)
+    pairs, vals = [], []
+    for r0 in range(0, n, batch_size):
+        batch = A[r0:r0 + batch_size]
+        if not (batch != 0).any():
+            continue
+        cols = np.unique(np.nonzero(batch)[1])
+        active = cols[mediator[cols]]
+        if len(active) == 0:
+            kept = batch
+        else:
+            indirect = np.zeros_like(batch)
+            for k in active:
+                indirect = np.maximum(indirect, np.minimum(batch[:, k:k + 1], A[k][None, :]))
+            if trim_threshold != 0.0:
+                indirect = (indirect + thr).astype(np.float32)
+            kept = np.where(batch >= indirect, batch, np.float32(0))
+        r, c = np.nonzero(kept)
+        pairs.append(np.stack([r + r0, c], 1).astype(np.int64))
+        vals.append(kept[r, c])
+    if not pairs:
+        return np.zeros((0, 2), np.int64), np.zeros(0, np.float32)
+    return np.concatenate(pairs), np.concatenate(vals).astype(np.float32)
+
+
 # ── MLP ────────────────────────────────────────────────────────────────────
 
 def mlp_n_params(D, H, O):

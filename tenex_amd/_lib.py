@@ -96,6 +96,8 @@ SIGNATURES = {
     "tnx_surrogate_run": (C.c_int, [C.POINTER(SurrArgs), p_]),
     "tnx_block_shuffle": (C.c_int, [p_, i64, i64, i64, p_, i32, i32, p_, p_]),
     "tnx_stats_accumulate": (C.c_int, [p_, i32, i64, p_, i64, p_, p_, p_, p_, p_]),
+    "tnx_dpi_maxmin": (C.c_int, [p_, i64, i64, i64, p_, i64, i64, p_, i64, C.c_float, p_, i64,
+                                 p_]),
     "tnx_mlp_forward": (C.c_int, [C.POINTER(Mlp), p_, i64, p_, p_]),
     "tnx_mlp_workspace_bytes": (i64, [C.c_int32, C.c_int32, i64]),
     "tnx_mlp_backward": (C.c_int, [C.POINTER(Mlp), p_, p_, p_, i64, p_, p_, p_, i64, p_]),

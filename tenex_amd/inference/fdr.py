@@ -2,6 +2,8 @@
 (tenex/inference/fdr.py:22-133)."""
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 import torch
 
@@ -47,49 +49,59 @@ class FDRMethod(InferenceMethod):
 
 def trim_dpi(te_matrix, variable_names, pairs, te_values, device, batch_size=512,
              trim_threshold=0.0):
-    """Data-processing-inequality trimming (tenex/inference/fdr.py:61-133):
-    an edge i->j survives iff A[i, j] >= max_k min(A[i, k], A[k, j]) +
-    trim_threshold over the thresholded network A."""
+    """Data-processing-inequality trimming (tenex/inference/fdr.py:61-133) of
+    the thresholded network A (edges `pairs`, weights `te_values`): edge
+    i -> j survives iff A[i,j] >= max(0, max_k min(A[i,k], A[k,j])) +
+    trim_threshold, k over the mediators (rows of A with positive sum).  As
+    in the reference, a block of `batch_size` rows none of whose edges
+    reach a mediator keeps every edge, and a network without mediators is
+    returned unchanged.  On a GPU the max-min product is the HIP kernel
+    tnx_dpi_maxmin over the gathered rows x mediators; on the CPU device it
+    is evaluated with torch in mediator chunks.  Output edges are in
+    row-major order."""
     n = te_matrix.shape[0]
-    arr_np = np.zeros((n, n), dtype=np.float32)
-    arr_np[pairs[:, 0], pairs[:, 1]] = np.asarray(te_values, dtype=np.float32)
-    arr = torch.from_numpy(arr_np).to(device)
-    mediators = (arr.sum(dim=1) > 0).nonzero(as_tuple=True)[0]
-    if len(mediators) == 0:
+    dev = torch.device(device)
+    A = torch.zeros((n, n), dtype=torch.float32, device=dev)
+    pairs = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    if len(pairs):
+        pd = torch.from_numpy(pairs).to(dev)
+        A[pd[:, 0], pd[:, 1]] = torch.from_numpy(np.asarray(te_values, np.float32)).to(dev)
+    mediator = A.sum(dim=1) > 0
+    if not bool(mediator.any()):
         return make_grn(variable_names, pairs, np.asarray(te_values))
-    if device.type == "cuda":
-        free_mem, _ = torch.cuda.mem_get_info(device)
-        budget = int(free_mem * 0.4)
-    else:
-        budget = 2 * 1024 ** 3
-    out_pairs, out_te = [], []
-    for r0 in range(0, n, batch_size):
-        r1 = min(r0 + batch_size, n)
-        batch = arr[r0:r1]
-        if not bool((batch != 0).any()):
-            continue
-        cols = batch.nonzero(as_tuple=True)[1].unique()
-        active = cols[torch.isin(cols, mediators)]
-        if len(active) == 0:
-            rows, cc = batch.nonzero(as_tuple=True)
-            out_pairs.append(torch.stack((rows + r0, cc), 1).cpu().numpy())
-            out_te.append(batch[rows, cc].cpu().numpy())
-            continue
-        bs = r1 - r0
-        kc = max(1, min(budget // (bs * n * 4), len(active)))
-        indirect = torch.zeros_like(batch)
-        for k0 in range(0, len(active), kc):
-            kidx = active[k0:k0 + kc]
-            paths = torch.min(batch[:, kidx].unsqueeze(2), arr[kidx].unsqueeze(0))
-            indirect = torch.max(indirect, paths.max(dim=1).values)
-        if trim_threshold != 0.0:
-            indirect += trim_threshold
-        trimmed = torch.where(batch >= indirect, batch, torch.zeros_like(batch))
-        rows, cc = torch.nonzero(trimmed, as_tuple=True)
-        if len(rows) == 0:
-            continue
-        out_pairs.append(torch.stack((rows + r0, cc), 1).cpu().numpy())
-        out_te.append(trimmed[rows, cc].cpu().numpy())
-    if not out_pairs:
-        return make_grn(variable_names, np.empty((0, 2), np.int32), np.empty(0, np.float32))
-    return make_grn(variable_names, np.concatenate(out_pairs), np.concatenate(out_te))
+    nz = A != 0
+    rows = torch.nonzero(nz.any(dim=1)).flatten()
+    med = torch.nonzero(mediator).flatten()
+    # blocks of rows whose edges reach no mediator are kept whole
+    reach = (nz & mediator[None, :]).any(dim=1).to(torch.int32)
+    blk = torch.arange(n, device=dev) // batch_size
+    blk_active = torch.zeros(int((n + batch_size - 1) // batch_size), dtype=torch.int32,
+                             device=dev).index_add_(0, blk, reach) > 0
+    E = A.index_select(0, rows).contiguous()
+    A_ik = E.index_select(1, med).contiguous()
+    B_kn = A.index_select(0, med).contiguous()
+    out = _maxmin_keep(A_ik, B_kn, E, float(trim_threshold), dev)
+    whole = ~blk_active[blk[rows]]
+    out[whole] = E[whole]
+    r, c = torch.nonzero(out, as_tuple=True)
+    kept = torch.stack([rows[r], c], 1).cpu().numpy()
+    return make_grn(variable_names, kept, out[r, c].cpu().numpy())
+
+
+def _maxmin_keep(A_ik, B_kn, E, thr, dev):
+    """out = E where E >= max(0, max_k min(A_ik[:, k], B_kn[k, :])) + thr."""
+    m, k = A_ik.shape
+    n = B_kn.shape[1]
+    out = torch.empty_like(E)
+    if dev.type == "cuda":
+        from .. import core
+        core.call("tnx_dpi_maxmin", core.ptr(A_ik), m, k, A_ik.stride(0) if k else 1,
+                  core.ptr(B_kn), n, n, core.ptr(E), n, C.c_float(thr), core.ptr(out), n,
+                  core._stream(dev))
+        return out
+    acc = torch.zeros_like(E)
+    kc = max(1, (256 << 20) // max(1, 4 * m * n))
+    for k0 in range(0, k, kc):
+        paths = torch.minimum(A_ik[:, k0:k0 + kc, None], B_kn[None, k0:k0 + kc, :])
+        acc = torch.maximum(acc, paths.amax(dim=1))
+    return torch.where(E >= acc + np.float32(thr), E, torch.zeros_like(E))

@@ -286,3 +286,33 @@ def test_bh_reject_matches_adjusted_pvalues():
         for a in (0.01, 0.05, 0.2):
             assert torch.equal(bh_reject_gpu(t, a), bh_fdr_gpu(t, a) < a), (m, a)
     assert bh_reject_gpu(torch.zeros(0), 0.05).numel() == 0
+
+
+def test_trim_dpi_known_answers_and_cpu_path():
+    """DPI trim: a -> c survives only when A[a,c] >= min(A[a,b], A[b,c]); the
+    oracle (fdr.py:61-133 restated) and the CPU device path agree on random
+    sparse networks with negative weights, small row blocks (whole blocks
+    kept when no edge reaches a mediator) and a nonzero threshold."""
+    from tenex_amd.inference.fdr import trim_dpi
+    names = np.array([f"g{i}" for i in range(4)])
+    A = np.zeros((4, 4), np.float32)
+    A[0, 1], A[1, 2], A[0, 2] = 0.5, 0.4, 0.3
+    pairs, vals = O.trim_dpi(A)
+    assert {tuple(p) for p in pairs.tolist()} == {(0, 1), (1, 2)}
+    A[0, 2] = 0.45
+    pairs, vals = O.trim_dpi(A)
+    assert {tuple(p) for p in pairs.tolist()} == {(0, 1), (1, 2), (0, 2)}
+    pairs, vals = O.trim_dpi(A, trim_threshold=0.1)   # 0.45 < 0.4 + 0.1
+    assert {tuple(p) for p in pairs.tolist()} == {(0, 1), (1, 2)}
+    assert O.trim_dpi(-np.abs(A)) is None                # no mediator: input unchanged
+    rng = np.random.default_rng(11)
+    for n, dens, bs, thr in [(40, 0.1, 512, 0.0), (90, 0.05, 16, 0.0), (70, 0.2, 8, 0.05)]:
+        A = np.where(rng.random((n, n)) < dens, rng.random((n, n)), 0).astype(np.float32)
+        A[rng.random((n, n)) < 0.02] *= -1
+        np.fill_diagonal(A, 0)
+        A[: n // 4] = np.where(A[: n // 4] > 0, 0, A[: n // 4])   # rows without mediators
+        p = np.argwhere(A != 0)
+        ref = O.trim_dpi(A, batch_size=bs, trim_threshold=thr)
+        got = trim_dpi(A, np.array([f"g{i}" for i in range(n)]), p, A[p[:, 0], p[:, 1]],
+                       torch.device("cpu"), batch_size=bs, trim_threshold=thr)
+        assert np.array_equal(got.pairs, ref[0]) and np.array_equal(got.te, ref[1]), (n, bs)
