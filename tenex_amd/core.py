@@ -641,7 +641,7 @@ def surrogate_plan(prep: Prepared, targets=None, sources=None) -> Plan:
 WAVE_CAP = 3200          # cells per wave histogram (te_hist.hip's default)
 FILL_WAVE_ITEMS = 256 * 3 * 4    # items that fill the chip: 4 per resident workgroup
 FILL_BLOCK_ITEMS = 256 * 2 * 4
-BLOCK_LDS = 80 * 1024 - 512
+BLOCK_LDS = 80 * 1024 - 512 - 8 * 256   # as te_hist.hip's kBlockLdsTarget
 
 
 def te_pairlist_plan(npv_host: np.ndarray, pairs: np.ndarray, device,

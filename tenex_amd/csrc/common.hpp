@@ -114,6 +114,19 @@ __device__ __forceinline__ void xlogx_add(DF &acc, const float2 *__restrict__ tb
     }
 }
 
+// c*log2(c) entries kept in LDS by kernels that look up many small counts
+// (the global table is L + 1 entries; small counts dominate the lookups).
+constexpr int kSmallTbl = 256;
+
+// As xlogx_add, with counts below kSmallTbl read from the LDS copy ltbl.
+__device__ __forceinline__ void xlogx_add_l(DF &acc, const float2 *ltbl,
+                                            const float2 *__restrict__ tbl, uint32_t c) {
+    if (c > 1) {
+        const float2 t = c < (uint32_t)kSmallTbl ? ltbl[c] : tbl[c];
+        df_add(acc, t.x, t.y);
+    }
+}
+
 // Wave-local ordering of LDS traffic: all of this wave's LDS operations
 // issued so far have completed before any later one is issued.
 __device__ __forceinline__ void wave_lds_fence() {

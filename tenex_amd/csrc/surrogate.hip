@@ -28,7 +28,6 @@
 namespace tnx {
 
 constexpr int kSurrWaves = 4;
-constexpr int kSmallTbl = 256;    // c log2 c of counts below this is read from LDS
 
 struct BlockMap {
     const int32_t *prow;  // perm[k, v, :]
@@ -175,17 +174,6 @@ __device__ __forceinline__ void count_pair16(uint32_t *hist, const ST *stp,
         }
     }
 }
-
-// acc += c log2 c: counts below kSmallTbl from the workgroup's LDS copy of
-// the table, larger ones from memory.
-__device__ __forceinline__ void xlogx_add_l(DF &acc, const float2 *ltbl,
-                                            const float2 *__restrict__ tbl, uint32_t c) {
-    if (c > 1) {
-        const float2 t = c < (uint32_t)kSmallTbl ? ltbl[c] : tbl[c];
-        df_add(acc, t.x, t.y);
-    }
-}
-
 
 // S3 - S2b of a u16-packed pair histogram (cells (a * b_x + b) * b_y + c)
 // counted by count_pair16 from L samples, summed over the wave; zeroes the

@@ -309,7 +309,8 @@ template <int NB, bool HZ>
 __device__ __forceinline__ DF hist_pair_block(uint32_t *hist, uint32_t *c2b, uint32_t *zero_row,
                                               int cap, const TargetRegs<NB> &regs,
                                               const uint4 *ys, int bx, int by, int L,
-                                              const float2 *__restrict__ tbl, float *red) {
+                                              const float2 *ltbl, const float2 *__restrict__ tbl,
+                                              float *red) {
     const int tid = threadIdx.x, nth = blockDim.x;
     uint32_t *zero = zero_row + (tid & 63);
     const int ncol = bx * by;
@@ -320,7 +321,6 @@ __device__ __forceinline__ DF hist_pair_block(uint32_t *hist, uint32_t *c2b, uin
     for (int a0 = 0; a0 < bx; a0 += apass) {
         const int a1 = (a0 + apass < bx) ? (a0 + apass) : bx;
         const uint32_t lo = (uint32_t)(a0 * bx), span = (uint32_t)((a1 - a0) * bx);
-        const int cells = (a1 - a0) * ncol;
         uint32_t *hc = hist - a0 * ncol;  // cell index of the full pair
         const bool single = a1 - a0 == bx;  // one pass: no range test
 #pragma unroll
@@ -356,41 +356,42 @@ __device__ __forceinline__ DF hist_pair_block(uint32_t *hist, uint32_t *c2b, uin
             if (tid < kWave) fold_zero_key<HZ>(hist, zero_row, tid);
             __syncthreads();
         }
-        // flattened scan over the pass's cells, four at a time (cap is a
-        // multiple of 4 and cells past the pass are zero): groups of four
-        // empty cells cost one read; only nonzero groups are cleared, and
-        // empty or single cells need no table lookup
-        const int ng = (cells + 3) >> 2;
-        const int step = (4 * nth) % ncol;
-        int col = (4 * tid) % ncol;
-        for (int g = tid; g < ng; g += nth) {
-            uint4 *p4 = reinterpret_cast<uint4 *>(hist) + g;
-            const uint4 v4 = *p4;
-            if (v4.x | v4.y | v4.z | v4.w) {
-                *p4 = make_uint4(0u, 0u, 0u, 0u);
-                const uint32_t vv[4] = {v4.x, v4.y, v4.z, v4.w};
-                int c = col;
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const uint32_t v = vv[k];
-                    if (v) {
-                        if (v > 1) {
-                            const float2 tv = tbl[v];
-                            df_add(a3, tv.x, tv.y);
-                        }
-                        __hip_atomic_fetch_add(c2b + c, v, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                    }
-                    if (++c == ncol) c = 0;
+        // column-owner scan: cell (a, col) of the pass is word a * ncol + col;
+        // a thread takes whole columns (or, when there are fewer columns than
+        // threads, every R-th row of one), so the b_t * b_s marginal of its
+        // column sums in a register and every cell costs one exchange (read
+        // and clear), a compare and an add; lanes of a wave take consecutive
+        // columns (conflict-free) and loop the same number of rows
+        const int na = a1 - a0;
+        if (ncol >= nth) {
+            for (int col = tid; col < ncol; col += nth) {
+                uint32_t c2 = 0;
+                for (int r = 0; r < na; r++) {
+                    const uint32_t v = lds_take(hist + r * ncol + col);
+                    c2 += v;
+                    xlogx_add_l(a3, ltbl, tbl, v);
                 }
+                c2b[col] += c2;  // the column's only writer in this pass
             }
-            col += step;
-            if (col >= ncol) col -= ncol;
+        } else {
+            const int R = nth / ncol;
+            if (tid < R * ncol) {
+                const int col = tid % ncol;
+                uint32_t c2 = 0;
+                for (int r = tid / ncol; r < na; r += R) {
+                    const uint32_t v = lds_take(hist + r * ncol + col);
+                    c2 += v;
+                    xlogx_add_l(a3, ltbl, tbl, v);
+                }
+                if (c2)
+                    __hip_atomic_fetch_add(c2b + col, c2, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
         }
         __syncthreads();
     }
     DF a2 = {0.f, 0.f};
-    for (int c = tid; c < ncol; c += nth) xlogx_add(a2, tbl, lds_take(c2b + c));
+    for (int c = tid; c < ncol; c += nth) xlogx_add_l(a2, ltbl, tbl, lds_take(c2b + c));
     df_add(a3, -a2.hi, -a2.lo);
     return df_block_sum(a3, red);
 }

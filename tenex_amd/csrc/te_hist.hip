@@ -41,7 +41,7 @@ constexpr int kDefaultCapWave = TNX_DEFAULT_CAP_WAVE;
 constexpr int kDefaultCapBlock = 16384;  // 64 KiB + the b_t b_s marginal
 // LDS the planner aims for per block-kernel workgroup, so that two
 // workgroups fit one CU's 160 KiB: histogram + marginal + reduction scratch.
-constexpr int kBlockLdsTarget = 80 * 1024 - 512;
+constexpr int kBlockLdsTarget = 80 * 1024 - 512 - 8 * kSmallTbl;
 constexpr int kWaveItemPairs = 256;  // pairs per wave-kernel work item
 constexpr int kXcds = 8;             // gfx950: 8 XCDs, workgroups dealt round-robin
 constexpr int kChunkSrc = 64;        // sources per locality chunk
@@ -204,15 +204,18 @@ template <int NB, bool HZ>
 __global__ __launch_bounds__(kBlockThreads) void te_block_kernel(tnx_te_args_t a) {
     extern __shared__ uint32_t lds_u32[];
     __shared__ float red[2 * (kBlockThreads / kWave)];
+    __shared__ float2 ltbl[kSmallTbl];  // c log2 c of small counts
     uint32_t *hist = lds_u32;
     uint32_t *c2b = lds_u32 + a.cap_block;
     uint32_t *zero_row = c2b + a.max_block_cols;
     const int tid = threadIdx.x;
+    const int L = (int)(a.T - a.tau);
+    const float2 *tbl = reinterpret_cast<const float2 *>(a.xlogx);
+    for (int i = tid; i < kSmallTbl; i += kBlockThreads)
+        ltbl[i] = i <= L ? tbl[i] : make_float2(0.f, 0.f);
     for (int i = tid; i < a.cap_block + a.max_block_cols + kWave; i += kBlockThreads)
         lds_u32[i] = 0;
     __syncthreads();
-    const int L = (int)(a.T - a.tau);
-    const float2 *tbl = reinterpret_cast<const float2 *>(a.xlogx);
     const double invN = 1.0 / (double)L;
     const tnx_work_t *work = a.work + a.n_wave_items;
     TargetRegs<NB> regs;
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(kBlockThreads) void te_block_kernel(tnx_te_args_t a
                 if (want && !fits && tid == 0) emit_te(a, slot, __builtin_nan(""));  // plan mismatch
                 if (want && fits) {
                     const DF d = hist_pair_block<NB, HZ>(hist, c2b, zero_row, a.cap_block, regs, ycur,
-                                                     bx, by, L, tbl, red);
+                                                     bx, by, L, ltbl, tbl, red);
                     if (tid == 0) emit_te(a, slot, (df_to_double(d) + s12) * invN);
                 }
                 y = yn;
@@ -570,7 +573,8 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
                       "tnx_te_run: cap_block %d must be a multiple of 4 and >= max_block_cols %d",
                       a.cap_block, a.max_block_cols);
         size_t shm = (size_t)(a.cap_block + a.max_block_cols + kWave) * 4;
-        TNX_CHECK_ARG(shm <= 160 * 1024, "tnx_te_run: block kernel LDS %zu exceeds 160 KiB", shm);
+        TNX_CHECK_ARG(shm + 8 * kSmallTbl + 256 <= 160 * 1024,
+                      "tnx_te_run: block kernel LDS %zu exceeds 160 KiB", shm);
         unsigned grid = (unsigned)std::min<int64_t>(a.n_block_items, 1 << 20);
         const int64_t L = a.T - a.tau;
         const int nb = (int)((L + 16 * kBlockThreads - 1) / (16 * kBlockThreads));

@@ -147,6 +147,30 @@ def test_materialized_stats_vs_oracle(dev, bl, tau):
     assert np.all(S.cpu().numpy().ravel()[mask] == 0)
 
 
+def test_staged_batch_equals_per_surrogate_runs(dev):
+    """K surrogates in one launch (TNX_OUT_STAGE + tnx_stats_accumulate) and
+    K one-surrogate runs (TNX_OUT_STATS) make the same additions in the same
+    order: identical bits, wave and block pairs."""
+    d, npv = _small(n=26, T=380, seed=9)
+    n, T = d.shape
+    prep = _prep(d, npv, dev)
+    obs = torch.from_numpy(core.te_matrix_internal(prep).cpu().numpy().T.copy()).to(dev)
+    perm, eff_bl, nb = core.block_perm_device(n, T, "block", 19, 0, 6, 3, dev, dev)
+    cells = npv[:, None].astype(np.int64) ** 2 * npv[None, :]
+    plan = core.get_plan(prep, cap_wave=int(np.median(cells)))
+    assert plan.info.n_block_items > 0
+    z = lambda dt: torch.zeros((n, n), dtype=dt, device=dev)
+    a = (z(torch.float64), z(torch.float64), z(torch.int32))
+    b = (z(torch.float64), z(torch.float64), z(torch.int32))
+    core.surrogate_stats_materialized(prep, plan, perm, eff_bl, obs, *a)
+    work = {}
+    for k in range(perm.shape[0]):
+        core.surrogate_stats_materialized(prep, plan, perm[k:k + 1], eff_bl, obs, *b, work)
+    for u, v in zip(a, b):
+        assert torch.equal(u, v)
+    assert float(a[0].abs().sum()) > 0
+
+
 def test_subset_stats_equal_full(dev):
     """Targets x sources touching a subset of the variables: only the subset
     is shuffled (the same block orders as rows of the full draw), and every
