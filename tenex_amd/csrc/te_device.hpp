@@ -105,8 +105,10 @@ __device__ __forceinline__ void hist_accumulate_mem(uint32_t *hist, const uint16
                                                     const int8_t *__restrict__ yr, uint32_t kmul,
                                                     uint32_t ysh, uint32_t *zero, int L,
                                                     int lane) {
+    // two whole 1024-sample steps per iteration while every lane's samples
+    // are inside L (a wave-uniform loop test); the tail is predicated
     int t = 16 * lane;
-    for (; t + 1024 + 16 <= L; t += 2048) {
+    for (int tb = 0; tb + 2048 <= L; tb += 2048, t += 2048) {
         const uint4 ya = *reinterpret_cast<const uint4 *>(yr + t);
         const uint4 sa0 = *reinterpret_cast<const uint4 *>(st + t);
         const uint4 sa1 = *reinterpret_cast<const uint4 *>(st + t + 8);
@@ -186,7 +188,10 @@ __device__ __forceinline__ void hist_accumulate_regs(uint32_t *hist, TargetRegs<
             const int t = 16 * lane + 1024 * i;
             if (i >= NI || t >= L) break;
             const uint32_t yw[4] = {ys[u].x, ys[u].y, ys[u].z, ys[u].w};
-            if (t + 16 <= L) {
+            // wave-uniform test (every lane's 16 samples inside L): a per-lane
+            // test here lets the compiler fold this path into the tail's
+            // per-sample predication
+            if (1024 * i + 1024 <= L) {
 #pragma unroll
                 for (int q = 0; q < 16; q++) {
                     const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
@@ -248,7 +253,33 @@ __device__ __forceinline__ DF hist_reduce_wave_copies(uint32_t *hist, int bx, in
     const int ncol = bx * by;
     const int cells = bx * ncol;
     DF a3 = {0.f, 0.f}, a2 = {0.f, 0.f};
-    if (cp.G > 1) {
+    if (cp.G >= 4) {
+        // four copies per 16-byte read: the 8 lanes of a read group (32
+        // banks) take chunks (j + f) of consecutive cells, f = k / (8 / (G/4))
+        // mod G/4, which lands them on 8 distinct 4-bank groups
+        const int G4 = cp.G >> 2;
+        uint32_t *tot = hist + cp.G * cells;
+        const int per_sh = TNX_LDS_BANK_SHIFT - cp.gsh;  // log2(8 / G4)
+        for (int k = lane; k < cells; k += kWave) {
+            const int f = (k >> per_sh) & (G4 - 1);
+            uint4 *row = reinterpret_cast<uint4 *>(hist + (k << cp.gsh));
+            uint32_t t = 0;
+            for (int j = 0; j < G4; j++) {
+                uint4 *q = row + ((j + f) & (G4 - 1));
+                const uint4 v = *q;
+                *q = make_uint4(0u, 0u, 0u, 0u);
+                t += (v.x + v.y) + (v.z + v.w);
+            }
+            tot[k] = t;
+            xlogx_add(a3, tbl, t);
+        }
+        wave_lds_fence();
+        for (int col = lane; col < ncol; col += kWave) {
+            uint32_t c2 = 0;
+            for (int aa = 0; aa < bx; aa++) c2 += lds_take(tot + aa * ncol + col);
+            xlogx_add(a2, tbl, c2);
+        }
+    } else if (cp.G > 1) {
         const int G = cp.G;
         uint32_t *tot = hist + G * cells;
         const int per_sh = TNX_LDS_BANK_SHIFT - cp.gsh;  // log2(banks / G)
@@ -328,14 +359,17 @@ __device__ __forceinline__ DF hist_pair_block(uint32_t *hist, uint32_t *c2b, uin
             const int t = 16 * tid + 16 * nth * i;
             if (t >= L) continue;
             const uint32_t yw[4] = {ys[i].x, ys[i].y, ys[i].z, ys[i].w};
-            if (single && t + 16 <= L) {
+            // the wave's last lane decides whole-chunk paths (wave-uniform;
+            // a per-lane test lets the compiler predicate every sample)
+            const bool full = __builtin_amdgcn_readfirstlane(16 * (tid | 63) + 16 * nth * i) + 16 <= L;
+            if (single && full) {
 #pragma unroll
                 for (int q = 0; q < 16; q++) {
                     const uint32_t sv = (regs.s[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
                     const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
                     lds_inc_key0<HZ>(hc, sv, by4, yv, 2u, zero);
                 }
-            } else if (t + 16 <= L) {
+            } else if (full) {
 #pragma unroll
                 for (int q = 0; q < 16; q++) {
                     const uint32_t sv = (regs.s[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;

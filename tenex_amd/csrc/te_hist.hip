@@ -46,6 +46,10 @@ constexpr int kWaveItemPairs = 256;  // pairs per wave-kernel work item
 constexpr int kXcds = 8;             // gfx950: 8 XCDs, workgroups dealt round-robin
 constexpr int kChunkSrc = 64;        // sources per locality chunk
 
+// Words of one wave's LDS slice: cap histogram words and the 64 lanes' key-0
+// words, rounded to 16 bytes (the copy fold reads four copies at a time).
+__host__ __device__ inline int wave_slice(int cap) { return (cap + kWave + 3) & ~3; }
+
 // Item of workgroup slot w in [0, n): workgroup w runs on XCD w % 8, and XCD
 // x takes the x-th contiguous slice of the items, in order (a bijection of
 // [0, n)).
@@ -112,9 +116,9 @@ __device__ __forceinline__ void te_wave_body(const tnx_te_args_t &a) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     // slice: cap_wave histogram words, then the lanes' key-0 words
-    uint32_t *hist = lds_u32 + wave * (a.cap_wave + kWave);
+    uint32_t *hist = lds_u32 + wave * wave_slice(a.cap_wave);
     uint32_t *zero_row = hist + a.cap_wave;
-    for (int i = lane; i < a.cap_wave + kWave; i += kWave) hist[i] = 0;
+    for (int i = lane; i < wave_slice(a.cap_wave); i += kWave) hist[i] = 0;
     wave_lds_fence();
     const int L = (int)(a.T - a.tau);
     const float2 *tbl = reinterpret_cast<const float2 *>(a.xlogx);
@@ -291,9 +295,9 @@ __global__ __launch_bounds__(256) void te_counts_kernel(
     extern __shared__ uint32_t lds_u32[];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    uint32_t *hist = lds_u32 + wave * (cap + kWave);
+    uint32_t *hist = lds_u32 + wave * wave_slice(cap);
     uint32_t *zero_row = hist + cap;
-    for (int i = lane; i < cap + kWave; i += kWave) hist[i] = 0;
+    for (int i = lane; i < wave_slice(cap); i += kWave) hist[i] = 0;
     wave_lds_fence();
     const int L = (int)(T - tau);
     const float2 *tbl = reinterpret_cast<const float2 *>(xlogx);
@@ -552,12 +556,12 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
                   "tnx_te_run: TNX_OUT_STAGE needs stage_vars >= 1, ld_out >= stage_vars and "
                   "stage_stride >= stage_vars * ld_out");
     TNX_CHECK_ARG(a.cap_wave >= 8 && a.cap_wave <= 8192, "tnx_te_run: cap_wave out of range");
-    TNX_CHECK_ARG((size_t)kWavesPerWG * (a.cap_wave + kWave) * 4 <= 160 * 1024,
+    TNX_CHECK_ARG((size_t)kWavesPerWG * wave_slice(a.cap_wave) * 4 <= 160 * 1024,
                   "tnx_te_run: cap_wave too large");
     hipStream_t s = as_stream(stream);
     if (a.n_wave_items > 0) {
         TNX_CHECK_ARG(a.work, "tnx_te_run: null work");
-        size_t shm = (size_t)kWavesPerWG * (a.cap_wave + kWave) * 4;
+        size_t shm = (size_t)kWavesPerWG * wave_slice(a.cap_wave) * 4;
         unsigned grid = (unsigned)std::min<int64_t>(a.n_wave_items, 1 << 20);
         const int ni = wave_variant(a.T - a.tau);
         const int rc = a.hot_zero ? launch_wave<true>(ni, grid, shm, s, a)
@@ -601,7 +605,7 @@ int tnx_te_counts_debug(const int8_t *bins, const uint16_t *states, const int32_
     TNX_CHECK_ARG(ld % 16 == 0 && lds % 8 == 0 && tau >= 1 && tau < T,
                   "tnx_te_counts_debug: bad strides / tau");
     if (n_pairs == 0) return TNX_OK;
-    size_t shm = (size_t)kWavesPerWG * (max_cells + kWave) * 4;
+    size_t shm = (size_t)kWavesPerWG * wave_slice(max_cells) * 4;
     unsigned grid = (unsigned)((n_pairs + kWavesPerWG - 1) / kWavesPerWG);
     const int ni = wave_variant(T - tau);
     switch (ni) {
