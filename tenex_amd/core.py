@@ -11,6 +11,8 @@ Layout in HBM (one GPU):
 from __future__ import annotations
 
 import ctypes as C
+import hashlib
+from collections import OrderedDict
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -256,16 +258,16 @@ def replicate_plan(plan: Plan, K: int, m: int) -> Plan:
     nw = int(i.n_wave_items)
     nt, ns = len(plan.tgt_host), len(plan.src_host)
     w = plan.work_host.astype(np.int64).reshape(-1, 4)
+    ks = np.arange(K, dtype=np.int64)
     reps = []
     for part in (w[:nw], w[nw:]):
-        for k in range(K):
-            q = part.copy()
-            q[:, 0] += k * nt
-            q[:, 2:] += k * ns
-            reps.append(q)
-    work = np.concatenate(reps).astype(np.int32) if reps else np.zeros((0, 4), np.int32)
-    tgt = np.concatenate([plan.tgt_host.astype(np.int64) + k * m for k in range(K)]).astype(np.int32)
-    src = np.concatenate([plan.src_host.astype(np.int64) + k * m for k in range(K)]).astype(np.int32)
+        q = np.repeat(part[None], K, axis=0)          # (K, items, 4)
+        q[:, :, 0] += (ks * nt)[:, None]
+        q[:, :, 2:] += (ks * ns)[:, None, None]
+        reps.append(q.reshape(-1, 4))
+    work = np.concatenate(reps).astype(np.int32)
+    tgt = (plan.tgt_host[None, :].astype(np.int64) + (ks * m)[:, None]).reshape(-1).astype(np.int32)
+    src = (plan.src_host[None, :].astype(np.int64) + (ks * m)[:, None]).reshape(-1).astype(np.int32)
     info = _lib.PlanInfo.from_buffer_copy(i)
     info.n_wave_items, info.n_block_items = nw * K, (len(w) - nw) * K
     info.n_pairs, info.n_tgt, info.n_src = i.n_pairs * K, len(tgt), len(src)
@@ -295,15 +297,39 @@ def make_plan(npv_host: np.ndarray, L: int, device, targets=None, sources=None,
                 tgt_host=tgt, src_host=src)
 
 
+# Plans depend only on the level counts, the pair set and the caps; the last
+# few are kept across Prepared objects (a surrogate test prepares its data
+# afresh on every call).
+_PLAN_CACHE: "OrderedDict[tuple, Plan]" = OrderedDict()
+_PLAN_CACHE_SIZE = 16
+
+
+def _cached_plan(key, build):
+    p = _PLAN_CACHE.get(key)
+    if p is None:
+        p = build()
+        _PLAN_CACHE[key] = p
+        while len(_PLAN_CACHE) > _PLAN_CACHE_SIZE:
+            _PLAN_CACHE.popitem(last=False)
+    else:
+        _PLAN_CACHE.move_to_end(key)
+    return p
+
+
+def _digest(a) -> bytes | None:
+    if a is None:
+        return None
+    return hashlib.blake2b(np.ascontiguousarray(a).tobytes(), digest_size=16).digest()
+
+
 def get_plan(prep: Prepared, targets=None, sources=None, cap_wave: int = 0,
              cap_block: int = 0) -> Plan:
-    key = (None if targets is None else hash(np.asarray(targets).tobytes()),
-           None if sources is None else hash(np.asarray(sources).tobytes()), cap_wave, cap_block)
-    p = prep.plans.get(key)
-    if p is None:
-        p = make_plan(prep.npv_host, prep.L, prep.device, targets, sources, cap_wave, cap_block)
-        prep.plans[key] = p
-    return p
+    key = ("te", _digest(prep.npv_host), int(prep.L), str(prep.device),
+           _digest(None if targets is None else np.asarray(targets, np.int32)),
+           _digest(None if sources is None else np.asarray(sources, np.int32)),
+           int(cap_wave), int(cap_block))
+    return _cached_plan(key, lambda: make_plan(prep.npv_host, prep.L, prep.device, targets,
+                                               sources, cap_wave, cap_block))
 
 
 # ── TE run ────────────────────────────────────────────────────────────────
@@ -646,6 +672,15 @@ BLOCK_LDS = 80 * 1024 - 512 - 8 * 256   # as te_hist.hip's kBlockLdsTarget
 
 def te_pairlist_plan(npv_host: np.ndarray, pairs: np.ndarray, device,
                      cap_wave: int = WAVE_CAP) -> Plan:
+    """Cached _te_pairlist_plan."""
+    pairs = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
+    key = ("pairs", _digest(np.asarray(npv_host, np.int32)), _digest(pairs), str(device),
+           int(cap_wave))
+    return _cached_plan(key, lambda: _te_pairlist_plan(npv_host, pairs, device, cap_wave))
+
+
+def _te_pairlist_plan(npv_host: np.ndarray, pairs: np.ndarray, device,
+                      cap_wave: int = WAVE_CAP) -> Plan:
     """TE plan for an explicit [target, source] pair list: pairs grouped by
     target, each target's sources sorted by level count; the wave-kernel
     pairs (b_t^2 b_s <= cap_wave) in items of up to 256 sources, the rest in

@@ -274,21 +274,35 @@ __global__ __launch_bounds__(256) void gene_prep_kernel(
     const int64_t L = T - tau;
     uint32_t *c1 = sh;          // bcap
     uint32_t *c2 = sh + bcap;   // bcap * bcap
-    for (int i = tid; i < b + b * b; i += 256) sh[i < b ? i : bcap + (i - b)] = 0;
+    for (int i = tid; i < b * b; i += 256) c2[i] = 0;
     __syncthreads();
     const int8_t *x = bins + g * ld;
     uint16_t *st = states + g * lds;
-    uint32_t same = 0;
+    // Only the joint (x[t+tau], x[t]) histogram is counted; its column sums
+    // give the x[t] marginal and its diagonal the lag-tau agreement.  State
+    // 0 (both at level 0, the bulk of zero-inflated data) is counted in a
+    // register: one LDS address taking every lane's increment would
+    // serialise the workgroup's atomics.
+    uint32_t z00 = 0;
     for (int64_t t = tid; t < lds; t += 256) {
         if (t < L) {
-            int a = x[t + tau], c = x[t];
-            st[t] = (uint16_t)(a * b + c);
-            atomicAdd(&c1[c], 1u);
-            atomicAdd(&c2[a * b + c], 1u);
-            same += (a == c);
+            const int a = x[t + tau], c = x[t];
+            const uint32_t key = (uint32_t)(a * b + c);
+            st[t] = (uint16_t)key;
+            if (key) atomicAdd(&c2[key], 1u);
+            else z00++;
         } else {
             st[t] = 0;
         }
+    }
+    if (z00) atomicAdd(&c2[0], z00);
+    __syncthreads();
+    uint32_t same = 0;
+    for (int i = tid; i < b; i += 256) {
+        uint32_t col = 0;
+        for (int a = 0; a < b; a++) col += c2[a * b + i];
+        c1[i] = col;
+        same += c2[i * b + i];
     }
     __syncthreads();
     double s1 = 0, s2 = 0, lvl = 0;

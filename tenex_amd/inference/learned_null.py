@@ -213,12 +213,11 @@ def plain_stats_grid(prep, targets, sources, obs_pub, n_surr, shuffle_method, bl
     return s, s2
 
 
-def training_grid(pub: np.ndarray, n_train: int, rng):
-    """Targets and sources of the training grid for `pub` ([source, target]
-    tested pairs): all sources when they are few (<= 2 sqrt(n_train)), else
-    sqrt(n_train) of them, and enough targets for n_train pairs."""
-    srcs = np.unique(pub[:, 0])
-    tgts = np.unique(pub[:, 1])
+def training_grid(srcs: np.ndarray, tgts: np.ndarray, n_train: int, rng):
+    """Targets and sources of the training grid, drawn from the tested
+    pairs' sources and targets (sorted unique): all sources when they are
+    few (<= 2 sqrt(n_train)), else sqrt(n_train) of them, and enough targets
+    for n_train pairs."""
     root = max(1, int(math.sqrt(n_train)))
     ns = len(srcs) if len(srcs) <= 2 * root else root
     nt = min(len(tgts), max(2, -(-n_train // ns)))
@@ -249,15 +248,14 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     pub_d = torch.nonzero(mask)                           # [source, target] of tested pairs
     if src_idx is not None:
         pub_d = pub_d[torch.isin(pub_d[:, 0], core.h2d(np.asarray(src_idx, np.int64), dev))]
-    pub = pub_d.cpu().numpy()
-    pairs = np.stack([pub[:, 1], pub[:, 0]], 1).astype(np.int64)   # [target, source]
-    P = len(pairs)
+    P = int(pub_d.shape[0])
     rng = np.random.default_rng(seed)
     n_want = min(P, max(min_train, int(train_frac * P)))
     # 1. plain surrogates on a grid of targets x sources; a fifth of the grid's
     #    targets is held out whole to measure how the model carries to
     #    unseen targets
-    gt, gs = training_grid(pub, n_want, rng)
+    gt, gs = training_grid(torch.unique(pub_d[:, 0]).cpu().numpy(),
+                           torch.unique(pub_d[:, 1]).cpu().numpy(), n_want, rng)
     S_g, S2_g = plain_stats_grid(prep, gt, gs, obs_d, n_surrogates, shuffle_method,
                                  block_length, seed, rng_device)
     n_ho = max(1, len(gt) // 5) if len(gt) > 1 else 0
@@ -320,8 +318,9 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     t_predict = time.perf_counter() - t0 - t_train_plain - t_fit
     # 4. plain surrogates for the borderline pairs
     if len(rr):
-        s_b, s2_b, _ = plain_stats_pairs(prep, pairs[rr], obs_d, n_surrogates, shuffle_method,
-                                         block_length, seed, rng_device)
+        rr_pub = pub_d[rr_d].cpu().numpy()
+        s_b, s2_b, _ = plain_stats_pairs(prep, rr_pub[:, ::-1], obs_d, n_surrogates,
+                                         shuffle_method, block_length, seed, rng_device)
         mean_p[rr_d], std_p[rr_d] = _moments(s_b, s2_b, n_surrogates)
     torch.cuda.synchronize(dev)
     t_rerun = time.perf_counter() - t0 - t_train_plain - t_fit - t_predict
