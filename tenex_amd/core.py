@@ -405,11 +405,12 @@ def counts_debug(prep: Prepared, pairs: np.ndarray, max_cells: int = 8192):
     return counts, te
 
 
-def gene_prep_into(prep: Prepared):
-    """Re-run the per-variable preparation into prep's existing buffers."""
+def gene_prep_into(prep: Prepared, features: bool = True):
+    """Re-run the per-variable preparation into prep's existing buffers
+    (features=False: states and entropies only)."""
     call("tnx_gene_prep", ptr(prep.bins), prep.n_vars, prep.T, prep.ld, int(prep.tau),
-         ptr(prep.npv), ptr(prep.states), prep.lds, ptr(prep.S1), ptr(prep.S2a), ptr(prep.feat),
-         _stream(prep.device))
+         ptr(prep.npv), ptr(prep.states), prep.lds, ptr(prep.S1), ptr(prep.S2a),
+         ptr(prep.feat) if features else None, _stream(prep.device))
 
 
 def lds_atomic_rate(device, mode: int = 0, n_blocks: int | None = None, iters: int = 4096,
@@ -467,15 +468,25 @@ def block_perm_device(n_vars: int, T: int, shuffle_method: str, block_length: in
     nk = k_end - k_start
     gen = torch.Generator(device=rng_device)
     if nk * n_vars * nb <= PERM_DRAW_FLOATS:
-        # the batch's draws side by side, then one gather and one sort
-        r = torch.empty((nk, n_vars, nb), dtype=torch.float32, device=rng_device)
-        for ki, k in enumerate(range(k_start, k_end)):
-            gen.manual_seed(seed + k)
-            torch.rand(n_vars, nb, device=rng_device, generator=gen, out=r[ki])
-        if rows_t is not None:
-            r = r.index_select(1, rows_t)
-        return torch.argsort(r, dim=2, stable=True).to(device=out_device,
-                                                       dtype=torch.int32), eff, nb
+        # the batch's draws side by side, one sort; the orders depend only on
+        # (seed, k, n_vars, n_blocks), so the last few batches are kept
+        key = (n_vars, nb, seed, k_start, k_end, str(torch.device(rng_device)),
+               str(torch.device(out_device)))
+        full = _PERM_CACHE.get(key)
+        if full is None:
+            r = torch.empty((nk, n_vars, nb), dtype=torch.float32, device=rng_device)
+            for ki, k in enumerate(range(k_start, k_end)):
+                gen.manual_seed(seed + k)
+                torch.rand(n_vars, nb, device=rng_device, generator=gen, out=r[ki])
+            full = torch.argsort(r, dim=2, stable=True).to(device=out_device, dtype=torch.int32)
+            _PERM_CACHE[key] = full
+            while sum(v.numel() for v in _PERM_CACHE.values()) > PERM_CACHE_INTS:
+                _PERM_CACHE.popitem(last=False)
+        else:
+            _PERM_CACHE.move_to_end(key)
+        if rows_t is None:
+            return full, eff, nb
+        return full.index_select(1, rows_t.to(full.device)), eff, nb
     out = torch.empty((nk, m, nb), dtype=torch.int32, device=out_device)
     for ki, k in enumerate(range(k_start, k_end)):
         gen.manual_seed(seed + k)
@@ -520,6 +531,8 @@ def block_shuffle(bins: torch.Tensor, T: int, perm_k: torch.Tensor, block_length
 
 STAGE_FLOATS = 64 << 20   # staging for batched surrogate copies: 256 MiB of float32
 PERM_DRAW_FLOATS = 32 << 20   # uniform draws of one batch of block orders held at once
+PERM_CACHE_INTS = 128 << 20   # block orders kept across calls (512 MiB of int32)
+_PERM_CACHE: "OrderedDict[tuple, torch.Tensor]" = OrderedDict()
 
 
 def surrogate_stats_materialized(prep: Prepared, plan: Plan, perm: torch.Tensor,
@@ -557,7 +570,7 @@ def surrogate_stats_materialized(prep: Prepared, plan: Plan, perm: torch.Tensor,
             src, sp, args, stage, _ = bw
             block_shuffle(src, prep.T, perm[k0:k0 + kk].reshape(kk * n, -1), block_length,
                           sp.bins)
-            gene_prep_into(sp)
+            gene_prep_into(sp, features=False)
             te_run(sp, None, None, _lib.TNX_OUT_STAGE, args=args)
             call("tnx_stats_accumulate", ptr(stage), kk, n * n, ptr(slots), slots.numel(),
                  ptr(observed), ptr(sum_te), ptr(sumsq_te), ptr(count_ge), _stream(prep.device))
@@ -574,7 +587,7 @@ def surrogate_stats_materialized(prep: Prepared, plan: Plan, perm: torch.Tensor,
                    stats=(sum_te, sumsq_te, count_ge, observed))
     for kk in range(n_k):
         block_shuffle(prep.bins, prep.T, perm[kk], block_length, sp.bins)
-        gene_prep_into(sp)
+        gene_prep_into(sp, features=False)
         te_run(sp, plan, None, _lib.TNX_OUT_STATS, args=args)
 
 

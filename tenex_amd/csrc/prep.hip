@@ -315,6 +315,13 @@ __global__ __launch_bounds__(256) void gene_prep_kernel(
     for (int i = tid; i < b * b; i += 256) s2 += xlog2x_d((double)c2[i]);
     double S1v = block_sum_d(s1, red);
     double S2v = block_sum_d(s2, red);
+    if (!feat) {  // entropies only (shuffled surrogate copies)
+        if (tid == 0) {
+            S1[g] = S1v;
+            S2a[g] = S2v;
+        }
+        return;
+    }
     double lv = block_sum_d(lvl, red);
     double sm = block_sum_d((double)same, red);
     // block max of mode
@@ -328,7 +335,7 @@ __global__ __launch_bounds__(256) void gene_prep_kernel(
     if (tid == 0) {
         S1[g] = S1v;
         S2a[g] = S2v;
-        if (feat) {
+        {
             uint32_t md = mred[0];
             for (int i = 1; i < 4; i++) md = mred[i] > md ? mred[i] : md;
             double Ld = (double)L, lg = log2(Ld);
