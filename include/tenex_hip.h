@@ -46,7 +46,7 @@ extern "C" {
 #define TNX_API
 #endif
 
-#define TNX_ABI_VERSION 4
+#define TNX_ABI_VERSION 5
 
 /* Status codes. */
 #define TNX_OK 0
@@ -111,10 +111,13 @@ TNX_API int tnx_xlogx_table(float *tbl, int64_t N, void *stream);
  *   S2a[v] = sum_ab c2a log2 c2a over (x_{t+tau}, x_t), t < L   (FP64)
  * feat (optional, float (n_vars, 8)): b, H(X_t), H(X_t+1|X_t),
  * fraction of x_t == 0, fraction of x_t == mode, log2 b, mean level / b,
- * lag-1 self agreement. */
+ * lag-1 self agreement.  b_max: an upper bound of n_per_var (it sizes the
+ * LDS histogram; 0 means 127); a variable above it gets NaN entropies.
+ * ld and lds must be multiples of 16 with lds <= ld.
+ * (Per-target marginals of every TE kernel, full_smem.cu:87-141.) */
 TNX_API int tnx_gene_prep(const int8_t *bins, int64_t n_vars, int64_t T, int64_t ld, int32_t tau,
                   const int32_t *n_per_var, uint16_t *states, int64_t lds,
-                  double *S1, double *S2a, float *feat, void *stream);
+                  double *S1, double *S2a, float *feat, int32_t b_max, void *stream);
 
 /* ── Pairwise TE (LDS histogram kernels) ─────────────────────────────── */
 

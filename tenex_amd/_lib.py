@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libtenex_hip.so")
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 TNX_OUT_PAIRFREE = 0
 TNX_OUT_MATRIX = 1
@@ -85,7 +85,7 @@ SIGNATURES = {
     "tnx_remap_dense": (C.c_int, [p_, i64, i64, p_, i64, p_, p_, p_]),
     "tnx_coarsen_bins": (C.c_int, [p_, i64, i64, i64, p_, i32, p_]),
     "tnx_xlogx_table": (C.c_int, [p_, i64, p_]),
-    "tnx_gene_prep": (C.c_int, [p_, i64, i64, i64, i32, p_, p_, i64, p_, p_, p_, p_]),
+    "tnx_gene_prep": (C.c_int, [p_, i64, i64, i64, i32, p_, p_, i64, p_, p_, p_, i32, p_]),
     "tnx_te_plan": (C.c_int, [p_, i64, i64, p_, i64, p_, i64, i32, i32, p_, p_, p_, i64,
                               C.POINTER(PlanInfo)]),
     "tnx_te_run": (C.c_int, [C.POINTER(TeArgs), p_]),

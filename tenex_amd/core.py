@@ -173,9 +173,10 @@ def prepare(bins: torch.Tensor, npv: torch.Tensor, T: int, tau: int) -> Prepared
     S1 = torch.empty(n, dtype=torch.float64, device=dev)
     S2a = torch.empty(n, dtype=torch.float64, device=dev)
     feat = torch.empty((n, 8), dtype=torch.float32, device=dev)
+    npv_host = npv.cpu().numpy().astype(np.int32)
     call("tnx_gene_prep", ptr(bins), n, T, ld, int(tau), ptr(npv), ptr(states), lds, ptr(S1),
-         ptr(S2a), ptr(feat), _stream(dev))
-    return Prepared(bins=bins, npv=npv, npv_host=npv.cpu().numpy().astype(np.int32), T=T,
+         ptr(S2a), ptr(feat), int(npv_host.max(initial=1)), _stream(dev))
+    return Prepared(bins=bins, npv=npv, npv_host=npv_host, T=T,
                     tau=tau, states=states, S1=S1, S2a=S2a, feat=feat,
                     xlogx=xlogx_table(L, dev),
                     hot_zero=key0_fraction(states, bins, L) >= HOT_ZERO_FRACTION)
@@ -258,9 +259,19 @@ def replicate_plan(plan: Plan, K: int, m: int) -> Plan:
     nw = int(i.n_wave_items)
     nt, ns = len(plan.tgt_host), len(plan.src_host)
     w = plan.work_host.astype(np.int64).reshape(-1, 4)
+    # the base plan's items were sized to fill the chip alone; K copies
+    # fill it with K times fewer, longer items
+    pw = lambda q: int((q[:, 1] * (q[:, 3] - q[:, 2])).sum())
+    wave = _merge_items(w[:nw], int(np.clip(-(-K * pw(w[:nw]) // FILL_WAVE_ITEMS), 4, 256)))
+    # wave items in the planner's L2 order: source chunk dealt round-robin to
+    # the XCD slices, then chunk, then target group (te_hist.hip, by_chunk)
+    ck = wave[:, 2] // CHUNK_SRC
+    wave = wave[np.lexsort((wave[:, 0], ck, ck % N_XCDS))]
+    block = _merge_items(w[nw:], int(np.clip(-(-K * pw(w[nw:]) // FILL_BLOCK_ITEMS), 1, 16)))
+    nw_m = len(wave)
     ks = np.arange(K, dtype=np.int64)
     reps = []
-    for part in (w[:nw], w[nw:]):
+    for part in (wave, block):
         q = np.repeat(part[None], K, axis=0)          # (K, items, 4)
         q[:, :, 0] += (ks * nt)[:, None]
         q[:, :, 2:] += (ks * ns)[:, None, None]
@@ -269,12 +280,34 @@ def replicate_plan(plan: Plan, K: int, m: int) -> Plan:
     tgt = (plan.tgt_host[None, :].astype(np.int64) + (ks * m)[:, None]).reshape(-1).astype(np.int32)
     src = (plan.src_host[None, :].astype(np.int64) + (ks * m)[:, None]).reshape(-1).astype(np.int32)
     info = _lib.PlanInfo.from_buffer_copy(i)
-    info.n_wave_items, info.n_block_items = nw * K, (len(w) - nw) * K
+    info.n_wave_items, info.n_block_items = nw_m * K, len(block) * K
     info.n_pairs, info.n_tgt, info.n_src = i.n_pairs * K, len(tgt), len(src)
     dev = plan.work.device
     wd = work if len(work) else np.zeros((1, 4), np.int32)
     return Plan(info=info, tgt_order=h2d(tgt, dev), src_order=h2d(src, dev), work=h2d(wd, dev),
                 work_host=work, tgt_host=tgt, src_host=src)
+
+
+def _merge_items(items: np.ndarray, per_pairs: int) -> np.ndarray:
+    """Merge work items (tgt_beg, cnt, src_beg, src_end) of the same target
+    group over contiguous source ranges into items of about per_pairs pairs
+    (items are never split)."""
+    if len(items) == 0:
+        return items.reshape(0, 4)
+    q = items[np.lexsort((items[:, 2], items[:, 1], items[:, 0]))]
+    brk = np.ones(len(q), bool)
+    brk[1:] = (q[1:, 0] != q[:-1, 0]) | (q[1:, 1] != q[:-1, 1]) | (q[1:, 2] != q[:-1, 3])
+    run = np.cumsum(brk) - 1
+    run_start = q[brk, 2][run]
+    ch = np.maximum(per_pairs // q[:, 1], 1)
+    chunk = (q[:, 2] - run_start) // ch
+    key = run * (int(chunk.max()) + 1) + chunk
+    first = np.ones(len(q), bool)
+    first[1:] = key[1:] != key[:-1]
+    idx = np.nonzero(first)[0]
+    out = q[idx].copy()
+    out[:, 3] = np.maximum.reduceat(q[:, 3], idx)
+    return out
 
 
 def make_plan(npv_host: np.ndarray, L: int, device, targets=None, sources=None,
@@ -410,7 +443,8 @@ def gene_prep_into(prep: Prepared, features: bool = True):
     (features=False: states and entropies only)."""
     call("tnx_gene_prep", ptr(prep.bins), prep.n_vars, prep.T, prep.ld, int(prep.tau),
          ptr(prep.npv), ptr(prep.states), prep.lds, ptr(prep.S1), ptr(prep.S2a),
-         ptr(prep.feat) if features else None, _stream(prep.device))
+         ptr(prep.feat) if features else None, int(prep.npv_host.max(initial=1)),
+         _stream(prep.device))
 
 
 def lds_atomic_rate(device, mode: int = 0, n_blocks: int | None = None, iters: int = 4096,
@@ -679,6 +713,7 @@ def surrogate_plan(prep: Prepared, targets=None, sources=None) -> Plan:
 
 WAVE_CAP = 3200          # cells per wave histogram (te_hist.hip's default)
 FILL_WAVE_ITEMS = 256 * 3 * 4    # items that fill the chip: 4 per resident workgroup
+CHUNK_SRC, N_XCDS = 64, 8        # the planner's source chunk and gfx950's XCD count
 FILL_BLOCK_ITEMS = 256 * 2 * 4
 BLOCK_LDS = 80 * 1024 - 512 - 8 * 256   # as te_hist.hip's kBlockLdsTarget
 

@@ -272,6 +272,10 @@ __global__ __launch_bounds__(256) void gene_prep_kernel(
     const int tid = threadIdx.x;
     const int b = npv[g];
     const int64_t L = T - tau;
+    if (b > bcap || b < 1) {  // level count above the caller's bound
+        if (tid == 0) S1[g] = S2a[g] = __builtin_nan("");
+        return;
+    }
     uint32_t *c1 = sh;          // bcap
     uint32_t *c2 = sh + bcap;   // bcap * bcap
     for (int i = tid; i < b * b; i += 256) c2[i] = 0;
@@ -283,17 +287,34 @@ __global__ __launch_bounds__(256) void gene_prep_kernel(
     // 0 (both at level 0, the bulk of zero-inflated data) is counted in a
     // register: one LDS address taking every lane's increment would
     // serialise the workgroup's atomics.
+    // 16 consecutive samples per thread and step: the 16 x[t] bytes in one
+    // 16-byte load, the 16 x[t+tau] bytes requested together before any is
+    // used, the 16 states written as two 16-byte stores (rows are padded to
+    // 64 elements, samples past L get state 0)
     uint32_t z00 = 0;
-    for (int64_t t = tid; t < lds; t += 256) {
-        if (t < L) {
-            const int a = x[t + tau], c = x[t];
-            const uint32_t key = (uint32_t)(a * b + c);
-            st[t] = (uint16_t)key;
-            if (key) atomicAdd(&c2[key], 1u);
-            else z00++;
-        } else {
-            st[t] = 0;
+    for (int64_t t0 = 16 * (int64_t)tid; t0 < lds; t0 += 16 * 256) {
+        const uint4 c16 = *reinterpret_cast<const uint4 *>(x + t0);
+        const uint32_t cw[4] = {c16.x, c16.y, c16.z, c16.w};
+        uint32_t av[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+            av[q] = t0 + q < L ? (uint32_t)(uint8_t)x[t0 + q + tau] : 0u;
+        uint32_t sw[8];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            uint32_t key = 0;
+            if (t0 + q < L) {
+                const uint32_t c = (cw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                key = av[q] * (uint32_t)b + c;
+                if (key) atomicAdd(&c2[key], 1u);
+                else z00++;
+            }
+            if (q & 1) sw[q >> 1] |= key << 16;
+            else sw[q >> 1] = key;
         }
+        uint4 *so = reinterpret_cast<uint4 *>(st + t0);
+        so[0] = make_uint4(sw[0], sw[1], sw[2], sw[3]);
+        so[1] = make_uint4(sw[4], sw[5], sw[6], sw[7]);
     }
     if (z00) atomicAdd(&c2[0], z00);
     __syncthreads();
@@ -437,14 +458,19 @@ int tnx_xlogx_table(float *tbl, int64_t N, void *stream) {
 
 int tnx_gene_prep(const int8_t *bins, int64_t n_vars, int64_t T, int64_t ld, int32_t tau,
                   const int32_t *n_per_var, uint16_t *states, int64_t lds, double *S1,
-                  double *S2a, float *feat, void *stream) {
+                  double *S2a, float *feat, int32_t b_max, void *stream) {
     clear_error();
     TNX_CHECK_ARG(bins && n_per_var && states && S1 && S2a, "tnx_gene_prep: null pointer");
     TNX_CHECK_ARG(tau >= 1 && tau < T, "tnx_gene_prep: need 1 <= tau < T");
     TNX_CHECK_ARG(ld >= T && lds >= T - tau, "tnx_gene_prep: row strides too small");
+    TNX_CHECK_ARG(ld % 16 == 0 && lds % 16 == 0 && lds <= ld,
+                  "tnx_gene_prep: ld and lds must be multiples of 16 with lds <= ld");
+    TNX_CHECK_ARG(b_max >= 0 && b_max <= 127, "tnx_gene_prep: b_max %d not in [0, 127]", b_max);
     if (n_vars == 0) return TNX_OK;
-    const int bcap = 128;
-    size_t shm = (size_t)(bcap + bcap * bcap) * 4;  // 64.5 KiB
+    // the histogram is sized for the largest level count: small b leaves LDS
+    // room for many workgroups per CU
+    const int bcap = b_max > 0 ? b_max : 127;
+    size_t shm = (size_t)(bcap + bcap * bcap) * 4;
     hipStream_t s = as_stream(stream);
     TNX_CHECK_HIP(hipFuncSetAttribute((const void *)gene_prep_kernel,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
