@@ -162,8 +162,14 @@ def run_c2(args):
     bins_np = synthetic.mixed_cardinality_bins(n, T, seed=args.seed)
     bins, npv, b_max, T = core.upload_bins(bins_np, dev)
     prep = core.prepare(bins, npv, T, tau)
+    tuned = None
+    cw = cb = 0
+    if args.autotune:
+        tuned = core.autotune(prep)
+        cw, cb = tuned["cap_wave"], tuned["cap_block"]
+        log(f"autotune: {tuned}")
     t0 = time.perf_counter()
-    plan = core.get_plan(prep)
+    plan = core.get_plan(prep, cap_wave=cw, cap_block=cb)
     plan_s = time.perf_counter() - t0
     info = plan.info
     n_pairs = n * (n - 1)
@@ -228,6 +234,8 @@ def run_c2(args):
         "wave_pairs": wave_pairs, "block_pairs": n_pairs - wave_pairs,
         "plan_host_ms": plan_s * 1e3,
     }
+    if tuned is not None:
+        result["config"]["autotune"] = tuned
     if not args.no_cpu_baseline:
         log("cpu baseline (scatter_add restatement) on a bounded sample ...")
         pairs = sample_pairs(prep.npv_host, args.cpu_sample, args.seed)
@@ -459,6 +467,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=20480)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="c2: time the launch candidates on a sample first (core.autotune)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run one C2 matrix and a 256 MiB copy (for rocprofv3 --pmc), no JSON")
     args = ap.parse_args()

@@ -393,6 +393,60 @@ def te_run(prep: Prepared, plan: Plan, out: torch.Tensor, out_mode: int, ld_out:
     call("tnx_te_run", C.byref(args), _stream(prep.device))
 
 
+# (cap_wave, cap_block) candidates of the autotuner: the wave kernel's LDS
+# histogram per wave (which pairs go to the block kernel, and how many
+# workgroups fit a CU) and the block kernel's histogram per pass.
+TUNE_CAPS = ((0, 0), (2400, 0), (4096, 0), (0, 8192))
+_TUNE_CACHE: dict = {}
+
+
+def autotune(prep: Prepared, targets=None, sources=None, sample_targets: int = 256,
+             reps: int = 3, min_gain: float = 0.02) -> dict:
+    """Launch parameters for the TE kernels on prep, chosen by timing: each
+    (cap_wave, cap_block) of TUNE_CAPS, with and without the key-0
+    diversion, runs the plan of an evenly spaced sample of at most
+    sample_targets targets against all sources; a candidate replaces the
+    defaults only when it is min_gain faster.  Cached per (levels, L,
+    targets, sources, device).  Returns dict(cap_wave, cap_block, hot_zero,
+    ms) and sets prep.hot_zero."""
+    tg_all = np.arange(prep.n_vars, dtype=np.int32) if targets is None else \
+        np.asarray(targets, dtype=np.int32)
+    key = (_digest(prep.npv_host), prep.L, str(prep.device), _digest(tg_all),
+           _digest(None if sources is None else np.asarray(sources, np.int32)))
+    best = _TUNE_CACHE.get(key)
+    if best is None:
+        # sample targets spread over the level counts (sorted by levels)
+        order = tg_all[np.argsort(prep.npv_host[tg_all], kind="stable")]
+        step = max(1, len(order) // sample_targets)
+        tg = np.sort(order[::step][:sample_targets])
+        out = torch.empty((prep.n_vars, prep.n_vars), dtype=torch.float32, device=prep.device)
+        hz0 = prep.hot_zero
+        timings = []
+        for hz in (hz0, not hz0):
+            prep.hot_zero = hz
+            for cw, cb in TUNE_CAPS:
+                plan = make_plan(prep.npv_host, prep.L, prep.device, tg, sources, cw, cb)
+                args = te_args(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=prep.n_vars)
+                te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, args=args)
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, args=args)
+                e1.record()
+                e1.synchronize()
+                timings.append((e0.elapsed_time(e1) / reps, cw, cb, hz))
+        prep.hot_zero = hz0
+        default = timings[0]
+        fastest = min(timings)
+        pick = fastest if fastest[0] < (1.0 - min_gain) * default[0] else default
+        best = dict(cap_wave=pick[1], cap_block=pick[2], hot_zero=pick[3], ms=pick[0],
+                    default_ms=default[0], sample_targets=len(tg))
+        _TUNE_CACHE[key] = best
+    prep.hot_zero = best["hot_zero"]
+    return best
+
+
 def te_matrix_internal(prep: Prepared, targets=None, sources=None, cap_wave: int = 0,
                        cap_block: int = 0) -> torch.Tensor:
     """Internal-layout TE matrix M[target, source] = TE(source -> target) on the

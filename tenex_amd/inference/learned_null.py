@@ -234,7 +234,7 @@ def _moments(s, s2, n_surr):
 def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogates,
                            shuffle_method, block_length, seed, rng_device, p_method, fdr, mask,
                            src_idx, dev, train_frac: float = 0.05, band: float = 1.0,
-                           steps: int = 300, lr: float = 3e-3, min_train: int = 2000,
+                           steps: int = 150, lr: float = 6e-3, min_train: int = 2000,
                            model_seed: int = 0, band_sigmas: float = 5.0,
                            fit_batch: int = 4096):
     """Learned-null surrogate test; returns a SurrogateTestResult with an

@@ -162,8 +162,10 @@ class TransferEntropyEngine:
         """Pairwise TE (signature of tenex/transferentropy.py:537-562).
 
         Returns a TransferEntropyResult with matrix[i, j] = TE(var_i -> var_j).
-        batch_size / autotune are accepted for API compatibility: the HIP
-        kernels take whole plans.  coarsening=True applies the reference's
+        batch_size is accepted for API compatibility (the HIP kernels take
+        whole plans); autotune=True times the TE kernels' LDS histogram sizes
+        and the key-0 diversion on a sample of targets and runs the fastest
+        (core.autotune).  coarsening=True applies the reference's
         lossy coarsening; the default keeps counts exact at any b_max <= 127.
         """
         set_verbose(verbose)
@@ -236,8 +238,12 @@ class TransferEntropyEngine:
             if profile:
                 ev[4].record(stream)
                 ev[5].record(stream)
-            internal = core.te_matrix_internal(prep, None, src_idx, kernel_obj.cap_wave,
-                                               kernel_obj.cap_block)
+            cw, cb = kernel_obj.cap_wave, kernel_obj.cap_block
+            if autotune:
+                tuned = core.autotune(prep, None, src_idx)
+                cw, cb = tuned["cap_wave"], tuned["cap_block"]
+                run_info["autotune"] = tuned
+            internal = core.te_matrix_internal(prep, None, src_idx, cw, cb)
         if profile:
             ev[6].record(stream)
         self._result_matrix = internal.t().contiguous().cpu().numpy()

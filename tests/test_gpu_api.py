@@ -41,6 +41,23 @@ def test_engine_sources_and_kernels():
         tnx.TransferEntropyEngine(x, names).compute(accelerator="gpu", kernel="GEMM-B2")
 
 
+def test_engine_autotune_matches_oracle():
+    """compute(autotune=True) times the launch candidates (LDS histogram
+    sizes, key-0 diversion) on a sample of targets, records its pick, and
+    whatever it picks the matrix stays within 1e-6 of the FP64 oracle."""
+    from tenex_amd import core
+    b = synthetic.mixed_cardinality_bins(150, 2000, seed=13)
+    names = np.array([f"g{i}" for i in range(150)])
+    eng = tnx.TransferEntropyEngine(b.astype(np.float32), names)
+    res = eng.compute(accelerator="gpu", autotune=True)
+    info = eng._last_run_info["autotune"]
+    assert (info["cap_wave"], info["cap_block"]) in core.TUNE_CAPS
+    assert info["ms"] <= info["default_ms"]
+    d, nb, npv, bm = O.discretize(b.astype(np.float32))
+    ref = O.te_matrix(d, npv)
+    assert np.abs(res.matrix - ref).max() < TOL
+
+
 def test_engine_binary_routes_to_gemm_b2():
     x, planted = synthetic.bernoulli_planted(200, 800, n_planted=20, seed=3)
     names = np.array([f"g{i}" for i in range(200)])
