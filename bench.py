@@ -181,6 +181,10 @@ def run_c2(args):
         core.gene_prep_into(prep)
         core.te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, args=targs)
 
+    if args.graph:  # the same launches captured once in a HIP graph, replayed per step
+        graph = core.capture_te_step(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n)
+        step = graph.replay  # noqa: F811
+
     log(f"plan: {info.n_wave_items} wave items, {info.n_block_items} block items, "
         f"{plan_s * 1e3:.1f} ms host")
     dt = timed_steps(step, args.steps, args.warmup)
@@ -236,6 +240,7 @@ def run_c2(args):
     }
     if tuned is not None:
         result["config"]["autotune"] = tuned
+    result["config"]["hip_graph"] = bool(args.graph)
     if not args.no_cpu_baseline:
         log("cpu baseline (scatter_add restatement) on a bounded sample ...")
         pairs = sample_pairs(prep.npv_host, args.cpu_sample, args.seed)
@@ -467,6 +472,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=20480)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph", action="store_true",
+                    help="c2: time replays of the step captured in a HIP graph")
     ap.add_argument("--autotune", action="store_true",
                     help="c2: time the launch candidates on a sample first (core.autotune)")
     ap.add_argument("--traffic-probe", action="store_true",

@@ -447,6 +447,26 @@ def autotune(prep: Prepared, targets=None, sources=None, sample_targets: int = 2
     return best
 
 
+def capture_te_step(prep: Prepared, plan: Plan, out: torch.Tensor, out_mode: int,
+                    ld_out: int = 0) -> torch.cuda.CUDAGraph:
+    """The per-variable preparation and the plan's TE launches captured once
+    in a HIP graph (torch.cuda.CUDAGraph): replay() re-runs them on prep's
+    current bins with no host launch work.  The launches are issued once
+    eagerly first (kernel attributes set outside the capture); nothing in
+    them is a memset, and the buffers are fixed, so every replay is
+    equivalent to the eager sequence."""
+    args = te_args(prep, plan, out, out_mode, ld_out=ld_out)
+    gene_prep_into(prep)
+    te_run(prep, plan, out, out_mode, args=args)
+    torch.cuda.synchronize(prep.device)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        gene_prep_into(prep)
+        te_run(prep, plan, out, out_mode, args=args)
+    g._tnx_keep = (args, plan, out)  # the graph's kernels read these
+    return g
+
+
 def te_matrix_internal(prep: Prepared, targets=None, sources=None, cap_wave: int = 0,
                        cap_block: int = 0) -> torch.Tensor:
     """Internal-layout TE matrix M[target, source] = TE(source -> target) on the
