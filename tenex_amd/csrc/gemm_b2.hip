@@ -102,23 +102,44 @@ __global__ __launch_bounds__(256) void b2_mfma_kernel(
     const int8_t *pb0 = P + (j0 + r) * Lp + 16 * h;
     const int8_t *pb1 = P + (j0 + 32 + r) * Lp + 16 * h;
     v16i accC0 = {}, accC1 = {}, accD0 = {}, accD1 = {}, accE0 = {}, accE1 = {};
-    for (int64_t k = 0; k < Lp; k += kB2K) {
+    // Two-stage software pipeline: the operands of step k + kB2K are
+    // requested before the MFMAs of step k are issued, so the memory latency
+    // of one step hides behind the matrix work of the previous one.
+    struct Ops {
+        v4i f[2], p[2], q0[2], q1[2];
+    };
+    auto load = [&](Ops &o, int64_t k) {
 #pragma unroll
         for (int u = 0; u < 2; u++) {
             const int64_t kk = k + 32 * u;
-            const v4i f = *reinterpret_cast<const v4i *>(fa + kk);
-            const v4i p = *reinterpret_cast<const v4i *>(pa + kk);
-            const v4i q0 = *reinterpret_cast<const v4i *>(pb0 + kk);
-            const v4i q1 = *reinterpret_cast<const v4i *>(pb1 + kk);
-            const v4i fp = f & p;
-            accC0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fp, q0, accC0, 0, 0, 0);
-            accC1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fp, q1, accC1, 0, 0, 0);
-            accD0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(p, q0, accD0, 0, 0, 0);
-            accD1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(p, q1, accD1, 0, 0, 0);
-            accE0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(f, q0, accE0, 0, 0, 0);
-            accE1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(f, q1, accE1, 0, 0, 0);
+            o.f[u] = *reinterpret_cast<const v4i *>(fa + kk);
+            o.p[u] = *reinterpret_cast<const v4i *>(pa + kk);
+            o.q0[u] = *reinterpret_cast<const v4i *>(pb0 + kk);
+            o.q1[u] = *reinterpret_cast<const v4i *>(pb1 + kk);
         }
+    };
+    auto mma = [&](const Ops &o) {
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const v4i fp = o.f[u] & o.p[u];
+            accC0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fp, o.q0[u], accC0, 0, 0, 0);
+            accC1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fp, o.q1[u], accC1, 0, 0, 0);
+            accD0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.p[u], o.q0[u], accD0, 0, 0, 0);
+            accD1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.p[u], o.q1[u], accD1, 0, 0, 0);
+            accE0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.f[u], o.q0[u], accE0, 0, 0, 0);
+            accE1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.f[u], o.q1[u], accE1, 0, 0, 0);
+        }
+    };
+    Ops a, b;
+    load(a, 0);
+    int64_t k = 0;
+    for (; k + 2 * kB2K <= Lp; k += 2 * kB2K) {  // ping-pong: no register copies
+        load(b, k + kB2K);
+        mma(a);
+        if (k + 2 * kB2K < Lp) load(a, k + 2 * kB2K);
+        mma(b);
     }
+    if (k < Lp) mma(a);  // Lp / kB2K odd: the last step was loaded into a
     // Epilogue.  32x32 accumulator layout: lane holds column (lane & 31),
     // register q holds row 8*(q/4) + 4*(lane/32) + (q%4).
     const float2 *tbl = reinterpret_cast<const float2 *>(xlogx);
