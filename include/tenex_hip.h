@@ -46,7 +46,7 @@ extern "C" {
 #define TNX_API
 #endif
 
-#define TNX_ABI_VERSION 5
+#define TNX_ABI_VERSION 6
 
 /* Status codes. */
 #define TNX_OK 0
@@ -161,11 +161,15 @@ TNX_API int tnx_te_plan(const int32_t *n_per_var, int64_t n_vars, int64_t L,
 #define TNX_OUT_MATRIX 1   /* out[target * ld_out + source] (internal layout) */
 #define TNX_OUT_STATS 2    /* surrogate statistics at the public slot source * ld_out + target:
                             * te = (float)TE; sum_te += te; sumsq_te += te^2 (FP64);
-                            * count_ge += (te >= observed[slot]) */
+                            * count_ge += (te >= observed[slot]) (skipped when count_ge is
+                            * NULL) */
 #define TNX_OUT_STAGE 3    /* K surrogate copies in one launch: variable v = k * m + j
                             * (m = stage_vars) is copy k of variable j; out[k * stage_stride
                             * + (source % m) * ld_out + (target % m)] = (float)TE, for
-                            * tnx_stats_accumulate to fold in copy order */
+                            * tnx_stats_accumulate to fold in copy order.  With stage_slot
+                            * set, the offset inside a copy is instead
+                            * stage_slot[(source % m) * ld_out + (target % m)] (pairs mapped
+                            * to -1 are skipped): a sparse pair list stages K x P floats */
 
 typedef struct {
     const int8_t *bins;       /* (n_vars, ld) */
@@ -189,7 +193,14 @@ typedef struct {
                         * per-lane words instead of one histogram cell.
                         * Results are identical either way; only speed differs. */
     int32_t stage_vars;    /* TNX_OUT_STAGE only: variables per copy (m) */
-    int64_t stage_stride;  /* TNX_OUT_STAGE only: floats per copy (>= m * ld_out) */
+    int64_t stage_stride;  /* TNX_OUT_STAGE only: floats per copy (>= m * ld_out, or
+                            * > every stage_slot entry) */
+    const int32_t *stage_slot; /* TNX_OUT_STAGE only, optional: (m * ld_out) map of a
+                                * copy's public slots to compact offsets (or -1) */
+    int32_t item_per_wave;     /* nonzero: each wave of a wave-kernel workgroup takes its own
+                                * work item (suits plans of one-target items: pair lists);
+                                * zero: the four waves split each item's pairs.  Results are
+                                * identical either way; only speed differs. */
 } tnx_te_args_t;
 
 /* Pairwise TE for every pair of the plan: FP32 entropy terms with
@@ -204,7 +215,8 @@ TNX_API int tnx_te_run(const tnx_te_args_t *args, void *stream);
  * slots[i] (ascending, < stage_stride) and copies k = 0..K-1 in order,
  * te = stage[k * stage_stride + slot]; sum_te[slot] += te, sumsq_te[slot] +=
  * te^2 (FP64), count_ge[slot] += (te >= observed[slot]): the same additions
- * in the same order as K TNX_OUT_STATS runs.  No reference counterpart: it
+ * in the same order as K TNX_OUT_STATS runs.  count_ge NULL: no counting
+ * (observed may then be NULL too).  No reference counterpart: it
  * batches the per-surrogate accumulation of tenex/inference/
  * surrogate_test.py:890-929 (sums over the surrogate TE matrices). */
 TNX_API int tnx_stats_accumulate(const float *stage, int32_t K, int64_t stage_stride,
@@ -249,6 +261,17 @@ TNX_API int tnx_te_gemm_b2(const int8_t *bins, int64_t n, int64_t T, int64_t ld,
 TNX_API int tnx_block_shuffle(const int8_t *bins, int64_t n_vars, int64_t T, int64_t ld,
                               const int32_t *perm, int32_t n_blocks, int32_t block_length,
                               int8_t *out, void *stream);
+
+/* K block-shuffled copies stacked: out row k * n_vars + v (ld bytes) is row
+ * v of bins shuffled with the block order in perm row k * n_vars + v (perm
+ * int32 (n_copies * n_vars, n_blocks)); n_vars * n_copies < 2^31.
+ * tnx_block_shuffle is n_copies = 1.  No reference counterpart: the batched
+ * form of _block_shuffle_gpu (tenex/inference/surrogate_test.py:107-148)
+ * for the surrogate copies that go through one TE launch. */
+TNX_API int tnx_block_shuffle_copies(const int8_t *bins, int64_t n_vars, int32_t n_copies,
+                                     int64_t T, int64_t ld, const int32_t *perm,
+                                     int32_t n_blocks, int32_t block_length, int8_t *out,
+                                     void *stream);
 
 typedef struct {
     const int8_t *bins;       /* (n_vars, ld) */

@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libtenex_hip.so")
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 TNX_OUT_PAIRFREE = 0
 TNX_OUT_MATRIX = 1
@@ -51,7 +51,8 @@ class TeArgs(C.Structure):
                 ("cap_wave", i32), ("cap_block", i32), ("max_block_cols", i32),
                 ("out", p_), ("ld_out", i64), ("pair_offset", i64), ("n_pairs_local", i64),
                 ("sum_te", p_), ("sumsq_te", p_), ("count_ge", p_), ("observed", p_),
-                ("hot_zero", C.c_int32), ("stage_vars", C.c_int32), ("stage_stride", C.c_int64)]
+                ("hot_zero", C.c_int32), ("stage_vars", C.c_int32), ("stage_stride", C.c_int64),
+                ("stage_slot", C.c_void_p), ("item_per_wave", C.c_int32)]
 
 
 class SurrArgs(C.Structure):
@@ -95,6 +96,7 @@ SIGNATURES = {
     "tnx_te_gemm_b2": (C.c_int, [p_, i64, i64, i64, i32, p_, p_, p_, p_, i64, p_, i64, p_, p_]),
     "tnx_surrogate_run": (C.c_int, [C.POINTER(SurrArgs), p_]),
     "tnx_block_shuffle": (C.c_int, [p_, i64, i64, i64, p_, i32, i32, p_, p_]),
+    "tnx_block_shuffle_copies": (C.c_int, [p_, i64, i32, i64, i64, p_, i32, i32, p_, p_]),
     "tnx_stats_accumulate": (C.c_int, [p_, i32, i64, p_, i64, p_, p_, p_, p_, p_]),
     "tnx_dpi_maxmin": (C.c_int, [p_, i64, i64, i64, p_, i64, i64, p_, i64, C.c_float, p_, i64,
                                  p_]),

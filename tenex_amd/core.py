@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 import hashlib
+import threading
 from collections import OrderedDict
 from dataclasses import dataclass, field
 
@@ -224,6 +225,50 @@ def h2d(a, device) -> torch.Tensor:
     return t.pin_memory().to(device, non_blocking=True)
 
 
+_SIDE_STREAMS: dict = {}
+
+
+class PendingUpload:
+    """A host array's copy to the device that overlaps other work: a
+    background thread stages it in pinned memory and starts the copy on a
+    side stream (default priority); get() waits for the thread and makes
+    the current stream wait for the copy, then returns the device tensor."""
+
+    def __init__(self, a, device):
+        self.device = torch.device(device)
+        self._host = torch.from_numpy(np.ascontiguousarray(a))
+        self._out = None
+        self._err = None
+        self._ev = torch.cuda.Event()
+        st = _SIDE_STREAMS.get(str(self.device))
+        if st is None:
+            st = torch.cuda.Stream(self.device)
+            _SIDE_STREAMS[str(self.device)] = st
+        self._stream = st
+        self._thread = threading.Thread(target=self._run, daemon=True)
+        self._thread.start()
+
+    def _run(self):
+        try:
+            pinned = self._host.pin_memory()
+            with torch.cuda.device(self.device), torch.cuda.stream(self._stream):
+                self._out = pinned.to(self.device, non_blocking=True)
+                self._ev.record(self._stream)
+        except BaseException as e:   # re-raised by get()
+            self._err = e
+
+    def get(self) -> torch.Tensor:
+        if self._thread is not None:
+            self._thread.join()
+            self._thread = None
+            if self._err is not None:
+                raise self._err
+            cur = torch.cuda.current_stream(self.device)
+            cur.wait_event(self._ev)
+            self._out.record_stream(cur)
+        return self._out
+
+
 def _ranges(beg: np.ndarray, end: np.ndarray) -> np.ndarray:
     """Concatenation of arange(beg[i], end[i])."""
     ln = np.maximum(end - beg, 0)
@@ -260,9 +305,12 @@ def replicate_plan(plan: Plan, K: int, m: int) -> Plan:
     nt, ns = len(plan.tgt_host), len(plan.src_host)
     w = plan.work_host.astype(np.int64).reshape(-1, 4)
     # the base plan's items were sized to fill the chip alone; K copies
-    # fill it with K times fewer, longer items
+    # fill it with K times fewer, longer items (one-target items, which a
+    # wave takes whole, at most WAVE_ITEM_PAIRS pairs)
     pw = lambda q: int((q[:, 1] * (q[:, 3] - q[:, 2])).sum())
-    wave = _merge_items(w[:nw], int(np.clip(-(-K * pw(w[:nw]) // FILL_WAVE_ITEMS), 4, 256)))
+    one_target = nw > 0 and bool(np.all(w[:nw, 1] == 1))
+    cap = WAVE_ITEM_PAIRS if one_target else 256
+    wave = _merge_items(w[:nw], int(np.clip(-(-K * pw(w[:nw]) // FILL_WAVE_ITEMS), 4, cap)))
     # wave items in the planner's L2 order: source chunk dealt round-robin to
     # the XCD slices, then chunk, then target group (te_hist.hip, by_chunk)
     ck = wave[:, 2] // CHUNK_SRC
@@ -369,10 +417,15 @@ def get_plan(prep: Prepared, targets=None, sources=None, cap_wave: int = 0,
 
 def te_args(prep: Prepared, plan: Plan, out: torch.Tensor | None, out_mode: int, ld_out: int = 0,
             pair_offset: int = 0, n_pairs_local: int = 0, stats=None, stage_vars: int = 0,
-            stage_stride: int = 0) -> _lib.TeArgs:
+            stage_stride: int = 0, stage_slot: torch.Tensor | None = None,
+            item_per_wave: bool | None = None) -> _lib.TeArgs:
     """stats: (sum, sumsq, count_ge, observed) public-layout tensors for
-    TNX_OUT_STATS; stage_vars / stage_stride for TNX_OUT_STAGE."""
+    TNX_OUT_STATS; stage_vars / stage_stride (and the optional int32 slot
+    map stage_slot) for TNX_OUT_STAGE.  item_per_wave None: chosen from the
+    plan (item_per_wave_default)."""
     i = plan.info
+    if item_per_wave is None:
+        item_per_wave = item_per_wave_default(plan)
     st = stats if stats is not None else (None, None, None, None)
     return _lib.TeArgs(
         bins=ptr(prep.bins), states=ptr(prep.states), n_per_var=ptr(prep.npv), S1=ptr(prep.S1),
@@ -383,7 +436,26 @@ def te_args(prep: Prepared, plan: Plan, out: torch.Tensor | None, out_mode: int,
         cap_block=i.cap_block, max_block_cols=max(i.max_block_cols, 1), out=ptr(out),
         ld_out=ld_out, pair_offset=pair_offset, n_pairs_local=n_pairs_local,
         sum_te=ptr(st[0]), sumsq_te=ptr(st[1]), count_ge=ptr(st[2]), observed=ptr(st[3]),
-        hot_zero=int(prep.hot_zero), stage_vars=int(stage_vars), stage_stride=int(stage_stride))
+        hot_zero=int(prep.hot_zero), stage_vars=int(stage_vars), stage_stride=int(stage_stride),
+        stage_slot=ptr(stage_slot), item_per_wave=int(bool(item_per_wave)))
+
+
+WAVE_ITEM_PAIRS = 32   # pairs of a one-target item that a single wave takes whole
+
+
+def item_per_wave_default(plan: Plan) -> bool:
+    """A wave per work item when every wave item holds one target and at
+    most WAVE_ITEM_PAIRS pairs (pair-list plans, where the four waves would
+    otherwise share one target's few pairs) and there are enough items to
+    fill the chip that way."""
+    got = plan.cache.get("item_per_wave")
+    if got is None:
+        nw = int(plan.info.n_wave_items)
+        w = np.asarray(plan.work_host).reshape(-1, 4)[:nw]
+        got = bool(nw >= 4 * FILL_WAVE_ITEMS and np.all(w[:, 1] == 1)
+                   and int((w[:, 3] - w[:, 2]).max(initial=0)) <= WAVE_ITEM_PAIRS)
+        plan.cache["item_per_wave"] = got
+    return got
 
 
 def te_run(prep: Prepared, plan: Plan, out: torch.Tensor, out_mode: int, ld_out: int = 0,
@@ -629,40 +701,46 @@ def surrogate_run(prep: Prepared, plan: Plan, perm: torch.Tensor, block_length: 
 
 
 def block_shuffle(bins: torch.Tensor, T: int, perm_k: torch.Tensor, block_length: int,
-                  out: torch.Tensor):
-    """out = one block-shuffled copy of bins (perm_k int32 (n_vars, n_blocks))."""
+                  out: torch.Tensor, copies: int = 1):
+    """out = block-shuffled copies of bins (n_vars rows): row k * n_vars + v
+    of out is row v shuffled with perm_k row k * n_vars + v (perm_k int32
+    (copies * n_vars, n_blocks))."""
     n, ld = bins.shape
     perm_k = perm_k.contiguous()
-    call("tnx_block_shuffle", ptr(bins), n, T, ld, ptr(perm_k), perm_k.shape[1],
-         int(block_length), ptr(out), _stream(bins.device))
+    call("tnx_block_shuffle_copies", ptr(bins), n, int(copies), T, ld, ptr(perm_k),
+         perm_k.shape[1], int(block_length), ptr(out), _stream(bins.device))
 
 
 STAGE_FLOATS = 64 << 20   # staging for batched surrogate copies: 256 MiB of float32
+COPY_BYTES = 6 << 30      # stacked surrogate copies (bins and states) held at once
 PERM_DRAW_FLOATS = 32 << 20   # uniform draws of one batch of block orders held at once
 PERM_CACHE_INTS = 128 << 20   # block orders kept across calls (512 MiB of int32)
 _PERM_CACHE: "OrderedDict[tuple, torch.Tensor]" = OrderedDict()
 
 
 def surrogate_stats_materialized(prep: Prepared, plan: Plan, perm: torch.Tensor,
-                                 block_length: int, observed: torch.Tensor,
+                                 block_length: int, observed: torch.Tensor | None,
                                  sum_te: torch.Tensor, sumsq_te: torch.Tensor,
-                                 count_ge: torch.Tensor, work: dict | None = None):
+                                 count_ge: torch.Tensor | None, work: dict | None = None):
     """Surrogate statistics through the TE kernels: for each surrogate k of
     perm (k, n_vars, n_blocks), a block-shuffled copy of the bins, its
     per-variable states and entropies (tnx_gene_prep), then the plan's TE,
     added in surrogate order into the public-layout sum / sumsq (FP64) and
-    count_ge (int32).  Up to STAGE_FLOATS / n_vars^2 surrogates go through
-    one launch of each kernel: their copies are stacked as variables
-    k * n_vars + j, the TE kernels write each copy's TE to a staging slab
-    (TNX_OUT_STAGE) and tnx_stats_accumulate folds the slab into the
-    statistics copy by copy; otherwise one TNX_OUT_STATS run per surrogate.
-    Both give the same additions in the same order.  `work` caches the
-    buffers across calls."""
+    count_ge (int32; not counted when count_ge or observed is None).  Up to
+    STAGE_FLOATS / n_vars^2 surrogates (and COPY_BYTES of copies) go
+    through one launch of each kernel: their copies are stacked as
+    variables k * n_vars + j, the TE kernels write each copy's TE to a
+    staging slab (TNX_OUT_STAGE) and tnx_stats_accumulate folds the slab
+    into the statistics copy by copy; otherwise one TNX_OUT_STATS run per
+    surrogate.  Both give the same additions in the same order.  `work`
+    caches the buffers across calls."""
     n = prep.n_vars
     if work is None:
         work = {}
+    if count_ge is None or observed is None:
+        count_ge = observed = None
     n_k = perm.shape[0]
-    kb = min(n_k, STAGE_FLOATS // max(1, n * n))
+    kb = copies_per_launch(prep, n_k, n * n)
     if kb >= 2:
         slots = plan.cache.get("slots")
         if slots is None:
@@ -671,17 +749,8 @@ def surrogate_stats_materialized(prep: Prepared, plan: Plan, perm: torch.Tensor,
             plan.cache["slots"] = slots
         for k0 in range(0, n_k, kb):
             kk = min(kb, n_k - k0)
-            bw = work.get(("batch", kk))
-            if bw is None:
-                bw = _stacked_copies(prep, plan, kk)
-                work[("batch", kk)] = bw
-            src, sp, args, stage, _ = bw
-            block_shuffle(src, prep.T, perm[k0:k0 + kk].reshape(kk * n, -1), block_length,
-                          sp.bins)
-            gene_prep_into(sp, features=False)
-            te_run(sp, None, None, _lib.TNX_OUT_STAGE, args=args)
-            call("tnx_stats_accumulate", ptr(stage), kk, n * n, ptr(slots), slots.numel(),
-                 ptr(observed), ptr(sum_te), ptr(sumsq_te), ptr(count_ge), _stream(prep.device))
+            _staged_batch(prep, plan, perm[k0:k0 + kk], block_length, work, slots, n * n,
+                          observed, sum_te, sumsq_te, count_ge)
         return
     sp = work.get("prep")
     if sp is None:
@@ -699,13 +768,44 @@ def surrogate_stats_materialized(prep: Prepared, plan: Plan, perm: torch.Tensor,
         te_run(sp, plan, None, _lib.TNX_OUT_STATS, args=args)
 
 
-def _stacked_copies(prep: Prepared, plan: Plan, K: int):
-    """Buffers for K stacked surrogate copies of prep's variables: the bins
-    repeated K times (the shuffle's input), the copies' Prepared, TE args
-    for the K-fold plan writing to a (K, n, n) staging slab, and the slab."""
+def copies_per_launch(prep: Prepared, n_k: int, stage_per_copy: int) -> int:
+    """Surrogate copies of prep's variables that go through one launch: up
+    to n_k, within STAGE_FLOATS of staging and COPY_BYTES of stacked bins
+    and states."""
+    per_copy = prep.n_vars * (prep.ld + 2 * prep.lds + 64)
+    return int(min(n_k, STAGE_FLOATS // max(1, stage_per_copy), COPY_BYTES // max(1, per_copy)))
+
+
+def _staged_batch(prep: Prepared, plan: Plan, perm: torch.Tensor, block_length: int, work: dict,
+                  slots: torch.Tensor, stride: int, observed, sum_te, sumsq_te, count_ge,
+                  slot_map: torch.Tensor | None = None):
+    """One launch of each kernel for the K = perm.shape[0] surrogate copies
+    of prep's variables: shuffle, prepare, TE to the staging slab (dense
+    public layout, or compact through slot_map with `stride` slots), fold
+    the slab into the statistics at `slots` in copy order."""
+    kk, n = perm.shape[0], prep.n_vars
+    key = ("batch", kk, slot_map is not None)
+    bw = work.get(key)
+    if bw is None:
+        bw = _stacked_copies(prep, plan, kk, slot_map, stride)
+        work[key] = bw
+    sp, args, stage = bw
+    block_shuffle(prep.bins, prep.T, perm.reshape(kk * n, -1), block_length, sp.bins, kk)
+    gene_prep_into(sp, features=False)
+    te_run(sp, None, None, _lib.TNX_OUT_STAGE, args=args)
+    call("tnx_stats_accumulate", ptr(stage), kk, stride, ptr(slots), slots.numel(),
+         ptr(observed), ptr(sum_te), ptr(sumsq_te), ptr(count_ge), _stream(prep.device))
+
+
+def _stacked_copies(prep: Prepared, plan: Plan, K: int, slot_map: torch.Tensor | None = None,
+                    stride: int = 0):
+    """Buffers for K stacked surrogate copies of prep's variables: the
+    copies' Prepared, TE args for the K-fold plan writing to a (K, n, n)
+    staging slab, or with slot_map (int32 (n, n), public layout, -1 off the
+    plan) to a compact (K, stride) one, and the slab."""
     n = prep.n_vars
-    src = prep.bins.repeat(K, 1).contiguous()
-    sp = Prepared(bins=torch.empty_like(src), npv=prep.npv.repeat(K).contiguous(),
+    sp = Prepared(bins=torch.empty((K * n, prep.ld), dtype=prep.bins.dtype, device=prep.device),
+                  npv=prep.npv.repeat(K).contiguous(),
                   npv_host=np.tile(prep.npv_host, K), T=prep.T, tau=prep.tau,
                   states=torch.empty((K * n, prep.lds), dtype=prep.states.dtype,
                                      device=prep.device),
@@ -718,16 +818,36 @@ def _stacked_copies(prep: Prepared, plan: Plan, K: int):
     if planK is None:
         planK = replicate_plan(plan, K, n)
         plan.cache[("copies", K)] = planK
-    stage = torch.empty(K * n * n, dtype=torch.float32, device=prep.device)
+    stride = n * n if slot_map is None else max(int(stride), 1)
+    stage = torch.empty(K * stride, dtype=torch.float32, device=prep.device)
     args = te_args(sp, planK, stage, _lib.TNX_OUT_STAGE, ld_out=n, stage_vars=n,
-                   stage_stride=n * n)
-    return src, sp, args, stage, planK
+                   stage_stride=stride, stage_slot=slot_map)
+    return sp, args, stage
+
+
+def copy_perms(n_vars: int, T: int, shuffle_method: str, block_length: int, k0: int, k1: int,
+               k_start: int, k_end: int, seed: int, rng_device, device, rows=None,
+               perm_batch: int = 32):
+    """Block orders of surrogates [k0, k1) inside a run over [k_start,
+    k_end): taken from the perm_batch-aligned draws block_perm_device makes
+    (and caches) for the run, so a launch may span several draws.  Returns
+    (perm int32 (k1 - k0, rows, n_blocks), eff_bl)."""
+    parts = []
+    q0 = k_start + (k0 - k_start) // perm_batch * perm_batch
+    eff = 1
+    while q0 < k1:
+        q1 = min(q0 + perm_batch, k_end)
+        p_, eff, _ = block_perm_device(n_vars, T, shuffle_method, block_length, q0, q1, seed,
+                                       rng_device, device, rows=rows)
+        parts.append(p_[max(k0, q0) - q0:min(k1, q1) - q0])
+        q0 = q1
+    return (parts[0] if len(parts) == 1 else torch.cat(parts)), eff
 
 
 def var_subset(prep: Prepared, vars_host: np.ndarray) -> Prepared:
     """The rows `vars_host` (sorted variable indices) of prep's bins as a
-    source for surrogate_stats_materialized: shuffling and re-preparing only
-    the variables a plan touches.  Cached on prep."""
+    source for the batched surrogate statistics: shuffling and re-preparing
+    only the variables a plan touches.  Cached on prep."""
     vars_host = np.ascontiguousarray(vars_host, dtype=np.int64)
     key = ("subset", hash(vars_host.tobytes()), len(vars_host))
     sub = prep.plans.get(key)
@@ -749,30 +869,124 @@ def var_subset(prep: Prepared, vars_host: np.ndarray) -> Prepared:
     return sub
 
 
+def _plan_slot_map(plan: Plan, m: int, device):
+    """(public slots, compact slot map) of the plan's pairs over m variables:
+    the sorted public slots source * m + target and an int32 (m * m) map
+    from each to its index there (-1 elsewhere).  Cached on the plan."""
+    got = plan.cache.get("slot_map")
+    if got is None:
+        x, y = plan_pairs(plan)
+        pub = np.unique(y * m + x)
+        mp = np.full(m * m, -1, np.int32)
+        mp[pub] = np.arange(len(pub), dtype=np.int32)
+        got = (h2d(pub, device), h2d(mp, device))
+        plan.cache["slot_map"] = got
+    return got
+
+
+def _stats_run(sub: Prepared, plan: Plan, rows, n_vars: int, observed, s, s2, cg, n_surr: int,
+               shuffle_method: str, block_length: int, seed: int, rng_device, k_start: int,
+               perm_batch: int, slots: torch.Tensor, stride: int, slot_map=None):
+    """Surrogates [k_start, k_start + n_surr) of sub's variables (rows of an
+    n_vars problem) in launches of copies_per_launch copies."""
+    k_end = k_start + n_surr
+    kb = max(1, copies_per_launch(sub, n_surr, stride))
+    work = {}
+    for k0 in range(k_start, k_end, kb):
+        k1 = min(k0 + kb, k_end)
+        perm, eff = copy_perms(n_vars, sub.T, shuffle_method, block_length, k0, k1, k_start,
+                               k_end, seed, rng_device, sub.device, rows=rows,
+                               perm_batch=perm_batch)
+        _staged_batch(sub, plan, perm, eff, work, slots, stride, observed, s, s2, cg, slot_map)
+
+
 def surrogate_stats_subset(prep: Prepared, vars_host: np.ndarray, plan: Plan,
-                           observed_pub: torch.Tensor, n_surr: int, shuffle_method: str,
+                           observed_pub: torch.Tensor | None, n_surr: int, shuffle_method: str,
                            block_length: int, seed: int, rng_device, k_start: int = 0,
                            perm_batch: int = 32):
     """Surrogate statistics of the pairs of `plan`, a plan over local indices
     of the variable subset vars_host (sorted): the subset's bins are
     block-shuffled with the same orders as in the full data (rows of
     block_perm_device), so every pair's statistics equal those of the full
-    computation.  Returns public-layout (m, m) sum, sumsq (float64) and
-    count_ge (int32) over the subset."""
+    computation.  Plans covering under half of the subset's slots stage
+    their pairs compactly.  Returns public-layout (m, m) sum, sumsq
+    (float64) and count_ge (int32; zero when observed_pub is None)."""
     sub = var_subset(prep, vars_host)
     m = len(vars_host)
     vd = sub.plans["rows"]
-    obs = observed_pub.index_select(0, vd).index_select(1, vd).contiguous()
-    s = torch.zeros((m, m), dtype=torch.float64, device=prep.device)
+    dev = prep.device
+    s = torch.zeros((m, m), dtype=torch.float64, device=dev)
     s2 = torch.zeros_like(s)
-    cg = torch.zeros((m, m), dtype=torch.int32, device=prep.device)
-    work = {}
-    k_end = k_start + n_surr
-    for k0 in range(k_start, k_end, perm_batch):
-        k1 = min(k0 + perm_batch, k_end)
-        perm, eff, nb = block_perm_device(prep.n_vars, prep.T, shuffle_method, block_length, k0,
-                                          k1, seed, rng_device, prep.device, rows=vd)
-        surrogate_stats_materialized(sub, plan, perm, eff, obs, s, s2, cg, work)
+    cg = torch.zeros((m, m), dtype=torch.int32, device=dev)
+    pub, slot_map = _plan_slot_map(plan, m, dev)
+    P = int(pub.numel())
+    if P == 0:
+        return s, s2, cg
+    compact = 2 * P < m * m
+    if compact:
+        obs = None if observed_pub is None else (
+            observed_pub.index_select(0, vd).index_select(1, vd).reshape(-1)[pub].contiguous())
+        sc = torch.zeros(P, dtype=torch.float64, device=dev)
+        s2c = torch.zeros_like(sc)
+        cgc = None if obs is None else torch.zeros(P, dtype=torch.int32, device=dev)
+        slots = torch.arange(P, dtype=torch.int64, device=dev)
+        _stats_run(sub, plan, vd, prep.n_vars, obs, sc, s2c, cgc, n_surr, shuffle_method,
+                   block_length, seed, rng_device, k_start, perm_batch, slots, P, slot_map)
+        s.view(-1)[pub] = sc
+        s2.view(-1)[pub] = s2c
+        if cgc is not None:
+            cg.view(-1)[pub] = cgc
+        return s, s2, cg
+    obs = None if observed_pub is None else (
+        observed_pub.index_select(0, vd).index_select(1, vd).contiguous())
+    _stats_run(sub, plan, vd, prep.n_vars, obs, s, s2, None if obs is None else cg, n_surr,
+               shuffle_method, block_length, seed, rng_device, k_start, perm_batch, pub, m * m)
+    return s, s2, cg
+
+
+def surrogate_stats_pairs(prep: Prepared, pairs: np.ndarray, observed: torch.Tensor | None,
+                          n_surr: int, shuffle_method: str, block_length: int, seed: int,
+                          rng_device, perm_batch: int = 32):
+    """Surrogate statistics of an explicit list of distinct [target, source]
+    pairs (host int (P, 2), target != source) as compact (P,) tensors: sum,
+    sumsq (float64) and count_ge (int32, against observed, float32 (P,);
+    zero when observed is None).  Only the variables the pairs touch are
+    shuffled, with the rows of the full draw's block orders, so each pair's
+    statistics equal those of the full computation; the TE kernels stage
+    the pairs compactly (K x P floats per launch)."""
+    pairs = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
+    P = len(pairs)
+    dev = prep.device
+    s = torch.zeros(P, dtype=torch.float64, device=dev)
+    s2 = torch.zeros_like(s)
+    cg = torch.zeros(P, dtype=torch.int32, device=dev)
+    if P == 0 or n_surr <= 0:
+        return s, s2, cg
+    if np.any(pairs[:, 0] == pairs[:, 1]):
+        raise ValueError("surrogate_stats_pairs: target == source pairs are not tested")
+    involved = np.unique(pairs)
+    loc = np.searchsorted(involved, pairs)
+    m = len(involved)
+    order = np.argsort(loc[:, 1] * m + loc[:, 0])
+    if len(np.unique(loc[:, 1] * m + loc[:, 0])) != P:
+        raise ValueError("surrogate_stats_pairs: pairs must be distinct")
+    sub = var_subset(prep, involved)
+    vd = sub.plans["rows"]
+    plan = te_pairlist_plan(sub.npv_host, loc, dev)
+    pub, slot_map = _plan_slot_map(plan, m, dev)   # compact index = rank of the public slot
+    od = h2d(order, dev)
+    obs = None if observed is None else (
+        observed.to(device=dev, dtype=torch.float32)[od].contiguous())
+    sc = torch.zeros(P, dtype=torch.float64, device=dev)
+    s2c = torch.zeros_like(sc)
+    cgc = None if obs is None else torch.zeros(P, dtype=torch.int32, device=dev)
+    slots = torch.arange(P, dtype=torch.int64, device=dev)
+    _stats_run(sub, plan, vd, prep.n_vars, obs, sc, s2c, cgc, n_surr, shuffle_method,
+               block_length, seed, rng_device, 0, perm_batch, slots, P, slot_map)
+    s[od] = sc
+    s2[od] = s2c
+    if cgc is not None:
+        cg[od] = cgc
     return s, s2, cg
 
 

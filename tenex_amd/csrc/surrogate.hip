@@ -400,19 +400,21 @@ __global__ __launch_bounds__(256) void surrogate_wave_kernel(tnx_surr_args_t a, 
     }
 }
 
-// One shuffled copy of every row: thread i of the grid writes 16 bytes of
-// one row (four words, zero-padded past T).
+// One shuffled copy of every output row: thread i of the grid writes 16
+// bytes of one row (four words, zero-padded past T).  Output row v reads
+// input row v % n_src (n_rows = copies x n_src, below 2^31).
 __global__ __launch_bounds__(256) void block_shuffle_kernel(const int8_t *__restrict__ bins,
-                                                            int64_t n_vars, int32_t T, int64_t ld,
+                                                            int64_t n_rows, int32_t n_src,
+                                                            int32_t T, int64_t ld,
                                                             const int32_t *__restrict__ perm,
                                                             int32_t n_blocks, int32_t bl,
                                                             int8_t *__restrict__ out) {
     const int64_t per_row = ld / 16;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_vars * per_row) return;
+    if (i >= n_rows * per_row) return;
     const int64_t v = i / per_row;
     const int32_t t16 = (int32_t)(i - v * per_row) * 16;
-    const int8_t *yr = bins + v * ld;
+    const int8_t *yr = bins + (int64_t)((uint32_t)v % (uint32_t)n_src) * ld;
     const BlockMap m{perm + v * n_blocks, bl, n_blocks * bl, n_blocks, 0xFFFFFFFFu / (uint32_t)bl};
     uint32_t w[4];
 #pragma unroll
@@ -434,23 +436,35 @@ __global__ __launch_bounds__(256) void block_shuffle_kernel(const int8_t *__rest
 
 using namespace tnx;
 
-extern "C" int tnx_block_shuffle(const int8_t *bins, int64_t n_vars, int64_t T, int64_t ld,
-                                 const int32_t *perm, int32_t n_blocks, int32_t block_length,
-                                 int8_t *out, void *stream) {
+extern "C" int tnx_block_shuffle_copies(const int8_t *bins, int64_t n_vars, int32_t n_copies,
+                                        int64_t T, int64_t ld, const int32_t *perm,
+                                        int32_t n_blocks, int32_t block_length, int8_t *out,
+                                        void *stream) {
     clear_error();
-    TNX_CHECK_ARG(bins && perm && out && n_vars >= 0 && T >= 1, "tnx_block_shuffle: bad arguments");
+    TNX_CHECK_ARG(bins && perm && out && n_vars >= 0 && n_copies >= 1 && T >= 1,
+                  "tnx_block_shuffle: bad arguments");
     TNX_CHECK_ARG(ld >= T && ld % 16 == 0, "tnx_block_shuffle: ld must be >= T, multiple of 16");
     TNX_CHECK_ARG(block_length >= 1 && n_blocks >= 1 && (int64_t)n_blocks * block_length <= T,
                   "tnx_block_shuffle: need n_blocks * block_length <= T");
     TNX_CHECK_ARG(T < (1LL << 31) / 2, "tnx_block_shuffle: T too large");
+    TNX_CHECK_ARG(n_vars * n_copies < (1LL << 31), "tnx_block_shuffle: too many rows");
     TNX_CHECK_ARG(bins != out, "tnx_block_shuffle: out must not alias bins");
     if (n_vars == 0) return TNX_OK;
-    const int64_t threads = n_vars * (ld / 16);
+    const int64_t rows = n_vars * n_copies;
+    const int64_t threads = rows * (ld / 16);
+    TNX_CHECK_ARG((threads + 255) / 256 < (1LL << 32), "tnx_block_shuffle: too many rows");
     hipLaunchKernelGGL(block_shuffle_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
-                       as_stream(stream), bins, n_vars, (int32_t)T, ld, perm, n_blocks,
-                       block_length, out);
+                       as_stream(stream), bins, rows, (int32_t)n_vars, (int32_t)T, ld, perm,
+                       n_blocks, block_length, out);
     TNX_CHECK_LAUNCH("block_shuffle_kernel");
     return TNX_OK;
+}
+
+extern "C" int tnx_block_shuffle(const int8_t *bins, int64_t n_vars, int64_t T, int64_t ld,
+                                 const int32_t *perm, int32_t n_blocks, int32_t block_length,
+                                 int8_t *out, void *stream) {
+    return tnx_block_shuffle_copies(bins, n_vars, 1, T, ld, perm, n_blocks, block_length, out,
+                                    stream);
 }
 
 extern "C" int tnx_surrogate_run(const tnx_surr_args_t *args, void *stream) {

@@ -212,6 +212,58 @@ __device__ __forceinline__ void hist_accumulate_regs(uint32_t *hist, TargetRegs<
     }
 }
 
+// The source chunks of one pair for the register path: chunk i of lane l is
+// bytes [16 l + 1024 i, +16) of the source row (zero past L).
+template <int NI>
+struct SourceChunks {
+    uint4 c[NI];
+};
+
+template <int NI>
+__device__ __forceinline__ void load_source_chunks(SourceChunks<NI> &ys,
+                                                   const int8_t *__restrict__ yr, int L,
+                                                   int lane) {
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+        const int t = 16 * lane + 1024 * i;
+        ys.c[i] = t < L ? *reinterpret_cast<const uint4 *>(yr + t) : make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
+// hist_accumulate_regs on source chunks already in registers (loaded by
+// load_source_chunks, possibly while the previous pair was reduced).
+template <int NI, bool HZ>
+__device__ __forceinline__ void hist_count_chunks(uint32_t *hist, TargetRegs<NI> &r,
+                                                  const SourceChunks<NI> &ys, uint32_t kmul,
+                                                  uint32_t ysh, uint32_t *zero, int L, int lane) {
+#pragma unroll
+    for (int k = 0; k < NI * 8; k++) asm volatile("" : "+v"(r.s[k]));
+    const uint32_t *sreg = r.s;
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+        const int t = 16 * lane + 1024 * i;
+        if (t >= L) break;
+        const uint32_t yw[4] = {ys.c[i].x, ys.c[i].y, ys.c[i].z, ys.c[i].w};
+        if (1024 * i + 1024 <= L) {   // wave-uniform: every lane's 16 samples inside L
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
+                const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                lds_inc_key0<HZ>(hist, sv, kmul, yv, ysh, zero);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                if (t + q < L) {
+                    const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
+                    const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                    lds_inc_key0<HZ>(hist, sv, kmul, yv, ysh, zero);
+                }
+            }
+        }
+    }
+}
+
 // Privatised histograms, cell-major: a pair with `cells` cells gets G copies
 // (G a power of two <= 32); cell k of copy r is word k * G + r and lane l
 // counts into copy l % G.  LDS has 32 banks of one word, so with G = 32 a

@@ -63,7 +63,7 @@ __device__ __forceinline__ void emit_te(const tnx_te_args_t &a, int64_t slot, do
         const double td = (double)tf;
         a.sum_te[slot] += td;
         a.sumsq_te[slot] += td * td;
-        a.count_ge[slot] += (tf >= a.observed[slot]) ? 1 : 0;
+        if (a.count_ge != nullptr) a.count_ge[slot] += (tf >= a.observed[slot]) ? 1 : 0;
     } else {
         a.out[slot] = (float)te;
     }
@@ -93,8 +93,14 @@ __device__ __forceinline__ bool out_slot(const tnx_te_args_t &a, int x, int y, i
     }
     if (a.out_mode == TNX_OUT_STAGE) {  // x, y in the same copy x / stage_vars
         const int k = x / a.stage_vars;
-        slot = (int64_t)k * a.stage_stride + (int64_t)(y - k * a.stage_vars) * a.ld_out +
-               (x - k * a.stage_vars);
+        const int64_t local =
+            (int64_t)(y - k * a.stage_vars) * a.ld_out + (x - k * a.stage_vars);
+        if (a.stage_slot != nullptr) {
+            const int32_t c = a.stage_slot[local];
+            slot = (int64_t)k * a.stage_stride + c;
+            return c >= 0;
+        }
+        slot = (int64_t)k * a.stage_stride + local;
         return true;
     }
     int64_t id = (int64_t)x * (a.n_vars - 1) + (y - (y > x ? 1 : 0));
@@ -126,28 +132,48 @@ __device__ __forceinline__ void te_wave_body(const tnx_te_args_t &a) {
     TargetRegs<(NI > 0 ? NI : 1)> regs;
     int cur_x = -1;
     double s12 = 0.0;  // S1[x] - S2a[x] of the target in registers
-    for (int64_t w = blockIdx.x; w < a.n_wave_items; w += gridDim.x) {
-        const tnx_work_t it = a.work[xcd_item(w, a.n_wave_items)];
+    // short series (NI <= 4): the next pair's source chunks are requested
+    // before the current pair's histogram is reduced, so their latency
+    // overlaps the reduction; pf_y names the row they hold (-1: none)
+    constexpr bool kPrefetch = NI > 0 && NI <= 4;
+    SourceChunks<(kPrefetch ? NI : 1)> ysrc;
+    int pf_y = -1;
+    // Items are dealt to workgroups (XCD-sliced order).  By default the four
+    // waves split an item's pairs; with item_per_wave each wave takes its
+    // own item of a group of four (plans of one-target items, where a wave
+    // then keeps its target in registers for all of the item's pairs).
+    const bool per_wave = a.item_per_wave != 0;
+    const int64_t n_groups =
+        per_wave ? (a.n_wave_items + kWavesPerWG - 1) / kWavesPerWG : a.n_wave_items;
+    const int pfirst = per_wave ? 0 : wave;
+    const int pstep = per_wave ? 1 : kWavesPerWG;
+    for (int64_t w = blockIdx.x; w < n_groups; w += gridDim.x) {
+        int64_t wi = xcd_item(w, n_groups);
+        if (per_wave) {
+            wi = wi * kWavesPerWG + wave;
+            if (wi >= a.n_wave_items) continue;
+        }
+        const tnx_work_t it = a.work[wi];
         const int cnt = it.tgt_cnt;
         const int npair = cnt * (it.src_end - it.src_beg);
-        // pair q = sj * cnt + ti; this wave takes q = wave, wave + 4, ...
-        // (ti, sj) advance without division, and the next pair's indices
-        // and level counts are loaded one pair ahead.
+        // pair q = sj * cnt + ti; this wave takes q = pfirst, pfirst + pstep,
+        // ... ((ti, sj) advance without division), and the next pair's
+        // indices and level counts are loaded one pair ahead.
         int ti = 0, sj = 0;
-        for (int u = 0; u < wave; u++)
+        for (int u = 0; u < pfirst; u++)
             if (++ti == cnt) { ti = 0; ++sj; }
         int xn = 0, yn = 0, bxn = 0, byn = 0;
-        if (wave < npair) {
+        if (pfirst < npair) {
             xn = a.tgt_order[it.tgt_beg + ti];
             yn = a.src_order[it.src_beg + sj];
             bxn = a.n_per_var[xn];
             byn = a.n_per_var[yn];
         }
-        for (int q = wave; q < npair; q += kWavesPerWG) {
+        for (int q = pfirst; q < npair; q += pstep) {
             const int x = xn, y = yn, bx = bxn, by = byn;
-            for (int u = 0; u < kWavesPerWG; u++)
+            for (int u = 0; u < pstep; u++)
                 if (++ti == cnt) { ti = 0; ++sj; }
-            if (q + kWavesPerWG < npair) {
+            if (q + pstep < npair) {
                 xn = a.tgt_order[it.tgt_beg + ti];
                 yn = a.src_order[it.src_beg + sj];
                 bxn = a.n_per_var[xn];
@@ -167,7 +193,15 @@ __device__ __forceinline__ void te_wave_body(const tnx_te_args_t &a) {
                 s12 = a.S1[x] - a.S2a[x];
                 cur_x = x;
             }
-            if constexpr (NI > 0) {
+            if constexpr (kPrefetch) {
+                if (pf_y != y) load_source_chunks<NI>(ysrc, yr, L, lane);
+                hist_count_chunks<NI, HZ>(hc, regs, ysrc, kmul, ysh, zero_row + lane, L, lane);
+                pf_y = -1;
+                if (q + pstep < npair) {
+                    load_source_chunks<NI>(ysrc, a.bins + (int64_t)yn * a.ld, L, lane);
+                    pf_y = yn;
+                }
+            } else if constexpr (NI > 0) {
                 hist_accumulate_regs<NI, HZ>(hc, regs, yr, kmul, ysh, zero_row + lane, L, lane);
             } else {
                 hist_accumulate_mem<HZ>(hc, a.states + (int64_t)x * a.lds, yr, kmul, ysh,
@@ -355,9 +389,10 @@ __global__ __launch_bounds__(256) void stats_accumulate_kernel(
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_slots) return;
     const int64_t sl = slots[i];
-    const float ob = observed[sl];
+    const bool counting = count != nullptr;
+    const float ob = counting ? observed[sl] : 0.f;
     double s = sum[sl], s2 = sumsq[sl];
-    int32_t c = count[sl];
+    int32_t c = counting ? count[sl] : 0;
     for (int k = 0; k < K; k++) {
         const float tf = stage[(int64_t)k * stride + sl];
         const double td = (double)tf;
@@ -367,7 +402,7 @@ __global__ __launch_bounds__(256) void stats_accumulate_kernel(
     }
     sum[sl] = s;
     sumsq[sl] = s2;
-    count[sl] = c;
+    if (counting) count[sl] = c;
 }
 
 template <bool HZ>
@@ -539,7 +574,8 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
                       a.src_order,
                   "tnx_te_run: null pointer");
     TNX_CHECK_ARG(a.out_mode == TNX_OUT_STATS
-                      ? (a.sum_te && a.sumsq_te && a.count_ge && a.observed && a.ld_out >= a.n_vars)
+                      ? (a.sum_te && a.sumsq_te && (a.count_ge == nullptr || a.observed) &&
+                         a.ld_out >= a.n_vars)
                       : (a.out != nullptr),
                   "tnx_te_run: null output pointer for out_mode %d", a.out_mode);
     TNX_CHECK_ARG(a.tau >= 1 && a.tau < a.T, "tnx_te_run: need 1 <= tau < T");
@@ -552,9 +588,10 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
                   "tnx_te_run: bad out_mode");
     TNX_CHECK_ARG(a.out_mode != TNX_OUT_STAGE ||
                       (a.stage_vars >= 1 && a.ld_out >= a.stage_vars &&
-                       a.stage_stride >= (int64_t)a.stage_vars * a.ld_out),
+                       (a.stage_slot != nullptr ? a.stage_stride >= 1
+                                                : a.stage_stride >= (int64_t)a.stage_vars * a.ld_out)),
                   "tnx_te_run: TNX_OUT_STAGE needs stage_vars >= 1, ld_out >= stage_vars and "
-                  "stage_stride >= stage_vars * ld_out");
+                  "stage_stride >= stage_vars * ld_out (>= 1 with stage_slot)");
     TNX_CHECK_ARG(a.cap_wave >= 8 && a.cap_wave <= 8192, "tnx_te_run: cap_wave out of range");
     TNX_CHECK_ARG((size_t)kWavesPerWG * wave_slice(a.cap_wave) * 4 <= 160 * 1024,
                   "tnx_te_run: cap_wave too large");
@@ -562,7 +599,10 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
     if (a.n_wave_items > 0) {
         TNX_CHECK_ARG(a.work, "tnx_te_run: null work");
         size_t shm = (size_t)kWavesPerWG * wave_slice(a.cap_wave) * 4;
-        unsigned grid = (unsigned)std::min<int64_t>(a.n_wave_items, 1 << 20);
+        const int64_t groups = a.item_per_wave
+                                   ? (a.n_wave_items + kWavesPerWG - 1) / kWavesPerWG
+                                   : a.n_wave_items;
+        unsigned grid = (unsigned)std::min<int64_t>(groups, 1 << 20);
         const int ni = wave_variant(a.T - a.tau);
         const int rc = a.hot_zero ? launch_wave<true>(ni, grid, shm, s, a)
                                   : launch_wave<false>(ni, grid, shm, s, a);
@@ -632,7 +672,7 @@ int tnx_stats_accumulate(const float *stage, int32_t K, int64_t stage_stride,
     clear_error();
     TNX_CHECK_ARG(K >= 0 && n_slots >= 0 && stage_stride >= 1, "tnx_stats_accumulate: bad sizes");
     if (K == 0 || n_slots == 0) return TNX_OK;
-    TNX_CHECK_ARG(stage && slots && observed && sum_te && sumsq_te && count_ge,
+    TNX_CHECK_ARG(stage && slots && sum_te && sumsq_te && (count_ge == nullptr || observed),
                   "tnx_stats_accumulate: null pointer");
     const unsigned grid = (unsigned)((n_slots + 255) / 256);
     hipLaunchKernelGGL(stats_accumulate_kernel, dim3(grid), dim3(256), 0, as_stream(stream), stage,

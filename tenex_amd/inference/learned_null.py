@@ -174,26 +174,28 @@ def _norm_stats(X: torch.Tensor):
 
 def plain_stats_pairs(prep, pairs, obs_pub, n_surr, shuffle_method, block_length, seed,
                       rng_device, perm_batch=32):
-    """Plain surrogate (sum, sumsq, count) of an explicit [target, source]
-    pair list (host int array (P, 2)), as device (P,) tensors (float64,
-    float64, int32): only the variables the pairs touch are shuffled."""
+    """Plain surrogate (sum, sumsq, count) of an explicit list of distinct
+    [target, source] pairs (host int array (P, 2)), as device (P,) tensors
+    (float64, float64, int32; no counting when obs_pub is None): only the
+    variables the pairs touch are shuffled, and the TE kernels stage the
+    pairs compactly."""
     pairs = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
-    involved = np.unique(pairs)
-    loc = np.searchsorted(involved, pairs)
-    sub = core.var_subset(prep, involved)
-    plan = core.te_pairlist_plan(sub.npv_host, loc, prep.device)
-    s, s2, cg = core.surrogate_stats_subset(prep, involved, plan, obs_pub, n_surr,
-                                            shuffle_method, block_length, seed, rng_device,
-                                            perm_batch=perm_batch)
-    ld = core.h2d(loc, prep.device)
-    return s[ld[:, 1], ld[:, 0]], s2[ld[:, 1], ld[:, 0]], cg[ld[:, 1], ld[:, 0]]
+    obs = None
+    if obs_pub is not None:
+        pd = core.h2d(pairs, prep.device)
+        obs = obs_pub[pd[:, 1], pd[:, 0]]
+    return core.surrogate_stats_pairs(prep, pairs, obs, n_surr, shuffle_method, block_length,
+                                      seed, rng_device, perm_batch=perm_batch)
 
 
 def plain_stats_grid(prep, targets, sources, obs_pub, n_surr, shuffle_method, block_length, seed,
                      rng_device, perm_batch=32):
-    """Plain surrogate (sum, sumsq) of every pair of targets x sources, as
-    device (n, n) public-layout tensors (other slots zero), through the TE
-    kernels on block-shuffled copies of the grid's variables."""
+    """Plain surrogate (sum, sumsq) of every pair of targets x sources,
+    through the TE kernels on block-shuffled copies of the grid's
+    variables.  Returns (sum, sumsq, involved, local): (m, m) public-layout
+    float64 tensors over the sorted variables `involved` (host) and the
+    device map local (n,) int64 from a variable to its row there (-1 when
+    not involved)."""
     n = prep.n_vars
     dev = prep.device
     tg = np.asarray(targets, dtype=np.int64)
@@ -205,12 +207,9 @@ def plain_stats_grid(prep, targets, sources, obs_pub, n_surr, shuffle_method, bl
     ss, ss2, _ = core.surrogate_stats_subset(prep, involved, plan, obs_pub, n_surr,
                                              shuffle_method, block_length, seed, rng_device,
                                              perm_batch=perm_batch)
-    s = torch.zeros((n, n), dtype=torch.float64, device=dev)
-    s2 = torch.zeros_like(s)
-    vd = sub.plans["rows"]
-    s[vd[:, None], vd[None, :]] = ss
-    s2[vd[:, None], vd[None, :]] = ss2
-    return s, s2
+    lm = np.full(n, -1, np.int64)
+    lm[involved] = np.arange(len(involved))
+    return ss, ss2, involved, core.h2d(lm, dev)
 
 
 def training_grid(srcs: np.ndarray, tgts: np.ndarray, n_train: int, rng):
@@ -241,11 +240,14 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     extra `learned_info` dict (training size, re-run size, timings, losses).
     `mask` is the (n, n) bool tensor of tested [source, target] pairs."""
     from . import bh_reject_gpu
-    from .surrogate_test import SurrogateTestResult, surrogate_decision
+    from .surrogate_test import (SurrogateTestResult, surrogate_decision,
+                                 surrogate_decision_pairs)
     t0 = time.perf_counter()
+    obs_src = obs_d       # a device tensor, or a core.PendingUpload still in flight
     n = prep.n_vars
     mask = torch.as_tensor(mask, dtype=torch.bool, device=dev)
     pub_d = torch.nonzero(mask)                           # [source, target] of tested pairs
+    n_mask = int(pub_d.shape[0])
     if src_idx is not None:
         pub_d = pub_d[torch.isin(pub_d[:, 0], core.h2d(np.asarray(src_idx, np.int64), dev))]
     P = int(pub_d.shape[0])
@@ -256,8 +258,8 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     #    unseen targets
     gt, gs = training_grid(torch.unique(pub_d[:, 0]).cpu().numpy(),
                            torch.unique(pub_d[:, 1]).cpu().numpy(), n_want, rng)
-    S_g, S2_g = plain_stats_grid(prep, gt, gs, obs_d, n_surrogates, shuffle_method,
-                                 block_length, seed, rng_device)
+    ss, ss2, _, lmap = plain_stats_grid(prep, gt, gs, None, n_surrogates, shuffle_method,
+                                        block_length, seed, rng_device)
     n_ho = max(1, len(gt) // 5) if len(gt) > 1 else 0
     ho_t = gt[:n_ho]
     in_gs = torch.zeros(n, dtype=torch.bool, device=dev)
@@ -269,10 +271,11 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     grid_rows = in_gs[pub_d[:, 0]] & in_gt[pub_d[:, 1]]       # tested pairs on the grid
     trd = torch.nonzero(grid_rows).flatten()
     n_train = int(trd.numel())
-    fit_rows = trd[~in_ho[pub_d[trd, 1]]]
-    ho_rows = trd[in_ho[pub_d[trd, 1]]]
-    mean_g, std_g = _moments(S_g, S2_g, n_surrogates)
-    mean_tr, std_tr = mean_g[pub_d[trd, 0], pub_d[trd, 1]], std_g[pub_d[trd, 0], pub_d[trd, 1]]
+    is_ho = in_ho[pub_d[trd, 1]]
+    fit_rows = trd[~is_ho]
+    ho_rows = trd[is_ho]
+    ls, lt = lmap[pub_d[trd, 0]], lmap[pub_d[trd, 1]]
+    mean_tr, std_tr = _moments(ss[ls, lt], ss2[ls, lt], n_surrogates)
     torch.cuda.synchronize(dev)
     t_train_plain = time.perf_counter() - t0
     # 2. train the MLP on (mean, log std)
@@ -292,12 +295,12 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     Yp = model.forward(Xn).to(torch.float64) * ysd + ymu
     mean_p = Yp[:, 0].contiguous()
     std_p = torch.exp(Yp[:, 1])
+    obs_d = obs_src.get() if isinstance(obs_src, core.PendingUpload) else obs_src
     obs = obs_d[pub_d[:, 0], pub_d[:, 1]].to(torch.float64)
     zf = lambda m, sdv: (obs - m) / torch.clamp(sdv, min=1e-12)
     z = zf(mean_p, std_p)
     if ho_rows.numel():
-        hs, ht = pub_d[ho_rows, 0], pub_d[ho_rows, 1]
-        z_true = (obs[ho_rows] - mean_g[hs, ht]) / torch.clamp(std_g[hs, ht], min=1e-12)
+        z_true = (obs[ho_rows] - mean_tr[is_ho]) / torch.clamp(std_tr[is_ho], min=1e-12)
         z_err = float(torch.sqrt(torch.mean((z[ho_rows] - z_true) ** 2)))
     else:
         z_err = 0.0
@@ -319,18 +322,25 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     # 4. plain surrogates for the borderline pairs
     if len(rr):
         rr_pub = pub_d[rr_d].cpu().numpy()
-        s_b, s2_b, _ = plain_stats_pairs(prep, rr_pub[:, ::-1], obs_d, n_surrogates,
+        s_b, s2_b, _ = plain_stats_pairs(prep, rr_pub[:, ::-1], None, n_surrogates,
                                          shuffle_method, block_length, seed, rng_device)
         mean_p[rr_d], std_p[rr_d] = _moments(s_b, s2_b, n_surrogates)
     torch.cuda.synchronize(dev)
     t_rerun = time.perf_counter() - t0 - t_train_plain - t_fit - t_predict
     # 5. decision with the reference arithmetic on the merged null
-    S = torch.zeros((n, n), dtype=torch.float64, device=dev)
-    S2 = torch.zeros_like(S)
-    S[pub_d[:, 0], pub_d[:, 1]] = mean_p * n_surrogates
-    S2[pub_d[:, 0], pub_d[:, 1]] = (std_p * std_p + mean_p * mean_p) * n_surrogates
-    mean_m, std_m, eff, pv, sigm = surrogate_decision(obs_d, S, S2, None, n_surrogates,
-                                                      "parametric", fdr, mask, dev, host=False)
+    S = mean_p * n_surrogates
+    S2 = (std_p * std_p + mean_p * mean_p) * n_surrogates
+    if P == n_mask:
+        mean_m, std_m, eff, pv, sigm = surrogate_decision_pairs(obs_d, pub_d, S, S2,
+                                                                n_surrogates, fdr)
+    else:   # tested pairs the source filter dropped keep zero sums, as in the dense form
+        Sd = torch.zeros((n, n), dtype=torch.float64, device=dev)
+        S2d = torch.zeros_like(Sd)
+        Sd[pub_d[:, 0], pub_d[:, 1]] = S
+        S2d[pub_d[:, 0], pub_d[:, 1]] = S2
+        mean_m, std_m, eff, pv, sigm = surrogate_decision(obs_d, Sd, S2d, None, n_surrogates,
+                                                          "parametric", fdr, mask, dev,
+                                                          host=False)
     edges = torch.nonzero(sigm)
     grn = make_grn(np.asarray(variable_names), edges.cpu().numpy().astype(np.int64),
                    eff[edges[:, 0], edges[:, 1]].cpu().numpy())

@@ -99,7 +99,8 @@ def surrogate_accumulate(prep: core.Prepared, observed_pub: torch.Tensor, n_surr
                          perm_batch: int = 32, k_start: int = 0,
                          te_k_dbg: torch.Tensor | None = None):
     """Device (n, n) public-layout sum, sumsq (float64) and count_ge (int32)
-    over surrogates [k_start, k_start + n_surrogates).
+    over surrogates [k_start, k_start + n_surrogates).  observed_pub None
+    (default path only): count_ge is not counted (left zero).
 
     fused=None (default): per surrogate a block-shuffled copy of the bins in
     HBM through the TE kernels with their statistics epilogue.  fused=True:
@@ -133,12 +134,13 @@ def surrogate_accumulate(prep: core.Prepared, observed_pub: torch.Tensor, n_surr
             return s, s2, cg
         plan = core.get_plan(prep, targets, sources)
         work = {}
-        for k0 in range(k_start, k_end, perm_batch):
-            k1 = min(k0 + perm_batch, k_end)
-            perm, eff_bl, nb = core.block_perm_device(n, T, shuffle_method, block_length, k0, k1,
-                                                      seed, rng_device, dev)
-            core.surrogate_stats_materialized(prep, plan, perm, eff_bl, observed_pub, s, s2, cg,
-                                              work)
+        kb = max(perm_batch, core.copies_per_launch(prep, n_surrogates, n * n))
+        for k0 in range(k_start, k_end, kb):
+            k1 = min(k0 + kb, k_end)
+            perm, eff_bl = core.copy_perms(n, T, shuffle_method, block_length, k0, k1, k_start,
+                                           k_end, seed, rng_device, dev, perm_batch=perm_batch)
+            core.surrogate_stats_materialized(prep, plan, perm, eff_bl, observed_pub, s, s2,
+                                              None if observed_pub is None else cg, work)
         return s, s2, cg
     if fused is None or fused:
         plan = core.surrogate_plan(prep, targets, sources)
@@ -204,6 +206,46 @@ def surrogate_decision(observed_te, sum_s, sumsq_s, count_ge, n_surr, p_method, 
     return tuple(x.cpu().numpy() for x in out) if host else out
 
 
+def surrogate_decision_pairs(observed_te, pairs, sum_s, sumsq_s, n_surr, fdr):
+    """surrogate_decision (parametric, on the device) for sums given only at
+    the tested pairs: pairs (P, 2) [source, target] device int64 (distinct,
+    off the diagonal), sum_s / sumsq_s (P,) float64.  The dense decision's
+    arithmetic on vectors: untested entries have zero sums, so their mean
+    and std are 0, eff = obs - 0 and z = eff / clamp(0, 1e-12), computed
+    over the dense matrix as there; the tested entries are computed on the
+    P vectors and scattered.  BH runs over the tested p-values (its
+    decisions do not depend on their order).  Returns the same (mean, std,
+    eff, p, sig) tensors as surrogate_decision(host=False) with the sums
+    scattered to dense matrices and mask = the pairs."""
+    obs = observed_te.to(torch.float32)
+    dev = obs.device
+    src, tgt = pairs[:, 0], pairs[:, 1]
+    mean = torch.zeros_like(obs)
+    std = torch.zeros_like(obs)
+    eff = obs - mean
+    tiny = torch.clamp(torch.zeros(1, dtype=torch.float32, device=dev), min=1e-12)
+    z = eff / tiny
+    p = (0.5 * torch.special.erfc(z.to(torch.float64) / np.sqrt(2.0))).to(torch.float32)
+    S = sum_s.to(torch.float64)
+    S2 = sumsq_s.to(torch.float64)
+    ov = obs[src, tgt]
+    mean64 = S / n_surr
+    mv = mean64.to(torch.float32)
+    var = S2 / n_surr - mean64 * mean64
+    sv = torch.sqrt(torch.clamp(var, min=0.0)).to(torch.float32)
+    ev = ov - mv
+    zv = (ov - mv) / torch.clamp(sv, min=1e-12)
+    pv = (0.5 * torch.special.erfc(zv.to(torch.float64) / np.sqrt(2.0))).to(torch.float32)
+    mean[src, tgt] = mv
+    std[src, tgt] = sv
+    eff[src, tgt] = ev
+    p[src, tgt] = pv
+    p.fill_diagonal_(1.0)
+    sig = torch.zeros(obs.shape, dtype=torch.bool, device=dev)
+    sig[src, tgt] = bh_reject_gpu(pv, fdr) & (ev > 0)
+    return mean, std, eff, p, sig
+
+
 class SurrogateTestMethod(InferenceMethod):
     """Surrogate-based test with effective-TE bias correction."""
 
@@ -257,6 +299,8 @@ class SurrogateTestMethod(InferenceMethod):
         observed_te = np.asarray(te_matrix, dtype=np.float32)
         dev = core.require_gpu(device if device is not None and torch.device(device).type == "cuda"
                                else None)
+        # the observed matrix travels while the masks, shuffles and TE launches are set up
+        obs_up = core.PendingUpload(observed_te, dev)
         rng_device = kwargs.get("rng_device") or dev
         sources = kwargs.get("sources")
         targets = kwargs.get("targets")
@@ -291,18 +335,21 @@ class SurrogateTestMethod(InferenceMethod):
         if npv_in.shape[0] == n_vars and bool((npv_in >= npv_d).all()):
             npv_d = npv_in.contiguous()
         prep = core.prepare(bins, npv_d, T, tau)
-        obs_d = core.h2d(observed_te, dev)
         learned = kwargs.get("learned")
         if learned is not None:
             from .learned_null import learned_surrogate_test
             return learned_surrogate_test(
-                prep, observed_te, obs_d, variable_names, n_surrogates, shuffle_method,
+                prep, observed_te, obs_up, variable_names, n_surrogates, shuffle_method,
                 block_length, seed, rng_device, p_method, fdr, mask, src_idx, dev,
                 **(kwargs.get("learned_kwargs") or {}))
         tg_plan = tgt_idx if (scope == "sources" and tgt_idx is not None) else None
-        s, s2, cg = surrogate_accumulate(prep, obs_d, n_surrogates, shuffle_method, block_length,
-                                         seed, rng_device, fused=fused_kw, targets=tg_plan,
-                                         sources=src_idx)
+        # the parametric decision reads only the sums: count_ge is needed for
+        # mc p-values (and the fused / loop paths always count)
+        counted = p_method == "mc" or fused_kw is not None
+        s, s2, cg = surrogate_accumulate(prep, obs_up.get() if counted else None, n_surrogates,
+                                         shuffle_method, block_length, seed, rng_device,
+                                         fused=fused_kw, targets=tg_plan, sources=src_idx)
+        obs_d = obs_up.get()
         if fused_kw is not False:
             cg.diagonal().add_(n_surrogates)  # the loop path's diagonal (te_k = obs = 0)
         mean, std, eff, p, sig = surrogate_decision(obs_d, s, s2, cg, n_surrogates, p_method,

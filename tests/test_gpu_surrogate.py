@@ -107,6 +107,16 @@ def test_block_shuffle_bit_exact(dev):
             exp[v, :head] = raw[v, :head].reshape(nb, bl)[p[v]].ravel()
         assert np.array_equal(got, exp.astype(np.int8)), (n, T, bl)
         assert np.all(out.cpu().numpy()[:, T:] == 0)
+        # K stacked copies: copy k of row v uses perm row k * n + v
+        K = 3
+        permK = torch.stack([torch.randperm(nb, generator=torch.Generator().manual_seed(100 + i))
+                             for i in range(K * n)]).to(torch.int32).to(dev)
+        outK = torch.full((K * n, bins.shape[1]), -1, dtype=torch.int8, device=dev)
+        core.block_shuffle(bins, T, permK, bl, outK, copies=K)
+        for k in range(K):
+            one = torch.full_like(bins, -1)
+            core.block_shuffle(bins, T, permK[k * n:(k + 1) * n], bl, one)
+            assert torch.equal(outK[k * n:(k + 1) * n], one), (n, T, bl, k)
 
 
 @pytest.mark.parametrize("bl,tau", [(20, 1), (33, 2)])
@@ -269,3 +279,78 @@ def test_surrogate_test_end_to_end(dev):
     exp = set(map(tuple, np.argwhere(dec["sig"]).tolist()))
     assert len(got ^ exp) <= 1
     assert (3, 7) in got
+
+
+def test_pair_stats_batches_straddle_draws(dev, monkeypatch):
+    """Compact pair-list statistics with a few copies per launch, so the
+    launches straddle the 32-surrogate block-order draws: every pair's
+    statistics still equal the full computation's bit for bit."""
+    from tenex_amd.inference.learned_null import plain_stats_pairs
+    d, npv = _small(n=30, T=400, seed=8)
+    prep = _prep(d, npv, dev)
+    obs = torch.from_numpy(core.te_matrix_internal(prep).cpu().numpy().T.copy()).to(dev)
+    full = surrogate_accumulate(prep, obs, 40, "block", 20, 11, dev)
+    rng = np.random.default_rng(1)
+    allp = np.array([[t, s] for t in range(30) for s in range(30) if t != s], np.int64)
+    pairs = allp[rng.choice(len(allp), 50, replace=False)]
+    monkeypatch.setattr(core, "STAGE_FLOATS", 7 * len(pairs))
+    s, s2, cg = plain_stats_pairs(prep, pairs, obs, 40, "block", 20, 11, dev)
+    for got, ref in zip((s, s2, cg), full):
+        assert np.array_equal(got.cpu().numpy(), ref.cpu().numpy()[pairs[:, 1], pairs[:, 0]])
+    with pytest.raises(ValueError):
+        core.surrogate_stats_pairs(prep, np.array([[3, 3]]), obs[:1, 0], 4, "block", 20, 11, dev)
+    with pytest.raises(ValueError):
+        core.surrogate_stats_pairs(prep, np.array([[3, 4], [3, 4]]), obs[:2, 0], 4, "block", 20,
+                                   11, dev)
+
+
+def test_decision_pairs_matches_dense(dev):
+    """The decision on the tested pairs' sums gives the dense decision's
+    matrices bit for bit, untested entries included."""
+    from tenex_amd.inference.surrogate_test import surrogate_decision, surrogate_decision_pairs
+    n, K = 60, 100
+    rng = np.random.default_rng(3)
+    obs = (rng.random((n, n)) * 0.05).astype(np.float32)
+    obs[rng.random((n, n)) < 0.1] = 0.0
+    obs[0, 5], obs[1, 7], obs[2, 9] = 1e-20, -1e-3, 3e-13
+    np.fill_diagonal(obs, 0.0)
+    mask = rng.random((n, n)) < 0.3
+    np.fill_diagonal(mask, False)
+    pairs = np.argwhere(mask)
+    mean = rng.random(len(pairs)) * 0.03
+    sd = rng.random(len(pairs)) * 0.004 + 1e-4
+    S = mean * K
+    S2 = (sd * sd + mean * mean) * K
+    Sd = np.zeros((n, n))
+    S2d = np.zeros((n, n))
+    Sd[pairs[:, 0], pairs[:, 1]] = S
+    S2d[pairs[:, 0], pairs[:, 1]] = S2
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    a = surrogate_decision(t(obs), t(Sd), t(S2d), None, K, "parametric", 0.05, t(mask), dev,
+                           host=False)
+    b = surrogate_decision_pairs(t(obs), t(pairs.astype(np.int64)), t(S), t(S2), K, 0.05)
+    assert int(a[4].sum()) > 0
+    for x, y in zip(a, b):
+        assert x.dtype == y.dtype and torch.equal(x, y)
+
+
+def test_uncounted_stats_and_pending_upload(dev):
+    """observed None skips count_ge and leaves the sums unchanged; the
+    observed matrix uploaded on a side stream arrives intact."""
+    d, npv = _small(n=28, T=360, seed=12)
+    n = d.shape[0]
+    prep = _prep(d, npv, dev)
+    obs_h = core.te_matrix_internal(prep).cpu().numpy().T.copy()
+    up = core.PendingUpload(obs_h, dev)
+    obs = up.get()
+    assert np.array_equal(obs.cpu().numpy(), obs_h)
+    tg = np.array([1, 4, 9, 20], np.int64)
+    sr = np.array([0, 4, 13, 27], np.int64)
+    a = surrogate_accumulate(prep, obs, 9, "block", 21, 8, dev, targets=tg, sources=sr)
+    b = surrogate_accumulate(prep, None, 9, "block", 21, 8, dev, targets=tg, sources=sr)
+    c = surrogate_accumulate(prep, None, 9, "block", 21, 8, dev)
+    full = surrogate_accumulate(prep, obs, 9, "block", 21, 8, dev)
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+    assert int(b[2].abs().sum()) == 0 and int(a[2].sum()) > 0
+    assert torch.equal(c[0], full[0]) and torch.equal(c[1], full[1])
+    assert int(c[2].abs().sum()) == 0

@@ -184,3 +184,32 @@ def test_hip_graph_replay_matches_eager(dev):
         got = out.clone()
         eager = core.te_matrix_internal(prep, cap_wave=512)
         assert torch.equal(got, eager), rep
+
+
+@pytest.mark.parametrize("kind", ["pairs", "grid"])
+def test_item_per_wave_matches_split(dev, kind):
+    """A wave per work item (the pair-list default once the items fill the
+    chip) and four waves splitting each item give the same bits; the TE
+    matches the FP64 oracle."""
+    b = synthetic.mixed_cardinality_bins(90, 1500, seed=21)
+    prep = _prep_from_dense(b, dev)
+    n = prep.n_vars
+    if kind == "pairs":
+        rng = np.random.default_rng(5)
+        allp = O.pairfree_pairs(n)
+        pairs = allp[rng.choice(len(allp), 2500, replace=False)]     # [target, source]
+        plan = core.te_pairlist_plan(prep.npv_host, pairs, dev)
+        assert np.all(plan.work_host[:plan.info.n_wave_items, 1] == 1)
+    else:
+        plan = core.get_plan(prep)
+        pairs = O.pairfree_pairs(n)
+    outs = []
+    for ipw in (False, True):
+        out = torch.full((n, n), -1.0, dtype=torch.float32, device=dev)
+        args = core.te_args(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n, item_per_wave=ipw)
+        core.te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, args=args)
+        outs.append(out.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+    te64 = O.te_pairs(b.astype(np.int32), prep.npv_host, pairs)
+    got = outs[1][pairs[:, 0], pairs[:, 1]]
+    assert np.abs(got.astype(np.float64) - te64).max() < TOL
