@@ -161,15 +161,11 @@ TNX_API int tnx_te_plan(const int32_t *n_per_var, int64_t n_vars, int64_t L,
 #define TNX_OUT_MATRIX 1   /* out[target * ld_out + source] (internal layout) */
 #define TNX_OUT_STATS 2    /* surrogate statistics at the public slot source * ld_out + target:
                             * te = (float)TE; sum_te += te; sumsq_te += te^2 (FP64);
-                            * count_ge += (te >= observed[slot]) (skipped when count_ge is
-                            * NULL) */
+                            * count_ge += (te >= observed[slot]) */
 #define TNX_OUT_STAGE 3    /* K surrogate copies in one launch: variable v = k * m + j
                             * (m = stage_vars) is copy k of variable j; out[k * stage_stride
                             * + (source % m) * ld_out + (target % m)] = (float)TE, for
-                            * tnx_stats_accumulate to fold in copy order.  With stage_slot
-                            * set, the offset inside a copy is instead
-                            * stage_slot[(source % m) * ld_out + (target % m)] (pairs mapped
-                            * to -1 are skipped): a sparse pair list stages K x P floats */
+                            * tnx_stats_accumulate to fold in copy order */
 
 typedef struct {
     const int8_t *bins;       /* (n_vars, ld) */
@@ -193,10 +189,7 @@ typedef struct {
                         * per-lane words instead of one histogram cell.
                         * Results are identical either way; only speed differs. */
     int32_t stage_vars;    /* TNX_OUT_STAGE only: variables per copy (m) */
-    int64_t stage_stride;  /* TNX_OUT_STAGE only: floats per copy (>= m * ld_out, or
-                            * > every stage_slot entry) */
-    const int32_t *stage_slot; /* TNX_OUT_STAGE only, optional: (m * ld_out) map of a
-                                * copy's public slots to compact offsets (or -1) */
+    int64_t stage_stride;  /* TNX_OUT_STAGE only: floats per copy (>= m * ld_out) */
     int32_t item_per_wave;     /* nonzero: each wave of a wave-kernel workgroup takes its own
                                 * work item (suits plans of one-target items: pair lists);
                                 * zero: the four waves split each item's pairs.  Results are

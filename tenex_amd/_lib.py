@@ -52,7 +52,7 @@ class TeArgs(C.Structure):
                 ("out", p_), ("ld_out", i64), ("pair_offset", i64), ("n_pairs_local", i64),
                 ("sum_te", p_), ("sumsq_te", p_), ("count_ge", p_), ("observed", p_),
                 ("hot_zero", C.c_int32), ("stage_vars", C.c_int32), ("stage_stride", C.c_int64),
-                ("stage_slot", C.c_void_p), ("item_per_wave", C.c_int32)]
+                ("item_per_wave", C.c_int32)]
 
 
 class SurrArgs(C.Structure):

@@ -417,11 +417,9 @@ def get_plan(prep: Prepared, targets=None, sources=None, cap_wave: int = 0,
 
 def te_args(prep: Prepared, plan: Plan, out: torch.Tensor | None, out_mode: int, ld_out: int = 0,
             pair_offset: int = 0, n_pairs_local: int = 0, stats=None, stage_vars: int = 0,
-            stage_stride: int = 0, stage_slot: torch.Tensor | None = None,
-            item_per_wave: bool | None = None) -> _lib.TeArgs:
+            stage_stride: int = 0, item_per_wave: bool | None = None) -> _lib.TeArgs:
     """stats: (sum, sumsq, count_ge, observed) public-layout tensors for
-    TNX_OUT_STATS; stage_vars / stage_stride (and the optional int32 slot
-    map stage_slot) for TNX_OUT_STAGE.  item_per_wave None: chosen from the
+    TNX_OUT_STATS; stage_vars / stage_stride for TNX_OUT_STAGE.  item_per_wave None: chosen from the
     plan (item_per_wave_default)."""
     i = plan.info
     if item_per_wave is None:
@@ -437,7 +435,7 @@ def te_args(prep: Prepared, plan: Plan, out: torch.Tensor | None, out_mode: int,
         ld_out=ld_out, pair_offset=pair_offset, n_pairs_local=n_pairs_local,
         sum_te=ptr(st[0]), sumsq_te=ptr(st[1]), count_ge=ptr(st[2]), observed=ptr(st[3]),
         hot_zero=int(prep.hot_zero), stage_vars=int(stage_vars), stage_stride=int(stage_stride),
-        stage_slot=ptr(stage_slot), item_per_wave=int(bool(item_per_wave)))
+        item_per_wave=int(bool(item_per_wave)))
 
 
 WAVE_ITEM_PAIRS = 32   # pairs of a one-target item that a single wave takes whole
@@ -711,7 +709,7 @@ def block_shuffle(bins: torch.Tensor, T: int, perm_k: torch.Tensor, block_length
          perm_k.shape[1], int(block_length), ptr(out), _stream(bins.device))
 
 
-STAGE_FLOATS = 64 << 20   # staging for batched surrogate copies: 256 MiB of float32
+STAGE_FLOATS = 1 << 29    # staging for batched surrogate copies: 2 GiB of float32
 COPY_BYTES = 6 << 30      # stacked surrogate copies (bins and states) held at once
 PERM_DRAW_FLOATS = 32 << 20   # uniform draws of one batch of block orders held at once
 PERM_CACHE_INTS = 128 << 20   # block orders kept across calls (512 MiB of int32)
@@ -749,8 +747,8 @@ def surrogate_stats_materialized(prep: Prepared, plan: Plan, perm: torch.Tensor,
             plan.cache["slots"] = slots
         for k0 in range(0, n_k, kb):
             kk = min(kb, n_k - k0)
-            _staged_batch(prep, plan, perm[k0:k0 + kk], block_length, work, slots, n * n,
-                          observed, sum_te, sumsq_te, count_ge)
+            _staged_batch(prep, plan, perm[k0:k0 + kk], block_length, work, slots, observed,
+                          sum_te, sumsq_te, count_ge)
         return
     sp = work.get("prep")
     if sp is None:
@@ -760,6 +758,12 @@ def surrogate_stats_materialized(prep: Prepared, plan: Plan, perm: torch.Tensor,
                       S2a=torch.empty_like(prep.S2a), feat=torch.empty_like(prep.feat),
                       xlogx=prep.xlogx, hot_zero=prep.hot_zero)  # block shuffles keep levels
         work["prep"] = sp
+    if count_ge is None:   # the kernels' statistics epilogue always counts: a discarded count
+        count_ge = work.get("discard_count")
+        if count_ge is None:
+            count_ge = torch.zeros_like(sum_te, dtype=torch.int32)
+            work["discard_count"] = count_ge
+        observed = torch.zeros_like(sum_te, dtype=torch.float32)
     args = te_args(sp, plan, None, _lib.TNX_OUT_STATS, ld_out=n,
                    stats=(sum_te, sumsq_te, count_ge, observed))
     for kk in range(n_k):
@@ -777,32 +781,29 @@ def copies_per_launch(prep: Prepared, n_k: int, stage_per_copy: int) -> int:
 
 
 def _staged_batch(prep: Prepared, plan: Plan, perm: torch.Tensor, block_length: int, work: dict,
-                  slots: torch.Tensor, stride: int, observed, sum_te, sumsq_te, count_ge,
-                  slot_map: torch.Tensor | None = None):
+                  slots: torch.Tensor, observed, sum_te, sumsq_te, count_ge):
     """One launch of each kernel for the K = perm.shape[0] surrogate copies
-    of prep's variables: shuffle, prepare, TE to the staging slab (dense
-    public layout, or compact through slot_map with `stride` slots), fold
-    the slab into the statistics at `slots` in copy order."""
+    of prep's variables: shuffle, prepare, TE to the (K, n, n) staging slab,
+    fold the slab into the public-layout statistics at `slots` in copy
+    order (count_ge None: no counting)."""
     kk, n = perm.shape[0], prep.n_vars
-    key = ("batch", kk, slot_map is not None)
-    bw = work.get(key)
+    bw = work.get(("batch", kk))
     if bw is None:
-        bw = _stacked_copies(prep, plan, kk, slot_map, stride)
-        work[key] = bw
+        bw = _stacked_copies(prep, plan, kk)
+        work[("batch", kk)] = bw
     sp, args, stage = bw
     block_shuffle(prep.bins, prep.T, perm.reshape(kk * n, -1), block_length, sp.bins, kk)
     gene_prep_into(sp, features=False)
     te_run(sp, None, None, _lib.TNX_OUT_STAGE, args=args)
-    call("tnx_stats_accumulate", ptr(stage), kk, stride, ptr(slots), slots.numel(),
+    call("tnx_stats_accumulate", ptr(stage), kk, n * n, ptr(slots), slots.numel(),
          ptr(observed), ptr(sum_te), ptr(sumsq_te), ptr(count_ge), _stream(prep.device))
 
 
-def _stacked_copies(prep: Prepared, plan: Plan, K: int, slot_map: torch.Tensor | None = None,
-                    stride: int = 0):
+def _stacked_copies(prep: Prepared, plan: Plan, K: int):
     """Buffers for K stacked surrogate copies of prep's variables: the
     copies' Prepared, TE args for the K-fold plan writing to a (K, n, n)
-    staging slab, or with slot_map (int32 (n, n), public layout, -1 off the
-    plan) to a compact (K, stride) one, and the slab."""
+    staging slab (only the plan's slots are written and read), and the
+    slab."""
     n = prep.n_vars
     sp = Prepared(bins=torch.empty((K * n, prep.ld), dtype=prep.bins.dtype, device=prep.device),
                   npv=prep.npv.repeat(K).contiguous(),
@@ -818,10 +819,9 @@ def _stacked_copies(prep: Prepared, plan: Plan, K: int, slot_map: torch.Tensor |
     if planK is None:
         planK = replicate_plan(plan, K, n)
         plan.cache[("copies", K)] = planK
-    stride = n * n if slot_map is None else max(int(stride), 1)
-    stage = torch.empty(K * stride, dtype=torch.float32, device=prep.device)
+    stage = torch.empty(K * n * n, dtype=torch.float32, device=prep.device)
     args = te_args(sp, planK, stage, _lib.TNX_OUT_STAGE, ld_out=n, stage_vars=n,
-                   stage_stride=stride, stage_slot=slot_map)
+                   stage_stride=n * n)
     return sp, args, stage
 
 
@@ -869,35 +869,34 @@ def var_subset(prep: Prepared, vars_host: np.ndarray) -> Prepared:
     return sub
 
 
-def _plan_slot_map(plan: Plan, m: int, device):
-    """(public slots, compact slot map) of the plan's pairs over m variables:
-    the sorted public slots source * m + target and an int32 (m * m) map
-    from each to its index there (-1 elsewhere).  Cached on the plan."""
-    got = plan.cache.get("slot_map")
+def _plan_slots(plan: Plan, m: int, device) -> torch.Tensor:
+    """Sorted public slots source * m + target of the plan's pairs over m
+    variables (device int64).  Cached on the plan."""
+    got = plan.cache.get("slots")
     if got is None:
         x, y = plan_pairs(plan)
-        pub = np.unique(y * m + x)
-        mp = np.full(m * m, -1, np.int32)
-        mp[pub] = np.arange(len(pub), dtype=np.int32)
-        got = (h2d(pub, device), h2d(mp, device))
-        plan.cache["slot_map"] = got
+        got = h2d(np.unique(y * m + x), device)
+        plan.cache["slots"] = got
     return got
 
 
 def _stats_run(sub: Prepared, plan: Plan, rows, n_vars: int, observed, s, s2, cg, n_surr: int,
                shuffle_method: str, block_length: int, seed: int, rng_device, k_start: int,
-               perm_batch: int, slots: torch.Tensor, stride: int, slot_map=None):
+               perm_batch: int):
     """Surrogates [k_start, k_start + n_surr) of sub's variables (rows of an
-    n_vars problem) in launches of copies_per_launch copies."""
+    n_vars problem) into sub's public-layout statistics, in launches of
+    copies_per_launch copies."""
+    m = sub.n_vars
     k_end = k_start + n_surr
-    kb = max(1, copies_per_launch(sub, n_surr, stride))
+    kb = max(1, copies_per_launch(sub, n_surr, m * m))
+    slots = _plan_slots(plan, m, sub.device)
     work = {}
     for k0 in range(k_start, k_end, kb):
         k1 = min(k0 + kb, k_end)
         perm, eff = copy_perms(n_vars, sub.T, shuffle_method, block_length, k0, k1, k_start,
                                k_end, seed, rng_device, sub.device, rows=rows,
                                perm_batch=perm_batch)
-        _staged_batch(sub, plan, perm, eff, work, slots, stride, observed, s, s2, cg, slot_map)
+        _staged_batch(sub, plan, perm, eff, work, slots, observed, s, s2, cg)
 
 
 def surrogate_stats_subset(prep: Prepared, vars_host: np.ndarray, plan: Plan,
@@ -908,9 +907,8 @@ def surrogate_stats_subset(prep: Prepared, vars_host: np.ndarray, plan: Plan,
     of the variable subset vars_host (sorted): the subset's bins are
     block-shuffled with the same orders as in the full data (rows of
     block_perm_device), so every pair's statistics equal those of the full
-    computation.  Plans covering under half of the subset's slots stage
-    their pairs compactly.  Returns public-layout (m, m) sum, sumsq
-    (float64) and count_ge (int32; zero when observed_pub is None)."""
+    computation.  Returns public-layout (m, m) sum, sumsq (float64) and
+    count_ge (int32; zero when observed_pub is None)."""
     sub = var_subset(prep, vars_host)
     m = len(vars_host)
     vd = sub.plans["rows"]
@@ -918,29 +916,10 @@ def surrogate_stats_subset(prep: Prepared, vars_host: np.ndarray, plan: Plan,
     s = torch.zeros((m, m), dtype=torch.float64, device=dev)
     s2 = torch.zeros_like(s)
     cg = torch.zeros((m, m), dtype=torch.int32, device=dev)
-    pub, slot_map = _plan_slot_map(plan, m, dev)
-    P = int(pub.numel())
-    if P == 0:
-        return s, s2, cg
-    compact = 2 * P < m * m
-    if compact:
-        obs = None if observed_pub is None else (
-            observed_pub.index_select(0, vd).index_select(1, vd).reshape(-1)[pub].contiguous())
-        sc = torch.zeros(P, dtype=torch.float64, device=dev)
-        s2c = torch.zeros_like(sc)
-        cgc = None if obs is None else torch.zeros(P, dtype=torch.int32, device=dev)
-        slots = torch.arange(P, dtype=torch.int64, device=dev)
-        _stats_run(sub, plan, vd, prep.n_vars, obs, sc, s2c, cgc, n_surr, shuffle_method,
-                   block_length, seed, rng_device, k_start, perm_batch, slots, P, slot_map)
-        s.view(-1)[pub] = sc
-        s2.view(-1)[pub] = s2c
-        if cgc is not None:
-            cg.view(-1)[pub] = cgc
-        return s, s2, cg
     obs = None if observed_pub is None else (
         observed_pub.index_select(0, vd).index_select(1, vd).contiguous())
     _stats_run(sub, plan, vd, prep.n_vars, obs, s, s2, None if obs is None else cg, n_surr,
-               shuffle_method, block_length, seed, rng_device, k_start, perm_batch, pub, m * m)
+               shuffle_method, block_length, seed, rng_device, k_start, perm_batch)
     return s, s2, cg
 
 
@@ -948,46 +927,42 @@ def surrogate_stats_pairs(prep: Prepared, pairs: np.ndarray, observed: torch.Ten
                           n_surr: int, shuffle_method: str, block_length: int, seed: int,
                           rng_device, perm_batch: int = 32):
     """Surrogate statistics of an explicit list of distinct [target, source]
-    pairs (host int (P, 2), target != source) as compact (P,) tensors: sum,
-    sumsq (float64) and count_ge (int32, against observed, float32 (P,);
-    zero when observed is None).  Only the variables the pairs touch are
+    pairs (host int (P, 2), target != source) as (P,) tensors: sum, sumsq
+    (float64) and count_ge (int32, against observed, float32 (P,); zero
+    when observed is None).  Only the variables the pairs touch are
     shuffled, with the rows of the full draw's block orders, so each pair's
-    statistics equal those of the full computation; the TE kernels stage
-    the pairs compactly (K x P floats per launch)."""
+    statistics equal those of the full computation."""
     pairs = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
     P = len(pairs)
     dev = prep.device
-    s = torch.zeros(P, dtype=torch.float64, device=dev)
-    s2 = torch.zeros_like(s)
-    cg = torch.zeros(P, dtype=torch.int32, device=dev)
     if P == 0 or n_surr <= 0:
-        return s, s2, cg
+        z = torch.zeros(P, dtype=torch.float64, device=dev)
+        return z, z.clone(), torch.zeros(P, dtype=torch.int32, device=dev)
     if np.any(pairs[:, 0] == pairs[:, 1]):
         raise ValueError("surrogate_stats_pairs: target == source pairs are not tested")
     involved = np.unique(pairs)
     loc = np.searchsorted(involved, pairs)
     m = len(involved)
-    order = np.argsort(loc[:, 1] * m + loc[:, 0])
-    if len(np.unique(loc[:, 1] * m + loc[:, 0])) != P:
+    pub = loc[:, 1] * m + loc[:, 0]
+    if len(np.unique(pub)) != P:
         raise ValueError("surrogate_stats_pairs: pairs must be distinct")
     sub = var_subset(prep, involved)
     vd = sub.plans["rows"]
     plan = te_pairlist_plan(sub.npv_host, loc, dev)
-    pub, slot_map = _plan_slot_map(plan, m, dev)   # compact index = rank of the public slot
-    od = h2d(order, dev)
-    obs = None if observed is None else (
-        observed.to(device=dev, dtype=torch.float32)[od].contiguous())
-    sc = torch.zeros(P, dtype=torch.float64, device=dev)
-    s2c = torch.zeros_like(sc)
-    cgc = None if obs is None else torch.zeros(P, dtype=torch.int32, device=dev)
-    slots = torch.arange(P, dtype=torch.int64, device=dev)
-    _stats_run(sub, plan, vd, prep.n_vars, obs, sc, s2c, cgc, n_surr, shuffle_method,
-               block_length, seed, rng_device, 0, perm_batch, slots, P, slot_map)
-    s[od] = sc
-    s2[od] = s2c
-    if cgc is not None:
-        cg[od] = cgc
-    return s, s2, cg
+    s = torch.zeros((m, m), dtype=torch.float64, device=dev)
+    s2 = torch.zeros_like(s)
+    cg = None
+    obs = None
+    pd = h2d(pub, dev)
+    if observed is not None:
+        cg = torch.zeros((m, m), dtype=torch.int32, device=dev)
+        obs = torch.zeros((m, m), dtype=torch.float32, device=dev)
+        obs.view(-1)[pd] = observed.to(device=dev, dtype=torch.float32)
+    _stats_run(sub, plan, vd, prep.n_vars, obs, s, s2, cg, n_surr, shuffle_method,
+               block_length, seed, rng_device, 0, perm_batch)
+    out_cg = (torch.zeros(P, dtype=torch.int32, device=dev) if cg is None
+              else cg.view(-1)[pd])
+    return s.view(-1)[pd], s2.view(-1)[pd], out_cg
 
 
 def surrogate_plan(prep: Prepared, targets=None, sources=None) -> Plan:

@@ -63,7 +63,7 @@ __device__ __forceinline__ void emit_te(const tnx_te_args_t &a, int64_t slot, do
         const double td = (double)tf;
         a.sum_te[slot] += td;
         a.sumsq_te[slot] += td * td;
-        if (a.count_ge != nullptr) a.count_ge[slot] += (tf >= a.observed[slot]) ? 1 : 0;
+        a.count_ge[slot] += (tf >= a.observed[slot]) ? 1 : 0;
     } else {
         a.out[slot] = (float)te;
     }
@@ -93,14 +93,8 @@ __device__ __forceinline__ bool out_slot(const tnx_te_args_t &a, int x, int y, i
     }
     if (a.out_mode == TNX_OUT_STAGE) {  // x, y in the same copy x / stage_vars
         const int k = x / a.stage_vars;
-        const int64_t local =
-            (int64_t)(y - k * a.stage_vars) * a.ld_out + (x - k * a.stage_vars);
-        if (a.stage_slot != nullptr) {
-            const int32_t c = a.stage_slot[local];
-            slot = (int64_t)k * a.stage_stride + c;
-            return c >= 0;
-        }
-        slot = (int64_t)k * a.stage_stride + local;
+        slot = (int64_t)k * a.stage_stride + (int64_t)(y - k * a.stage_vars) * a.ld_out +
+               (x - k * a.stage_vars);
         return true;
     }
     int64_t id = (int64_t)x * (a.n_vars - 1) + (y - (y > x ? 1 : 0));
@@ -142,7 +136,8 @@ __device__ __forceinline__ void te_wave_body(const tnx_te_args_t &a) {
     // waves split an item's pairs; with item_per_wave each wave takes its
     // own item of a group of four (plans of one-target items, where a wave
     // then keeps its target in registers for all of the item's pairs).
-    const bool per_wave = a.item_per_wave != 0;
+    constexpr bool kPerWaveOK = NI > 0 && NI <= 4;   // a hint, honoured for short series
+    const bool per_wave = kPerWaveOK && a.item_per_wave != 0;
     const int64_t n_groups =
         per_wave ? (a.n_wave_items + kWavesPerWG - 1) / kWavesPerWG : a.n_wave_items;
     const int pfirst = per_wave ? 0 : wave;
@@ -574,7 +569,7 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
                       a.src_order,
                   "tnx_te_run: null pointer");
     TNX_CHECK_ARG(a.out_mode == TNX_OUT_STATS
-                      ? (a.sum_te && a.sumsq_te && (a.count_ge == nullptr || a.observed) &&
+                      ? (a.sum_te && a.sumsq_te && a.count_ge && a.observed &&
                          a.ld_out >= a.n_vars)
                       : (a.out != nullptr),
                   "tnx_te_run: null output pointer for out_mode %d", a.out_mode);
@@ -588,10 +583,9 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
                   "tnx_te_run: bad out_mode");
     TNX_CHECK_ARG(a.out_mode != TNX_OUT_STAGE ||
                       (a.stage_vars >= 1 && a.ld_out >= a.stage_vars &&
-                       (a.stage_slot != nullptr ? a.stage_stride >= 1
-                                                : a.stage_stride >= (int64_t)a.stage_vars * a.ld_out)),
+                       a.stage_stride >= (int64_t)a.stage_vars * a.ld_out),
                   "tnx_te_run: TNX_OUT_STAGE needs stage_vars >= 1, ld_out >= stage_vars and "
-                  "stage_stride >= stage_vars * ld_out (>= 1 with stage_slot)");
+                  "stage_stride >= stage_vars * ld_out");
     TNX_CHECK_ARG(a.cap_wave >= 8 && a.cap_wave <= 8192, "tnx_te_run: cap_wave out of range");
     TNX_CHECK_ARG((size_t)kWavesPerWG * wave_slice(a.cap_wave) * 4 <= 160 * 1024,
                   "tnx_te_run: cap_wave too large");
@@ -599,11 +593,11 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
     if (a.n_wave_items > 0) {
         TNX_CHECK_ARG(a.work, "tnx_te_run: null work");
         size_t shm = (size_t)kWavesPerWG * wave_slice(a.cap_wave) * 4;
-        const int64_t groups = a.item_per_wave
-                                   ? (a.n_wave_items + kWavesPerWG - 1) / kWavesPerWG
-                                   : a.n_wave_items;
-        unsigned grid = (unsigned)std::min<int64_t>(groups, 1 << 20);
         const int ni = wave_variant(a.T - a.tau);
+        const bool per_wave = a.item_per_wave && ni > 0 && ni <= 4;  // as te_wave_body
+        const int64_t groups =
+            per_wave ? (a.n_wave_items + kWavesPerWG - 1) / kWavesPerWG : a.n_wave_items;
+        unsigned grid = (unsigned)std::min<int64_t>(groups, 1 << 20);
         const int rc = a.hot_zero ? launch_wave<true>(ni, grid, shm, s, a)
                                   : launch_wave<false>(ni, grid, shm, s, a);
         if (rc != TNX_OK) return rc;

@@ -282,7 +282,7 @@ def test_surrogate_test_end_to_end(dev):
 
 
 def test_pair_stats_batches_straddle_draws(dev, monkeypatch):
-    """Compact pair-list statistics with a few copies per launch, so the
+    """Pair-list statistics with a few copies per launch, so the
     launches straddle the 32-surrogate block-order draws: every pair's
     statistics still equal the full computation's bit for bit."""
     from tenex_amd.inference.learned_null import plain_stats_pairs
@@ -293,7 +293,8 @@ def test_pair_stats_batches_straddle_draws(dev, monkeypatch):
     rng = np.random.default_rng(1)
     allp = np.array([[t, s] for t in range(30) for s in range(30) if t != s], np.int64)
     pairs = allp[rng.choice(len(allp), 50, replace=False)]
-    monkeypatch.setattr(core, "STAGE_FLOATS", 7 * len(pairs))
+    m = len(np.unique(pairs))
+    monkeypatch.setattr(core, "STAGE_FLOATS", 7 * m * m)   # 7 copies per launch
     s, s2, cg = plain_stats_pairs(prep, pairs, obs, 40, "block", 20, 11, dev)
     for got, ref in zip((s, s2, cg), full):
         assert np.array_equal(got.cpu().numpy(), ref.cpu().numpy()[pairs[:, 1], pairs[:, 0]])
