@@ -230,34 +230,67 @@ def _moments(s, s2, n_surr):
     return mean, torch.sqrt(torch.clamp(s2 / n_surr - mean * mean, min=0.0))
 
 
+def _tested_pairs(rows: np.ndarray, cols: np.ndarray, device=None):
+    """The tested [source, target] pairs rows x cols off the diagonal, in
+    row-major order (the order of nonzero over the mask), built on `device`
+    (None: host numpy), and the sources and targets that occur in them
+    (sorted, host)."""
+    R = np.sort(np.asarray(rows, np.int64))
+    Cc = np.sort(np.asarray(cols, np.int64))
+    if device is None:
+        pub = np.stack([np.repeat(R, len(Cc)), np.tile(Cc, len(R))], 1)
+        pub = pub[pub[:, 0] != pub[:, 1]]
+    else:
+        Rd, Cd = core.h2d(R, device), core.h2d(Cc, device)
+        pub = torch.stack([Rd.repeat_interleave(len(Cc)), Cd.repeat(len(R))], 1)
+        if len(np.intersect1d(R, Cc)):
+            pub = pub[pub[:, 0] != pub[:, 1]]
+    # a row (column) occurs unless its only partner is itself
+    srcs = R[(len(Cc) > 1) | ((len(Cc) == 1) & (Cc[:1] != R))] if len(Cc) else R[:0]
+    tgts = Cc[(len(R) > 1) | ((len(R) == 1) & (R[:1] != Cc))] if len(R) else Cc[:0]
+    return pub, srcs, tgts
+
+
 def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogates,
                            shuffle_method, block_length, seed, rng_device, p_method, fdr, mask,
                            src_idx, dev, train_frac: float = 0.05, band: float = 1.0,
                            steps: int = 150, lr: float = 6e-3, min_train: int = 2000,
                            model_seed: int = 0, band_sigmas: float = 5.0,
-                           fit_batch: int = 4096):
+                           fit_batch: int = 4096, tested=None):
     """Learned-null surrogate test; returns a SurrogateTestResult with an
     extra `learned_info` dict (training size, re-run size, timings, losses).
-    `mask` is the (n, n) bool tensor of tested [source, target] pairs."""
+    `mask` is the (n, n) bool tensor of tested [source, target] pairs (or a
+    callable that builds it when needed);
+    `tested` = (rows, cols), when given, says that the mask is rows x cols
+    off the diagonal (then the pair list is built from the two sets rather
+    than by a scan of the mask)."""
     from . import bh_reject_gpu
     from .surrogate_test import (SurrogateTestResult, surrogate_decision,
                                  surrogate_decision_pairs)
     t0 = time.perf_counter()
     obs_src = obs_d       # a device tensor, or a core.PendingUpload still in flight
     n = prep.n_vars
-    mask = torch.as_tensor(mask, dtype=torch.bool, device=dev)
-    pub_d = torch.nonzero(mask)                           # [source, target] of tested pairs
-    n_mask = int(pub_d.shape[0])
-    if src_idx is not None:
-        pub_d = pub_d[torch.isin(pub_d[:, 0], core.h2d(np.asarray(src_idx, np.int64), dev))]
+    if tested is not None and (src_idx is None or
+                               np.array_equal(np.sort(np.asarray(src_idx, np.int64)),
+                                              np.sort(np.asarray(tested[0], np.int64)))):
+        pub_d, srcs_u, tgts_u = _tested_pairs(*tested, device=dev)
+        n_mask = int(pub_d.shape[0])
+    else:
+        mask = torch.as_tensor(mask() if callable(mask) else mask, dtype=torch.bool, device=dev)
+        pub_d = torch.nonzero(mask)                       # [source, target] of tested pairs
+        n_mask = int(pub_d.shape[0])
+        if src_idx is not None:
+            pub_d = pub_d[torch.isin(pub_d[:, 0],
+                                     core.h2d(np.asarray(src_idx, np.int64), dev))]
+        srcs_u = torch.unique(pub_d[:, 0]).cpu().numpy()
+        tgts_u = torch.unique(pub_d[:, 1]).cpu().numpy()
     P = int(pub_d.shape[0])
     rng = np.random.default_rng(seed)
     n_want = min(P, max(min_train, int(train_frac * P)))
     # 1. plain surrogates on a grid of targets x sources; a fifth of the grid's
     #    targets is held out whole to measure how the model carries to
     #    unseen targets
-    gt, gs = training_grid(torch.unique(pub_d[:, 0]).cpu().numpy(),
-                           torch.unique(pub_d[:, 1]).cpu().numpy(), n_want, rng)
+    gt, gs = training_grid(srcs_u, tgts_u, n_want, rng)
     ss, ss2, _, lmap = plain_stats_grid(prep, gt, gs, None, n_surrogates, shuffle_method,
                                         block_length, seed, rng_device)
     n_ho = max(1, len(gt) // 5) if len(gt) > 1 else 0
@@ -268,13 +301,13 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     in_gt[core.h2d(gt.astype(np.int64), dev)] = True
     in_ho = torch.zeros(n, dtype=torch.bool, device=dev)
     in_ho[core.h2d(ho_t.astype(np.int64), dev)] = True
-    grid_rows = in_gs[pub_d[:, 0]] & in_gt[pub_d[:, 1]]       # tested pairs on the grid
+    grid_rows = in_gs[pub_d[:, 0]] & in_gt[pub_d[:, 1]]   # tested pairs on the grid
     trd = torch.nonzero(grid_rows).flatten()
-    n_train = int(trd.numel())
     is_ho = in_ho[pub_d[trd, 1]]
     fit_rows = trd[~is_ho]
     ho_rows = trd[is_ho]
     ls, lt = lmap[pub_d[trd, 0]], lmap[pub_d[trd, 1]]
+    n_train = int(trd.numel())
     mean_tr, std_tr = _moments(ss[ls, lt], ss2[ls, lt], n_surrogates)
     torch.cuda.synchronize(dev)
     t_train_plain = time.perf_counter() - t0
@@ -295,23 +328,33 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     Yp = model.forward(Xn).to(torch.float64) * ysd + ymu
     mean_p = Yp[:, 0].contiguous()
     std_p = torch.exp(Yp[:, 1])
-    obs_d = obs_src.get() if isinstance(obs_src, core.PendingUpload) else obs_src
-    obs = obs_d[pub_d[:, 0], pub_d[:, 1]].to(torch.float64)
+    if tested is not None and isinstance(obs_src, core.PendingUpload):
+        # the tested rows of the observed matrix alone (the whole matrix is
+        # still in flight; the decision waits for it)
+        R = np.sort(np.asarray(tested[0], np.int64))
+        obs_rows = core.h2d(np.asarray(observed_te, np.float32)[R], dev)
+        rpos = torch.searchsorted(core.h2d(R, dev), pub_d[:, 0].contiguous())
+        obs = obs_rows[rpos, pub_d[:, 1]].to(torch.float64)
+    else:
+        obs_d = obs_src.get() if isinstance(obs_src, core.PendingUpload) else obs_src
+        obs = obs_d[pub_d[:, 0], pub_d[:, 1]].to(torch.float64)
     zf = lambda m, sdv: (obs - m) / torch.clamp(sdv, min=1e-12)
     z = zf(mean_p, std_p)
     if ho_rows.numel():
         z_true = (obs[ho_rows] - mean_tr[is_ho]) / torch.clamp(std_tr[is_ho], min=1e-12)
-        z_err = float(torch.sqrt(torch.mean((z[ho_rows] - z_true) ** 2)))
+        z_err_d = torch.sqrt(torch.mean((z[ho_rows] - z_true) ** 2))
     else:
-        z_err = 0.0
-    band = max(band, band_sigmas * z_err)
+        z_err_d = torch.zeros((), dtype=torch.float64, device=dev)
     mean_p[trd] = mean_tr
     std_p[trd] = std_tr
     z = zf(mean_p, std_p)
     p_pred = (0.5 * torch.special.erfc(z / math.sqrt(2.0))).to(torch.float32)
     sig = bh_reject_gpu(p_pred, fdr) & (obs - mean_p > 0)
-    if bool(sig.any()):
-        z_thr = float(z[sig].min())
+    z_sig_min = torch.where(sig, z, torch.full_like(z, math.inf)).min()
+    z_err, z_min = (float(v) for v in torch.stack([z_err_d, z_sig_min]).cpu())
+    band = max(band, band_sigmas * z_err)
+    if math.isfinite(z_min):
+        z_thr = z_min
     else:
         z_thr = float(-math.log(fdr / max(P, 1)))  # no predicted edge: a high bar
     border = torch.abs(z - z_thr) < band
@@ -328,20 +371,22 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     torch.cuda.synchronize(dev)
     t_rerun = time.perf_counter() - t0 - t_train_plain - t_fit - t_predict
     # 5. decision with the reference arithmetic on the merged null
+    obs_d = obs_src.get() if isinstance(obs_src, core.PendingUpload) else obs_src
     S = mean_p * n_surrogates
     S2 = (std_p * std_p + mean_p * mean_p) * n_surrogates
     if P == n_mask:
         mean_m, std_m, eff, pv, sigm = surrogate_decision_pairs(obs_d, pub_d, S, S2,
                                                                 n_surrogates, fdr)
+        edges = pub_d[sigm[pub_d[:, 0], pub_d[:, 1]]]     # row-major, as nonzero(sigm)
     else:   # tested pairs the source filter dropped keep zero sums, as in the dense form
         Sd = torch.zeros((n, n), dtype=torch.float64, device=dev)
         S2d = torch.zeros_like(Sd)
         Sd[pub_d[:, 0], pub_d[:, 1]] = S
         S2d[pub_d[:, 0], pub_d[:, 1]] = S2
-        mean_m, std_m, eff, pv, sigm = surrogate_decision(obs_d, Sd, S2d, None, n_surrogates,
-                                                          "parametric", fdr, mask, dev,
-                                                          host=False)
-    edges = torch.nonzero(sigm)
+        mean_m, std_m, eff, pv, sigm = surrogate_decision(
+            obs_d, Sd, S2d, None, n_surrogates, "parametric", fdr,
+            mask() if callable(mask) else mask, dev, host=False)
+        edges = torch.nonzero(sigm)
     grn = make_grn(np.asarray(variable_names), edges.cpu().numpy().astype(np.int64),
                    eff[edges[:, 0], edges[:, 1]].cpu().numpy())
     t_total = time.perf_counter() - t0

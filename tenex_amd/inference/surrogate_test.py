@@ -299,49 +299,60 @@ class SurrogateTestMethod(InferenceMethod):
         observed_te = np.asarray(te_matrix, dtype=np.float32)
         dev = core.require_gpu(device if device is not None and torch.device(device).type == "cuda"
                                else None)
-        # the observed matrix travels while the masks, shuffles and TE launches are set up
-        obs_up = core.PendingUpload(observed_te, dev)
         rng_device = kwargs.get("rng_device") or dev
         sources = kwargs.get("sources")
         targets = kwargs.get("targets")
         scope = kwargs.get("pairs_scope", "all")
+        learned = kwargs.get("learned")
         names_arr = np.asarray(variable_names)
-        mask = torch.zeros((n_vars, n_vars), dtype=torch.bool, device=dev)
+        s_rows = tgt_idx = None
         if sources is not None:
             # rows = the sources, every other variable a target (build_pairs)
             _, s_rows, _ = np.intersect1d(names_arr, sources, return_indices=True)
-            mask[core.h2d(s_rows.astype(np.int64), dev)] = True
-        else:
-            mask.fill_(True)
-        mask.fill_diagonal_(False)
-        tgt_idx = None
         if targets is not None:
             # extension: restrict the tested pairs to these target names too
             _, tgt_idx, _ = np.intersect1d(names_arr, targets, return_indices=True)
-            tmask = torch.zeros(n_vars, dtype=torch.bool, device=dev)
-            tmask[core.h2d(tgt_idx.astype(np.int64), dev)] = True
-            mask &= tmask[None, :]
             tgt_idx = tgt_idx.astype(np.int32)
         src_idx = None
         if scope == "sources" and sources is not None:
-            _, src_idx, _ = np.intersect1d(np.asarray(variable_names), sources,
-                                           return_indices=True)
-            src_idx = src_idx.astype(np.int32)
+            src_idx = s_rows.astype(np.int32)
         vprint(f"[TENEX] surrogate_test: {n_surrogates} shuffles (method={shuffle_method}, "
                f"block_length={block_length}, kernel={kernel_name}, p={p_method}, "
                f"device={dev})")
         bins, npv_d, _, T_ = core.upload_bins(bin_data, dev)
+        # the observed matrix travels while the shuffles and TE launches are set up
+        obs_up = core.PendingUpload(observed_te, dev)
         npv_in = core.h2d(np.asarray(n_per_var, dtype=np.int32), dev)
         if npv_in.shape[0] == n_vars and bool((npv_in >= npv_d).all()):
             npv_d = npv_in.contiguous()
         prep = core.prepare(bins, npv_d, T, tau)
-        learned = kwargs.get("learned")
+        rows = np.sort(s_rows) if s_rows is not None else np.arange(n_vars)
+        cols = np.sort(tgt_idx) if tgt_idx is not None else np.arange(n_vars)
+        tested = (rows, cols) if len(rows) * len(cols) <= (1 << 24) else None
+
+        def tested_mask():
+            """(n, n) bool of the tested [source, target] pairs: rows x cols
+            off the diagonal."""
+            m = torch.zeros((n_vars, n_vars), dtype=torch.bool, device=dev)
+            if s_rows is not None:
+                m[core.h2d(s_rows.astype(np.int64), dev)] = True
+            else:
+                m.fill_(True)
+            m.fill_diagonal_(False)
+            if tgt_idx is not None:
+                tm = torch.zeros(n_vars, dtype=torch.bool, device=dev)
+                tm[core.h2d(tgt_idx.astype(np.int64), dev)] = True
+                m &= tm[None, :]
+            return m
+
         if learned is not None:
             from .learned_null import learned_surrogate_test
             return learned_surrogate_test(
                 prep, observed_te, obs_up, variable_names, n_surrogates, shuffle_method,
-                block_length, seed, rng_device, p_method, fdr, mask, src_idx, dev,
-                **(kwargs.get("learned_kwargs") or {}))
+                block_length, seed, rng_device, p_method, fdr,
+                tested_mask if tested is not None else tested_mask(), src_idx, dev,
+                tested=tested, **(kwargs.get("learned_kwargs") or {}))
+        mask = tested_mask()
         tg_plan = tgt_idx if (scope == "sources" and tgt_idx is not None) else None
         # the parametric decision reads only the sums: count_ge is needed for
         # mc p-values (and the fused / loop paths always count)

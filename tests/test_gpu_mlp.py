@@ -146,3 +146,32 @@ def test_learned_surrogate_end_to_end(dev):
     b = set(map(tuple, lr.grn.pairs.tolist()))
     jac = len(a & b) / max(1, len(a | b))
     assert jac >= 0.9, (jac, len(a), len(b))
+
+
+def test_learned_host_and_device_pair_lists_agree(dev, monkeypatch):
+    """Sources and targets given: the learned null builds its pair lists on
+    the host; forcing the device path (nonzero over the mask) gives the same
+    p-values, edges and re-run set."""
+    import tenex_amd as tnx
+    from tenex_amd.inference import learned_null as LN
+    n, T = 150, 1000
+    x = synthetic.zero_inflated_lognormal(n, T, seed=14)
+    rng = np.random.default_rng(6)
+    for s, t in [(1, 2), (10, 11), (20, 33), (40, 41)]:
+        x[t, 1:] = x[s, :-1] + 0.2 * rng.random(T - 1).astype(np.float32)
+    names = np.array([f"g{i}" for i in range(n)])
+    src = names[[1, 10, 20, 40, 55, 70, 90, 100, 120, 140, 2]]
+    tgt = names[np.r_[2:60, 100:150]]              # overlaps the sources
+    res = tnx.TransferEntropyEngine(x, names, sources=src).compute(accelerator="gpu")
+    nw = tnx.NetWeaver(res, fdr=0.05)
+    kw = dict(n_surrogates=20, targets=tgt, pairs_scope="sources", learned="mlp",
+              learned_kwargs=dict(min_train=300, steps=100))
+    a = nw.infer("surrogate_test", **kw)
+    orig = LN.learned_surrogate_test
+    monkeypatch.setattr(LN, "learned_surrogate_test",
+                        lambda *args, **k: orig(*args, **{**k, "tested": None}))
+    b = nw.infer("surrogate_test", **kw)
+    assert a.learned_info["n_rerun"] == b.learned_info["n_rerun"]
+    assert np.array_equal(a.p_values, b.p_values)
+    assert np.array_equal(a.grn.pairs, b.grn.pairs)
+    assert len(a.grn.pairs) > 0

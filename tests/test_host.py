@@ -316,3 +316,28 @@ def test_trim_dpi_known_answers_and_cpu_path():
         got = trim_dpi(A, np.array([f"g{i}" for i in range(n)]), p, A[p[:, 0], p[:, 1]],
                        torch.device("cpu"), batch_size=bs, trim_threshold=thr)
         assert np.array_equal(got.pairs, ref[0]) and np.array_equal(got.te, ref[1]), (n, bs)
+
+
+def test_tested_pairs_match_mask_nonzero():
+    """The learned null's host-built pair list equals nonzero over the
+    rows x cols mask off the diagonal (row-major), and its sources and
+    targets equal the unique ones of that list, overlaps and singletons
+    included."""
+    from tenex_amd.inference.learned_null import _tested_pairs
+    rng = np.random.default_rng(0)
+    n = 40
+    cases = [(rng.choice(n, 7, replace=False), rng.choice(n, 12, replace=False)),
+             (np.arange(n), np.arange(n)), (np.array([5]), np.array([5, 9])),
+             (np.array([5, 9]), np.array([5])), (np.array([3]), np.array([3])),
+             (np.array([1, 2, 3]), np.array([2, 3, 4, 30]))]
+    for rows, cols in cases:
+        m = np.zeros((n, n), bool)
+        m[np.ix_(rows, cols)] = True
+        np.fill_diagonal(m, False)
+        exp = np.argwhere(m)
+        pub, srcs, tgts = _tested_pairs(rows, cols)
+        assert np.array_equal(pub, exp), (rows, cols)
+        pub_t, _, _ = _tested_pairs(rows, cols, device="cpu")
+        assert np.array_equal(pub_t.numpy(), exp), (rows, cols)
+        assert np.array_equal(srcs, np.unique(exp[:, 0]))
+        assert np.array_equal(tgts, np.unique(exp[:, 1]))
