@@ -120,6 +120,7 @@ struct BwdShared {
 
 struct ParamLayout {
     int W1, b1, W2, b2, W3, b3, P;
+    int part_ld;  // floats per workgroup row of partial gradients (P + loss, padded to 4)
     __host__ __device__ ParamLayout(int D, int O) {
         W1 = 0;
         b1 = kMlpH * D;
@@ -128,6 +129,7 @@ struct ParamLayout {
         W3 = b2 + kMlpH;
         b3 = W3 + O * kMlpH;
         P = b3 + O;
+        part_ld = (P + 1 + 3) & ~3;
     }
 };
 
@@ -330,15 +332,24 @@ __global__ __launch_bounds__(kBwdThreads) void mlp_backward_kernel(
         }
         __syncthreads();  // the next chunk overwrites x, h1, h2, d1, d2, dy
     }
-    float *pr = part + (int64_t)blockIdx.x * (L.P + 1);
+    float *pr = part + (int64_t)blockIdx.x * L.part_ld;
+    // W2 rows as 16-byte stores (the row stride and W2's offset are multiples
+    // of four floats when D is; W1 rows are D long)
+    const bool w2_vec = ((L.W2 | L.part_ld) & 3) == 0;
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int u = 0; u < 4; u++) {
+        const int i = i0 + u;
+        if (w2_vec) {
+            *reinterpret_cast<float4 *>(pr + L.W2 + i * kMlpH + k0) =
+                make_float4(gw2[u][0], gw2[u][1], gw2[u][2], gw2[u][3]);
+        } else {
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int i = i0 + u, k = k0 + v;
-            pr[L.W2 + i * kMlpH + k] = gw2[u][v];
-            if (k < D) pr[L.W1 + i * D + k] = gw1[u][v];
+            for (int v = 0; v < 4; v++) pr[L.W2 + i * kMlpH + k0 + v] = gw2[u][v];
         }
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            if (k0 + v < D) pr[L.W1 + i * D + k0 + v] = gw1[u][v];
+    }
     if (kb == 0) {
         for (int u = 0; u < 4; u++) pr[L.b2 + i0 + u] = gb[u];
     } else if (kb == 1) {
@@ -393,7 +404,7 @@ __global__ __launch_bounds__(256) void mlp_reduce_adam_kernel(
     float acc = 0.f;
     if (p <= L.P) {
 #pragma unroll 4
-        for (int g = gs; g < G; g += kRedSlices) acc += part[(int64_t)g * (L.P + 1) + p];
+        for (int g = gs; g < G; g += kRedSlices) acc += part[(int64_t)g * L.part_ld + p];
     }
     red[gs][pl] = acc;
     __syncthreads();
@@ -451,7 +462,7 @@ int tnx_mlp_forward(const tnx_mlp_t *m, const float *X, int64_t n_rows, float *Y
 int64_t tnx_mlp_workspace_bytes(int32_t D, int32_t O, int64_t n_rows) {
     if (D < 1 || D > kMlpMaxD || O < 1 || O > kMlpMaxO || n_rows < 0) return -1;
     const ParamLayout L(D, O);
-    return bwd_groups(n_rows) * (int64_t)(L.P + 1) * (int64_t)sizeof(float);
+    return bwd_groups(n_rows) * (int64_t)L.part_ld * (int64_t)sizeof(float);
 }
 
 static int launch_backward(const tnx_mlp_t *m, const float *X, const float *tgt,
