@@ -335,9 +335,10 @@ def test_decision_pairs_matches_dense(dev):
         assert x.dtype == y.dtype and torch.equal(x, y)
 
 
-def test_uncounted_stats_and_pending_upload(dev):
-    """observed None skips count_ge and leaves the sums unchanged; the
-    observed matrix uploaded on a side stream arrives intact."""
+def test_uncounted_stats_and_pending_upload(dev, monkeypatch):
+    """observed None skips count_ge and leaves the sums unchanged (staged
+    batches and one-surrogate runs); the observed matrix uploaded on a side
+    stream arrives intact."""
     d, npv = _small(n=28, T=360, seed=12)
     n = d.shape[0]
     prep = _prep(d, npv, dev)
@@ -355,3 +356,7 @@ def test_uncounted_stats_and_pending_upload(dev):
     assert int(b[2].abs().sum()) == 0 and int(a[2].sum()) > 0
     assert torch.equal(c[0], full[0]) and torch.equal(c[1], full[1])
     assert int(c[2].abs().sum()) == 0
+    monkeypatch.setattr(core, "STAGE_FLOATS", n * n)     # one surrogate per run (TNX_OUT_STATS)
+    d1 = surrogate_accumulate(prep, None, 9, "block", 21, 8, dev)
+    assert torch.equal(d1[0], full[0]) and torch.equal(d1[1], full[1])
+    assert int(d1[2].abs().sum()) == 0
