@@ -75,24 +75,26 @@ __global__ __launch_bounds__(256) void b2_pack_kernel(const int8_t *__restrict__
     }
 }
 
-// c*log2(c) of an inclusion-exclusion count; counts outside [0, N] can only
-// come from non-binary input and are ignored (the table has N + 1 entries).
-__device__ __forceinline__ void xl_add(DF &acc, const float2 *__restrict__ tbl, int32_t c,
-                                       int32_t N) {
-    if (c > 1 && c <= N) {
-        float2 t = tbl[c];
-        df_add(acc, t.x, t.y);
-    }
-}
+// Entries of the c*log2(c) table the epilogue keeps in LDS (8 bytes each);
+// longer series read it from memory.
+constexpr int kB2LdsTbl = 8192;
 
-// Workgroup: 4 waves as 2 (i) x 2 (j); wave tile 32 (i) x 64 (j).
+// Workgroup: 4 waves as 2 (i) x 2 (j); wave tile 32 (i) x 64 (j).  LT: the
+// table (N + 1 <= kB2LdsTbl entries) is copied to LDS while the MFMA loop
+// runs, so the epilogue's twelve lookups per pair are LDS reads.
+template <bool LT>
 __global__ __launch_bounds__(256) void b2_mfma_kernel(
     const int8_t *__restrict__ F, const int8_t *__restrict__ P, const int32_t *__restrict__ nf,
     const int32_t *__restrict__ np_, const int32_t *__restrict__ nfp,
     const double *__restrict__ S1, const double *__restrict__ S2a,
     const float *__restrict__ xlogx, int64_t n, int64_t L, int64_t Lp,
     float *__restrict__ te_out, int64_t ld_out, int32_t *__restrict__ counts_dbg) {
+    extern __shared__ unsigned long long b2_ltbl[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if constexpr (LT) {
+        const unsigned long long *gt = reinterpret_cast<const unsigned long long *>(xlogx);
+        for (int e = threadIdx.x; e <= (int)L; e += 256) b2_ltbl[e] = gt[e];
+    }
     const int64_t i0 = (int64_t)blockIdx.y * kB2TileI + (wave >> 1) * 32;
     const int64_t j0 = (int64_t)blockIdx.x * kB2TileJ + (wave & 1) * 64;
     const int r = lane & 31, h = lane >> 5;
@@ -140,10 +142,24 @@ __global__ __launch_bounds__(256) void b2_mfma_kernel(
         mma(b);
     }
     if (k < Lp) mma(a);  // Lp / kB2K odd: the last step was loaded into a
+    if constexpr (LT) __syncthreads();  // the LDS table is complete
     // Epilogue.  32x32 accumulator layout: lane holds column (lane & 31),
     // register q holds row 8*(q/4) + 4*(lane/32) + (q%4).
     const float2 *tbl = reinterpret_cast<const float2 *>(xlogx);
     const int32_t N = (int32_t)L;
+    // c*log2(c) of an inclusion-exclusion count; counts outside [0, N] can
+    // only come from non-binary input and are ignored
+    auto xl_add = [&](DF &acc, int32_t c) {
+        if (c > 1 && c <= N) {
+            if constexpr (LT) {
+                const unsigned long long w = b2_ltbl[c];
+                df_add(acc, __uint_as_float((uint32_t)w), __uint_as_float((uint32_t)(w >> 32)));
+            } else {
+                const float2 t = tbl[c];
+                df_add(acc, t.x, t.y);
+            }
+        }
+    };
     const double invN = 1.0 / (double)L;
 #pragma unroll
     for (int half = 0; half < 2; half++) {
@@ -168,18 +184,18 @@ __global__ __launch_bounds__(256) void b2_mfma_kernel(
             }
             const int32_t nfi = nf[i], npi = np_[i], nfpi = nfp[i];
             DF s3 = {0.f, 0.f}, s2 = {0.f, 0.f};
-            xl_add(s3, tbl, Cc, N);
-            xl_add(s3, tbl, Dd - Cc, N);
-            xl_add(s3, tbl, Ee - Cc, N);
-            xl_add(s3, tbl, npj - Dd - Ee + Cc, N);
-            xl_add(s3, tbl, nfpi - Cc, N);
-            xl_add(s3, tbl, npi - Dd - nfpi + Cc, N);
-            xl_add(s3, tbl, nfi - Ee - nfpi + Cc, N);
-            xl_add(s3, tbl, N - nfi - npi - npj + nfpi + Dd + Ee - Cc, N);
-            xl_add(s2, tbl, Dd, N);
-            xl_add(s2, tbl, npj - Dd, N);
-            xl_add(s2, tbl, npi - Dd, N);
-            xl_add(s2, tbl, N - npi - npj + Dd, N);
+            xl_add(s3, Cc);
+            xl_add(s3, Dd - Cc);
+            xl_add(s3, Ee - Cc);
+            xl_add(s3, npj - Dd - Ee + Cc);
+            xl_add(s3, nfpi - Cc);
+            xl_add(s3, npi - Dd - nfpi + Cc);
+            xl_add(s3, nfi - Ee - nfpi + Cc);
+            xl_add(s3, N - nfi - npi - npj + nfpi + Dd + Ee - Cc);
+            xl_add(s2, Dd);
+            xl_add(s2, npj - Dd);
+            xl_add(s2, npi - Dd);
+            xl_add(s2, N - npi - npj + Dd);
             const double te = (df_to_double(s3) + S1[i] - S2a[i] - df_to_double(s2)) * invN;
             te_out[i * ld_out + j] = (float)te;
         }
@@ -222,8 +238,13 @@ int tnx_te_gemm_b2(const int8_t *bins, int64_t n, int64_t T, int64_t ld, int32_t
                        Lp, F, P, nf, npp, nfp);
     TNX_CHECK_LAUNCH("b2_pack_kernel");
     dim3 grid((unsigned)((n + kB2TileJ - 1) / kB2TileJ), (unsigned)((n + kB2TileI - 1) / kB2TileI));
-    hipLaunchKernelGGL(b2_mfma_kernel, grid, dim3(256), 0, s, F, P, nf, npp, nfp, S1, S2a, xlogx, n,
-                       L, Lp, te_out, ld_out, counts_dbg);
+    if (L + 1 <= kB2LdsTbl) {
+        hipLaunchKernelGGL(b2_mfma_kernel<true>, grid, dim3(256), (size_t)(L + 1) * 8, s, F, P, nf,
+                           npp, nfp, S1, S2a, xlogx, n, L, Lp, te_out, ld_out, counts_dbg);
+    } else {
+        hipLaunchKernelGGL(b2_mfma_kernel<false>, grid, dim3(256), 0, s, F, P, nf, npp, nfp, S1,
+                           S2a, xlogx, n, L, Lp, te_out, ld_out, counts_dbg);
+    }
     TNX_CHECK_LAUNCH("b2_mfma_kernel");
     return TNX_OK;
 }
