@@ -33,7 +33,18 @@ constexpr int kB2K = 64;        // bytes of t per main-loop step (2 MFMAs)
 __host__ __device__ inline int64_t b2_npad(int64_t n) { return (n + 127) / 128 * 128; }
 __host__ __device__ inline int64_t b2_lp(int64_t L) { return (L + kB2K - 1) / kB2K * kB2K; }
 
-// One workgroup per variable: operand rows and marginals.
+// Operand layout: 32-row x 32-byte blocks of 1 KB (row block major, then the
+// 32-byte k chunk).  Inside a block, the 16-byte run h = (k % 32) / 16 of row
+// r sits at byte (h * 32 + r) * 16: exactly where lane h * 32 + r of a
+// v_mfma_i32_32x32x32_i8 operand load reads it, so each wave's operand load
+// is one contiguous kilobyte.
+__device__ __forceinline__ int64_t b2_tiled(int64_t row, int64_t k, int64_t Lp) {
+    const int64_t blk = (row >> 5) * (Lp >> 5) + (k >> 5);
+    return blk * 1024 + ((((k & 31) >> 4) * 32 + (row & 31)) << 4) + (k & 15);
+}
+
+// One workgroup per variable: operand rows (tiled, 16 bytes per thread and
+// step) and marginals.
 __global__ __launch_bounds__(256) void b2_pack_kernel(const int8_t *__restrict__ bins, int64_t n,
                                                       int64_t ld, int32_t tau, int64_t L,
                                                       int64_t Lp, int8_t *__restrict__ F,
@@ -45,17 +56,25 @@ __global__ __launch_bounds__(256) void b2_pack_kernel(const int8_t *__restrict__
     const int64_t i = blockIdx.x;
     const int tid = threadIdx.x;
     int32_t sf = 0, sp = 0, sfp = 0;
-    for (int64_t t = tid; t < Lp; t += 256) {
-        int8_t f = 0, p = 0;
-        if (i < n && t < L) {
-            f = bins[i * ld + t + tau];
-            p = bins[i * ld + t];
+    for (int64_t t0 = 16 * (int64_t)tid; t0 < Lp; t0 += 16 * 256) {
+        uint32_t fw[4] = {0u, 0u, 0u, 0u}, pw[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int64_t t = t0 + q;
+            uint32_t f = 0, p = 0;
+            if (i < n && t < L) {
+                f = (uint32_t)(uint8_t)bins[i * ld + t + tau];
+                p = (uint32_t)(uint8_t)bins[i * ld + t];
+            }
+            fw[q >> 2] |= f << (8 * (q & 3));
+            pw[q >> 2] |= p << (8 * (q & 3));
+            sf += (int32_t)f;
+            sp += (int32_t)p;
+            sfp += (int32_t)(f & p);
         }
-        F[i * Lp + t] = f;
-        P[i * Lp + t] = p;
-        sf += f;
-        sp += p;
-        sfp += f & p;
+        const int64_t o = b2_tiled(i, t0, Lp);
+        *reinterpret_cast<uint4 *>(F + o) = make_uint4(fw[0], fw[1], fw[2], fw[3]);
+        *reinterpret_cast<uint4 *>(P + o) = make_uint4(pw[0], pw[1], pw[2], pw[3]);
     }
     for (int off = 32; off >= 1; off >>= 1) {
         sf += __shfl_xor(sf, off, 64);
@@ -97,12 +116,14 @@ __global__ __launch_bounds__(256) void b2_mfma_kernel(
     }
     const int64_t i0 = (int64_t)blockIdx.y * kB2TileI + (wave >> 1) * 32;
     const int64_t j0 = (int64_t)blockIdx.x * kB2TileJ + (wave & 1) * 64;
-    const int r = lane & 31, h = lane >> 5;
-    // operand rows held by this lane (padded rows are zero, so no masking)
-    const int8_t *fa = F + (i0 + r) * Lp + 16 * h;
-    const int8_t *pa = P + (i0 + r) * Lp + 16 * h;
-    const int8_t *pb0 = P + (j0 + r) * Lp + 16 * h;
-    const int8_t *pb1 = P + (j0 + 32 + r) * Lp + 16 * h;
+    // this lane's 16 operand bytes of each 32-row block (tiled layout: the
+    // wave reads one contiguous kilobyte per block and k chunk; padded rows
+    // are zero, so no masking)
+    const int64_t kblk = Lp >> 5;  // 32-byte k chunks per row block
+    const int8_t *fa = F + (i0 >> 5) * kblk * 1024 + 16 * lane;
+    const int8_t *pa = P + (i0 >> 5) * kblk * 1024 + 16 * lane;
+    const int8_t *pb0 = P + (j0 >> 5) * kblk * 1024 + 16 * lane;
+    const int8_t *pb1 = P + ((j0 + 32) >> 5) * kblk * 1024 + 16 * lane;
     v16i accC0 = {}, accC1 = {}, accD0 = {}, accD1 = {}, accE0 = {}, accE1 = {};
     // Two-stage software pipeline: the operands of step k + kB2K are
     // requested before the MFMAs of step k are issued, so the memory latency
@@ -113,7 +134,7 @@ __global__ __launch_bounds__(256) void b2_mfma_kernel(
     auto load = [&](Ops &o, int64_t k) {
 #pragma unroll
         for (int u = 0; u < 2; u++) {
-            const int64_t kk = k + 32 * u;
+            const int64_t kk = ((k >> 5) + u) * 1024;  // this k chunk's block
             o.f[u] = *reinterpret_cast<const v4i *>(fa + kk);
             o.p[u] = *reinterpret_cast<const v4i *>(pa + kk);
             o.q0[u] = *reinterpret_cast<const v4i *>(pb0 + kk);
