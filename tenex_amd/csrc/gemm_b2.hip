@@ -56,21 +56,45 @@ __global__ __launch_bounds__(256) void b2_pack_kernel(const int8_t *__restrict__
     const int64_t i = blockIdx.x;
     const int tid = threadIdx.x;
     int32_t sf = 0, sp = 0, sfp = 0;
+    // sum of the four 0/1 bytes of a word
+    auto bsum = [](uint32_t w) { return (int32_t)((w * 0x01010101u) >> 24); };
+    const int8_t *row = bins + i * ld;
     for (int64_t t0 = 16 * (int64_t)tid; t0 < Lp; t0 += 16 * 256) {
         uint32_t fw[4] = {0u, 0u, 0u, 0u}, pw[4] = {0u, 0u, 0u, 0u};
+        const int64_t tf = t0 + tau, base = tf & ~(int64_t)15;
+        if (i < n && t0 + 16 <= L && base + 32 <= ld) {
+            // 16-byte loads: P from the aligned run at t0, F from the two
+            // aligned runs around t0 + tau, shifted by (t0 + tau) % 16 bytes
+            const uint4 pv = *reinterpret_cast<const uint4 *>(row + t0);
+            const uint4 a = *reinterpret_cast<const uint4 *>(row + base);
+            const uint4 b = *reinterpret_cast<const uint4 *>(row + base + 16);
+            const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            const int sh = (int)(tf & 15), ws = sh >> 2, bs = 8 * (sh & 3);
+            pw[0] = pv.x; pw[1] = pv.y; pw[2] = pv.z; pw[3] = pv.w;
+            // word selection by compares (static register indices)
+            auto pick = [&](int j) {
+                return ws == 0 ? w[j] : ws == 1 ? w[j + 1] : ws == 2 ? w[j + 2] : w[j + 3];
+            };
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int64_t t = t0 + q;
-            uint32_t f = 0, p = 0;
-            if (i < n && t < L) {
-                f = (uint32_t)(uint8_t)bins[i * ld + t + tau];
-                p = (uint32_t)(uint8_t)bins[i * ld + t];
+            for (int q = 0; q < 4; q++) {
+                const uint32_t lo = pick(q), hi = pick(q + 1);
+                fw[q] = bs ? ((lo >> bs) | (hi << (32 - bs))) : lo;
             }
-            fw[q >> 2] |= f << (8 * (q & 3));
-            pw[q >> 2] |= p << (8 * (q & 3));
-            sf += (int32_t)f;
-            sp += (int32_t)p;
-            sfp += (int32_t)(f & p);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int64_t t = t0 + q;
+                if (i < n && t < L) {
+                    fw[q >> 2] |= (uint32_t)(uint8_t)row[t + tau] << (8 * (q & 3));
+                    pw[q >> 2] |= (uint32_t)(uint8_t)row[t] << (8 * (q & 3));
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            sf += bsum(fw[q]);
+            sp += bsum(pw[q]);
+            sfp += bsum(fw[q] & pw[q]);
         }
         const int64_t o = b2_tiled(i, t0, Lp);
         *reinterpret_cast<uint4 *>(F + o) = make_uint4(fw[0], fw[1], fw[2], fw[3]);
