@@ -134,9 +134,13 @@ __global__ __launch_bounds__(256) void b2_mfma_kernel(
     float *__restrict__ te_out, int64_t ld_out, int32_t *__restrict__ counts_dbg) {
     extern __shared__ unsigned long long b2_ltbl[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if constexpr (LT) {
-        const unsigned long long *gt = reinterpret_cast<const unsigned long long *>(xlogx);
-        for (int e = threadIdx.x; e <= (int)L; e += 256) b2_ltbl[e] = gt[e];
+    if constexpr (LT) {  // hi + lo of each entry as one double
+        const float2 *gt = reinterpret_cast<const float2 *>(xlogx);
+        double *lt = reinterpret_cast<double *>(b2_ltbl);
+        for (int e = threadIdx.x; e <= (int)L; e += 256) {
+            const float2 v = gt[e];
+            lt[e] = (double)v.x + (double)v.y;
+        }
     }
     const int64_t i0 = (int64_t)blockIdx.y * kB2TileI + (wave >> 1) * 32;
     const int64_t j0 = (int64_t)blockIdx.x * kB2TileJ + (wave & 1) * 64;
@@ -192,32 +196,57 @@ __global__ __launch_bounds__(256) void b2_mfma_kernel(
     // register q holds row 8*(q/4) + 4*(lane/32) + (q%4).
     const float2 *tbl = reinterpret_cast<const float2 *>(xlogx);
     const int32_t N = (int32_t)L;
-    // c*log2(c) of an inclusion-exclusion count; counts outside [0, N] can
-    // only come from non-binary input and are ignored
-    auto xl_add = [&](DF &acc, int32_t c) {
-        if (c > 1 && c <= N) {
-            if constexpr (LT) {
-                const unsigned long long w = b2_ltbl[c];
-                df_add(acc, __uint_as_float((uint32_t)w), __uint_as_float((uint32_t)(w >> 32)));
-            } else {
-                const float2 t = tbl[c];
-                df_add(acc, t.x, t.y);
-            }
+    const double invN = 1.0 / (double)L;
+    const int64_t jc[2] = {j0 + (lane & 31), j0 + 32 + (lane & 31)};
+    const int32_t npj[2] = {jc[0] < n ? np_[jc[0]] : 0, jc[1] < n ? np_[jc[1]] : 0};
+    // S3 - S2b of one pair from its three counts and the marginals: the 8
+    // joint cells and 4 (x_t, y_t) cells by inclusion-exclusion, c*log2(c)
+    // each (counts outside [0, N] come only from non-binary input and are
+    // ignored).  With the LDS table the terms add in FP64; the table in
+    // memory keeps its hi/lo pairs and double-float sums.
+    auto s3_minus_s2 = [&](int32_t Cc, int32_t Dd, int32_t Ee, int32_t nfi, int32_t npi,
+                           int32_t nfpi, int32_t npjv) -> double {
+        const int32_t c3[8] = {Cc, Dd - Cc, Ee - Cc, npjv - Dd - Ee + Cc, nfpi - Cc,
+                               npi - Dd - nfpi + Cc, nfi - Ee - nfpi + Cc,
+                               N - nfi - npi - npjv + nfpi + Dd + Ee - Cc};
+        const int32_t c2[4] = {Dd, npjv - Dd, npi - Dd, N - npi - npjv + Dd};
+        if constexpr (LT) {
+            const double *lt = reinterpret_cast<const double *>(b2_ltbl);
+            double a3 = 0.0, a2 = 0.0;
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                if (c3[e] > 1 && c3[e] <= N) a3 += lt[c3[e]];
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (c2[e] > 1 && c2[e] <= N) a2 += lt[c2[e]];
+            return a3 - a2;
+        } else {
+            DF a3 = {0.f, 0.f}, a2 = {0.f, 0.f};
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                if (c3[e] > 1 && c3[e] <= N) df_add(a3, tbl[c3[e]].x, tbl[c3[e]].y);
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (c2[e] > 1 && c2[e] <= N) df_add(a2, tbl[c2[e]].x, tbl[c2[e]].y);
+            return df_to_double(a3) - df_to_double(a2);
         }
     };
-    const double invN = 1.0 / (double)L;
+    // 32x32 accumulator layout: lane holds column (lane & 31), register q
+    // holds row 8*(q/4) + 4*(lane/32) + (q%4); each row's marginals are
+    // read once for both column halves
 #pragma unroll
-    for (int half = 0; half < 2; half++) {
-        const int64_t j = j0 + 32 * half + (lane & 31);
-        const v16i &aC = half ? accC1 : accC0;
-        const v16i &aD = half ? accD1 : accD0;
-        const v16i &aE = half ? accE1 : accE0;
-        const int32_t npj = j < n ? np_[j] : 0;
+    for (int q = 0; q < 16; q++) {
+        const int64_t i = i0 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+        if (i >= n) continue;
+        const int32_t nfi = nf[i], npi = np_[i], nfpi = nfp[i];
+        const double s12 = S1[i] - S2a[i];
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int64_t i = i0 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-            if (i >= n || j >= n) continue;
-            const int32_t Cc = aC[q], Dd = aD[q], Ee = aE[q];
+        for (int half = 0; half < 2; half++) {
+            const int64_t j = jc[half];
+            if (j >= n) continue;
+            const int32_t Cc = half ? accC1[q] : accC0[q];
+            const int32_t Dd = half ? accD1[q] : accD0[q];
+            const int32_t Ee = half ? accE1[q] : accE0[q];
             if (counts_dbg) {
                 counts_dbg[i * n + j] = Cc;
                 counts_dbg[n * n + i * n + j] = Dd;
@@ -227,22 +256,8 @@ __global__ __launch_bounds__(256) void b2_mfma_kernel(
                 te_out[i * ld_out + j] = 0.0f;
                 continue;
             }
-            const int32_t nfi = nf[i], npi = np_[i], nfpi = nfp[i];
-            DF s3 = {0.f, 0.f}, s2 = {0.f, 0.f};
-            xl_add(s3, Cc);
-            xl_add(s3, Dd - Cc);
-            xl_add(s3, Ee - Cc);
-            xl_add(s3, npj - Dd - Ee + Cc);
-            xl_add(s3, nfpi - Cc);
-            xl_add(s3, npi - Dd - nfpi + Cc);
-            xl_add(s3, nfi - Ee - nfpi + Cc);
-            xl_add(s3, N - nfi - npi - npj + nfpi + Dd + Ee - Cc);
-            xl_add(s2, Dd);
-            xl_add(s2, npj - Dd);
-            xl_add(s2, npi - Dd);
-            xl_add(s2, N - npi - npj + Dd);
-            const double te = (df_to_double(s3) + S1[i] - S2a[i] - df_to_double(s2)) * invN;
-            te_out[i * ld_out + j] = (float)te;
+            const double d = s3_minus_s2(Cc, Dd, Ee, nfi, npi, nfpi, npj[half]);
+            te_out[i * ld_out + j] = (float)((d + s12) * invN);
         }
     }
 }
