@@ -53,7 +53,11 @@ __global__ __launch_bounds__(256) void b2_pack_kernel(const int8_t *__restrict__
                                                       int32_t *__restrict__ np_,
                                                       int32_t *__restrict__ nfp) {
     __shared__ int32_t red[3][4];
-    const int64_t i = blockIdx.x;
+    // Workgroup w runs on XCD w % 8; XCD x packs the x-th contiguous slice of
+    // rows (the grid is a multiple of 128), so the eight rows whose 16-byte
+    // runs share one 128-byte line of the tiled layout are written through
+    // one L2
+    const int64_t i = (int64_t)(blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
     const int tid = threadIdx.x;
     int32_t sf = 0, sp = 0, sfp = 0;
     // sum of the four 0/1 bytes of a word
