@@ -12,8 +12,8 @@
 // and applies the epilogue in registers:
 //     TE(j -> i) = (S3 + S1[i] - S2a[i] - S2b) / L
 // with S3 over the 8 joint counts and S2b over the 4 (x_i[t], x_j[t]) counts,
-// c*log2(c) from the hi/lo table with compensated fp32 sums and an fp64
-// final combination.
+// c*log2(c) from the hi/lo table (as doubles in LDS with fp64 sums when the
+// table fits, else compensated fp32 sums) and an fp64 final combination.
 //
 // Workspace (tnx_gemm_b2_workspace): F and P operand rows (n_pad x Lp int8,
 // zero padded), then nf, np, nfp (int32 n_pad each).
