@@ -1,6 +1,6 @@
 """torch-CPU restatement of the reference's scatter_add TE path: the CPU baseline.
 
-TEST / BASELINE INFRASTRUCTURE ONLY (bench.py cpu_baseline leg and tests).
+TEST AND CPU-BASELINE INFRASTRUCTURE ONLY (bench.py cpu_baseline leg and tests).
 
 Follows tenex/kernels/scatter_add.py:33-119 (_compute_te_batch) op for op:
 int64 flat index xt1*B^2 + xt*B + yt + pair*B^3 with a global B = b_max,
