@@ -46,7 +46,7 @@ extern "C" {
 #define TNX_API
 #endif
 
-#define TNX_ABI_VERSION 6
+#define TNX_ABI_VERSION 7
 
 /* Status codes. */
 #define TNX_OK 0
@@ -194,7 +194,51 @@ typedef struct {
                                 * work item (suits plans of one-target items: pair lists);
                                 * zero: the four waves split each item's pairs.  Results are
                                 * identical either way; only speed differs. */
+    /* One persistent launch (from ABI 7).  queue_items == NULL: the wave and
+     * block items run as two grid-striding launches.  Otherwise one grid of
+     * resident workgroups takes entries from 2 * TNX_QUEUES queues built by
+     * tnx_te_queue: queue q holds queue_items[queue_off[q] .. queue_off[q +
+     * 1]); queues 0 .. TNX_QUEUES - 1 hold wave entries (a wave takes one at
+     * a time: entry e is wave item e / 4, its pairs e % 4, e % 4 + 4, ...,
+     * or with item_per_wave wave item e whole), queues TNX_QUEUES .. hold
+     * block entries (the workgroup takes one at a time: block item e).  A
+     * workgroup drains its own XCD's queues first, then the others.  Needs
+     * T - tau <= 12288, four cap_wave slices within 62 KiB and a block
+     * histogram that holds max_block_cols; wave pairs give the counts and TE
+     * of the two-launch path, block pairs the same counts (TE summed in
+     * another order). */
+    const int32_t *queue_items;
+    const int32_t *queue_off;  /* device int32[2 * TNX_QUEUES + 1] */
+    int32_t *queue_ctr;        /* device int32[TNX_QUEUE_CTR_INTS] scratch: the queues' take
+                                * counters, cleared by tnx_te_run on the stream before the
+                                * launch (a kernel, not a memset) */
+    /* Test hook: when counts_map != NULL, each computed pair (x, y) with
+     * r = counts_map[x * n_vars + y] >= 0 also writes its joint counts to
+     * counts_dbg[r * counts_ld + ((a * b_x + b) * b_y + c)] from inside the
+     * kernel that computes it (persistent launch only). */
+    const int32_t *counts_map;
+    int32_t *counts_dbg;
+    int64_t counts_ld;
 } tnx_te_args_t;
+
+#define TNX_QUEUES 8            /* work queues of the persistent launch (one per XCD) */
+#define TNX_QUEUE_CTR_INTS 256  /* ints of queue_ctr */
+
+/* Host-only: the persistent launch's work queues for a plan (work, orders
+ * from tnx_te_plan, all host).  Wave entries (see queue_items above) go to
+ * queues 0 .. TNX_QUEUES - 1, block entries to the next TNX_QUEUES, each by
+ * the source chunk of its item (the workgroups of one XCD then share the
+ * source rows in their L2), chunks balanced over the queues by estimated
+ * cost; inside a queue chunk by chunk, heaviest items first.  qitems
+ * (capacity qcap) receives the entries, qoff (2 * TNX_QUEUES + 1 ints) the
+ * queue bounds, *n_wave_entries the number of wave entries.  qitems == NULL:
+ * size query (only *n_wave_entries is written).  No reference counterpart:
+ * the reference launches one grid over all pairs (full_smem.cu:350-381). */
+TNX_API int tnx_te_queue(const tnx_work_t *work, int64_t n_wave_items, int64_t n_block_items,
+                         const int32_t *n_per_var, int64_t n_vars, const int32_t *tgt_order,
+                         const int32_t *src_order, int64_t L, int32_t cap_wave,
+                         int32_t item_per_wave, int32_t *qitems, int64_t qcap, int32_t *qoff,
+                         int64_t *n_wave_entries);
 
 /* Pairwise TE for every pair of the plan: FP32 entropy terms with
  * compensated summation, FP64 final combination

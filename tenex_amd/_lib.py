@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libtenex_hip.so")
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 TNX_OUT_PAIRFREE = 0
 TNX_OUT_MATRIX = 1
@@ -52,7 +52,13 @@ class TeArgs(C.Structure):
                 ("out", p_), ("ld_out", i64), ("pair_offset", i64), ("n_pairs_local", i64),
                 ("sum_te", p_), ("sumsq_te", p_), ("count_ge", p_), ("observed", p_),
                 ("hot_zero", C.c_int32), ("stage_vars", C.c_int32), ("stage_stride", C.c_int64),
-                ("item_per_wave", C.c_int32)]
+                ("item_per_wave", C.c_int32),
+                ("queue_items", p_), ("queue_off", p_), ("queue_ctr", p_),
+                ("counts_map", p_), ("counts_dbg", p_), ("counts_ld", C.c_int64)]
+
+
+TNX_QUEUES = 8
+TNX_QUEUE_CTR_INTS = 256
 
 
 class SurrArgs(C.Structure):
@@ -90,6 +96,8 @@ SIGNATURES = {
     "tnx_te_plan": (C.c_int, [p_, i64, i64, p_, i64, p_, i64, i32, i32, p_, p_, p_, i64,
                               C.POINTER(PlanInfo)]),
     "tnx_te_run": (C.c_int, [C.POINTER(TeArgs), p_]),
+    "tnx_te_queue": (C.c_int, [p_, i64, i64, p_, i64, p_, p_, i64, i32, i32, p_, i64, p_,
+                               C.POINTER(C.c_int64)]),
     "tnx_te_counts_debug": (C.c_int, [p_, p_, p_, p_, p_, p_, i64, i64, i64, i64, i32, p_, i64,
                                       i32, p_, p_, p_]),
     "tnx_gemm_b2_workspace": (i64, [i64, i64, i32]),

@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 import hashlib
+import os
 import threading
 from collections import OrderedDict
 from dataclasses import dataclass, field
@@ -415,15 +416,75 @@ def get_plan(prep: Prepared, targets=None, sources=None, cap_wave: int = 0,
 
 # ── TE run ────────────────────────────────────────────────────────────────
 
+# The persistent single launch (te_persist.hip) takes series of up to this
+# many samples; TNX_PERSIST=0 in the environment selects the two-launch path.
+PERSIST_MAX_L = 12 * 1024
+PERSIST_MAX_SLICES_BYTES = 156 * 1024  # four wave slices per workgroup
+
+
+def _wave_slice(cap: int) -> int:
+    return (cap + 64 + 3) & ~3
+
+
+def persistent_ok(prep: Prepared, plan: Plan) -> bool:
+    """Whether the plan runs as one persistent launch: short enough series
+    and a block-role histogram (four wave slices) that holds the widest
+    block pair's (x_t, y_t) marginal."""
+    if os.environ.get("TNX_PERSIST", "1") == "0" or prep.L > PERSIST_MAX_L:
+        return False
+    i = plan.info
+    slice_words = _wave_slice(int(i.cap_wave))
+    if 4 * slice_words * 4 > PERSIST_MAX_SLICES_BYTES:
+        return False
+    if i.n_block_items == 0:
+        return True
+    mc = max(int(i.max_block_cols), 1)
+    return 4 * slice_words - mc - 128 >= max(mc, 4)
+
+
+def plan_queue(plan: Plan, npv_host: np.ndarray, L: int, item_per_wave: bool):
+    """Device work queues of the persistent launch for plan (tnx_te_queue),
+    cached on the plan: (items, offsets, counters)."""
+    key = ("queue", bool(item_per_wave))
+    got = plan.cache.get(key)
+    if got is None:
+        i = plan.info
+        nw, nb = int(i.n_wave_items), int(i.n_block_items)
+        work = np.ascontiguousarray(plan.work_host.reshape(-1, 4)[: nw + nb], dtype=np.int32)
+        npv = np.ascontiguousarray(npv_host, dtype=np.int32)
+        n_we = C.c_int64(0)
+        args = (ptr(work), nw, nb, ptr(npv), len(npv), ptr(plan.tgt_host), ptr(plan.src_host),
+                int(L), int(i.cap_wave), int(bool(item_per_wave)))
+        call("tnx_te_queue", *args, None, 0, None, C.byref(n_we))
+        n_ent = int(n_we.value) + nb
+        items = np.empty(max(n_ent, 1), np.int32)
+        off = np.zeros(2 * _lib.TNX_QUEUES + 1, np.int32)
+        call("tnx_te_queue", *args, ptr(items), n_ent, ptr(off), C.byref(n_we))
+        dev = plan.work.device
+        ctr = torch.zeros(_lib.TNX_QUEUE_CTR_INTS, dtype=torch.int32, device=dev)
+        got = (h2d(items, dev), h2d(off, dev), ctr)
+        plan.cache[key] = got
+    return got
+
+
 def te_args(prep: Prepared, plan: Plan, out: torch.Tensor | None, out_mode: int, ld_out: int = 0,
             pair_offset: int = 0, n_pairs_local: int = 0, stats=None, stage_vars: int = 0,
-            stage_stride: int = 0, item_per_wave: bool | None = None) -> _lib.TeArgs:
+            stage_stride: int = 0, item_per_wave: bool | None = None,
+            persistent: bool | None = None, counts=None) -> _lib.TeArgs:
     """stats: (sum, sumsq, count_ge, observed) public-layout tensors for
     TNX_OUT_STATS; stage_vars / stage_stride for TNX_OUT_STAGE.  item_per_wave None: chosen from the
-    plan (item_per_wave_default)."""
+    plan (item_per_wave_default).  persistent None: one persistent launch
+    whenever persistent_ok.  counts = (map int32 (n, n), out int32 (rows,
+    ld)) turns on the kernels' joint-count dump (persistent launch)."""
     i = plan.info
     if item_per_wave is None:
         item_per_wave = item_per_wave_default(plan)
+    if persistent is None:
+        persistent = persistent_ok(prep, plan)
+    if counts is not None and not persistent:
+        raise ValueError("the joint-count dump needs the persistent launch")
+    q = plan_queue(plan, prep.npv_host, prep.L, item_per_wave) if persistent else (None,) * 3
+    cm, cd = (counts[0], counts[1]) if counts is not None else (None, None)
     st = stats if stats is not None else (None, None, None, None)
     return _lib.TeArgs(
         bins=ptr(prep.bins), states=ptr(prep.states), n_per_var=ptr(prep.npv), S1=ptr(prep.S1),
@@ -435,7 +496,9 @@ def te_args(prep: Prepared, plan: Plan, out: torch.Tensor | None, out_mode: int,
         ld_out=ld_out, pair_offset=pair_offset, n_pairs_local=n_pairs_local,
         sum_te=ptr(st[0]), sumsq_te=ptr(st[1]), count_ge=ptr(st[2]), observed=ptr(st[3]),
         hot_zero=int(prep.hot_zero), stage_vars=int(stage_vars), stage_stride=int(stage_stride),
-        item_per_wave=int(bool(item_per_wave)))
+        item_per_wave=int(bool(item_per_wave)), queue_items=ptr(q[0]), queue_off=ptr(q[1]),
+        queue_ctr=ptr(q[2]), counts_map=ptr(cm), counts_dbg=ptr(cd),
+        counts_ld=int(cd.shape[1]) if cd is not None else 0)
 
 
 WAVE_ITEM_PAIRS = 32   # pairs of a one-target item that a single wave takes whole
@@ -569,8 +632,39 @@ def te_pairfree(prep: Prepared, pair_offset: int = 0, n_pairs_local: int | None 
     return out
 
 
+def counts_plan(prep: Prepared, plan: Plan, pairs: np.ndarray, max_cells: int):
+    """Joint counts of the given [target, source] pairs as the production
+    launch of `plan` counts them (the persistent kernel's count dump): int32
+    (P, max_cells) in layout (a * b_t + b) * b_s + c, and the launch's TE
+    matrix (internal layout)."""
+    pairs = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
+    n, dev = prep.n_vars, prep.device
+    cells = prep.npv_host[pairs[:, 0]].astype(np.int64) ** 2 * prep.npv_host[pairs[:, 1]]
+    if len(cells) and cells.max() > max_cells:
+        raise ValueError(f"a pair has {int(cells.max())} cells > max_cells {max_cells}")
+    key, inv = np.unique(pairs[:, 0] * n + pairs[:, 1], return_inverse=True)
+    cmap = torch.full((n * n,), -1, dtype=torch.int32, device=dev)
+    cmap[h2d(key, dev)] = torch.arange(len(key), dtype=torch.int32, device=dev)
+    counts = torch.zeros((max(len(key), 1), max_cells), dtype=torch.int32, device=dev)
+    out = torch.zeros((n, n), dtype=torch.float32, device=dev)
+    args = te_args(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n, persistent=True,
+                   counts=(cmap, counts))
+    te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, args=args)
+    return counts[h2d(inv.reshape(-1), dev)], out
+
+
 def counts_debug(prep: Prepared, pairs: np.ndarray, max_cells: int = 8192):
-    """Raw joint counts of explicit [target, source] pairs (wave path)."""
+    """Raw joint counts of explicit [target, source] pairs through the
+    production launch (a pair-list plan in the persistent kernel, wave and
+    block roles); returns (counts (P, max_cells), TE (P,)).  Series longer
+    than the persistent launch takes use tnx_te_counts_debug (wave path)."""
+    pairs_np = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
+    if prep.L <= PERSIST_MAX_L and os.environ.get("TNX_PERSIST", "1") != "0":
+        plan = te_pairlist_plan(prep.npv_host, pairs_np, prep.device)
+        if persistent_ok(prep, plan):
+            counts, m = counts_plan(prep, plan, pairs_np, max_cells)
+            te = m[h2d(pairs_np[:, 0], prep.device), h2d(pairs_np[:, 1], prep.device)]
+            return counts, te
     pairs = np.ascontiguousarray(pairs, dtype=np.int32)
     p_d = torch.from_numpy(pairs).to(prep.device)
     P = len(pairs)
@@ -974,7 +1068,7 @@ def surrogate_plan(prep: Prepared, targets=None, sources=None) -> Plan:
     return get_plan(prep, targets, sources, cap_wave=cap, cap_block=0)
 
 
-WAVE_CAP = 3200          # cells per wave histogram (te_hist.hip's default)
+WAVE_CAP = 3136          # cells per wave histogram (te_hist.hip's default)
 FILL_WAVE_ITEMS = 256 * 3 * 4    # items that fill the chip: 4 per resident workgroup
 CHUNK_SRC, N_XCDS = 64, 8        # the planner's source chunk and gfx950's XCD count
 FILL_BLOCK_ITEMS = 256 * 2 * 4

@@ -336,10 +336,12 @@ __global__ __launch_bounds__(256) void surrogate_wave_kernel(tnx_surr_args_t a, 
                         const DF d = reduce_pair16(hist, bx, by, L, ltbl, tbl, lane);
                         const double te = (df_to_double(d) + S12k) * invN;
                         const int64_t slot = (int64_t)y * n + x;
-                        if (lane == sj) {
-                            acc_s += te;
-                            acc_q += te * te;
-                            acc_c += ((float)te >= a.observed[slot]) ? 1 : 0;
+                        if (lane == sj) {  // the float32 TE_k, as emit_te accumulates it
+                            const float tf = (float)te;
+                            const double td = (double)tf;
+                            acc_s += td;
+                            acc_q += td * td;
+                            acc_c += (tf >= a.observed[slot]) ? 1 : 0;
                         }
                         if (a.te_k_dbg && lane == 0) a.te_k_dbg[slot * a.n_surr + k] = (float)te;
                     }
@@ -371,10 +373,12 @@ __global__ __launch_bounds__(256) void surrogate_wave_kernel(tnx_surr_args_t a, 
                         const DF d = reduce_pair16(hist, bx, by, L, ltbl, tbl, lane);
                         const double te = (df_to_double(d) + S12k) * invN;
                         const int64_t slot = (int64_t)y * n + x;
-                        if (lane == (sj / kSurrWaves)) {
-                            acc_s += te;
-                            acc_q += te * te;
-                            acc_c += ((float)te >= a.observed[slot]) ? 1 : 0;
+                        if (lane == (sj / kSurrWaves)) {  // the float32 TE_k
+                            const float tf = (float)te;
+                            const double td = (double)tf;
+                            acc_s += td;
+                            acc_q += td * td;
+                            acc_c += (tf >= a.observed[slot]) ? 1 : 0;
                         }
                         if (a.te_k_dbg && lane == 0) a.te_k_dbg[slot * a.n_surr + k] = (float)te;
                     }

@@ -17,6 +17,21 @@
 
 namespace tnx {
 
+// Raw buffer access to one row: the base address and size live in four
+// scalar registers and each load adds a per-lane 32-bit offset, so a wave
+// streaming a row keeps no 64-bit address per chunk; loads past `bytes`
+// return zero.  (Buffer resource word 3 of the gfx9 family, raw addressing.)
+typedef int32_t i32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                             0x00020000);
+}
+__device__ __forceinline__ uint4 row_load16(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                            uint32_t soff) {
+    const i32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
+    return make_uint4((uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z, (uint32_t)v.w);
+}
+
 __device__ __forceinline__ void lds_inc(uint32_t *p) {
     __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -146,12 +161,13 @@ __device__ __forceinline__ void load_target_regs(TargetRegs<NI> &r,
 #pragma unroll
     for (int i = 0; i < NI; i++) {
         const int t = 16 * lane + 1024 * i;
+        uint4 a = make_uint4(0u, 0u, 0u, 0u), b = a;  // state 0 past L
         if (t < L) {
-            const uint4 a = *reinterpret_cast<const uint4 *>(st + t);
-            const uint4 b = *reinterpret_cast<const uint4 *>(st + t + 8);
-            r.s[8 * i + 0] = a.x; r.s[8 * i + 1] = a.y; r.s[8 * i + 2] = a.z; r.s[8 * i + 3] = a.w;
-            r.s[8 * i + 4] = b.x; r.s[8 * i + 5] = b.y; r.s[8 * i + 6] = b.z; r.s[8 * i + 7] = b.w;
+            a = *reinterpret_cast<const uint4 *>(st + t);
+            b = *reinterpret_cast<const uint4 *>(st + t + 8);
         }
+        r.s[8 * i + 0] = a.x; r.s[8 * i + 1] = a.y; r.s[8 * i + 2] = a.z; r.s[8 * i + 3] = a.w;
+        r.s[8 * i + 4] = b.x; r.s[8 * i + 5] = b.y; r.s[8 * i + 6] = b.z; r.s[8 * i + 7] = b.w;
     }
 }
 
@@ -299,9 +315,10 @@ __device__ __forceinline__ uint32_t lds_take(uint32_t *p) {
 // copies and give S3.  Then per (x_t, y_t) column the b_x totals of
 // x_{t+tau} give S2b (without copies, this column pass gives S3 too).
 // Leaves the slice zero.
+template <class XL>
 __device__ __forceinline__ DF hist_reduce_wave_copies(uint32_t *hist, int bx, int by,
-                                                      const WaveCopies &cp,
-                                                      const float2 *__restrict__ tbl, int lane) {
+                                                      const WaveCopies &cp, const XL &xl,
+                                                      int lane) {
     const int ncol = bx * by;
     const int cells = bx * ncol;
     DF a3 = {0.f, 0.f}, a2 = {0.f, 0.f};
@@ -323,13 +340,13 @@ __device__ __forceinline__ DF hist_reduce_wave_copies(uint32_t *hist, int bx, in
                 t += (v.x + v.y) + (v.z + v.w);
             }
             tot[k] = t;
-            xlogx_add(a3, tbl, t);
+            xl(a3, t);
         }
         wave_lds_fence();
         for (int col = lane; col < ncol; col += kWave) {
             uint32_t c2 = 0;
             for (int aa = 0; aa < bx; aa++) c2 += lds_take(tot + aa * ncol + col);
-            xlogx_add(a2, tbl, c2);
+            xl(a2, c2);
         }
     } else if (cp.G > 1) {
         const int G = cp.G;
@@ -341,13 +358,13 @@ __device__ __forceinline__ DF hist_reduce_wave_copies(uint32_t *hist, int bx, in
             uint32_t t = 0;
             for (int j = 0; j < G; j++) t += lds_take(row + ((j + f) & (G - 1)));
             tot[k] = t;
-            xlogx_add(a3, tbl, t);
+            xl(a3, t);
         }
         wave_lds_fence();
         for (int col = lane; col < ncol; col += kWave) {
             uint32_t c2 = 0;
             for (int aa = 0; aa < bx; aa++) c2 += lds_take(tot + aa * ncol + col);
-            xlogx_add(a2, tbl, c2);
+            xl(a2, c2);
         }
     } else {
         for (int col = lane; col < ncol; col += kWave) {
@@ -355,9 +372,9 @@ __device__ __forceinline__ DF hist_reduce_wave_copies(uint32_t *hist, int bx, in
             for (int aa = 0; aa < bx; aa++) {
                 const uint32_t v = lds_take(hist + aa * ncol + col);
                 c2 += v;
-                xlogx_add(a3, tbl, v);
+                xl(a3, v);
             }
-            xlogx_add(a2, tbl, c2);
+            xl(a2, c2);
         }
     }
     df_add(a3, -a2.hi, -a2.lo);
@@ -387,13 +404,14 @@ __device__ __forceinline__ DF df_block_sum(DF u, float *red) {
 // are in registers (thread tid owns samples 16 tid + 16 nth i + q, NB chunks,
 // L <= 16 nth NB) and so is the source row (ys, same samples).  Returns
 // S3 - S2b, valid in thread 0.  hist and c2b must be zero on entry and are
-// zero on exit; the caller's next pair may start right away.
-template <int NB, bool HZ>
+// zero on exit; the caller's next pair may start right away.  cdbg (test
+// hook, may be NULL) receives the pair's joint counts, pass by pass.
+template <int NB, bool HZ, class XL>
 __device__ __forceinline__ DF hist_pair_block(uint32_t *hist, uint32_t *c2b, uint32_t *zero_row,
                                               int cap, const TargetRegs<NB> &regs,
                                               const uint4 *ys, int bx, int by, int L,
-                                              const float2 *ltbl, const float2 *__restrict__ tbl,
-                                              float *red) {
+                                              const XL &xl, float *red,
+                                              int32_t *cdbg = nullptr) {
     const int tid = threadIdx.x, nth = blockDim.x;
     uint32_t *zero = zero_row + (tid & 63);
     const int ncol = bx * by;
@@ -442,6 +460,11 @@ __device__ __forceinline__ DF hist_pair_block(uint32_t *hist, uint32_t *c2b, uin
             if (tid < kWave) fold_zero_key<HZ>(hist, zero_row, tid);
             __syncthreads();
         }
+        if (cdbg) {  // test hook: the pass's counts, read-only, before the scan clears them
+            const int pc = (a1 - a0) * ncol;
+            for (int i = tid; i < pc; i += nth) cdbg[a0 * ncol + i] = (int32_t)hist[i];
+            __syncthreads();
+        }
         // column-owner scan: cell (a, col) of the pass is word a * ncol + col;
         // a thread takes whole columns (or, when there are fewer columns than
         // threads, every R-th row of one), so the b_t * b_s marginal of its
@@ -455,7 +478,7 @@ __device__ __forceinline__ DF hist_pair_block(uint32_t *hist, uint32_t *c2b, uin
                 for (int r = 0; r < na; r++) {
                     const uint32_t v = lds_take(hist + r * ncol + col);
                     c2 += v;
-                    xlogx_add_l(a3, ltbl, tbl, v);
+                    xl(a3, v);
                 }
                 c2b[col] += c2;  // the column's only writer in this pass
             }
@@ -467,7 +490,7 @@ __device__ __forceinline__ DF hist_pair_block(uint32_t *hist, uint32_t *c2b, uin
                 for (int r = tid / ncol; r < na; r += R) {
                     const uint32_t v = lds_take(hist + r * ncol + col);
                     c2 += v;
-                    xlogx_add_l(a3, ltbl, tbl, v);
+                    xl(a3, v);
                 }
                 if (c2)
                     __hip_atomic_fetch_add(c2b + col, c2, __ATOMIC_RELAXED,
@@ -477,7 +500,7 @@ __device__ __forceinline__ DF hist_pair_block(uint32_t *hist, uint32_t *c2b, uin
         __syncthreads();
     }
     DF a2 = {0.f, 0.f};
-    for (int c = tid; c < ncol; c += nth) xlogx_add_l(a2, ltbl, tbl, lds_take(c2b + c));
+    for (int c = tid; c < ncol; c += nth) xl(a2, lds_take(c2b + c));
     df_add(a3, -a2.hi, -a2.lo);
     return df_block_sum(a3, red);
 }

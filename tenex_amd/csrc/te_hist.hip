@@ -28,79 +28,11 @@
 #include <vector>
 
 #include "common.hpp"
+#include "te_args.hpp"
 #include "te_device.hpp"
 
 namespace tnx {
 
-constexpr int kWavesPerWG = 4;
-constexpr int kBlockThreads = 512;
-#ifndef TNX_DEFAULT_CAP_WAVE
-#define TNX_DEFAULT_CAP_WAVE 3200  // 12.8 KiB per wave: 3 workgroups of 4 waves per CU
-#endif
-constexpr int kDefaultCapWave = TNX_DEFAULT_CAP_WAVE;
-constexpr int kDefaultCapBlock = 16384;  // 64 KiB + the b_t b_s marginal
-// LDS the planner aims for per block-kernel workgroup, so that two
-// workgroups fit one CU's 160 KiB: histogram + marginal + reduction scratch.
-constexpr int kBlockLdsTarget = 80 * 1024 - 512 - 8 * kSmallTbl;
-constexpr int kWaveItemPairs = 256;  // pairs per wave-kernel work item
-constexpr int kXcds = 8;             // gfx950: 8 XCDs, workgroups dealt round-robin
-constexpr int kChunkSrc = 64;        // sources per locality chunk
-
-// Words of one wave's LDS slice: cap histogram words and the 64 lanes' key-0
-// words, rounded to 16 bytes (the copy fold reads four copies at a time).
-__host__ __device__ inline int wave_slice(int cap) { return (cap + kWave + 3) & ~3; }
-
-// Item of workgroup slot w in [0, n): workgroup w runs on XCD w % 8, and XCD
-// x takes the x-th contiguous slice of the items, in order (a bijection of
-// [0, n)).
-// Writes TE of the pair at slot: the value, or (surrogate statistics) its
-// float32 rounding added to the FP64 sum and sum of squares and the count of
-// values >= the observed TE (float32 compare), as the reference accumulates
-// its per-surrogate float32 TE matrices.
-__device__ __forceinline__ void emit_te(const tnx_te_args_t &a, int64_t slot, double te) {
-    if (a.out_mode == TNX_OUT_STATS) {
-        const float tf = (float)te;
-        const double td = (double)tf;
-        a.sum_te[slot] += td;
-        a.sumsq_te[slot] += td * td;
-        a.count_ge[slot] += (tf >= a.observed[slot]) ? 1 : 0;
-    } else {
-        a.out[slot] = (float)te;
-    }
-}
-
-__device__ __forceinline__ int64_t xcd_item(int64_t w, int64_t n) {
-    const int64_t x = w % kXcds, i = w / kXcds;
-    const int64_t q = n / kXcds, r = n % kXcds;
-    return x * q + (x < r ? x : r) + i;
-}
-constexpr int kBlockItemPairs = 16;  // pairs per block-kernel work item (at most)
-// Small plans split their pairs into more, shorter items so that every wave
-// slot gets work: about four items per resident workgroup (256 CUs x 3
-// wave-kernel or 2 block-kernel workgroups).
-constexpr int64_t kFillWaveItems = 256 * 3 * 4;
-constexpr int64_t kFillBlockItems = 256 * 2 * 4;
-
-// Output slot of pair (x, y); returns false if the pair is filtered out.
-__device__ __forceinline__ bool out_slot(const tnx_te_args_t &a, int x, int y, int64_t &slot) {
-    if (a.out_mode == TNX_OUT_MATRIX) {
-        slot = (int64_t)x * a.ld_out + y;
-        return true;
-    }
-    if (a.out_mode == TNX_OUT_STATS) {
-        slot = (int64_t)y * a.ld_out + x;  // public layout [source, target]
-        return true;
-    }
-    if (a.out_mode == TNX_OUT_STAGE) {  // x, y in the same copy x / stage_vars
-        const int k = x / a.stage_vars;
-        slot = (int64_t)k * a.stage_stride + (int64_t)(y - k * a.stage_vars) * a.ld_out +
-               (x - k * a.stage_vars);
-        return true;
-    }
-    int64_t id = (int64_t)x * (a.n_vars - 1) + (y - (y > x ? 1 : 0));
-    slot = id - a.pair_offset;
-    return slot >= 0 && slot < a.n_pairs_local;
-}
 
 // Wave-per-pair histogram kernel.  The four waves of a workgroup take the
 // (up to four) targets of a work item and stream the same source rows, which
@@ -204,7 +136,7 @@ __device__ __forceinline__ void te_wave_body(const tnx_te_args_t &a) {
             }
             wave_lds_fence();
             fold_zero_key<HZ>(hist, zero_row, lane);
-            const DF d = hist_reduce_wave_copies(hist, bx, by, cp, tbl, lane);
+            const DF d = hist_reduce_wave_copies(hist, bx, by, cp, XlTable{tbl}, lane);
             wave_lds_fence();
             if (lane == 0) emit_te(a, slot, (df_to_double(d) + s12) * invN);
         }
@@ -298,7 +230,7 @@ __global__ __launch_bounds__(kBlockThreads) void te_block_kernel(tnx_te_args_t a
                 if (want && !fits && tid == 0) emit_te(a, slot, __builtin_nan(""));  // plan mismatch
                 if (want && fits) {
                     const DF d = hist_pair_block<NB, HZ>(hist, c2b, zero_row, a.cap_block, regs, ycur,
-                                                     bx, by, L, ltbl, tbl, red);
+                                                     bx, by, L, XlSmall{ltbl, tbl}, red);
                     if (tid == 0) emit_te(a, slot, (df_to_double(d) + s12) * invN);
                 }
                 y = yn;
@@ -359,7 +291,7 @@ __global__ __launch_bounds__(256) void te_counts_kernel(
         counts[p * cap + i] = (int32_t)v;
     }
     wave_lds_fence();
-    const DF d = hist_reduce_wave_copies(hist, bx, by, cp, tbl, lane);
+    const DF d = hist_reduce_wave_copies(hist, bx, by, cp, XlTable{tbl}, lane);
     if (lane == 0) te_out[p] = (float)((df_to_double(d) + (S1[x] - S2a[x])) / (double)L);
 }
 
@@ -575,8 +507,8 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
                   "tnx_te_run: null output pointer for out_mode %d", a.out_mode);
     TNX_CHECK_ARG(a.tau >= 1 && a.tau < a.T, "tnx_te_run: need 1 <= tau < T");
     TNX_CHECK_ARG(a.ld >= a.T && a.ld % 16 == 0, "tnx_te_run: ld must be >= T and a multiple of 16");
-    TNX_CHECK_ARG(a.lds >= a.T - a.tau && a.lds % 8 == 0,
-                  "tnx_te_run: lds must be >= T - tau and a multiple of 8");
+    TNX_CHECK_ARG(a.lds >= a.T - a.tau && a.lds % 16 == 0,
+                  "tnx_te_run: lds must be >= T - tau and a multiple of 16");
     TNX_CHECK_ARG(a.T - a.tau < (1LL << 31), "tnx_te_run: L too large");
     TNX_CHECK_ARG(a.out_mode == TNX_OUT_MATRIX || a.out_mode == TNX_OUT_PAIRFREE ||
                       a.out_mode == TNX_OUT_STATS || a.out_mode == TNX_OUT_STAGE,
@@ -590,6 +522,12 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
     TNX_CHECK_ARG((size_t)kWavesPerWG * wave_slice(a.cap_wave) * 4 <= 160 * 1024,
                   "tnx_te_run: cap_wave too large");
     hipStream_t s = as_stream(stream);
+    if (a.queue_items != nullptr) {
+        if (a.n_wave_items + a.n_block_items == 0) return TNX_OK;
+        const int ni = wave_variant(a.T - a.tau);
+        return launch_persist(a, ni, s);
+    }
+    TNX_CHECK_ARG(a.counts_map == nullptr, "tnx_te_run: counts_map needs the persistent launch");
     if (a.n_wave_items > 0) {
         TNX_CHECK_ARG(a.work, "tnx_te_run: null work");
         size_t shm = (size_t)kWavesPerWG * wave_slice(a.cap_wave) * 4;

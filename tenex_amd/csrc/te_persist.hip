@@ -1,0 +1,768 @@
+// te_persist.hip -- every pair of a plan in one persistent launch.
+//
+// One grid of resident workgroups (256 threads, four waves, three per CU)
+// takes entries from eight work queues with an atomic take counter each.  A
+// workgroup drains the queue of its own XCD first (the planner deals entries
+// to queues by source chunk, so an XCD's workgroups share source rows in its
+// L2), then the other queues, so the launch ends when the last entry does.
+// An entry is either
+//   a wave item: the four waves split its pairs, one pair per wave at a time,
+//                each wave in a private LDS slice (te_hist.hip's wave kernel
+//                layout), or
+//   a block item: the whole workgroup on one pair at a time, the four
+//                slices as one histogram, several passes over ranges of
+//                x_{t+tau} when the pair has more cells.
+// Both kinds share CUs, so the latency-bound block pairs (barriers, scans)
+// overlap the LDS-atomic-bound wave pairs instead of running after them.
+//
+// Per wave pair the source row is read through a buffer resource in groups
+// of 1024-sample chunks that are software-pipelined across pairs: before a
+// group is counted the next one (after the last group, the next pair's
+// first) is requested.  c*log2(c) of counts below 256 comes from an LDS copy
+// of the table, larger ones are computed in FP64 exactly as the table is, so
+// the reduction reads no global memory.  Wave pairs give the same counts and
+// the same TE bits as the two-launch wave kernel (same accumulation, same
+// summation order); block pairs the same counts, their TE summed over 256
+// instead of 512 threads.
+//
+// Replaces the pair-free launches te_smem_launch_pairfree
+// (tenex/csrc/full_smem.cu:350-381) and te_adaptive_smem_launch_pairfree
+// (tenex/csrc/adaptive_smem.cu:351-376) for series of up to 12,288 samples.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <tuple>
+#include <vector>
+
+#include "common.hpp"
+#include "te_args.hpp"
+#include "te_device.hpp"
+
+namespace tnx {
+
+__device__ __noinline__ float2 xlogx_hilo_call(uint32_t c) { return xlogx_hilo(c); }
+
+#ifndef TNX_PERSIST_WAVE_XL
+#define TNX_PERSIST_WAVE_XL 1  // wave-role c log2 c: 0 LDS table + FP64, 1 global table
+#endif
+
+constexpr int kQueueStride = 16;  // take counters 64 bytes apart
+static_assert(2 * TNX_QUEUES * kQueueStride <= TNX_QUEUE_CTR_INTS, "queue counters");
+constexpr int kPersistThreads = kWavesPerWG * kWave;
+
+__global__ void queue_reset_kernel(int32_t *ctr) {
+    if (threadIdx.x < TNX_QUEUE_CTR_INTS) ctr[threadIdx.x] = 0;
+}
+
+// Source chunks [C0, C0 + CN) of one pair for lane `lane`: chunk i covers
+// bytes [16 lane + 1024 i, +16) of the row (rsrc spans the padded row, whose
+// bytes past T are zero; chunks past the row read zero).
+template <int CN>
+__device__ __forceinline__ void load_group(uint4 *ys, __amdgpu_buffer_rsrc_t yr, int C0,
+                                           int lane) {
+#pragma unroll
+    for (int u = 0; u < CN; u++) ys[u] = row_load16(yr, 16u * lane, 1024u * (C0 + u));
+}
+
+// Target states of a register-resident wave pass (load_target_regs through
+// a buffer resource: zero past the padded row).
+template <int NI>
+__device__ __forceinline__ void load_target_buf(TargetRegs<NI> &r, __amdgpu_buffer_rsrc_t st,
+                                                int L, int lane) {
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+        const int t = 16 * lane + 1024 * i;
+        uint4 a = make_uint4(0u, 0u, 0u, 0u), b = a;  // state 0 past L
+        if (1024 * i < L) {  // wave-uniform; lanes past L read the zero padding or zero
+            a = row_load16(st, 32u * lane, 2048u * i);
+            b = row_load16(st, 32u * lane + 16u, 2048u * i);
+        }
+        (void)t;
+        r.s[8 * i + 0] = a.x; r.s[8 * i + 1] = a.y; r.s[8 * i + 2] = a.z; r.s[8 * i + 3] = a.w;
+        r.s[8 * i + 4] = b.x; r.s[8 * i + 5] = b.y; r.s[8 * i + 6] = b.z; r.s[8 * i + 7] = b.w;
+    }
+}
+
+// Count chunks [C0, C0 + CN) of the pair (target states in regs, source
+// chunks in ys; C0 is a constant once the caller's group loop is unrolled)
+// into the privatised histogram.  Only the chunk holding sample L - 1 can be
+// partial; its samples past L are not predicated one by one (per-sample exec
+// masks cost scalar registers the register-resident target cannot spare)
+// but sent by a select: with HZ to the lane's key-0 word, whose surplus is
+// taken off cell 0 after the fold (pad_count); without HZ to `trash`, a
+// lane-private word the caller clears.  Chunks wholly
+// past L, or past NI, are skipped (wave-uniform).
+template <int NI, int CN, bool HZ>
+__device__ __forceinline__ void count_group(uint32_t *hist, TargetRegs<NI> &r, const uint4 *ys,
+                                            int C0, uint32_t kmul, uint32_t ysh, uint32_t *zero,
+                                            uint32_t trash_off, int L, int lane) {
+#pragma unroll
+    for (int k = 0; k < NI * 8; k++) asm volatile("" : "+v"(r.s[k]));
+    const uint32_t *sreg = r.s;
+#pragma unroll
+    for (int u = 0; u < CN; u++) {
+        const int i = C0 + u;
+        if (i >= NI || 1024 * i >= L) break;  // wave-uniform
+        const uint32_t yw[4] = {ys[u].x, ys[u].y, ys[u].z, ys[u].w};
+        if (1024 * i + 1024 <= L) {
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
+                const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                lds_inc_key0<HZ>(hist, sv, kmul, yv, ysh, zero);
+            }
+        } else {
+            // samples past L (state 0 past L, but the source bytes between
+            // L and T are real): with HZ to the lane's key-0 word, taken off
+            // cell 0 after the fold; without HZ to the trash word
+            const int nv = L - (16 * lane + 1024 * i);  // valid samples of this lane's 16
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
+                const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                const uint32_t off = __umul24(sv, kmul) + (yv << ysh);
+                if constexpr (HZ) {
+                    const uint32_t o = q < nv ? off : 0u;
+                    lds_inc(o ? reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) + o)
+                              : zero);
+                } else {
+                    lds_inc(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) +
+                                                         (q < nv ? off : trash_off)));
+                }
+            }
+        }
+    }
+}
+
+// Samples past L that a register-resident pass counts in the key-0 words (HZ).
+__device__ __forceinline__ uint32_t pad_count(int L) {
+    return (uint32_t)(((L + 1023) / 1024) * 1024 - L);
+}
+
+// Source-row pipeline of a wave pair: NG groups of GS chunks in two
+// alternating buffers; before group g is counted, group g + 1 (after the
+// last group: group 0 of the next pair) is requested into the other buffer.
+// NG is even, so a pair always starts in buffer 0.  Long series use four
+// groups to keep the in-flight chunks (two groups) within registers.
+#ifndef TNX_PERSIST_NG_LONG
+#define TNX_PERSIST_NG_LONG 4  // source groups per pair for NI >= 10
+#endif
+template <int NI>
+struct SrcPipe {
+    static constexpr int NG = NI >= 10 ? TNX_PERSIST_NG_LONG : 2;
+    static constexpr int GS = (NI + NG - 1) / NG;
+};
+
+template <int NI, int G, bool HZ>
+__device__ __forceinline__ void pipe_count(uint4 (&b0)[SrcPipe<NI>::GS],
+                                           uint4 (&b1)[SrcPipe<NI>::GS], TargetRegs<NI> &regs,
+                                           __amdgpu_buffer_rsrc_t yr, __amdgpu_buffer_rsrc_t yr_next,
+                                           bool more, bool run,
+                                           uint32_t *hc, uint32_t kmul, uint32_t ysh,
+                                           uint32_t *zero, uint32_t trash_off, int L, int lane) {
+    constexpr int NG = SrcPipe<NI>::NG, GS = SrcPipe<NI>::GS;
+    if constexpr (G < NG) {
+        uint4(&cur)[GS] = (G & 1) ? b1 : b0;
+        uint4(&nxt)[GS] = (G & 1) ? b0 : b1;
+        if constexpr (G + 1 < NG) {
+            load_group<GS>(nxt, yr, (G + 1) * GS, lane);
+        } else {
+            if (more) load_group<GS>(nxt, yr_next, 0, lane);
+        }
+        if (run) count_group<NI, GS, HZ>(hc, regs, cur, G * GS, kmul, ysh, zero, trash_off, L, lane);
+        pipe_count<NI, G + 1, HZ>(b0, b1, regs, yr, yr_next, more, run, hc, kmul, ysh, zero,
+                                  trash_off, L, lane);
+    }
+}
+
+// Test hook: this pair's folded joint counts (the G copies summed) to
+// counts_dbg, read-only on the slice (one wavefront).
+__device__ __forceinline__ void dump_wave_counts(const tnx_te_args_t &a, int x, int y,
+                                                 const uint32_t *hist, int cells,
+                                                 const WaveCopies &cp, int lane) {
+    const int r = a.counts_map[(int64_t)x * a.n_vars + y];
+    if (r < 0 || cells > a.counts_ld) return;
+    int32_t *dst = a.counts_dbg + (int64_t)r * a.counts_ld;
+    for (int k = lane; k < cells; k += kWave) {
+        uint32_t v = 0;
+        for (int j = 0; j < cp.G; j++) v += hist[(k << cp.gsh) + j];
+        dst[k] = (int32_t)v;
+    }
+}
+
+// The pairs q = pfirst, pfirst + pstep, ... of a wave item (pair q is
+// target tgt_order[tgt_beg + q % cnt] against source src_order[src_beg +
+// q / cnt]), one wave.
+template <int NI, bool HZ>
+__device__ __forceinline__ void persist_wave_item(const tnx_te_args_t &a, const tnx_work_t it,
+                                                  int pfirst, int pstep, uint32_t *hist,
+                                                  uint32_t *zero_row, const float2 *ltbl, int L,
+                                                  double invN, int lane) {
+    constexpr int GS = SrcPipe<NI>::GS;
+    const int cnt = it.tgt_cnt;
+    const int npair = cnt * (it.src_end - it.src_beg);
+    if (pfirst >= npair) return;
+    int ti = 0, sj = 0;
+    for (int u = 0; u < pfirst; u++)
+        if (++ti == cnt) { ti = 0; ++sj; }
+    int x = a.tgt_order[it.tgt_beg + ti];
+    int y = a.src_order[it.src_beg + sj];
+    int bx = a.n_per_var[x], by = a.n_per_var[y];
+    TargetRegs<NI> regs;
+    int cur_x = -1;
+    double s12 = 0.0;
+    const uint32_t row_bytes = (uint32_t)a.ld;
+    uint4 b0[GS], b1[GS];
+    load_group<GS>(b0, row_rsrc(a.bins + (int64_t)y * a.ld, row_bytes), 0, lane);
+    for (int q = pfirst; q < npair; q += pstep) {
+        for (int u = 0; u < pstep; u++)
+            if (++ti == cnt) { ti = 0; ++sj; }
+        const bool more = q + pstep < npair;
+        int xn = x, yn = y, bxn = bx, byn = by;
+        if (more) {
+            xn = a.tgt_order[it.tgt_beg + ti];
+            yn = a.src_order[it.src_beg + sj];
+            bxn = a.n_per_var[xn];
+            byn = a.n_per_var[yn];
+        }
+        if (x != cur_x) {
+            load_target_buf<NI>(regs, row_rsrc(a.states + (int64_t)x * a.lds, 2u * (uint32_t)a.lds),
+                                L, lane);
+            s12 = a.S1[x] - a.S2a[x];
+            cur_x = x;
+        }
+        int64_t slot = 0;
+        const bool want = (x != y) && out_slot(a, x, y, slot);
+        const int cells = bx * bx * by;
+        const bool fits = cells <= a.cap_wave;
+        const bool run = want && fits;
+        const WaveCopies cp = wave_copies(cells, a.cap_wave, TNX_WAVE_COPY_BUDGET);
+        uint32_t *hc = hist + (lane & (cp.G - 1));
+        const uint32_t kmul = (uint32_t)by << (2 + cp.gsh), ysh = 2 + cp.gsh;
+        // without HZ the lane's key-0 word is free: padded samples go there
+        const uint32_t trash_off = (uint32_t)(a.cap_wave + lane - (lane & (cp.G - 1))) * 4u;
+        pipe_count<NI, 0, HZ>(b0, b1, regs, row_rsrc(a.bins + (int64_t)y * a.ld, row_bytes),
+                              row_rsrc(a.bins + (int64_t)yn * a.ld, row_bytes), more, run, hc,
+                              kmul, ysh, zero_row + lane, trash_off, L, lane);
+        if (run) {
+            wave_lds_fence();
+            if constexpr (HZ) {
+                fold_zero_key<HZ>(hist, zero_row, lane);
+                if (lane == 0) hist[0] -= pad_count(L);
+                wave_lds_fence();
+            } else {
+                zero_row[lane] = 0u;  // the padded samples' counts
+            }
+            if (a.counts_map) dump_wave_counts(a, x, y, hist, cells, cp, lane);
+#if TNX_PERSIST_WAVE_XL == 1
+            const DF d = hist_reduce_wave_copies(
+                hist, bx, by, cp, XlTable{reinterpret_cast<const float2 *>(a.xlogx)}, lane);
+#else
+            const DF d = hist_reduce_wave_copies(hist, bx, by, cp, XlComputed{ltbl}, lane);
+#endif
+            wave_lds_fence();
+            if (lane == 0) emit_te(a, slot, (df_to_double(d) + s12) * invN);
+        } else if (want && lane == 0) {
+            emit_te(a, slot, __builtin_nan(""));  // plan mismatch: the pair is not counted
+        }
+        x = xn; y = yn; bx = bxn; by = byn;
+    }
+}
+
+// One pair, whole workgroup (hist_pair_block's passes over ranges of
+// x_{t+tau}) with a branch-free accumulate and a cell-parallel scan: samples of the pair's other passes and
+// samples past L are sent to `trash`, a thread-private word, by a select
+// instead of an exec mask per sample.  hist, c2b and zero_row are zero on
+// entry and exit; trash holds garbage (the caller clears it before the
+// region serves wave slices again).
+template <int NB, bool HZ>
+__device__ __forceinline__ DF block_pair(uint32_t *hist, uint32_t *c2b, uint32_t *zero_row,
+                                         uint32_t *trash, int cap, const TargetRegs<NB> &regs,
+                                         const uint4 *ys, int bx, int by, int L,
+                                         const XlComputed &xl, float *red, int32_t *cdbg) {
+    const int tid = threadIdx.x, nth = kPersistThreads;
+    uint32_t *zero = zero_row + (tid & 63);
+    const int ncol = bx * by;
+    const uint32_t by4 = (uint32_t)by << 2;
+    int apass = cap / ncol;
+    if (apass < 1) apass = 1;
+    DF a3 = {0.f, 0.f};
+    for (int a0 = 0; a0 < bx; a0 += apass) {
+        const int a1 = (a0 + apass < bx) ? (a0 + apass) : bx;
+        const uint32_t lo = (uint32_t)(a0 * bx), span = (uint32_t)((a1 - a0) * bx);
+        uint32_t *hc = hist - a0 * ncol;  // cell index of the full pair
+        // byte offsets from hc of this thread's trash and key-0 words
+        const uint32_t trash_rel = (uint32_t)((trash + (tid & 63)) - hc) * 4u;
+        const uint32_t zero_rel = (uint32_t)(zero - hc) * 4u;
+        const bool single = a1 - a0 == bx;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            const int c0 = 16 * nth * i;  // first sample of chunk row i
+            if (c0 >= L) break;           // wave-uniform
+            uint32_t yw[4] = {ys[i].x, ys[i].y, ys[i].z, ys[i].w};
+            uint32_t sr[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) sr[k] = regs.s[8 * i + k];
+            // one chunk row at a time: keeps the compiler from computing the
+            // addresses of several rows ahead in registers
+            asm volatile("" : "+v"(yw[0]), "+v"(yw[1]), "+v"(yw[2]), "+v"(yw[3]));
+            asm volatile("" : "+v"(sr[0]), "+v"(sr[1]), "+v"(sr[2]), "+v"(sr[3]), "+v"(sr[4]),
+                         "+v"(sr[5]), "+v"(sr[6]), "+v"(sr[7]));
+            if (single && c0 + 16 * nth <= L) {
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    const uint32_t sv = (sr[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
+                    const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                    lds_inc_key0<HZ>(hc, sv, by4, yv, 2u, zero);
+                }
+            } else {
+                const int nv = L - (16 * tid + c0);  // valid samples of this thread's 16
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    const uint32_t sv = (sr[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
+                    const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+                    const uint32_t off = __umul24(sv, by4) + (yv << 2);
+                    uint32_t rel = (sv - lo < span && q < nv) ? off : trash_rel;
+                    if (HZ) rel = rel ? rel : zero_rel;  // key 0 (first pass only)
+                    lds_inc(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hc) + rel));
+                }
+            }
+        }
+        __syncthreads();
+        if (HZ && a0 == 0) {  // key 0 only occurs in the first pass
+            if (tid < kWave) fold_zero_key<HZ>(hist, zero_row, tid);
+            __syncthreads();
+        }
+        if (cdbg) {  // test hook: the pass's counts, read-only, before the scan clears them
+            const int pc = (a1 - a0) * ncol;
+            for (int k = tid; k < pc; k += nth) cdbg[a0 * ncol + k] = (int32_t)hist[k];
+            __syncthreads();
+        }
+        // cell-parallel scan: thread tid takes cells k = tid + nth j of the
+        // pass with four reads (read and clear) in flight; a nonzero cell
+        // adds its count to its (x_t, y_t) column, c2b[k mod ncol], the
+        // column index kept incrementally (nth mod ncol per step)
+        const int pc = (a1 - a0) * ncol;
+        const int step = nth % ncol;
+        int col = tid % ncol;
+        int k = tid;
+        for (; k + 3 * nth < pc; k += 4 * nth) {
+            uint32_t v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) v[u] = lds_take(hist + k + u * nth);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (v[u]) {
+                    __hip_atomic_fetch_add(c2b + col, v[u], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    xl(a3, v[u]);
+                }
+                col += step;
+                if (col >= ncol) col -= ncol;
+            }
+        }
+        for (; k < pc; k += nth) {
+            const uint32_t v = lds_take(hist + k);
+            if (v) {
+                __hip_atomic_fetch_add(c2b + col, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                xl(a3, v);
+            }
+            col += step;
+            if (col >= ncol) col -= ncol;
+        }
+        __syncthreads();
+    }
+    DF a2 = {0.f, 0.f};
+    for (int c = tid; c < ncol; c += nth) xl(a2, lds_take(c2b + c));
+    df_add(a3, -a2.hi, -a2.lo);
+    return df_block_sum(a3, red);
+}
+
+// Block-role source row: thread tid holds samples 16 tid + 4096 i, i < NB.
+template <int NB>
+__device__ __forceinline__ void load_row256(uint4 *ys, const int8_t *__restrict__ yr, int L) {
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        const int t = 16 * threadIdx.x + 16 * kPersistThreads * i;
+        ys[i] = t < L ? *reinterpret_cast<const uint4 *>(yr + t) : make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
+// A block item: its targets one by one against sources [src_beg, src_end),
+// the whole workgroup on each pair (hist_pair_block), the next source row
+// loaded while the current pair is counted.
+template <int NI, bool HZ>
+__device__ __forceinline__ void persist_block_item(const tnx_te_args_t &a, const tnx_work_t it,
+                                                   uint32_t *region, int capB,
+                                                   const float2 *ltbl, float *red, int L,
+                                                   double invN) {
+    constexpr int NB = (NI + 3) / 4;  // 16-sample chunks per thread (4096 samples per chunk row)
+    const int tid = threadIdx.x;
+    uint32_t *hist = region;
+    uint32_t *c2b = region + capB;
+    uint32_t *zero_row = c2b + a.max_block_cols;
+    uint32_t *trash = zero_row + kWave;
+    TargetRegs<NB> regs;
+    uint4 ycur[NB], ynext[NB];
+    for (int ti = 0; ti < it.tgt_cnt; ti++) {
+        const int x = a.tgt_order[it.tgt_beg + ti];
+        const int bx = a.n_per_var[x];
+        const uint16_t *st = a.states + (int64_t)x * a.lds;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            const int t = 16 * tid + 16 * kPersistThreads * i;
+            if (t < L) {
+                const uint4 u = *reinterpret_cast<const uint4 *>(st + t);
+                const uint4 v = *reinterpret_cast<const uint4 *>(st + t + 8);
+                regs.s[8 * i + 0] = u.x; regs.s[8 * i + 1] = u.y;
+                regs.s[8 * i + 2] = u.z; regs.s[8 * i + 3] = u.w;
+                regs.s[8 * i + 4] = v.x; regs.s[8 * i + 5] = v.y;
+                regs.s[8 * i + 6] = v.z; regs.s[8 * i + 7] = v.w;
+            }
+        }
+        const double s12 = a.S1[x] - a.S2a[x];
+        int y = a.src_order[it.src_beg];
+        int by = a.n_per_var[y];
+        load_row256<NB>(ycur, a.bins + (int64_t)y * a.ld, L);
+        for (int sj = it.src_beg; sj < it.src_end; sj++) {
+            int yn = 0, byn = 0;
+            if (sj + 1 < it.src_end) {
+                yn = a.src_order[sj + 1];
+                byn = a.n_per_var[yn];
+                load_row256<NB>(ynext, a.bins + (int64_t)yn * a.ld, L);
+            }
+            int64_t slot = 0;
+            const bool want = (x != y) && out_slot(a, x, y, slot);
+            const bool fits = bx * by <= a.max_block_cols;
+            if (want && fits) {
+                int32_t *cdbg = nullptr;
+                if (a.counts_map) {
+                    const int r = a.counts_map[(int64_t)x * a.n_vars + y];
+                    if (r >= 0 && (int64_t)bx * bx * by <= a.counts_ld)
+                        cdbg = a.counts_dbg + (int64_t)r * a.counts_ld;
+                }
+                const DF d = block_pair<NB, HZ>(hist, c2b, zero_row, trash, capB, regs, ycur, bx,
+                                                by, L, XlComputed{ltbl}, red, cdbg);
+                if (tid == 0) emit_te(a, slot, (df_to_double(d) + s12) * invN);
+            } else if (want && !fits && tid == 0) {
+                emit_te(a, slot, __builtin_nan(""));  // plan mismatch
+            }
+            y = yn;
+            by = byn;
+#pragma unroll
+            for (int i = 0; i < NB; i++) ycur[i] = ynext[i];
+        }
+    }
+    __syncthreads();  // every thread's trash increments are done
+    if (tid < kWave) trash[tid] = 0u;  // the region serves wave slices next
+}
+
+// Next entry of the queues [q0, q0 + TNX_QUEUES) (wave queues q0 = 0, block
+// queues q0 = TNX_QUEUES), starting at queue q0 + home, or -1 when all are
+// drained; `tries` counts the queues found empty (kept by the caller).
+__device__ __forceinline__ int take_entry(const tnx_te_args_t &a, int q0, int home, int &tries) {
+    while (tries < TNX_QUEUES) {
+        const int qq = q0 + (home + tries) % TNX_QUEUES;
+        const int beg = a.queue_off[qq], n_q = a.queue_off[qq + 1] - beg;
+        if (n_q > 0) {
+            const int pos = atomicAdd(a.queue_ctr + qq * kQueueStride, 1);
+            if (pos < n_q) return a.queue_items[beg + pos];
+        }
+        ++tries;
+    }
+    return -1;
+}
+
+template <int NI, bool HZ>
+__global__ __launch_bounds__(kPersistThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void
+te_persist_kernel(tnx_te_args_t a) {
+    // 16-byte aligned: the copy fold reads four copies per ds_read_b128
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds_u32[];
+    __shared__ float red[2 * kWavesPerWG];
+    __shared__ int s_entry;
+    __shared__ float2 ltbl[kSmallTbl];  // c log2 c of counts < kSmallTbl
+    uint32_t *region = lds_u32;         // four wave slices
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int L = (int)(a.T - a.tau);
+    const double invN = 1.0 / (double)L;
+    const float2 *tbl = reinterpret_cast<const float2 *>(a.xlogx);
+    for (int i = tid; i < kSmallTbl; i += kPersistThreads)
+        ltbl[i] = i <= L ? tbl[i] : make_float2(0.f, 0.f);
+    const int slice = wave_slice(a.cap_wave);
+    for (int i = tid; i < kWavesPerWG * slice; i += kPersistThreads) region[i] = 0;
+    __syncthreads();
+    const int capB = (kWavesPerWG * slice - a.max_block_cols - 2 * kWave) & ~3;
+    const int home = (int)(blockIdx.x % TNX_QUEUES);  // this workgroup's XCD
+    // Block entries: the whole workgroup takes one at a time (thread 0 takes,
+    // LDS broadcast), its own XCD's block queue first.
+    auto block_loop = [&]() {
+        int tries = 0;  // thread 0: block queues found empty so far
+        while (true) {
+            if (tid == 0) s_entry = take_entry(a, TNX_QUEUES, home, tries);
+            __syncthreads();
+            const int e = __builtin_amdgcn_readfirstlane(s_entry);  // uniform: scalar loads
+            __syncthreads();  // s_entry is rewritten by the next take
+            if (e < 0) break;
+            persist_block_item<NI, HZ>(a, a.work[a.n_wave_items + e], region, capB, ltbl, red, L,
+                                       invN);
+        }
+    };
+    // One workgroup in three per XCD starts on block entries, so the
+    // latency-bound block pairs run beside LDS-bound wave pairs on most CUs.
+    if ((blockIdx.x / TNX_QUEUES) % 3 == 0) block_loop();
+    // Wave entries: each wave takes its own, no workgroup barrier (the waves
+    // of a workgroup drift apart instead of waiting for the slowest).  An
+    // entry is item e / 4, pairs q = e % 4, e % 4 + 4, ... (or, with
+    // item_per_wave, item e whole).
+    {
+        uint32_t *hist = region + wave * slice;
+        uint32_t *zero_row = hist + a.cap_wave;
+        const bool per_wave = a.item_per_wave != 0;
+        int tries = 0;
+        while (true) {
+            int e = -1;
+            if (lane == 0) e = take_entry(a, 0, home, tries);
+            e = __builtin_amdgcn_readfirstlane(e);
+            tries = __builtin_amdgcn_readfirstlane(tries);
+            if (e < 0) break;
+            const int64_t wi = per_wave ? e : e / kWavesPerWG;
+            persist_wave_item<NI, HZ>(a, a.work[wi], per_wave ? 0 : e % kWavesPerWG,
+                                      per_wave ? 1 : kWavesPerWG, hist, zero_row, ltbl, L, invN,
+                                      lane);
+        }
+    }
+    __syncthreads();  // every wave slice is clear: the region may serve block pairs
+    block_loop();     // block entries still queued
+}
+
+// Resident workgroups per CU of a persistent kernel times the CUs of the
+// current device (cached per device, kernel and LDS size).
+static int persist_grid(const void *kfn, size_t shm) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    static std::mutex mu;
+    static std::map<std::tuple<int, const void *, size_t>, int> cache;
+    const auto key = std::make_tuple(dev, kfn, shm);
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    int n_cu = 0;
+    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return -1;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kPersistThreads, shm) !=
+        hipSuccess)
+        return -1;
+    const int grid = std::max(1, per_cu) * n_cu;
+    std::lock_guard<std::mutex> g(mu);
+    cache[key] = grid;
+    return grid;
+}
+
+template <bool HZ>
+static const void *persist_fn(int ni) {
+    switch (ni) {
+#define TNX_PERSIST_FN(NI) \
+    case NI: return (const void *)te_persist_kernel<NI, HZ>;
+        TNX_PERSIST_FN(1) TNX_PERSIST_FN(2) TNX_PERSIST_FN(3) TNX_PERSIST_FN(4)
+        TNX_PERSIST_FN(6) TNX_PERSIST_FN(8) TNX_PERSIST_FN(10) TNX_PERSIST_FN(12)
+#undef TNX_PERSIST_FN
+        default: return nullptr;
+    }
+}
+
+int launch_persist(const tnx_te_args_t &a, int ni, hipStream_t s) {
+    TNX_CHECK_ARG(a.queue_off && a.queue_ctr && a.work,
+                  "tnx_te_run: persistent launch needs queue_off, queue_ctr and work");
+    TNX_CHECK_ARG(ni >= 1 && ni <= 12 && a.T - a.tau <= kPersistMaxL,
+                  "tnx_te_run: the persistent launch takes T - tau <= %lld",
+                  (long long)kPersistMaxL);
+    const size_t slice = (size_t)wave_slice(a.cap_wave);
+    const size_t shm = (size_t)kWavesPerWG * slice * 4;  // + the static 2 KiB table
+    TNX_CHECK_ARG(shm <= 156 * 1024,
+                  "tnx_te_run: cap_wave %d too large for the persistent launch", a.cap_wave);
+    if (a.n_block_items > 0)
+        TNX_CHECK_ARG(a.max_block_cols >= 1 &&
+                          kWavesPerWG * (int64_t)slice - a.max_block_cols - 2 * kWave >=
+                              std::max<int64_t>(a.max_block_cols, 4),
+                      "tnx_te_run: max_block_cols %d too large for the persistent launch",
+                      a.max_block_cols);
+    TNX_CHECK_ARG(!a.counts_map || (a.counts_dbg && a.counts_ld >= 1),
+                  "tnx_te_run: counts_map needs counts_dbg and counts_ld");
+    const void *kfn = a.hot_zero ? persist_fn<true>(ni) : persist_fn<false>(ni);
+    TNX_CHECK_ARG(kfn != nullptr, "tnx_te_run: no persistent variant for NI = %d", ni);
+    TNX_CHECK_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    const int grid = persist_grid(kfn, shm);
+    TNX_CHECK_ARG(grid > 0, "tnx_te_run: occupancy query failed");
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(TNX_QUEUE_CTR_INTS), 0, s, a.queue_ctr);
+    TNX_CHECK_LAUNCH("queue_reset_kernel");
+    tnx_te_args_t args = a;
+    void *kargs[] = {&args};
+    TNX_CHECK_HIP(hipLaunchKernel(kfn, dim3((unsigned)grid), dim3(kPersistThreads), kargs, shm, s));
+    TNX_CHECK_LAUNCH("te_persist_kernel");
+    return TNX_OK;
+}
+
+}  // namespace tnx
+
+using namespace tnx;
+
+namespace {
+
+// Relative cost of a pair in sample-update units: the wave path's atomics
+// slowed by the bank conflicts of G copies plus a per-pair reduction; the
+// block path's atomics, its passes' sample scans and its cell scan.
+double wave_pair_cost(int64_t bx, int64_t by, int64_t L, int32_t cap) {
+    const int64_t cells = bx * bx * by;
+    int G = 1;
+    while (G < TNX_MAX_COPIES && 2 * G * cells <= TNX_WAVE_COPY_BUDGET &&
+           (2 * G + 1) * cells <= cap)
+        G *= 2;
+    static const double conflict[6] = {2.4, 2.0, 1.7, 1.4, 1.15, 1.0};  // G = 1 .. 32
+    int lg = 0;
+    while ((1 << lg) < G) lg++;
+    return (double)L * conflict[std::min(lg, 5)] + 4.0 * (double)(G * cells) + 1500.0;
+}
+
+double block_pair_cost(int64_t bx, int64_t by, int64_t L, int64_t capB) {
+    const int64_t ncol = bx * by, cells = bx * ncol;
+    const int64_t per = std::max<int64_t>(1, capB / ncol);
+    const int64_t passes = (bx + per - 1) / per;
+    return (double)L * (1.8 + 0.35 * (double)passes) + 2.0 * (double)cells + 6000.0;
+}
+
+}  // namespace
+
+extern "C" int tnx_te_queue(const tnx_work_t *work, int64_t n_wave_items, int64_t n_block_items,
+                            const int32_t *npv, int64_t n_vars, const int32_t *tgt_order,
+                            const int32_t *src_order, int64_t L, int32_t cap_wave,
+                            int32_t item_per_wave, int32_t *qitems, int64_t qcap, int32_t *qoff,
+                            int64_t *n_wave_entries) {
+    clear_error();
+    TNX_CHECK_ARG(npv && tgt_order && src_order && n_wave_entries &&
+                      (work || n_wave_items + n_block_items == 0),
+                  "tnx_te_queue: null pointer");
+    TNX_CHECK_ARG(n_wave_items >= 0 && n_block_items >= 0 && L >= 1,
+                  "tnx_te_queue: bad sizes");
+    if (cap_wave <= 0) cap_wave = kDefaultCapWave;
+    // wave entries: item * 4 + slot (a wave takes the item's pairs q = slot,
+    // slot + 4, ...), or with item_per_wave the item itself
+    std::vector<int32_t> wave_e;
+    wave_e.reserve(item_per_wave ? n_wave_items : 4 * n_wave_items);
+    for (int64_t i = 0; i < n_wave_items; i++) {
+        if (item_per_wave) {
+            wave_e.push_back((int32_t)i);
+            continue;
+        }
+        const int64_t np = (int64_t)work[i].tgt_cnt * (work[i].src_end - work[i].src_beg);
+        for (int slot = 0; slot < kWavesPerWG && slot < np; slot++)
+            wave_e.push_back((int32_t)(i * kWavesPerWG + slot));
+    }
+    const int64_t nwe = (int64_t)wave_e.size();
+    const int64_t n_entries = nwe + n_block_items;
+    TNX_CHECK_ARG(n_entries < (1LL << 31) && n_wave_items * kWavesPerWG < (1LL << 31),
+                  "tnx_te_queue: too many entries");
+    *n_wave_entries = nwe;
+    if (!qitems) return TNX_OK;  // size query
+    TNX_CHECK_ARG(qoff != nullptr, "tnx_te_queue: null qoff");
+    if (n_entries > qcap) {
+        set_error("tnx_te_queue: qcap %lld < %lld entries", (long long)qcap, (long long)n_entries);
+        return TNX_E_CAP;
+    }
+    // the block role's histogram: four default slices minus the widest marginal
+    int32_t maxcols = 1;
+    for (int64_t b = 0; b < n_block_items; b++) {
+        const tnx_work_t &w = work[n_wave_items + b];
+        for (int32_t t = 0; t < w.tgt_cnt; t++)
+            for (int32_t j = w.src_beg; j < w.src_end; j++)
+                maxcols = std::max(maxcols, npv[tgt_order[w.tgt_beg + t]] * npv[src_order[j]]);
+    }
+    const int64_t capB =
+        std::max<int64_t>(4, kWavesPerWG * wave_slice(cap_wave) - maxcols - 2 * kWave);
+    auto pair_cost = [&](const tnx_work_t &w, int32_t t, int32_t j, bool block) {
+        const int64_t bx = npv[tgt_order[w.tgt_beg + t]], by = npv[src_order[j]];
+        return block ? block_pair_cost(bx, by, L, capB) : wave_pair_cost(bx, by, L, cap_wave);
+    };
+    struct Entry {
+        int32_t id, chunk;
+        int64_t item;
+        double cost, item_cost;
+    };
+    // cost of a wave entry: its pairs (every 4th of the item, or all)
+    std::vector<Entry> went, bent;
+    went.reserve(nwe);
+    for (int32_t e : wave_e) {
+        const int64_t i = item_per_wave ? e : e / kWavesPerWG;
+        const tnx_work_t &w = work[i];
+        const int32_t first = item_per_wave ? 0 : e % kWavesPerWG;
+        const int32_t step = item_per_wave ? 1 : kWavesPerWG;
+        const int64_t np = (int64_t)w.tgt_cnt * (w.src_end - w.src_beg);
+        double c = 0;
+        for (int64_t q = first; q < np; q += step)
+            c += pair_cost(w, (int32_t)(q % w.tgt_cnt), w.src_beg + (int32_t)(q / w.tgt_cnt), false);
+        went.push_back({e, w.src_beg / kChunkSrc, i, c, 0.0});
+    }
+    // an item's slots stay adjacent: order by the item's total cost
+    {
+        std::vector<double> icost(n_wave_items, 0.0);
+        for (const Entry &x : went) icost[x.item] += x.cost;
+        for (Entry &x : went) x.item_cost = icost[x.item];
+    }
+    bent.reserve(n_block_items);
+    for (int64_t b = 0; b < n_block_items; b++) {
+        const tnx_work_t &w = work[n_wave_items + b];
+        double c = 0;
+        for (int32_t t = 0; t < w.tgt_cnt; t++)
+            for (int32_t j = w.src_beg; j < w.src_end; j++) c += pair_cost(w, t, j, true);
+        bent.push_back({(int32_t)b, w.src_beg / kChunkSrc, b, c, c});
+    }
+    // source chunks to XCD queues: heaviest chunk first onto the least loaded
+    int32_t n_chunks = 1;
+    for (const Entry &x : went) n_chunks = std::max(n_chunks, x.chunk + 1);
+    for (const Entry &x : bent) n_chunks = std::max(n_chunks, x.chunk + 1);
+    std::vector<double> chunk_cost(n_chunks, 0.0);
+    for (const Entry &x : went) chunk_cost[x.chunk] += x.cost;
+    for (const Entry &x : bent) chunk_cost[x.chunk] += x.cost;
+    std::vector<int32_t> order(n_chunks);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t p, int32_t q) { return chunk_cost[p] > chunk_cost[q]; });
+    std::vector<int32_t> chunk_q(n_chunks, 0);
+    double load[TNX_QUEUES] = {0};
+    for (int32_t c : order) {
+        int best = 0;
+        for (int qq = 1; qq < TNX_QUEUES; qq++)
+            if (load[qq] < load[best]) best = qq;
+        chunk_q[c] = best;
+        load[best] += chunk_cost[c];
+    }
+    // inside a queue: chunk by chunk (ascending), heaviest items first
+    auto by_queue = [&](const Entry &u, const Entry &v) {
+        const int32_t qu = chunk_q[u.chunk], qv = chunk_q[v.chunk];
+        if (qu != qv) return qu < qv;
+        if (u.chunk != v.chunk) return u.chunk < v.chunk;
+        if (u.item_cost != v.item_cost) return u.item_cost > v.item_cost;
+        return u.id < v.id;
+    };
+    std::stable_sort(went.begin(), went.end(), by_queue);
+    std::stable_sort(bent.begin(), bent.end(), by_queue);
+    int64_t pos = 0;
+    for (int part = 0; part < 2; part++) {
+        const std::vector<Entry> &ent = part ? bent : went;
+        size_t k = 0;
+        for (int qq = 0; qq < TNX_QUEUES; qq++) {
+            qoff[part * TNX_QUEUES + qq] = (int32_t)pos;
+            while (k < ent.size() && chunk_q[ent[k].chunk] == qq) qitems[pos++] = ent[k++].id;
+        }
+    }
+    qoff[2 * TNX_QUEUES] = (int32_t)pos;
+    return TNX_OK;
+}

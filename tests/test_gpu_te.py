@@ -158,7 +158,7 @@ def test_short_and_edge_series(dev):
         pairs = O.pairfree_pairs(12)
         te64 = O.te_pairs(d, npv, pairs, tau=tau)
         assert np.abs(m[pairs[:, 0], pairs[:, 1]] - te64).max() < TOL
-        assert np.all(m[0, :] == 0) or True
+        assert np.abs(m[0, :]).max() < TOL  # a constant target: TE 0 from every source
         # constant source gives zero TE
         col = m[pairs[:, 0], pairs[:, 1]][pairs[:, 1] == 0]
         assert np.all(np.abs(col) < TOL)
