@@ -133,9 +133,9 @@ def measured_traffic():
 
 
 def run_traffic_probe(args):
-    """One C2 TE matrix, then a 256 MiB device copy: the program the
-    FETCH_SIZE / WRITE_SIZE counter passes run (the copy calibrates the
-    counters)."""
+    """One C2 TE matrix (the persistent launch of the headline step), then a
+    256 MiB device copy: the program the FETCH_SIZE / WRITE_SIZE counter
+    passes run (the copy calibrates the counters)."""
     import torch
     from tenex_amd import _lib, core, synthetic
     dev = core.require_gpu("cuda:0")
@@ -151,6 +151,13 @@ def run_traffic_probe(args):
     dst.copy_(src)
     torch.cuda.synchronize()
     log("traffic probe done: one C2 matrix, one 256 MiB copy")
+
+
+def lds_peaks(dev):
+    """Measured chip-wide LDS atomic rates (updates/s): conflict-free and
+    random keys in 2048 cells (tnx_bench_lds_atomics)."""
+    from tenex_amd import core
+    return core.lds_atomic_rate(dev, mode=0), core.lds_atomic_rate(dev, mode=1)
 
 
 def run_c2(args):
@@ -170,6 +177,8 @@ def run_c2(args):
         log(f"autotune: {tuned}")
     t0 = time.perf_counter()
     plan = core.get_plan(prep, cap_wave=cw, cap_block=cb)
+    persistent = core.persistent_ok(prep, plan)
+    targs = core.te_args(prep, plan, None, _lib.TNX_OUT_MATRIX, ld_out=n)  # builds the queues
     plan_s = time.perf_counter() - t0
     info = plan.info
     n_pairs = n * (n - 1)
@@ -181,37 +190,32 @@ def run_c2(args):
         core.gene_prep_into(prep)
         core.te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, args=targs)
 
-    if args.graph:  # the same launches captured once in a HIP graph, replayed per step
+    if not args.no_graph:  # the same launches captured once in a HIP graph, replayed per step
         graph = core.capture_te_step(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n)
         step = graph.replay  # noqa: F811
 
     log(f"plan: {info.n_wave_items} wave items, {info.n_block_items} block items, "
-        f"{plan_s * 1e3:.1f} ms host")
+        f"{'one persistent launch' if persistent else 'two launches'}, {plan_s * 1e3:.1f} ms host")
     dt = timed_steps(step, args.steps, args.warmup)
-    # per-kernel durations: the wave items alone, the block items alone
-    wave_args = core.te_args(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n)
-    wave_args.n_block_items = 0
-    block_args = core.te_args(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n)
-    block_args.n_wave_items = 0
-    block_args.work = plan.work.data_ptr() + 16 * info.n_wave_items  # block items follow
     reps = max(2, args.steps)
-    t_wave = time_launches(lambda: core.te_run(prep, plan, out, 1, args=wave_args), reps)
-    t_block = time_launches(lambda: core.te_run(prep, plan, out, 1, args=block_args), reps)
+    # the TE launch alone (HIP events on the stream it runs on), and the
+    # earlier two-launch kernels on the same plan for comparison
+    t_te = time_launches(lambda: core.te_run(prep, plan, out, 1, args=targs), reps)
     t_prep = time_launches(lambda: core.gene_prep_into(prep), reps)
-    log(f"te_wave kernel {t_wave * 1e3:.2f} ms, te_block kernel {t_block * 1e3:.2f} ms, "
-        f"gene_prep {t_prep * 1e3:.3f} ms")
-    wh = plan.work_host[: info.n_wave_items]
-    # pair slots of the wave items (at most n of them are target == source)
-    wave_pairs = int(np.sum(wh[:, 1].astype(np.int64) * (wh[:, 3] - wh[:, 2])))
+    a2 = core.te_args(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n, persistent=False)
+    t_two = time_launches(lambda: core.te_run(prep, plan, out, 1, args=a2), reps)
+    core.te_run(prep, plan, out, 1, args=targs)
+    log(f"TE launch {t_te * 1e3:.2f} ms ({'persistent' if persistent else 'two launches'}), "
+        f"two-launch kernels {t_two * 1e3:.2f} ms, gene_prep {t_prep * 1e3:.3f} ms")
     L = T - tau
-    # algorithmic HBM bytes of one wave-kernel launch: each bin row read once
-    # (target states 2 B / sample, source 1 B / sample) + 4 B per TE written
-    alg_bytes = n * L * 3 + 4 * wave_pairs
-    hbm_achieved = alg_bytes / t_wave / 1e9
-    lds_peak = core.lds_atomic_rate(dev, mode=0)
-    lds_rand = core.lds_atomic_rate(dev, mode=1)
-    upd_rate = wave_pairs * L / t_wave
+    upd = n_pairs * L            # LDS histogram updates of one TE launch
+    lds_peak, lds_rand = lds_peaks(dev)
+    # algorithmic HBM bytes of one TE launch: each bin row read once (target
+    # states 2 B / sample, source 1 B / sample) + 4 B per TE written
+    alg_bytes = n * L * 3 + 4 * n_pairs
     te_host = out.cpu().numpy()
+    kname = (f"te_persist_kernel<{-(-L // 1024)}, {str(prep.hot_zero).lower()}>" if persistent
+             else "te_wave_reg_kernel + te_block_kernel")
     result = {
         "metric": "TE gene-pairs/sec at 4k genes x 10k cells (mixed-cardinality, all ordered pairs)",
         "value": n_pairs / dt, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
@@ -221,26 +225,27 @@ def run_c2(args):
         "config": {"workload": "C2 mixed-cardinality TE matrix, 4000 genes x 10000 cells",
                    "n_genes": n, "n_cells": T, "tau": tau, "n_pairs": n_pairs,
                    "b_max": int(b_max), "levels": "80% [2,8], 18% [9,32], 2% [33,64]",
-                   "seed": args.seed,
+                   "seed": args.seed, "persistent_launch": persistent,
                    "arith": "int32 counts, fp32 compensated entropy terms, fp64 final combine"},
-        "roofline": {"bound": "hbm", "kernel": "te_wave_reg_kernel", "achieved": hbm_achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(),
-                     "traffic_source": TRAFFIC_FILE,
-                     "note": "histogram path is LDS-atomic bound; see lds_roofline"},
-        "lds_roofline": {"bound": "lds_atomic", "kernel": "te_wave_reg_kernel",
-                         "achieved": upd_rate / 1e9, "peak": lds_peak / 1e9,
-                         "peak_random_keys": lds_rand / 1e9, "unit": "Gupdates/s",
-                         "frac": upd_rate / lds_peak, "updates_per_launch": wave_pairs * L,
-                         "step_frac": n_pairs * L / dt / lds_peak},
-        "kernels_ms": {"te_wave_kernel": t_wave * 1e3, "te_block_kernel": t_block * 1e3,
+        "roofline": {"bound": "lds_atomic", "kernel": kname,
+                     "achieved": upd / t_te / 1e9, "peak": lds_peak / 1e9, "unit": "Gupdates/s",
+                     "frac": upd / t_te / lds_peak, "updates_per_launch": upd,
+                     "peak_random_keys": lds_rand / 1e9,
+                     "step_frac": upd / dt / lds_peak,
+                     "peak_source": "tnx_bench_lds_atomics mode 0 (conflict-free ds_add_u32, "
+                                    "all CUs), measured in this run",
+                     "traffic": measured_traffic(), "traffic_source": TRAFFIC_FILE},
+        "hbm_roofline": {"bound": "hbm", "achieved": alg_bytes / t_te / 1e9,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": alg_bytes / t_te / 1e9 / HBM_PEAK_GBS,
+                         "algorithmic_bytes": alg_bytes, "traffic": measured_traffic()},
+        "kernels_ms": {"te_launch": t_te * 1e3, "te_two_launch_kernels": t_two * 1e3,
                        "gene_prep_kernel": t_prep * 1e3},
-        "wave_pairs": wave_pairs, "block_pairs": n_pairs - wave_pairs,
         "plan_host_ms": plan_s * 1e3,
     }
     if tuned is not None:
         result["config"]["autotune"] = tuned
-    result["config"]["hip_graph"] = bool(args.graph)
+    result["config"]["hip_graph"] = not args.no_graph
     if not args.no_cpu_baseline:
         log("cpu baseline (scatter_add restatement) on a bounded sample ...")
         pairs = sample_pairs(prep.npv_host, args.cpu_sample, args.seed)
@@ -329,10 +334,48 @@ def run_c1(args):
     return result
 
 
+def surrogate_cpu_baseline(d, npv, n_pairs_total, n_tests, sample_pairs_n, seed):
+    """scatter_add restatement timed on one batch of pairs (a surrogate's TE
+    costs what an observed TE costs), extrapolated to n_pairs_total x
+    n_tests TE evaluations."""
+    from oracle import scatter_add_ref as S
+    rng = np.random.default_rng(seed)
+    n = d.shape[0]
+    tg = rng.integers(0, n, sample_pairs_n * 2)
+    sr = rng.integers(0, n, sample_pairs_n * 2)
+    keep = tg != sr
+    pairs = np.stack([tg[keep], sr[keep]], 1)[:sample_pairs_n].astype(np.int64)
+    t_cpu, _ = S.time_pairs(d, int(npv.max()), pairs, tau=1, repeats=1, warmup=0)
+    rate = len(pairs) / t_cpu
+    aff, threads = host_cores()
+    return {"value": n_pairs_total * n_tests / rate, "unit": "s", "cores": int(threads),
+            "kind": "port",
+            "sample": (f"{len(pairs)} random pairs through the scatter_add restatement "
+                       f"({t_cpu:.2f} s on {threads} threads, {aff} cores in affinity), "
+                       f"extrapolated to {n_pairs_total} pairs x {n_tests} TE evaluations "
+                       f"(observed + surrogates)")}
+
+
+def surrogate_oracle_err(d, npv, res, n, T, n_surr, bl, seed, dev, n_check, rng_seed):
+    """max |mean surrogate TE (GPU) - FP64 oracle| over n_check random tested
+    pairs, on the block orders the test drew (same generator and seeds)."""
+    from oracle import oracle as O
+    from tenex_amd import core
+    perm, eff, nb = core.block_perm_device(n, T, "block", bl, 0, n_surr, seed, dev, "cpu")
+    rng = np.random.default_rng(rng_seed)
+    allp = O.pairfree_pairs(n)
+    pairs = allp[rng.choice(len(allp), n_check, replace=False)]       # [target, source]
+    obs = res.observed_te[pairs[:, 1], pairs[:, 0]]                     # public [src, tgt]
+    s, s2, cg = O.surrogate_stats(d, npv, perm.numpy(), eff, pairs, obs)
+    gpu = res.mean_surrogate_te[pairs[:, 1], pairs[:, 0]].astype(np.float64)
+    return float(np.abs(gpu - s / n_surr).max())
+
+
 def run_c3(args):
     import torch
     import tenex_amd as tnx
-    from tenex_amd import synthetic
+    from tenex_amd import core, synthetic
+    dev = core.require_gpu("cuda:0")
     n, T = 1000, 3000
     x = synthetic.zero_inflated_lognormal(n, T, seed=3)
     names = np.array([f"g{i}" for i in range(n)])
@@ -346,6 +389,7 @@ def run_c3(args):
     st = step()
     L = T - 1
     upd = n * (n - 1) * 100 * L
+    lds_peak, lds_rand = lds_peaks(dev)
     result = {"metric": "surrogate-test seconds, 1000 genes x 3000 cells, 100 block shuffles",
               "value": dt, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
               "ms_per_step": dt * 1e3, "higher_is_better": False, "scaling": "weak",
@@ -353,15 +397,26 @@ def run_c3(args):
               "data": "synthetic zero-inflated log-normal (seed 3)",
               "config": {"workload": "C3 plain surrogate test", "b_max": int(res.b_max),
                          "n_surrogates": 100, "block_length": 50, "edges": len(st.grn)},
-              "lds_roofline": {"bound": "lds_atomic", "achieved": upd / dt / 1e9,
-                               "unit": "Gupdates/s"}}
+              "roofline": {"bound": "lds_atomic", "kernel": "whole surrogate test",
+                           "achieved": upd / dt / 1e9, "peak": lds_peak / 1e9,
+                           "unit": "Gupdates/s", "frac": upd / dt / lds_peak,
+                           "updates_per_test": upd, "peak_random_keys": lds_rand / 1e9,
+                           "traffic": None}}
+    d = res.bin_arrs[:, :, 0].astype(np.int32)
+    result["max_abs_err_mean_surrogate_vs_fp64_oracle"] = surrogate_oracle_err(
+        d, res.n_per_var, st, n, T, 100, 50, 42, dev, 200, args.seed)
+    if not args.no_cpu_baseline:
+        log("cpu baseline (scatter_add restatement, per-surrogate TE) on a bounded sample ...")
+        result["cpu_baseline"] = surrogate_cpu_baseline(d, res.n_per_var, n * (n - 1), 101,
+                                                        2048, args.seed)
     return result
 
 
 def run_c4(args):
     import torch
     import tenex_amd as tnx
-    from tenex_amd import synthetic
+    from tenex_amd import core, synthetic
+    dev = core.require_gpu("cuda:0")
     n, T = 6000, 3000
     x = synthetic.zero_inflated_lognormal(n, T, seed=4)
     rng = np.random.default_rng(4)
@@ -385,23 +440,48 @@ def run_c4(args):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / args.steps, out
 
+    def jaccard(p, q):
+        a = set(map(tuple, p.grn.pairs.tolist()))
+        b = set(map(tuple, q.grn.pairs.tolist()))
+        return len(a & b) / max(1, len(a | b)), len(a), len(b)
+
     t_plain, plain = timed(lambda: nw.infer("surrogate_test", **kw))
     t_learn, learned = timed(lambda: nw.infer("surrogate_test", learned="mlp", **kw))
-    a = set(map(tuple, plain.grn.pairs.tolist()))
-    b = set(map(tuple, learned.grn.pairs.tolist()))
-    jac = len(a & b) / max(1, len(a | b))
-    info = {k: v for k, v in learned.learned_info.items() if k != "losses"}
-    return {"metric": "surrogate-test seconds, learned vs plain (6000x3000, 200 TF x 2000 targets)",
-            "value": t_learn, "unit": "s", "n_gpus": 1, "steps": args.steps,
-            "warmup": max(1, args.warmup),
-            "ms_per_step": t_learn * 1e3, "higher_is_better": False, "scaling": "weak",
-            "vs_baseline": None, "dtype": "int32 counts; bf16 weights / fp32 accumulate MLP",
-            "data": "synthetic zero-inflated log-normal with 100 planted TF->target edges",
-            "config": {"workload": "C4 learned-null surrogate test", "n_surrogates": 100,
-                       "block_length": 50},
-            "plain_s": t_plain, "learned_s": t_learn, "speedup": t_plain / t_learn,
-            "edge_jaccard": jac, "edges_plain": len(a), "edges_learned": len(b),
-            "learned_info": info}
+    # the configuration's literal rule: re-run pairs with |z - z_thr| < 1.0
+    t_lit, literal = timed(lambda: nw.infer("surrogate_test", learned="mlp",
+                                            learned_kwargs=dict(band_sigmas=0.0), **kw))
+    jac, ea, eb = jaccard(plain, learned)
+    jac_lit, _, el = jaccard(plain, literal)
+    info = {k: v for k, v in learned.learned_info.items() if k not in ("losses", "rerun_pairs")}
+    L = T - 1
+    n_pairs = 200 * 2000
+    upd = n_pairs * 100 * L
+    lds_peak, _ = lds_peaks(dev)
+    out = {"metric": "surrogate-test seconds, learned vs plain (6000x3000, 200 TF x 2000 targets)",
+           "value": t_learn, "unit": "s", "n_gpus": 1, "steps": args.steps,
+           "warmup": max(1, args.warmup),
+           "ms_per_step": t_learn * 1e3, "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "int32 counts; bf16 weights / fp32 accumulate MLP",
+           "data": "synthetic zero-inflated log-normal with 100 planted TF->target edges",
+           "config": {"workload": "C4 learned-null surrogate test", "n_surrogates": 100,
+                      "block_length": 50, "band": "max(1.0, 5 x held-out RMS z error)"},
+           "roofline": {"bound": "lds_atomic", "kernel": "plain surrogate test (the learned "
+                        "test measures a fraction of these histograms)",
+                        "achieved": upd / t_plain / 1e9, "peak": lds_peak / 1e9,
+                        "unit": "Gupdates/s", "frac": upd / t_plain / lds_peak,
+                        "updates_per_test": upd, "traffic": None},
+           "plain_s": t_plain, "learned_s": t_learn, "speedup": t_plain / t_learn,
+           "edge_jaccard": jac, "edges_plain": ea, "edges_learned": eb,
+           "literal_band": {"band": 1.0, "learned_s": t_lit, "speedup": t_plain / t_lit,
+                            "edge_jaccard": jac_lit, "edges": el,
+                            "n_rerun": int(literal.learned_info["n_rerun"])},
+           "learned_info": info}
+    d = res.bin_arrs[:, :, 0].astype(np.int32)
+    if not args.no_cpu_baseline:
+        log("cpu baseline (scatter_add restatement, per-surrogate TE) on a bounded sample ...")
+        out["cpu_baseline"] = surrogate_cpu_baseline(d, res.n_per_var, n_pairs, 101, 2048,
+                                                     args.seed)
+    return out
 
 
 # fp32 VALU issue: 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-instructions/s
@@ -472,8 +552,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=20480)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--graph", action="store_true",
-                    help="c2: time replays of the step captured in a HIP graph")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="c2: launch the step eagerly instead of replaying it from a HIP graph")
     ap.add_argument("--autotune", action="store_true",
                     help="c2: time the launch candidates on a sample first (core.autotune)")
     ap.add_argument("--traffic-probe", action="store_true",

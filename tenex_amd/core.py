@@ -98,6 +98,13 @@ def upload_bins(bin_arr, device) -> tuple[torch.Tensor, torch.Tensor, int, int]:
             raise ValueError(f"only K=1 bin arrays are supported (got K={t.shape[2]})")
         t = t[:, :, 0]
     n, T = t.shape
+    if t.is_floating_point() or t.dtype == torch.bool or t.is_complex():
+        raise ValueError(f"bins must be an integer array (got {t.dtype})")
+    if n and T:
+        lo, hi = int(t.min().item()), int(t.max().item())
+        if lo < 0 or hi > 126:
+            raise ValueError(f"bins must satisfy 0 <= bin < n_per_var <= 127 (got values in "
+                             f"[{lo}, {hi}]); int8 bins cannot hold them")
     ld = round_up(T, ROW_ALIGN)
     bins = torch.zeros((n, ld), dtype=torch.int8, device=device)
     bins[:, :T] = t.to(device=device, dtype=torch.int8)

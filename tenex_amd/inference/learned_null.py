@@ -362,18 +362,24 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     rr_d = torch.nonzero(border).flatten()
     rr = rr_d.cpu().numpy()
     t_predict = time.perf_counter() - t0 - t_train_plain - t_fit
+    # the sums the decision reads: predicted pairs from the predicted moments,
+    # measured pairs (grid, re-runs) their plain sums as they are, so that
+    # their statistics equal the plain test's bit for bit
+    S = mean_p * n_surrogates
+    S2 = (std_p * std_p + mean_p * mean_p) * n_surrogates
+    S[trd] = ss[ls, lt]
+    S2[trd] = ss2[ls, lt]
     # 4. plain surrogates for the borderline pairs
+    rr_pub = pub_d[rr_d].cpu().numpy() if len(rr) else np.zeros((0, 2), np.int64)
     if len(rr):
-        rr_pub = pub_d[rr_d].cpu().numpy()
         s_b, s2_b, _ = plain_stats_pairs(prep, rr_pub[:, ::-1], None, n_surrogates,
                                          shuffle_method, block_length, seed, rng_device)
-        mean_p[rr_d], std_p[rr_d] = _moments(s_b, s2_b, n_surrogates)
+        S[rr_d] = s_b.to(torch.float64)
+        S2[rr_d] = s2_b.to(torch.float64)
     torch.cuda.synchronize(dev)
     t_rerun = time.perf_counter() - t0 - t_train_plain - t_fit - t_predict
     # 5. decision with the reference arithmetic on the merged null
     obs_d = obs_src.get() if isinstance(obs_src, core.PendingUpload) else obs_src
-    S = mean_p * n_surrogates
-    S2 = (std_p * std_p + mean_p * mean_p) * n_surrogates
     if P == n_mask:
         mean_m, std_m, eff, pv, sigm = surrogate_decision_pairs(obs_d, pub_d, S, S2,
                                                                 n_surrogates, fdr)
@@ -401,5 +407,6 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
                             z_err_holdout=z_err,
                             plain_fraction=(n_train + len(rr)) / max(P, 1), z_threshold=z_thr,
                             t_train_plain=t_train_plain, t_fit=t_fit, t_predict=t_predict,
-                            t_rerun=t_rerun, t_total=t_total, losses=losses)
+                            t_rerun=t_rerun, t_total=t_total, losses=losses,
+                            rerun_pairs=rr_pub)
     return res

@@ -283,6 +283,9 @@ class SurrogateTestMethod(InferenceMethod):
                              f"two samples to be informative.")
         if p_method not in ("parametric", "mc"):
             raise ValueError(f"p_method must be 'parametric' or 'mc' (got {p_method!r})")
+        if kwargs.get("learned") is not None and p_method != "parametric":
+            raise ValueError(f"learned={kwargs.get('learned')!r} predicts a parametric null: "
+                             f"p_method must be 'parametric' (got {p_method!r})")
         bin_data = np.asarray(bin_data)
         if bin_data.ndim == 2:
             bin_data = bin_data[..., None]

@@ -341,3 +341,26 @@ def test_tested_pairs_match_mask_nonzero():
         assert np.array_equal(pub_t.numpy(), exp), (rows, cols)
         assert np.array_equal(srcs, np.unique(exp[:, 0]))
         assert np.array_equal(tgts, np.unique(exp[:, 1]))
+
+
+def test_upload_bins_rejects_out_of_range_levels():
+    """Bins outside 0 <= bin <= 126 would reach the kernels as wrapped int8
+    bytes: upload_bins refuses them (checked on the host tensor)."""
+    ok = np.array([[0, 1, 2], [3, 0, 1]], np.int32)
+    bins, npv, b_max, T = core.upload_bins(ok, "cpu")
+    assert b_max == 4 and T == 3 and list(npv.numpy()) == [3, 4]
+    for bad in (np.array([[0, -1, 2]], np.int32), np.array([[0, 127, 3]], np.int32),
+                np.array([[0, 200, 3]], np.int64)):
+        with pytest.raises(ValueError):
+            core.upload_bins(bad, "cpu")
+    with pytest.raises(ValueError):
+        core.upload_bins(np.zeros((2, 3), np.float32), "cpu")
+
+
+def test_learned_null_requires_parametric_p_values():
+    from tenex_amd.inference.surrogate_test import SurrogateTestMethod
+    names = np.array(list("abc"))
+    kw = dict(bin_data=np.zeros((3, 40), np.int32), n_per_var=np.ones(3, np.int32), b_max=1,
+              kernel="Adaptive-SMEM", learned="mlp", p_method="mc")
+    with pytest.raises(ValueError, match="parametric"):
+        SurrogateTestMethod().infer(np.zeros((3, 3), np.float32), names, None, **kw)
