@@ -18,12 +18,11 @@
 // Per wave pair the source row is read through a buffer resource in groups
 // of 1024-sample chunks that are software-pipelined across pairs: before a
 // group is counted the next one (after the last group, the next pair's
-// first) is requested.  c*log2(c) of counts below 256 comes from an LDS copy
-// of the table, larger ones are computed in FP64 exactly as the table is, so
-// the reduction reads no global memory.  Wave pairs give the same counts and
-// the same TE bits as the two-launch wave kernel (same accumulation, same
-// summation order); block pairs the same counts, their TE summed over 256
-// instead of 512 threads.
+// first) is requested.  The reduction reads each column's cells four at a
+// time and takes c*log2(c) of counts below 256 from an LDS copy of the
+// table.  Counts are those of the two-launch kernels (te_wave_reg_kernel /
+// te_block_kernel); TE differs from theirs only by the order of the
+// compensated sums.
 //
 // Replaces the pair-free launches te_smem_launch_pairfree
 // (tenex/csrc/full_smem.cu:350-381) and te_adaptive_smem_launch_pairfree
@@ -47,7 +46,7 @@ namespace tnx {
 __device__ __noinline__ float2 xlogx_hilo_call(uint32_t c) { return xlogx_hilo(c); }
 
 #ifndef TNX_PERSIST_WAVE_XL
-#define TNX_PERSIST_WAVE_XL 1  // wave-role c log2 c: 0 LDS table + FP64, 1 global table
+#define TNX_PERSIST_WAVE_XL 2  // wave-role c log2 c: 0 LDS table + FP64, 1 global table, 2 split + batched reduce
 #endif
 
 constexpr int kQueueStride = 16;  // take counters 64 bytes apart
@@ -179,6 +178,100 @@ __device__ __forceinline__ void pipe_count(uint4 (&b0)[SrcPipe<NI>::GS],
     }
 }
 
+// c*log2(c): counts below kSmallTbl from the workgroup's LDS copy of the
+// table, larger ones from the table in memory.  The two pointers live in
+// distinct address spaces, so the lookup is a ds_read or a global load,
+// never a generic (flat) one.
+struct XlSplit {
+    const float2 *ltbl;
+    const float2 *__restrict__ tbl;
+    __device__ __forceinline__ void operator()(DF &acc, uint32_t c) const {
+        if (c > 1) {
+            float2 t;
+            if (c < (uint32_t)kSmallTbl) {
+                t = ltbl[c];
+            } else {
+                t = tbl[c];
+                // keeps the two loads apart (merged, they would become one
+                // select of generic pointers and a flat load)
+                asm volatile("" : "+v"(t.x), "+v"(t.y));
+            }
+            df_add(acc, t.x, t.y);
+        }
+    }
+};
+
+// hist_reduce_wave_copies with the LDS reads of a column issued four at a
+// time (their latencies overlap instead of adding up), the lookups of the
+// four counts after them.  Returns S3 - S2b (every lane); leaves the slice
+// zero.
+template <class XL>
+__device__ __forceinline__ DF reduce_wave_ilp(uint32_t *hist, int bx, int by, const WaveCopies &cp,
+                                              const XL &xl, int lane) {
+    const int ncol = bx * by;
+    const int cells = bx * ncol;
+    DF a3 = {0.f, 0.f}, a2 = {0.f, 0.f};
+    uint32_t *col_base = hist;  // where the column pass reads the cell totals
+    if (cp.G > 1) {
+        uint32_t *tot = hist + cp.G * cells;
+        if (cp.G >= 4) {
+            // four copies per 16-byte read, rotated so that the 8 lanes of a
+            // read group land on 8 distinct 4-bank groups
+            const int G4 = cp.G >> 2;
+            const int per_sh = TNX_LDS_BANK_SHIFT - cp.gsh;
+            for (int k = lane; k < cells; k += kWave) {
+                const int f = (k >> per_sh) & (G4 - 1);
+                uint4 *row = reinterpret_cast<uint4 *>(hist + (k << cp.gsh));
+                uint32_t t = 0;
+                for (int j = 0; j < G4; j++) {
+                    uint4 *q = row + ((j + f) & (G4 - 1));
+                    const uint4 v = *q;
+                    *q = make_uint4(0u, 0u, 0u, 0u);
+                    t += (v.x + v.y) + (v.z + v.w);
+                }
+                tot[k] = t;
+                xl(a3, t);
+            }
+        } else {
+            const int G = cp.G;
+            const int per_sh = TNX_LDS_BANK_SHIFT - cp.gsh;
+            for (int k = lane; k < cells; k += kWave) {
+                const int f = k >> per_sh;
+                uint32_t *row = hist + (k << cp.gsh);
+                uint32_t t = 0;
+                for (int j = 0; j < G; j++) t += lds_take(row + ((j + f) & (G - 1)));
+                tot[k] = t;
+                xl(a3, t);
+            }
+        }
+        wave_lds_fence();
+        col_base = tot;
+    }
+    const bool s3_here = cp.G == 1;  // without copies the column pass also gives S3
+    for (int col = lane; col < ncol; col += kWave) {
+        uint32_t c2 = 0;
+        int aa = 0;
+        for (; aa + 4 <= bx; aa += 4) {
+            uint32_t v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) v[u] = lds_take(col_base + (aa + u) * ncol + col);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                c2 += v[u];
+                if (s3_here) xl(a3, v[u]);
+            }
+        }
+        for (; aa < bx; aa++) {
+            const uint32_t v = lds_take(col_base + aa * ncol + col);
+            c2 += v;
+            if (s3_here) xl(a3, v);
+        }
+        xl(a2, c2);
+    }
+    df_add(a3, -a2.hi, -a2.lo);
+    return df_wave_sum(a3);
+}
+
 // Test hook: this pair's folded joint counts (the G copies summed) to
 // counts_dbg, read-only on the slice (one wavefront).
 __device__ __forceinline__ void dump_wave_counts(const tnx_te_args_t &a, int x, int y,
@@ -261,6 +354,10 @@ __device__ __forceinline__ void persist_wave_item(const tnx_te_args_t &a, const 
 #if TNX_PERSIST_WAVE_XL == 1
             const DF d = hist_reduce_wave_copies(
                 hist, bx, by, cp, XlTable{reinterpret_cast<const float2 *>(a.xlogx)}, lane);
+#elif TNX_PERSIST_WAVE_XL == 2
+            const DF d = reduce_wave_ilp(
+                hist, bx, by, cp,
+                XlSplit{ltbl, reinterpret_cast<const float2 *>(a.xlogx)}, lane);
 #else
             const DF d = hist_reduce_wave_copies(hist, bx, by, cp, XlComputed{ltbl}, lane);
 #endif

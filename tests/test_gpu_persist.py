@@ -37,10 +37,10 @@ def _wave_mask(prep, plan):
 
 @pytest.mark.parametrize("cap_wave", [0, 1536])
 def test_persistent_matches_two_launch_and_oracle(dev, cap_wave):
-    """Wave pairs: the same bits as the two-launch wave kernel (same
-    accumulation and summation order); block pairs (256 instead of 512
-    threads per pair, so another summation order) within 1e-6 of it; all
-    pairs within 1e-6 of the FP64 oracle."""
+    """The same counts as the two-launch kernels, hence TE that differs only
+    by the order of the compensated sums: wave pairs within 1e-9, block pairs
+    (256 instead of 512 threads per pair) within 1e-6; all pairs within
+    1e-6 of the FP64 oracle."""
     b = synthetic.mixed_cardinality_bins(150, 3000, seed=31)
     prep = _prep(b, dev)
     plan = core.get_plan(prep, cap_wave=cap_wave)
@@ -49,7 +49,7 @@ def test_persistent_matches_two_launch_and_oracle(dev, cap_wave):
     m_2 = _matrix(prep, plan, False)
     w = _wave_mask(prep, plan)
     np.fill_diagonal(w, False)
-    assert np.array_equal(m_p[w], m_2[w])
+    assert np.abs(m_p[w].astype(np.float64) - m_2[w]).max() < 1e-9
     off = ~np.eye(150, dtype=bool)
     assert np.abs(m_p[off].astype(np.float64) - m_2[off]).max() < TOL
     pairs = O.pairfree_pairs(150)
