@@ -105,6 +105,13 @@ TNX_API int tnx_coarsen_bins(int8_t *bins, int64_t n_vars, int64_t T, int64_t ld
  * hi + lo == the FP64 value to ~2^-48 relative.  tbl: float (2*(N+1)). */
 TNX_API int tnx_xlogx_table(float *tbl, int64_t N, void *stream);
 
+/* c*log2(c) for c in [0, N] as 64-bit fixed point: tbl[c] = round(c log2 c
+ * * 2^shift) (the FP64 value tnx_xlogx_table splits).  Sums of entries are
+ * exact integers, independent of their order; shift must keep N log2 N *
+ * 2^shift below 2^62 (the Python layer uses 55 - ceil(log2(N + 1)), 41 fraction
+ * bits at N = 9,999).  The persistent TE launch reduces with it. */
+TNX_API int tnx_xlogx_table_fx(int64_t *tbl, int64_t N, int32_t shift, void *stream);
+
 /* Per variable v with b = n_per_var[v], L = T - tau:
  *   states[v, t] = x[t+tau] * b + x[t]          (uint16, row stride lds)
  *   S1[v]  = sum_b  c1 log2 c1   over x_t,             t < L   (FP64)
@@ -219,6 +226,11 @@ typedef struct {
     const int32_t *counts_map;
     int32_t *counts_dbg;
     int64_t counts_ld;
+    /* The persistent launch's c*log2(c) table (tnx_xlogx_table_fx, N = T -
+     * tau) and its fraction bits: per pair S3 - S2b is summed exactly in
+     * 64-bit fixed point, then scaled and combined with S1 - S2a in FP64. */
+    const int64_t *xlogx_fx;
+    int32_t fx_shift;
 } tnx_te_args_t;
 
 #define TNX_QUEUES 8            /* work queues of the persistent launch (one per XCD) */

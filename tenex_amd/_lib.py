@@ -54,7 +54,8 @@ class TeArgs(C.Structure):
                 ("hot_zero", C.c_int32), ("stage_vars", C.c_int32), ("stage_stride", C.c_int64),
                 ("item_per_wave", C.c_int32),
                 ("queue_items", p_), ("queue_off", p_), ("queue_ctr", p_),
-                ("counts_map", p_), ("counts_dbg", p_), ("counts_ld", C.c_int64)]
+                ("counts_map", p_), ("counts_dbg", p_), ("counts_ld", C.c_int64),
+                ("xlogx_fx", p_), ("fx_shift", C.c_int32)]
 
 
 TNX_QUEUES = 8
@@ -92,6 +93,7 @@ SIGNATURES = {
     "tnx_remap_dense": (C.c_int, [p_, i64, i64, p_, i64, p_, p_, p_]),
     "tnx_coarsen_bins": (C.c_int, [p_, i64, i64, i64, p_, i32, p_]),
     "tnx_xlogx_table": (C.c_int, [p_, i64, p_]),
+    "tnx_xlogx_table_fx": (C.c_int, [p_, i64, i32, p_]),
     "tnx_gene_prep": (C.c_int, [p_, i64, i64, i64, i32, p_, p_, i64, p_, p_, p_, i32, p_]),
     "tnx_te_plan": (C.c_int, [p_, i64, i64, p_, i64, p_, i64, i32, i32, p_, p_, p_, i64,
                               C.POINTER(PlanInfo)]),

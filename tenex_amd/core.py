@@ -134,6 +134,26 @@ def xlogx_table(N: int, device) -> torch.Tensor:
     return t
 
 
+_xlogx_fx_cache: dict = {}
+
+
+def fx_shift(N: int) -> int:
+    """Fraction bits of the fixed-point c*log2(c) table for series of N
+    samples: sums of up to N log2 N stay below 2^60."""
+    return int(min(48, max(16, 55 - int(np.ceil(np.log2(N + 1))))))
+
+
+def xlogx_fx_table(N: int, device) -> torch.Tensor:
+    """int64 (N + 1,) fixed-point c*log2(c) with fx_shift(N) fraction bits."""
+    key = (int(N), str(device))
+    t = _xlogx_fx_cache.get(key)
+    if t is None:
+        t = torch.empty(N + 1, dtype=torch.int64, device=device)
+        call("tnx_xlogx_table_fx", ptr(t), int(N), fx_shift(N), _stream(device))
+        _xlogx_fx_cache[key] = t
+    return t
+
+
 @dataclass
 class Prepared:
     """Device-resident discretised data ready for the TE kernels."""
@@ -505,7 +525,8 @@ def te_args(prep: Prepared, plan: Plan, out: torch.Tensor | None, out_mode: int,
         hot_zero=int(prep.hot_zero), stage_vars=int(stage_vars), stage_stride=int(stage_stride),
         item_per_wave=int(bool(item_per_wave)), queue_items=ptr(q[0]), queue_off=ptr(q[1]),
         queue_ctr=ptr(q[2]), counts_map=ptr(cm), counts_dbg=ptr(cd),
-        counts_ld=int(cd.shape[1]) if cd is not None else 0)
+        counts_ld=int(cd.shape[1]) if cd is not None else 0,
+        xlogx_fx=ptr(xlogx_fx_table(prep.L, prep.device)), fx_shift=fx_shift(prep.L))
 
 
 WAVE_ITEM_PAIRS = 32   # pairs of a one-target item that a single wave takes whole

@@ -129,9 +129,7 @@ __device__ __forceinline__ void xlogx_add_l(DF &acc, const float2 *ltbl,
 
 // c*log2(c) lookups as objects, so one reduction body serves every table
 // placement: XlTable reads the global hi/lo table, XlSmall takes counts below
-// kSmallTbl from an LDS copy, XlComputed takes them from the LDS copy and
-// computes larger ones in FP64 exactly as tnx_xlogx_table does (the same
-// hi/lo pair, without a memory access).
+// kSmallTbl from an LDS copy.
 struct XlTable {
     const float2 *__restrict__ tbl;
     __device__ __forceinline__ void operator()(DF &acc, uint32_t c) const { xlogx_add(acc, tbl, c); }
@@ -142,31 +140,6 @@ struct XlSmall {
     const float2 *__restrict__ tbl;
     __device__ __forceinline__ void operator()(DF &acc, uint32_t c) const {
         xlogx_add_l(acc, ltbl, tbl, c);
-    }
-};
-
-__device__ __forceinline__ float2 xlogx_hilo(uint32_t c) {
-    const double v = (double)c * log2((double)c);
-    const float hi = (float)v;
-    return make_float2(hi, (float)(v - (double)hi));
-}
-
-// Out of line: large counts are rare inside a pair, and the inlined FP64
-// log2 would hold its registers across the callers' unrolled loops.
-__device__ __noinline__ float2 xlogx_hilo_call(uint32_t c);
-
-struct XlComputed {
-    const float2 *ltbl;
-    __device__ __forceinline__ void operator()(DF &acc, uint32_t c) const {
-        if (c < (uint32_t)kSmallTbl) {
-            if (c > 1) {
-                const float2 t = ltbl[c];
-                df_add(acc, t.x, t.y);
-            }
-        } else {
-            const float2 t = xlogx_hilo_call(c);
-            df_add(acc, t.x, t.y);
-        }
     }
 };
 

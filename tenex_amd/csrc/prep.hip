@@ -247,6 +247,15 @@ __global__ void xlogx_kernel(float *__restrict__ tbl, int64_t N) {
     tbl[2 * c + 1] = lo;
 }
 
+// c*log2(c) as 64-bit fixed point with `shift` fraction bits (rounded to
+// nearest from the same FP64 value the hi/lo table splits).
+__global__ void xlogx_fx_kernel(int64_t *__restrict__ tbl, int64_t N, int shift) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > N) return;
+    const double v = c > 1 ? (double)c * log2((double)c) : 0.0;
+    tbl[c] = __double2ll_rn(ldexp(v, shift));
+}
+
 // ── per-variable preparation ───────────────────────────────────────────────
 __device__ __forceinline__ double xlog2x_d(double c) { return c > 1.0 ? c * log2(c) : 0.0; }
 
@@ -453,6 +462,20 @@ int tnx_xlogx_table(float *tbl, int64_t N, void *stream) {
     hipLaunchKernelGGL(xlogx_kernel, dim3((unsigned)((N + 256) / 256)), dim3(256), 0,
                        as_stream(stream), tbl, N);
     TNX_CHECK_LAUNCH("xlogx_kernel");
+    return TNX_OK;
+}
+
+int tnx_xlogx_table_fx(int64_t *tbl, int64_t N, int32_t shift, void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(tbl && N >= 0 && N < (1LL << 31), "tnx_xlogx_table_fx: bad arguments");
+    // the largest sum of entries is at most N log2 N: it must stay below 2^62
+    const double top = N > 1 ? (double)N * log2((double)N) : 1.0;
+    TNX_CHECK_ARG(shift >= 0 && ldexp(top, shift) < ldexp(1.0, 62),
+                  "tnx_xlogx_table_fx: shift %d overflows 64 bits for N = %lld", shift,
+                  (long long)N);
+    hipLaunchKernelGGL(xlogx_fx_kernel, dim3((unsigned)((N + 256) / 256)), dim3(256), 0,
+                       as_stream(stream), tbl, N, (int)shift);
+    TNX_CHECK_LAUNCH("xlogx_fx_kernel");
     return TNX_OK;
 }
 

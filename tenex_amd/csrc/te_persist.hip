@@ -43,12 +43,6 @@
 
 namespace tnx {
 
-__device__ __noinline__ float2 xlogx_hilo_call(uint32_t c) { return xlogx_hilo(c); }
-
-#ifndef TNX_PERSIST_WAVE_XL
-#define TNX_PERSIST_WAVE_XL 2  // wave-role c log2 c: 0 LDS table + FP64, 1 global table, 2 split + batched reduce
-#endif
-
 constexpr int kQueueStride = 16;  // take counters 64 bytes apart
 static_assert(2 * TNX_QUEUES * kQueueStride <= TNX_QUEUE_CTR_INTS, "queue counters");
 constexpr int kPersistThreads = kWavesPerWG * kWave;
@@ -178,45 +172,64 @@ __device__ __forceinline__ void pipe_count(uint4 (&b0)[SrcPipe<NI>::GS],
     }
 }
 
-// c*log2(c): counts below kSmallTbl from the workgroup's LDS copy of the
-// table, larger ones from the table in memory.  The two pointers live in
-// distinct address spaces, so the lookup is a ds_read or a global load,
-// never a generic (flat) one.
-struct XlSplit {
-    const float2 *ltbl;
-    const float2 *__restrict__ tbl;
-    __device__ __forceinline__ void operator()(DF &acc, uint32_t c) const {
-        if (c > 1) {
-            float2 t;
-            if (c < (uint32_t)kSmallTbl) {
-                t = ltbl[c];
-            } else {
-                t = tbl[c];
-                // keeps the two loads apart (merged, they would become one
-                // select of generic pointers and a flat load)
-                asm volatile("" : "+v"(t.x), "+v"(t.y));
-            }
-            df_add(acc, t.x, t.y);
-        }
+// c*log2(c) as 64-bit fixed point (tnx_xlogx_table_fx): counts below
+// kSmallTbl from the workgroup's LDS copy, read by every lane without a
+// branch; the rare larger counts from the table in memory.  Sums of these
+// integers are exact, so a pair's S3 - S2b does not depend on the order
+// the cells are visited in (wave or block role, any launch).
+struct XfLookup {
+    const int64_t *lfx;          // LDS, kSmallTbl entries
+    __amdgpu_buffer_rsrc_t gfx;  // the N + 1 entries in memory
+    __device__ __forceinline__ int64_t operator()(uint32_t c) const {
+        const bool small = c < (uint32_t)kSmallTbl;
+        int64_t v = lfx[small ? c : 0u];
+        // a buffer load: distinct from the LDS read, so the two are never
+        // merged into one generic (flat) load
+        if (!small) v = (int64_t)__builtin_amdgcn_raw_buffer_load_b64(gfx, (int)(c * 8u), 0, 0);
+        return v;
     }
 };
 
-// hist_reduce_wave_copies with the LDS reads of a column issued four at a
-// time (their latencies overlap instead of adding up), the lookups of the
-// four counts after them.  Returns S3 - S2b (every lane); leaves the slice
+// Sum of an int64 over the 64 lanes (every lane gets it): rotate-and-add
+// inside each row of 16 lanes on the two 32-bit halves (DPP), then the four
+// row totals.
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+    uint64_t u = (uint64_t)v;
+#define TNX_ROW_ROR_ADD(N)                                                                      \
+    {                                                                                          \
+        const uint32_t lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x120 + N, 0xF,   \
+                                                        0xF, false);                            \
+        const uint32_t hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x120 + N, \
+                                                        0xF, 0xF, false);                       \
+        u += ((uint64_t)hi << 32) | lo;                                                        \
+    }
+    TNX_ROW_ROR_ADD(1) TNX_ROW_ROR_ADD(2) TNX_ROW_ROR_ADD(4) TNX_ROW_ROR_ADD(8)
+#undef TNX_ROW_ROR_ADD
+    auto lane_u64 = [&](int l) {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+        return ((uint64_t)hi << 32) | lo;
+    };
+    return (int64_t)(lane_u64(0) + lane_u64(16) + lane_u64(32) + lane_u64(48));
+}
+
+// S3 - S2b of the pair in the wave's slice, exact in fixed point (every
+// lane): with copies, lane l first folds cells k = l, l + 64, ... (the G
+// copies of cell k read in a rotated order that spreads a half-wave over
+// the banks) and writes their totals after the copies; then per (x_t, y_t)
+// column the b_x totals are read four at a time (their latencies overlap)
+// and give S2b (without copies the same pass gives S3).  Leaves the slice
 // zero.
-template <class XL>
-__device__ __forceinline__ DF reduce_wave_ilp(uint32_t *hist, int bx, int by, const WaveCopies &cp,
-                                              const XL &xl, int lane) {
+template <class XF>
+__device__ __forceinline__ int64_t reduce_wave_fx(uint32_t *hist, int bx, int by,
+                                                  const WaveCopies &cp, const XF &xf, int lane) {
     const int ncol = bx * by;
     const int cells = bx * ncol;
-    DF a3 = {0.f, 0.f}, a2 = {0.f, 0.f};
+    int64_t a3 = 0, a2 = 0;
     uint32_t *col_base = hist;  // where the column pass reads the cell totals
     if (cp.G > 1) {
         uint32_t *tot = hist + cp.G * cells;
         if (cp.G >= 4) {
-            // four copies per 16-byte read, rotated so that the 8 lanes of a
-            // read group land on 8 distinct 4-bank groups
             const int G4 = cp.G >> 2;
             const int per_sh = TNX_LDS_BANK_SHIFT - cp.gsh;
             for (int k = lane; k < cells; k += kWave) {
@@ -230,7 +243,7 @@ __device__ __forceinline__ DF reduce_wave_ilp(uint32_t *hist, int bx, int by, co
                     t += (v.x + v.y) + (v.z + v.w);
                 }
                 tot[k] = t;
-                xl(a3, t);
+                a3 += xf(t);
             }
         } else {
             const int G = cp.G;
@@ -241,7 +254,7 @@ __device__ __forceinline__ DF reduce_wave_ilp(uint32_t *hist, int bx, int by, co
                 uint32_t t = 0;
                 for (int j = 0; j < G; j++) t += lds_take(row + ((j + f) & (G - 1)));
                 tot[k] = t;
-                xl(a3, t);
+                a3 += xf(t);
             }
         }
         wave_lds_fence();
@@ -258,18 +271,17 @@ __device__ __forceinline__ DF reduce_wave_ilp(uint32_t *hist, int bx, int by, co
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 c2 += v[u];
-                if (s3_here) xl(a3, v[u]);
+                if (s3_here) a3 += xf(v[u]);
             }
         }
         for (; aa < bx; aa++) {
             const uint32_t v = lds_take(col_base + aa * ncol + col);
             c2 += v;
-            if (s3_here) xl(a3, v);
+            if (s3_here) a3 += xf(v);
         }
-        xl(a2, c2);
+        a2 += xf(c2);
     }
-    df_add(a3, -a2.hi, -a2.lo);
-    return df_wave_sum(a3);
+    return wave_sum_i64(a3 - a2);
 }
 
 // Test hook: this pair's folded joint counts (the G copies summed) to
@@ -293,7 +305,7 @@ __device__ __forceinline__ void dump_wave_counts(const tnx_te_args_t &a, int x, 
 template <int NI, bool HZ>
 __device__ __forceinline__ void persist_wave_item(const tnx_te_args_t &a, const tnx_work_t it,
                                                   int pfirst, int pstep, uint32_t *hist,
-                                                  uint32_t *zero_row, const float2 *ltbl, int L,
+                                                  uint32_t *zero_row, const XfLookup &xf, int L,
                                                   double invN, int lane) {
     constexpr int GS = SrcPipe<NI>::GS;
     const int cnt = it.tgt_cnt;
@@ -351,18 +363,9 @@ __device__ __forceinline__ void persist_wave_item(const tnx_te_args_t &a, const 
                 zero_row[lane] = 0u;  // the padded samples' counts
             }
             if (a.counts_map) dump_wave_counts(a, x, y, hist, cells, cp, lane);
-#if TNX_PERSIST_WAVE_XL == 1
-            const DF d = hist_reduce_wave_copies(
-                hist, bx, by, cp, XlTable{reinterpret_cast<const float2 *>(a.xlogx)}, lane);
-#elif TNX_PERSIST_WAVE_XL == 2
-            const DF d = reduce_wave_ilp(
-                hist, bx, by, cp,
-                XlSplit{ltbl, reinterpret_cast<const float2 *>(a.xlogx)}, lane);
-#else
-            const DF d = hist_reduce_wave_copies(hist, bx, by, cp, XlComputed{ltbl}, lane);
-#endif
+            const int64_t d = reduce_wave_fx(hist, bx, by, cp, xf, lane);
             wave_lds_fence();
-            if (lane == 0) emit_te(a, slot, (df_to_double(d) + s12) * invN);
+            if (lane == 0) emit_te(a, slot, (ldexp((double)d, -a.fx_shift) + s12) * invN);
         } else if (want && lane == 0) {
             emit_te(a, slot, __builtin_nan(""));  // plan mismatch: the pair is not counted
         }
@@ -377,17 +380,17 @@ __device__ __forceinline__ void persist_wave_item(const tnx_te_args_t &a, const 
 // entry and exit; trash holds garbage (the caller clears it before the
 // region serves wave slices again).
 template <int NB, bool HZ>
-__device__ __forceinline__ DF block_pair(uint32_t *hist, uint32_t *c2b, uint32_t *zero_row,
+__device__ __forceinline__ int64_t block_pair(uint32_t *hist, uint32_t *c2b, uint32_t *zero_row,
                                          uint32_t *trash, int cap, const TargetRegs<NB> &regs,
                                          const uint4 *ys, int bx, int by, int L,
-                                         const XlComputed &xl, float *red, int32_t *cdbg) {
+                                         const XfLookup &xf, int64_t *red, int32_t *cdbg) {
     const int tid = threadIdx.x, nth = kPersistThreads;
     uint32_t *zero = zero_row + (tid & 63);
     const int ncol = bx * by;
     const uint32_t by4 = (uint32_t)by << 2;
     int apass = cap / ncol;
     if (apass < 1) apass = 1;
-    DF a3 = {0.f, 0.f};
+    int64_t a3 = 0;
     for (int a0 = 0; a0 < bx; a0 += apass) {
         const int a1 = (a0 + apass < bx) ? (a0 + apass) : bx;
         const uint32_t lo = (uint32_t)(a0 * bx), span = (uint32_t)((a1 - a0) * bx);
@@ -456,7 +459,7 @@ __device__ __forceinline__ DF block_pair(uint32_t *hist, uint32_t *c2b, uint32_t
                 if (v[u]) {
                     __hip_atomic_fetch_add(c2b + col, v[u], __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
-                    xl(a3, v[u]);
+                    a3 += xf(v[u]);
                 }
                 col += step;
                 if (col >= ncol) col -= ncol;
@@ -466,17 +469,24 @@ __device__ __forceinline__ DF block_pair(uint32_t *hist, uint32_t *c2b, uint32_t
             const uint32_t v = lds_take(hist + k);
             if (v) {
                 __hip_atomic_fetch_add(c2b + col, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                xl(a3, v);
+                a3 += xf(v);
             }
             col += step;
             if (col >= ncol) col -= ncol;
         }
         __syncthreads();
     }
-    DF a2 = {0.f, 0.f};
-    for (int c = tid; c < ncol; c += nth) xl(a2, lds_take(c2b + c));
-    df_add(a3, -a2.hi, -a2.lo);
-    return df_block_sum(a3, red);
+    int64_t a2 = 0;
+    for (int c = tid; c < ncol; c += nth) a2 += xf(lds_take(c2b + c));
+    // workgroup sum: waves, then thread 0 over the waves' partials (read
+    // before the caller's next barrier, so none is needed after)
+    const int64_t w = wave_sum_i64(a3 - a2);
+    if ((tid & 63) == 0) red[tid >> 6] = w;
+    __syncthreads();
+    int64_t tot = 0;
+    if (tid == 0)
+        for (int i = 0; i < kWavesPerWG; i++) tot += red[i];
+    return tot;
 }
 
 // Block-role source row: thread tid holds samples 16 tid + 4096 i, i < NB.
@@ -495,7 +505,7 @@ __device__ __forceinline__ void load_row256(uint4 *ys, const int8_t *__restrict_
 template <int NI, bool HZ>
 __device__ __forceinline__ void persist_block_item(const tnx_te_args_t &a, const tnx_work_t it,
                                                    uint32_t *region, int capB,
-                                                   const float2 *ltbl, float *red, int L,
+                                                   const XfLookup &xf, int64_t *red, int L,
                                                    double invN) {
     constexpr int NB = (NI + 3) / 4;  // 16-sample chunks per thread (4096 samples per chunk row)
     const int tid = threadIdx.x;
@@ -542,9 +552,9 @@ __device__ __forceinline__ void persist_block_item(const tnx_te_args_t &a, const
                     if (r >= 0 && (int64_t)bx * bx * by <= a.counts_ld)
                         cdbg = a.counts_dbg + (int64_t)r * a.counts_ld;
                 }
-                const DF d = block_pair<NB, HZ>(hist, c2b, zero_row, trash, capB, regs, ycur, bx,
-                                                by, L, XlComputed{ltbl}, red, cdbg);
-                if (tid == 0) emit_te(a, slot, (df_to_double(d) + s12) * invN);
+                const int64_t d = block_pair<NB, HZ>(hist, c2b, zero_row, trash, capB, regs, ycur,
+                                                     bx, by, L, xf, red, cdbg);
+                if (tid == 0) emit_te(a, slot, (ldexp((double)d, -a.fx_shift) + s12) * invN);
             } else if (want && !fits && tid == 0) {
                 emit_te(a, slot, __builtin_nan(""));  // plan mismatch
             }
@@ -579,18 +589,17 @@ __global__ __launch_bounds__(kPersistThreads) __attribute__((amdgpu_waves_per_eu
 te_persist_kernel(tnx_te_args_t a) {
     // 16-byte aligned: the copy fold reads four copies per ds_read_b128
     extern __shared__ __attribute__((aligned(16))) uint32_t lds_u32[];
-    __shared__ float red[2 * kWavesPerWG];
+    __shared__ int64_t red[kWavesPerWG];
     __shared__ int s_entry;
-    __shared__ float2 ltbl[kSmallTbl];  // c log2 c of counts < kSmallTbl
+    __shared__ int64_t lfx[kSmallTbl];  // fixed-point c log2 c of counts < kSmallTbl
     uint32_t *region = lds_u32;         // four wave slices
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
     const int L = (int)(a.T - a.tau);
     const double invN = 1.0 / (double)L;
-    const float2 *tbl = reinterpret_cast<const float2 *>(a.xlogx);
-    for (int i = tid; i < kSmallTbl; i += kPersistThreads)
-        ltbl[i] = i <= L ? tbl[i] : make_float2(0.f, 0.f);
+    for (int i = tid; i < kSmallTbl; i += kPersistThreads) lfx[i] = i <= L ? a.xlogx_fx[i] : 0;
+    const XfLookup xf{lfx, row_rsrc(a.xlogx_fx, (uint32_t)(8 * (L + 1)))};
     const int slice = wave_slice(a.cap_wave);
     for (int i = tid; i < kWavesPerWG * slice; i += kPersistThreads) region[i] = 0;
     __syncthreads();
@@ -606,7 +615,7 @@ te_persist_kernel(tnx_te_args_t a) {
             const int e = __builtin_amdgcn_readfirstlane(s_entry);  // uniform: scalar loads
             __syncthreads();  // s_entry is rewritten by the next take
             if (e < 0) break;
-            persist_block_item<NI, HZ>(a, a.work[a.n_wave_items + e], region, capB, ltbl, red, L,
+            persist_block_item<NI, HZ>(a, a.work[a.n_wave_items + e], region, capB, xf, red, L,
                                        invN);
         }
     };
@@ -630,7 +639,7 @@ te_persist_kernel(tnx_te_args_t a) {
             if (e < 0) break;
             const int64_t wi = per_wave ? e : e / kWavesPerWG;
             persist_wave_item<NI, HZ>(a, a.work[wi], per_wave ? 0 : e % kWavesPerWG,
-                                      per_wave ? 1 : kWavesPerWG, hist, zero_row, ltbl, L, invN,
+                                      per_wave ? 1 : kWavesPerWG, hist, zero_row, xf, L, invN,
                                       lane);
         }
     }
@@ -677,8 +686,9 @@ static const void *persist_fn(int ni) {
 }
 
 int launch_persist(const tnx_te_args_t &a, int ni, hipStream_t s) {
-    TNX_CHECK_ARG(a.queue_off && a.queue_ctr && a.work,
-                  "tnx_te_run: persistent launch needs queue_off, queue_ctr and work");
+    TNX_CHECK_ARG(a.queue_off && a.queue_ctr && a.work && a.xlogx_fx,
+                  "tnx_te_run: persistent launch needs queue_off, queue_ctr, work and xlogx_fx");
+    TNX_CHECK_ARG(a.fx_shift >= 0 && a.fx_shift <= 62, "tnx_te_run: bad fx_shift %d", a.fx_shift);
     TNX_CHECK_ARG(ni >= 1 && ni <= 12 && a.T - a.tau <= kPersistMaxL,
                   "tnx_te_run: the persistent launch takes T - tau <= %lld",
                   (long long)kPersistMaxL);
