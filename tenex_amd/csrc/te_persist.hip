@@ -180,15 +180,39 @@ __device__ __forceinline__ void pipe_count(uint4 (&b0)[SrcPipe<NI>::GS],
 struct XfLookup {
     const int64_t *lfx;          // LDS, kSmallTbl entries
     __amdgpu_buffer_rsrc_t gfx;  // the N + 1 entries in memory
-    __device__ __forceinline__ int64_t operator()(uint32_t c) const {
-        const bool small = c < (uint32_t)kSmallTbl;
-        int64_t v = lfx[small ? c : 0u];
-        // a buffer load: distinct from the LDS read, so the two are never
-        // merged into one generic (flat) load
-        if (!small) v = (int64_t)__builtin_amdgcn_raw_buffer_load_b64(gfx, (int)(c * 8u), 0, 0);
-        return v;
-    }
 };
+
+// Sum of the fixed-point c log2 c of N counts: the LDS reads for all of
+// them; then, only if a lane of the wave holds a count >= kSmallTbl, one
+// buffer load per count (a small count's offset lies past the table and
+// reads zero) and a single wait.  Branch-free per lane; a buffer load
+// cannot be merged with the LDS read into a generic (flat) load.
+template <int N>
+__device__ __forceinline__ int64_t xf_sum(const XfLookup &xf, const uint32_t (&c)[N]) {
+    int64_t s = 0;
+    bool big = false;
+#pragma unroll
+    for (int u = 0; u < N; u++) {
+        const bool small = c[u] < (uint32_t)kSmallTbl;
+        s += xf.lfx[small ? c[u] : 0u];
+        big |= !small;
+    }
+    if (__builtin_amdgcn_ballot_w64(big)) {  // wave-uniform
+        int64_t w[N];
+#pragma unroll
+        for (int u = 0; u < N; u++)
+            w[u] = (int64_t)__builtin_amdgcn_raw_buffer_load_b64(
+                xf.gfx, (int)(c[u] < (uint32_t)kSmallTbl ? 0x80000000u : c[u] * 8u), 0, 0);
+#pragma unroll
+        for (int u = 0; u < N; u++) s += w[u];
+    }
+    return s;
+}
+
+__device__ __forceinline__ int64_t xf_one(const XfLookup &xf, uint32_t c) {
+    const uint32_t cc[1] = {c};
+    return xf_sum<1>(xf, cc);
+}
 
 // Sum of an int64 over the 64 lanes (every lane gets it): rotate-and-add
 // inside each row of 16 lanes on the two 32-bit halves (DPP), then the four
@@ -220,66 +244,72 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
 // column the b_x totals are read four at a time (their latencies overlap)
 // and give S2b (without copies the same pass gives S3).  Leaves the slice
 // zero.
-template <class XF>
 __device__ __forceinline__ int64_t reduce_wave_fx(uint32_t *hist, int bx, int by,
-                                                  const WaveCopies &cp, const XF &xf, int lane) {
+                                                  const WaveCopies &cp, const XfLookup &xf,
+                                                  int lane) {
     const int ncol = bx * by;
     const int cells = bx * ncol;
     int64_t a3 = 0, a2 = 0;
     uint32_t *col_base = hist;  // where the column pass reads the cell totals
     if (cp.G > 1) {
         uint32_t *tot = hist + cp.G * cells;
-        if (cp.G >= 4) {
-            const int G4 = cp.G >> 2;
-            const int per_sh = TNX_LDS_BANK_SHIFT - cp.gsh;
-            for (int k = lane; k < cells; k += kWave) {
-                const int f = (k >> per_sh) & (G4 - 1);
-                uint4 *row = reinterpret_cast<uint4 *>(hist + (k << cp.gsh));
-                uint32_t t = 0;
-                for (int j = 0; j < G4; j++) {
-                    uint4 *q = row + ((j + f) & (G4 - 1));
-                    const uint4 v = *q;
-                    *q = make_uint4(0u, 0u, 0u, 0u);
-                    t += (v.x + v.y) + (v.z + v.w);
+        const int per_sh = TNX_LDS_BANK_SHIFT - cp.gsh;
+        // four cells per step, so that their folds and lookups overlap
+        for (int k0 = lane; k0 < cells; k0 += 4 * kWave) {
+            uint32_t t[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int k = k0 + u * kWave;
+                t[u] = 0;
+                if (k < cells) {
+                    if (cp.G >= 4) {
+                        const int G4 = cp.G >> 2;
+                        const int f = (k >> per_sh) & (G4 - 1);
+                        uint4 *row = reinterpret_cast<uint4 *>(hist + (k << cp.gsh));
+                        for (int j = 0; j < G4; j++) {
+                            uint4 *q = row + ((j + f) & (G4 - 1));
+                            const uint4 v = *q;
+                            *q = make_uint4(0u, 0u, 0u, 0u);
+                            t[u] += (v.x + v.y) + (v.z + v.w);
+                        }
+                    } else {
+                        const int f = k >> per_sh;
+                        uint32_t *row = hist + (k << cp.gsh);
+                        for (int j = 0; j < cp.G; j++)
+                            t[u] += lds_take(row + ((j + f) & (cp.G - 1)));
+                    }
+                    tot[k] = t[u];
                 }
-                tot[k] = t;
-                a3 += xf(t);
             }
-        } else {
-            const int G = cp.G;
-            const int per_sh = TNX_LDS_BANK_SHIFT - cp.gsh;
-            for (int k = lane; k < cells; k += kWave) {
-                const int f = k >> per_sh;
-                uint32_t *row = hist + (k << cp.gsh);
-                uint32_t t = 0;
-                for (int j = 0; j < G; j++) t += lds_take(row + ((j + f) & (G - 1)));
-                tot[k] = t;
-                a3 += xf(t);
-            }
+            a3 += xf_sum<4>(xf, t);
         }
         wave_lds_fence();
         col_base = tot;
     }
     const bool s3_here = cp.G == 1;  // without copies the column pass also gives S3
-    for (int col = lane; col < ncol; col += kWave) {
-        uint32_t c2 = 0;
-        int aa = 0;
-        for (; aa + 4 <= bx; aa += 4) {
-            uint32_t v[4];
+    // four columns per step; each column's b_x cells read four at a time
+    for (int c0 = lane; c0 < ncol; c0 += 4 * kWave) {
+        uint32_t c2[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) v[u] = lds_take(col_base + (aa + u) * ncol + col);
+        for (int u = 0; u < 4; u++) {
+            const int col = c0 + u * kWave;
+            c2[u] = 0;
+            if (col >= ncol) continue;
+            int aa = 0;
+            for (; aa + 4 <= bx; aa += 4) {
+                uint32_t v[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                c2 += v[u];
-                if (s3_here) a3 += xf(v[u]);
+                for (int w = 0; w < 4; w++) v[w] = lds_take(col_base + (aa + w) * ncol + col);
+                c2[u] += (v[0] + v[1]) + (v[2] + v[3]);
+                if (s3_here) a3 += xf_sum<4>(xf, v);
+            }
+            for (; aa < bx; aa++) {
+                const uint32_t v = lds_take(col_base + aa * ncol + col);
+                c2[u] += v;
+                if (s3_here) a3 += xf_one(xf, v);
             }
         }
-        for (; aa < bx; aa++) {
-            const uint32_t v = lds_take(col_base + aa * ncol + col);
-            c2 += v;
-            if (s3_here) a3 += xf(v);
-        }
-        a2 += xf(c2);
+        a2 += xf_sum<4>(xf, c2);
     }
     return wave_sum_i64(a3 - a2);
 }
@@ -454,13 +484,12 @@ __device__ __forceinline__ int64_t block_pair(uint32_t *hist, uint32_t *c2b, uin
             uint32_t v[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) v[u] = lds_take(hist + k + u * nth);
+            a3 += xf_sum<4>(xf, v);
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                if (v[u]) {
+                if (v[u])
                     __hip_atomic_fetch_add(c2b + col, v[u], __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
-                    a3 += xf(v[u]);
-                }
                 col += step;
                 if (col >= ncol) col -= ncol;
             }
@@ -469,7 +498,7 @@ __device__ __forceinline__ int64_t block_pair(uint32_t *hist, uint32_t *c2b, uin
             const uint32_t v = lds_take(hist + k);
             if (v) {
                 __hip_atomic_fetch_add(c2b + col, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                a3 += xf(v);
+                a3 += xf_one(xf, v);
             }
             col += step;
             if (col >= ncol) col -= ncol;
@@ -477,7 +506,12 @@ __device__ __forceinline__ int64_t block_pair(uint32_t *hist, uint32_t *c2b, uin
         __syncthreads();
     }
     int64_t a2 = 0;
-    for (int c = tid; c < ncol; c += nth) a2 += xf(lds_take(c2b + c));
+    for (int c0 = tid; c0 < ncol; c0 += 4 * nth) {
+        uint32_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = c0 + u * nth < ncol ? lds_take(c2b + c0 + u * nth) : 0u;
+        a2 += xf_sum<4>(xf, v);
+    }
     // workgroup sum: waves, then thread 0 over the waves' partials (read
     // before the caller's next barrier, so none is needed after)
     const int64_t w = wave_sum_i64(a3 - a2);
