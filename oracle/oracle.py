@@ -330,6 +330,25 @@ def bh_fdr(p: np.ndarray) -> np.ndarray:
     return out
 
 
+def fdr_threshold(te: np.ndarray, pairs: np.ndarray, fdr: float):
+    """fdr_threshold_gpu (inference/__init__.py:129-138) restated in numpy:
+    z = (te - mean) / std with the unbiased (n - 1) standard deviation,
+    p = erfc(z / sqrt 2) / 2, BH-adjusted (bh_fdr above), kept where the
+    adjusted p < fdr; a constant vector keeps nothing.  Mean and std are
+    taken in float64 and rounded to float32 (torch accumulates float32
+    reductions in wider precision), p in float64 rounded to float32."""
+    from scipy.special import erfc
+    te = np.asarray(te, np.float32)
+    std = np.float32(te.astype(np.float64).std(ddof=1))
+    if std == 0:
+        return te[:0], pairs[:0]
+    mean = np.float32(te.astype(np.float64).mean())
+    z = ((te - mean) / std).astype(np.float32)
+    p = (0.5 * erfc(z.astype(np.float64) / np.sqrt(2.0))).astype(np.float32)
+    keep = bh_fdr(p) < np.float32(fdr)
+    return te[keep], pairs[keep]
+
+
 # ── DPI trimming ───────────────────────────────────────────────────────────
 
 def trim_dpi(A: np.ndarray, batch_size: int = 512, trim_threshold: float = 0.0):

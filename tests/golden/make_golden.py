@@ -15,6 +15,20 @@ surrogate_40x400.npz   NetWeaver(result).infer("surrogate_test") on the CPU loop
                        block-shuffle surrogates, block length 25, seed 42,
                        FDR 0.05: observed, mean, std, effective TE, p-values,
                        edge list.
+surrogate_120x1000.npz the same at 120 genes x 1,000 cells with 8 planted
+                       lagged dependencies, 50 surrogates, block length 40,
+                       seed 42, FDR 0.05 (more edges pinned than the 40-gene
+                       fixture).
+fdr_300x600.npz        the FDR method's arithmetic on the 300-gene x 600-cell
+                       TE matrix of the reference's CPU engine (seed 5, 24
+                       planted lagged dependencies of graded strength and 8
+                       chains a -> b -> c with a weaker direct a -> c): the
+                       reference's
+                       fdr_threshold_gpu (z score, erfc p-values, BH at 0.05,
+                       cut at FDR 0.01) evaluated on CPU tensors in the
+                       public pair order of build_pairs, then its DPI _trim
+                       on the CPU device: thresholded and trimmed edge lists
+                       with their weights.
 """
 import os
 import sys
@@ -32,10 +46,22 @@ def zero_inflated_lognormal(n, T, seed):
     return x
 
 
-def main():
+def main(which):
     sys.path.insert(0, REF)
     import tenex as tnx  # the reference package
 
+    if "te_c0" in which:
+        write_te_c0(tnx)
+    if "surrogate_40x400" in which:
+        write_surrogate_40(tnx)
+    if "surrogate_120x1000" in which:
+        write_surrogate_120(tnx)
+    if "fdr_300x600" in which:
+        write_fdr_300(tnx)
+    print("golden fixtures written:", ", ".join(which))
+
+
+def write_te_c0(tnx):
     x = zero_inflated_lognormal(100, 500, 0)
     names = np.array([f"g{i}" for i in range(100)])
     res = tnx.TransferEntropyEngine(x, names).compute(accelerator="cpu")
@@ -45,6 +71,8 @@ def main():
                         matrix=np.asarray(res.matrix, np.float32), b_max=res.b_max,
                         kernel=str(res.kernel))
 
+
+def write_surrogate_40(tnx):
     x2 = zero_inflated_lognormal(40, 400, 8)
     x2[7, 1:] = x2[3, :-1] + 0.1 * np.random.default_rng(1).random(399).astype(np.float32)
     names2 = np.array([f"g{i}" for i in range(40)])
@@ -58,8 +86,62 @@ def main():
                         std=st.std_surrogate_te, effective=st.effective_te, p=st.p_values,
                         edges=np.asarray(st.grn.pairs, np.int64), n_surrogates=20,
                         block_length=25, seed=42, fdr=0.05)
-    print("golden fixtures written")
 
+
+def write_surrogate_120(tnx):
+    x3 = zero_inflated_lognormal(120, 1000, 13)
+    rng = np.random.default_rng(3)
+    for s_, t_ in rng.choice(120, (8, 2), replace=False):
+        x3[t_, 1:] = x3[s_, :-1] * (0.5 + rng.random(999).astype(np.float32))
+    names3 = np.array([f"g{i}" for i in range(120)])
+    r3 = tnx.TransferEntropyEngine(x3, names3).compute(accelerator="cpu")
+    st3 = tnx.NetWeaver(r3, fdr=0.05).infer(method="surrogate_test", device="cpu",
+                                             n_surrogates=50, block_length=40, seed=42)
+    np.savez_compressed(os.path.join(HERE, "surrogate_120x1000.npz"), data=x3,
+                        bins=np.asarray(r3.bin_arrs)[:, :, 0].astype(np.int8),
+                        n_per_var=np.asarray(r3.n_per_var, np.int32),
+                        observed=st3.observed_te, mean=st3.mean_surrogate_te,
+                        std=st3.std_surrogate_te, effective=st3.effective_te, p=st3.p_values,
+                        edges=np.asarray(st3.grn.pairs, np.int64), n_surrogates=50,
+                        block_length=40, seed=42, fdr=0.05)
+
+
+def write_fdr_300(tnx):
+    import torch
+    from tenex.inference import build_pairs, fdr_threshold_gpu
+    from tenex.inference.fdr import _trim
+    x4 = zero_inflated_lognormal(300, 600, 5)
+    rng = np.random.default_rng(5)
+    g = rng.choice(300, 24 * 2 + 8 * 3, replace=False)
+    # 24 single lagged dependencies of graded strength (some near the cut)
+    for i, w in enumerate(np.linspace(0.15, 1.0, 24)):
+        s_, t_ = g[2 * i], g[2 * i + 1]
+        x4[t_, 1:] = (1 - w) * x4[t_, 1:] + w * x4[s_, :-1] * (0.5 + rng.random(599))
+    # 8 chains a -> b -> c with a weaker direct a -> c: DPI removes a -> c
+    for i in range(8):
+        a, b, c = g[48 + 3 * i: 51 + 3 * i]
+        x4[b, 1:] = x4[a, :-1] * (0.5 + rng.random(599))
+        x4[c, 1:] = x4[b, :-1] * (0.5 + rng.random(599)) + 0.4 * x4[a, :-1]
+    x4 = x4.astype(np.float32)
+    names4 = np.array([f"g{i}" for i in range(300)])
+    r4 = tnx.TransferEntropyEngine(x4, names4).compute(accelerator="cpu")
+    M = np.asarray(r4.matrix, np.float32)
+    pairs = build_pairs(names4)
+    pairs_t = torch.from_numpy(pairs).to(torch.long)
+    te_t = torch.from_numpy(M)[pairs_t[:, 0], pairs_t[:, 1]]
+    te_cut, pairs_cut = fdr_threshold_gpu(te_t, pairs_t, 0.01)
+    te_cut, pairs_cut = te_cut.numpy(), pairs_cut.numpy()
+    trimmed = _trim(M, names4, pairs_cut, te_cut, device=torch.device("cpu"), batch_size=512,
+                    trim_threshold=0.0)
+    np.savez_compressed(os.path.join(HERE, "fdr_300x600.npz"), matrix=M,
+                        pairs_cut=np.asarray(pairs_cut, np.int64),
+                        te_cut=np.asarray(te_cut, np.float32),
+                        trimmed_pairs=np.asarray(trimmed.pairs, np.int64),
+                        trimmed_te=np.asarray(trimmed.te, np.float32), fdr=0.01)
+
+
+ALL = ("te_c0", "surrogate_40x400", "surrogate_120x1000", "fdr_300x600")
 
 if __name__ == "__main__":
-    main()
+    # python make_golden.py [fixture ...]   (default: all of ALL)
+    main(sys.argv[1:] or ALL)
