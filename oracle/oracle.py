@@ -129,6 +129,29 @@ def discretize(arr: np.ndarray, kp: float = 0.5):
     return dense, nb, npv, bmax
 
 
+def coarsen_bins(bins: np.ndarray, npv: np.ndarray, limit: int):
+    """coarsen_bins (tenex/preprocess.py:462-551, CPU branch :531-534):
+    variables with more than `limit` levels map level v to v * limit // ng
+    and report exactly `limit` levels; the rest are unchanged.  Returns
+    (bins int32, npv int32, b_max)."""
+    bins = np.array(bins, dtype=np.int64, copy=True)
+    npv = np.array(npv, dtype=np.int64, copy=True)
+    big = npv > limit
+    bins[big] = bins[big] * int(limit) // npv[big][:, None]
+    npv[big] = int(limit)
+    return bins.astype(np.int32), npv.astype(np.int32), int(npv.max())
+
+
+def smem_bin_limit(smem_optin: int, warps: int = 32) -> int:
+    """compute_smem_bin_limit (tenex/kernels/adaptive_smem.py:701-726): the
+    largest B >= 2 with (B^3 + 2 B^2 + B + warps) * 4 <= smem_optin, by
+    downward search from the cube-root estimate."""
+    B = max(2, int((smem_optin / 4) ** (1.0 / 3.0)))
+    while B > 2 and (B ** 3 + 2 * B ** 2 + B + warps) * 4 > smem_optin:
+        B -= 1
+    return B
+
+
 # ── pairs ──────────────────────────────────────────────────────────────────
 
 def pairfree_pairs(n: int) -> np.ndarray:

@@ -639,6 +639,38 @@ def te_matrix_internal(prep: Prepared, targets=None, sources=None, cap_wave: int
     return out
 
 
+def te_pair_list(prep: Prepared, pairs: np.ndarray, cap_wave: int = 0) -> torch.Tensor:
+    """TE of an explicit [target, source] pair list as a (P,) float32 tensor
+    (0 where target == source), costing the listed pairs only: a pair-list
+    plan over the variables the pairs touch, run into a matrix over those
+    variables (a private re-prepared copy of their rows when they are fewer
+    than all)."""
+    p = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
+    P = len(p)
+    out = torch.zeros(P, dtype=torch.float32, device=prep.device)
+    if P and (p.min() < 0 or p.max() >= prep.n_vars):
+        raise ValueError(f"pair indices outside [0, {prep.n_vars})")
+    valid = p[:, 0] != p[:, 1]
+    if not valid.any():
+        return out
+    pv = p[valid]
+    involved = np.unique(pv)
+    m = len(involved)
+    if m == prep.n_vars:
+        sub, loc = prep, pv
+    else:
+        sub = var_subset(prep, involved, cache=False)
+        gene_prep_into(sub, features=False)
+        loc = np.searchsorted(involved, pv)
+    plan = te_pairlist_plan(sub.npv_host, np.unique(loc, axis=0), prep.device,
+                            cap_wave or WAVE_CAP)
+    M = torch.zeros((m, m), dtype=torch.float32, device=prep.device)
+    te_run(sub, plan, M, _lib.TNX_OUT_MATRIX, ld_out=m)
+    lt = h2d(loc, prep.device)
+    out[h2d(np.nonzero(valid)[0], prep.device)] = M[lt[:, 0], lt[:, 1]]
+    return out
+
+
 def te_pairfree(prep: Prepared, pair_offset: int = 0, n_pairs_local: int | None = None,
                 cap_wave: int = 0, cap_block: int = 0) -> torch.Tensor:
     """TE in pair-free linear order for ids [pair_offset, pair_offset + n_local)."""
@@ -966,13 +998,14 @@ def copy_perms(n_vars: int, T: int, shuffle_method: str, block_length: int, k0: 
     return (parts[0] if len(parts) == 1 else torch.cat(parts)), eff
 
 
-def var_subset(prep: Prepared, vars_host: np.ndarray) -> Prepared:
+def var_subset(prep: Prepared, vars_host: np.ndarray, cache: bool = True) -> Prepared:
     """The rows `vars_host` (sorted variable indices) of prep's bins as a
     source for the batched surrogate statistics: shuffling and re-preparing
-    only the variables a plan touches.  Cached on prep."""
+    only the variables a plan touches.  Cached on prep unless cache=False
+    (then a private copy whose states are left to the caller)."""
     vars_host = np.ascontiguousarray(vars_host, dtype=np.int64)
     key = ("subset", hash(vars_host.tobytes()), len(vars_host))
-    sub = prep.plans.get(key)
+    sub = prep.plans.get(key) if cache else None
     if sub is None:
         vd = h2d(vars_host, prep.device)
         m = len(vars_host)
@@ -987,7 +1020,8 @@ def var_subset(prep: Prepared, vars_host: np.ndarray) -> Prepared:
                                         device=prep.device),
                        xlogx=prep.xlogx, hot_zero=prep.hot_zero)
         sub.plans["rows"] = vd    # the subset's variable indices on the device
-        prep.plans[key] = sub
+        if cache:
+            prep.plans[key] = sub
     return sub
 
 

@@ -70,11 +70,7 @@ class _LDSKernel(TEKernel):
         p = p.astype(np.int64).reshape(-1, 2)
         if len(p) == 0:
             return np.empty(0, np.float32)
-        tg = np.unique(p[:, 0]).astype(np.int32)
-        sr = np.unique(p[:, 1]).astype(np.int32)
-        m = core.te_matrix_internal(prep, tg, sr, self.cap_wave, self.cap_block)
-        pt = torch.from_numpy(p).to(prep.device)
-        return m[pt[:, 0], pt[:, 1]].cpu().numpy()
+        return core.te_pair_list(prep, p, self.cap_wave).cpu().numpy()
 
     def log_description(self, **kwargs):
         return f"{self.name} (gfx950 LDS histograms, b_max={kwargs.get('b_max')})"
@@ -112,11 +108,15 @@ class AdaptiveSMEMKernel(_LDSKernel):
         if use_coarsening is not True:
             return bin_arrs, n_per_var, b_max
         smem_optin = kwargs.get("smem_optin") or LDS_BYTES_GFX950
-        limit = compute_smem_bin_limit(int(smem_optin), warps=16)
-        if b_max <= limit:
-            return bin_arrs, n_per_var, b_max
         if not isinstance(bin_arrs, core.Prepared):
             raise TypeError("prepare() expects core.Prepared data")
+        # the limit the reference derives for its block of next_pow2(L)
+        # threads, clamped to [128, 1024] (adaptive_smem.py:1289-1290)
+        L = max(1, int(bin_arrs.L))
+        warps = max(128, min(1024, 1 << (L - 1).bit_length())) // 32
+        limit = compute_smem_bin_limit(int(smem_optin), warps=warps)
+        if b_max <= limit:
+            return bin_arrs, n_per_var, b_max
         vprint(f"[TENEX] bin coarsening: b_max={b_max}->{limit}")
         bins = bin_arrs.bins.clone()
         npv = bin_arrs.npv.clone()

@@ -229,3 +229,50 @@ def test_mlp_oracle_gradient():
         pm = params.copy(); pm[i] -= eps
         lm, _ = O.mlp_loss_grad(pm, D, H, Oo, X, Tg, w)
         assert abs((lp - lm) / (2 * eps) - g[i]) < 1e-6 * max(1.0, abs(g[i]))
+
+
+def test_coarsen_restatement_properties():
+    """Coarsening maps the levels of a variable with ng > limit onto every
+    level 0..limit-1 (the surjectivity the reference's docstring proves),
+    monotonically, merging ceil/floor(ng / limit) old levels per new level;
+    variables at or below the limit are unchanged."""
+    rng = np.random.default_rng(0)
+    for limit in (2, 3, 7, 12, 33):
+        ng = np.arange(2, 90)
+        dense = np.tile(np.arange(90), (len(ng), 1)) % ng[:, None]
+        d2, npv2, bm = O.coarsen_bins(dense, ng, limit)
+        for g, b in enumerate(ng):
+            old = np.arange(b)
+            new = d2[g, :b]
+            if b <= limit:
+                assert np.array_equal(new, old) and npv2[g] == b
+                continue
+            assert npv2[g] == limit
+            assert np.array_equal(np.unique(new), np.arange(limit))
+            assert np.all(np.diff(new) >= 0)
+            sizes = np.bincount(new, minlength=limit)
+            assert sizes.min() >= b // limit and sizes.max() <= -(-b // limit)
+        assert bm == max(min(limit, int(ng.max())), 2)
+    # the joint counts of coarsened data are the sums of the original counts
+    # over the merged levels (brute force on a small case)
+    d = np.stack([rng.integers(0, 9, 400), rng.integers(0, 5, 400)]).astype(np.int32)
+    npv = np.array([9, 5], np.int32)
+    c = O.joint_counts(d, npv, 0, 1)
+    dc, npc, _ = O.coarsen_bins(d, npv, 4)
+    cc = O.joint_counts(dc, npc, 0, 1)
+    m, ms = np.arange(9) * 4 // 9, np.arange(5) * 4 // 5
+    agg = np.zeros_like(cc)
+    for a in range(9):
+        for b in range(9):
+            for s in range(5):
+                agg[m[a], m[b], ms[s]] += c[a, b, s]
+    assert np.array_equal(agg, cc)
+
+
+def test_smem_bin_limit_brute_force():
+    for optin in (48 * 1024, 64 * 1024, 99 * 1024, 160 * 1024, 228 * 1024):
+        for warps in (4, 16, 32):
+            best = max(B for B in range(2, 200)
+                       if (B ** 3 + 2 * B ** 2 + B + warps) * 4 <= optin)
+            assert O.smem_bin_limit(optin, warps) == best
+    assert O.smem_bin_limit(160 * 1024, 32) == 33
