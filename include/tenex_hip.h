@@ -260,6 +260,20 @@ TNX_API int tnx_te_queue(const tnx_work_t *work, int64_t n_wave_items, int64_t n
  * gmem.cu:190-230). */
 TNX_API int tnx_te_run(const tnx_te_args_t *args, void *stream);
 
+/* tnx_te_run's persistent launch without any device memory of the caller's
+ * beyond inputs and outputs: no counterpart in the reference; for callers
+ * that cannot allocate (HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  args as for tnx_te_run with queue_items set and
+ * queue_ctr NULL: the queue take counters live in a library-owned device
+ * array of TNX_NOALLOC_SCRATCH_INTS int32 per GPU (1 KiB, held until the
+ * process ends), cleared by a kernel before each launch.  Calls nothing
+ * that allocates.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (concurrent streams or threads) race and give wrong
+ * results; serialise them.  TNX_E_ARG without a queue or with queue_ctr
+ * set, TNX_E_CAP if the queues need more counters than the array holds. */
+#define TNX_NOALLOC_SCRATCH_INTS 256
+TNX_API int tnx_te_run_noalloc(const tnx_te_args_t *args, void *stream);
+
 /* Surrogate statistics from a TNX_OUT_STAGE run: for each public slot
  * slots[i] (ascending, < stage_stride) and copies k = 0..K-1 in order,
  * te = stage[k * stage_stride + slot]; sum_te[slot] += te, sumsq_te[slot] +=

@@ -60,6 +60,7 @@ class TeArgs(C.Structure):
 
 TNX_QUEUES = 8
 TNX_QUEUE_CTR_INTS = 256
+TNX_NOALLOC_SCRATCH_INTS = 256
 
 
 class SurrArgs(C.Structure):
@@ -98,6 +99,7 @@ SIGNATURES = {
     "tnx_te_plan": (C.c_int, [p_, i64, i64, p_, i64, p_, i64, i32, i32, p_, p_, p_, i64,
                               C.POINTER(PlanInfo)]),
     "tnx_te_run": (C.c_int, [C.POINTER(TeArgs), p_]),
+    "tnx_te_run_noalloc": (C.c_int, [C.POINTER(TeArgs), p_]),
     "tnx_te_queue": (C.c_int, [p_, i64, i64, p_, i64, p_, p_, i64, i32, i32, p_, i64, p_,
                                C.POINTER(C.c_int64)]),
     "tnx_te_counts_debug": (C.c_int, [p_, p_, p_, p_, p_, p_, i64, i64, i64, i64, i32, p_, i64,

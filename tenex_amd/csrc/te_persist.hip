@@ -752,9 +752,63 @@ int launch_persist(const tnx_te_args_t &a, int ni, hipStream_t s) {
     return TNX_OK;
 }
 
+// ── tnx_te_run_noalloc: the queue take counters in a device-side array ────
+// One copy per device (each GPU the library runs on holds it until the
+// process ends).  Host code never passes the name `scratch` to a kernel or a
+// copy: the name is a host-side stand-in; its device address comes from
+// hipGetSymbolAddress, looked up once per device with that device current.
+static __device__ __align__(256) int32_t scratch[TNX_NOALLOC_SCRATCH_INTS];
+static_assert(2 * TNX_QUEUES * kQueueStride <= TNX_NOALLOC_SCRATCH_INTS,
+              "the no-alloc scratch holds every queue's take counter");
+
+constexpr int kMaxDevices = 64;
+
+static int32_t *scratch_address() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error("tnx_te_run_noalloc: no current device (or device index >= %d)", kMaxDevices);
+        return nullptr;
+    }
+    static std::mutex mu;
+    static int32_t *addr[kMaxDevices] = {};
+    std::lock_guard<std::mutex> g(mu);
+    if (addr[dev] == nullptr) {
+        void *p = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (rc != hipSuccess || p == nullptr) {
+            set_error("tnx_te_run_noalloc: hipGetSymbolAddress failed on device %d: %s", dev,
+                      hipGetErrorString(rc));
+            return nullptr;
+        }
+        addr[dev] = static_cast<int32_t *>(p);
+    }
+    return addr[dev];
+}
+
 }  // namespace tnx
 
 using namespace tnx;
+
+extern "C" int tnx_te_run_noalloc(const tnx_te_args_t *args, void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(args != nullptr, "tnx_te_run_noalloc: args is NULL");
+    TNX_CHECK_ARG(args->queue_items != nullptr,
+                  "tnx_te_run_noalloc: needs a persistent plan (queue_items from tnx_te_queue)");
+    TNX_CHECK_ARG(args->queue_ctr == nullptr,
+                  "tnx_te_run_noalloc: queue_ctr must be NULL (the counters live in the "
+                  "library's device array)");
+    const int64_t need = 2 * TNX_QUEUES * kQueueStride;
+    if (need > TNX_NOALLOC_SCRATCH_INTS) {
+        set_error("tnx_te_run_noalloc: %lld queue counters exceed the scratch maximum of %d",
+                  (long long)need, TNX_NOALLOC_SCRATCH_INTS);
+        return TNX_E_CAP;
+    }
+    int32_t *ctr = scratch_address();
+    if (ctr == nullptr) return TNX_E_HIP;
+    tnx_te_args_t a = *args;
+    a.queue_ctr = ctr;  // cleared by queue_reset_kernel on the stream, as in tnx_te_run
+    return tnx_te_run(&a, stream);
+}
 
 namespace {
 
