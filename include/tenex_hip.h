@@ -260,6 +260,25 @@ TNX_API int tnx_te_queue(const tnx_work_t *work, int64_t n_wave_items, int64_t n
  * gmem.cu:190-230). */
 TNX_API int tnx_te_run(const tnx_te_args_t *args, void *stream);
 
+/* TRACE OutTE / InTE for n_jobs variables (replaces fused_outte_4way_launch
+ * and fused_inte_4way_launch, tenex/csrc/trace_sort_entropy.cu:191-304, and
+ * the 4-entropy combination of TRACEMethod._compute_cuda,
+ * tenex/inference/trace.py:913-966).  Job j: variable var_ids[j] with the d_j
+ * connected variables conn[conn_off[j] .. conn_off[j + 1]) (mode 0 OutTE:
+ * its network targets; mode 1 InTE: its network sources), keyed with the
+ * reference's 64-bit polynomial state hash.  sums_out[4 j + e] (device
+ * int64) receives sum over distinct keys of c*log2(c) in fixed point
+ * (xlogx_fx from tnx_xlogx_table_fx with N = T - tau and fx_shift) for the
+ * entropies e = 0..3 of the reference's order; te_out (device float, may be
+ * NULL) receives max(0, (-S0 + S1 + S2 - S3) 2^-fx_shift / (T - tau)).
+ * bins: device int8 rows of stride ld (a multiple of 16).  Needs
+ * T - tau <= 16384 (the keys of one entropy are sorted in LDS). */
+TNX_API int tnx_trace_entropy(const int8_t *bins, int64_t n_vars, int64_t T, int64_t ld,
+                              int32_t tau, int32_t mode, const int32_t *var_ids,
+                              const int32_t *conn, const int64_t *conn_off, int64_t n_jobs,
+                              const int64_t *xlogx_fx, int32_t fx_shift, int64_t *sums_out,
+                              float *te_out, void *stream);
+
 /* tnx_te_run's persistent launch without any device memory of the caller's
  * beyond inputs and outputs: no counterpart in the reference; for callers
  * that cannot allocate (HIP graph capture, tight serving loops, harnesses

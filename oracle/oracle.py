@@ -415,6 +415,153 @@ def trim_dpi(A: np.ndarray, batch_size: int = 512, trim_threshold: float = 0.0):
     return np.concatenate(pairs), np.concatenate(vals).astype(np.float32)
 
 
+# ── TRACE ──────────────────────────────────────────────────────────────────
+
+TRACE_P = np.uint64(1000000007)
+
+
+def trace_hash(rows) -> np.ndarray:
+    """Horner polynomial hash of the rows (d, L) with wrapping 64-bit
+    arithmetic: h = h * P + v per row in order (trace.py:497-502,
+    trace_sort_entropy.cu:46-50); uint64 bits equal the reference's int64."""
+    rows = np.atleast_2d(np.asarray(rows)).astype(np.uint64)
+    h = np.zeros(rows.shape[1], np.uint64)
+    with np.errstate(over="ignore"):
+        for r in rows:
+            h = h * TRACE_P + r
+    return h
+
+
+def _pow_p(d: int) -> np.uint64:
+    v = np.uint64(1)
+    with np.errstate(over="ignore"):
+        for _ in range(d):
+            v = v * TRACE_P
+    return v
+
+
+def trace_keys(bins, var: int, conn, mode: int, tau: int = 1):
+    """The four key vectors of one TRACE job as the reference's 4-way hash
+    fusion forms them (compute_outte_cpu / compute_inte_cpu, trace.py:475-547):
+    mode 0 (OutTE, conn = X's targets): h(Yf,Yp), h(Yp), h(Yf,Xp,Yp), h(Xp,Yp);
+    mode 1 (InTE, conn = X's sources): h(Xf,Xp), h(Xp), h(Xf,Xp,Yp), h(Xp,Yp)."""
+    bins = np.asarray(bins)
+    if bins.ndim == 3:
+        bins = bins[:, :, 0]
+    T = bins.shape[1]
+    L = T - tau
+    conn = [int(c) for c in conn]
+    Pd = _pow_p(len(conn))
+    xp = bins[var, :L].astype(np.uint64)
+    with np.errstate(over="ignore"):
+        if mode == 0:
+            h_yf = trace_hash(bins[conn, tau:tau + L])
+            h_yp = trace_hash(bins[conn, :L])
+            return (h_yf * Pd + h_yp, h_yp, (h_yf * TRACE_P + xp) * Pd + h_yp, xp * Pd + h_yp)
+        xf = bins[var, tau:tau + L].astype(np.uint64)
+        h_yp = trace_hash(bins[conn, :L])
+        h_xfxp = xf * TRACE_P + xp
+        return (h_xfxp, xp, h_xfxp * Pd + h_yp, xp * Pd + h_yp)
+
+
+def _entropy_np(h) -> float:
+    _, counts = np.unique(h, return_counts=True)
+    p = counts / len(h)
+    return -float(np.sum(p * np.log2(p)))
+
+
+def trace_te(bins, var: int, conn, mode: int, tau: int = 1) -> float:
+    """OutTE (mode 0) or InTE (mode 1) of one variable as the reference's CPU
+    path computes it: FP64 plug-in entropies of the hashed states,
+    max(e0 - e1 - e2 + e3, 0) (trace.py:505-510, :541-547); 0 without
+    connections."""
+    if len(conn) == 0:
+        return 0.0
+    e = [_entropy_np(k) for k in trace_keys(bins, var, conn, mode, tau)]
+    return max(e[0] - e[1] - e[2] + e[3], 0.0)
+
+
+def trace_te_tuples(bins, var: int, conn, mode: int, tau: int = 1) -> float:
+    """The same quantity from the states themselves (np.unique over rows of
+    the (L, k) state matrix, no hashing), the structured-dtype entropy the
+    reference validates its hash against (trace.py:417-440)."""
+    bins = np.asarray(bins)
+    if bins.ndim == 3:
+        bins = bins[:, :, 0]
+    if len(conn) == 0:
+        return 0.0
+    L = bins.shape[1] - tau
+    conn = [int(c) for c in conn]
+    f = lambda rows: bins[rows, tau:tau + L]  # noqa: E731
+    pst = lambda rows: bins[rows, :L]  # noqa: E731
+
+    def H(*blocks):
+        m = np.concatenate([np.atleast_2d(b) for b in blocks], axis=0).T
+        _, c = np.unique(m, axis=0, return_counts=True)
+        p = c / L
+        return -float(np.sum(p * np.log2(p)))
+    if mode == 0:
+        e = (H(f(conn), pst(conn)), H(pst(conn)), H(f(conn), pst([var]), pst(conn)),
+             H(pst([var]), pst(conn)))
+    else:
+        e = (H(f([var]), pst([var])), H(pst([var])), H(f([var]), pst([var]), pst(conn)),
+             H(pst([var]), pst(conn)))
+    return max(e[0] - e[1] - e[2] + e[3], 0.0)
+
+
+def trace_network(te_matrix, n_surrogates: int = 100, significance: float = 2.0,
+                  seed: int = 42, gpu: bool = True):
+    """build_causal_network (trace.py:550-612): the threshold mean +
+    significance * std (numpy float32, ddof 0) of n_surrogates off-diagonal
+    entries drawn with default_rng(seed); edges te[i, j] > threshold with the
+    diagonal zeroed first (the GPU branch, gpu=True) or excluded after the
+    comparison (the CPU branch).  Returns (network dict target -> sorted
+    sources, threshold)."""
+    te = np.asarray(te_matrix, np.float32)
+    n = te.shape[0]
+    rng = np.random.default_rng(seed)
+    rows = rng.integers(0, n, size=n_surrogates)
+    cols = rng.integers(0, n, size=n_surrogates)
+    m = rows == cols
+    cols[m] = (cols[m] + 1) % n
+    sv = te[rows, cols]
+    thr = float(sv.mean() + significance * sv.std())
+    if gpu:
+        t2 = te.copy()
+        np.fill_diagonal(t2, 0.0)
+        edges = t2 > thr
+    else:
+        edges = te > thr
+        np.fill_diagonal(edges, False)
+    src, tgt = np.nonzero(edges)
+    network = {j: [] for j in range(n)}
+    order = np.argsort(tgt, kind="stable")
+    for s_, t_ in zip(src[order], tgt[order]):
+        network[int(t_)].append(int(s_))
+    return network, thr
+
+
+def trace_infer(te_matrix, bins, tau: int = 1, n_surrogates: int = 100,
+                significance: float = 2.0, seed: int = 42, gpu: bool = True):
+    """TRACEMethod.infer (trace.py:621-760) on the CPU: network, then OutTE
+    over each variable's targets and InTE over its sources (float32).
+    Returns (outte, inte, network, edge pairs [src, tgt] target-major)."""
+    network, _ = trace_network(te_matrix, n_surrogates, significance, seed, gpu)
+    n = len(network)
+    outgoing = {i: [] for i in range(n)}
+    pairs = []
+    for j in range(n):
+        for i in network[j]:
+            outgoing[i].append(j)
+            pairs.append((i, j))
+    outte = np.zeros(n, np.float32)
+    inte = np.zeros(n, np.float32)
+    for i in range(n):
+        outte[i] = trace_te(bins, i, sorted(outgoing[i]), 0, tau)
+        inte[i] = trace_te(bins, i, network[i], 1, tau)
+    return outte, inte, network, np.asarray(pairs, np.int64).reshape(-1, 2)
+
+
 # ── MLP ────────────────────────────────────────────────────────────────────
 
 def mlp_n_params(D, H, O):

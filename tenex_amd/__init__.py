@@ -11,6 +11,7 @@ __version__ = "0.1.0"
 from ._log import is_verbose, set_verbose
 from .inference import GRN, NetWeaver, available_methods
 from .inference.surrogate_test import SurrogateTestResult
+from .inference.trace import TRACEResult
 from .kernels import TEKernel, auto_select, get_kernel, registered_kernels
 from .pipeline import Pipeline, PipelineResult
 from .result import TransferEntropyResult
@@ -18,6 +19,6 @@ from .transferentropy import TransferEntropyEngine
 
 __all__ = [
     "TransferEntropyEngine", "TransferEntropyResult", "Pipeline", "PipelineResult", "NetWeaver",
-    "GRN", "SurrogateTestResult", "available_methods", "TEKernel", "auto_select", "get_kernel",
-    "registered_kernels", "set_verbose", "is_verbose", "__version__",
+    "GRN", "SurrogateTestResult", "TRACEResult", "available_methods", "TEKernel", "auto_select",
+    "get_kernel", "registered_kernels", "set_verbose", "is_verbose", "__version__",
 ]

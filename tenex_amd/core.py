@@ -671,6 +671,42 @@ def te_pair_list(prep: Prepared, pairs: np.ndarray, cap_wave: int = 0) -> torch.
     return out
 
 
+TRACE_MAX_L = 16384     # trace.hip sorts one entropy's keys in LDS
+
+
+def trace_te(bins: torch.Tensor, T: int, tau: int, mode: int, var_ids, conn_lists,
+             return_sums: bool = False):
+    """TRACE OutTE (mode 0, conn = each variable's targets) or InTE (mode 1,
+    conn = its sources) of the variables var_ids on device int8 bins (n, ld)
+    through tnx_trace_entropy; returns float32 (len(var_ids),) on the device
+    (and the int64 (jobs, 4) fixed-point entropy sums with return_sums)."""
+    dev = bins.device
+    var_ids = np.ascontiguousarray(var_ids, dtype=np.int32)
+    J = len(var_ids)
+    te = torch.zeros(J, dtype=torch.float32, device=dev)
+    sums = torch.zeros((J, 4), dtype=torch.int64, device=dev)
+    if J == 0:
+        return (te, sums) if return_sums else te
+    L = int(T) - int(tau)
+    if L > TRACE_MAX_L:
+        raise NotImplementedError(f"TRACE on the GPU takes T - tau <= {TRACE_MAX_L} (got {L})")
+    n = bins.shape[0]
+    lens = np.array([len(c) for c in conn_lists], dtype=np.int64)
+    off = np.zeros(J + 1, dtype=np.int64)
+    np.cumsum(lens, out=off[1:])
+    flat = (np.concatenate([np.asarray(c, dtype=np.int64) for c in conn_lists])
+            if off[-1] else np.zeros(1, np.int64))
+    if (var_ids.min() < 0 or var_ids.max() >= n or flat.min() < 0 or flat.max() >= n):
+        raise ValueError(f"TRACE variable indices outside [0, {n})")
+    tbl = xlogx_fx_table(L, dev)
+    v_d = h2d(var_ids, dev)
+    c_d = h2d(flat.astype(np.int32), dev)
+    o_d = h2d(off, dev)
+    call("tnx_trace_entropy", ptr(bins), n, int(T), bins.shape[1], int(tau), int(mode), ptr(v_d),
+         ptr(c_d), ptr(o_d), J, ptr(tbl), fx_shift(L), ptr(sums), ptr(te), _stream(dev))
+    return (te, sums) if return_sums else te
+
+
 def te_pairfree(prep: Prepared, pair_offset: int = 0, n_pairs_local: int | None = None,
                 cap_wave: int = 0, cap_block: int = 0) -> torch.Tensor:
     """TE in pair-free linear order for ids [pair_offset, pair_offset + n_local)."""

@@ -1,11 +1,12 @@
 """Link inference from a TE matrix: method registry and shared utilities
 (tenex/inference/__init__.py:32-199).
 
-Implemented methods: 'fdr' (z-score -> BH -> optional DPI trim) and
+Implemented methods: 'fdr' (z-score -> BH -> optional DPI trim),
 'surrogate_test' (block-shuffle surrogates on the gfx950 kernels, optional
-learned null).  The reference's 'clr', 'nd', 'trace' and 'point' are outside
-this build's hot path; they are registered so available_methods() keeps the
-reference's list, and raise NotImplementedError when called.
+learned null) and 'trace' (thresholded network, OutTE / InTE).  The
+reference's 'clr', 'nd' and 'point' are outside this build's hot path; they
+are registered so available_methods() keeps the reference's list, and raise
+NotImplementedError when called.
 """
 from __future__ import annotations
 
@@ -181,9 +182,11 @@ class _OutOfScope(InferenceMethod):
 def _register_all():
     from .fdr import FDRMethod
     from .surrogate_test import SurrogateTestMethod
+    from .trace import TRACEMethod
     register(FDRMethod())
     register(SurrogateTestMethod())
-    for nm in ("clr", "nd", "trace", "point"):
+    register(TRACEMethod())
+    for nm in ("clr", "nd", "point"):
         register(_OutOfScope(nm))
 
 

@@ -49,6 +49,19 @@ class NetWeaver:
         method_obj = get_method(method)
         params = dict(fdr=self.fdr, links=self.links, sources=self.sources,
                       is_trimming=self.is_trimming, trim_threshold=self.trim_threshold)
+        if method == "trace":
+            if "bin_data" not in kwargs and self.bin_arrs is not None:
+                ba = np.asarray(self.bin_arrs)
+                if ba.ndim == 3 and ba.shape[2] != 1:
+                    raise ValueError(
+                        f"TRACE requires K=1 bin arrays; the result has shape {ba.shape}. "
+                        f"Re-run engine.compute() with a K=1 binning method (e.g. FSBW-L) "
+                        f"before passing to TRACE.")
+                params["bin_data"] = self.bin_arrs
+            if "tau" not in kwargs:
+                params["tau"] = self.tau
+            for drop in ("fdr", "links", "sources", "is_trimming", "trim_threshold"):
+                params.pop(drop, None)
         if method == "surrogate_test":
             if "bin_data" not in kwargs and self.bin_arrs is not None:
                 params["bin_data"] = self.bin_arrs

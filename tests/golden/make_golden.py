@@ -29,6 +29,12 @@ fdr_300x600.npz        the FDR method's arithmetic on the 300-gene x 600-cell
                        public pair order of build_pairs, then its DPI _trim
                        on the CPU device: thresholded and trimmed edge lists
                        with their weights.
+trace_80x700.npz       NetWeaver(result).infer("trace", device="cpu") on 80
+                       genes x 700 cells (seed 21, 10 hubs driving two
+                       targets each): the TE matrix, bins, OutTE, InTE, the
+                       thresholded network and its GRN (100 surrogate
+                       entries, significance 2.0, the reference's CPU
+                       hash-entropy path).
 """
 import os
 import sys
@@ -58,6 +64,8 @@ def main(which):
         write_surrogate_120(tnx)
     if "fdr_300x600" in which:
         write_fdr_300(tnx)
+    if "trace_80x700" in which:
+        write_trace_80(tnx)
     print("golden fixtures written:", ", ".join(which))
 
 
@@ -140,7 +148,32 @@ def write_fdr_300(tnx):
                         trimmed_te=np.asarray(trimmed.te, np.float32), fdr=0.01)
 
 
-ALL = ("te_c0", "surrogate_40x400", "surrogate_120x1000", "fdr_300x600")
+def write_trace_80(tnx):
+    x5 = zero_inflated_lognormal(80, 700, 21)
+    rng = np.random.default_rng(21)
+    g = rng.choice(80, 30, replace=False)
+    for i in range(10):                       # hubs: one source drives two targets
+        s_, t1, t2 = g[3 * i: 3 * i + 3]
+        x5[t1, 1:] = x5[s_, :-1] * (0.5 + rng.random(699))
+        x5[t2, 1:] = 0.5 * x5[t2, 1:] + x5[s_, :-1] * (0.5 + rng.random(699))
+    x5 = x5.astype(np.float32)
+    names5 = np.array([f"g{i}" for i in range(80)])
+    r5 = tnx.TransferEntropyEngine(x5, names5).compute(accelerator="cpu")
+    tr = tnx.NetWeaver(r5).infer(method="trace", device="cpu", n_surrogates=100,
+                                 significance=2.0)
+    net = np.array([[int(s_), j] for j in range(80) for s_ in tr.network[j]],
+                   np.int64).reshape(-1, 2)
+    np.savez_compressed(os.path.join(HERE, "trace_80x700.npz"), data=x5,
+                        bins=np.asarray(r5.bin_arrs)[:, :, 0].astype(np.int8),
+                        matrix=np.asarray(r5.matrix, np.float32),
+                        outte=np.asarray(tr.outte, np.float32),
+                        inte=np.asarray(tr.inte, np.float32), network=net,
+                        grn_pairs=np.asarray(tr.grn.pairs, np.int64),
+                        grn_te=np.asarray(tr.grn.te, np.float32), n_surrogates=100,
+                        significance=2.0)
+
+
+ALL = ("te_c0", "surrogate_40x400", "surrogate_120x1000", "fdr_300x600", "trace_80x700")
 
 if __name__ == "__main__":
     # python make_golden.py [fixture ...]   (default: all of ALL)
