@@ -46,7 +46,7 @@ extern "C" {
 #define TNX_API
 #endif
 
-#define TNX_ABI_VERSION 7
+#define TNX_ABI_VERSION 8
 
 /* Status codes. */
 #define TNX_OK 0
@@ -231,6 +231,14 @@ typedef struct {
      * 64-bit fixed point, then scaled and combined with S1 - S2a in FP64. */
     const int64_t *xlogx_fx;
     int32_t fx_shift;
+    /* Per-variable series length (from ABI 8; persistent launch only):
+     * var_L != NULL gives target variable v the L = var_L[v] samples
+     * (1 <= var_L[v] <= T - tau; larger values are clamped) and 1 / var_L[v]
+     * as its normaliser, so stacked copies prepared with different lags
+     * (tnx_gene_prep per copy with its own tau, T - tau = the copy's L) run
+     * in one launch; tau is then the smallest lag and xlogx_fx the table
+     * for N = T - tau. */
+    const int32_t *var_L;
 } tnx_te_args_t;
 
 #define TNX_QUEUES 8            /* work queues of the persistent launch (one per XCD) */

@@ -497,12 +497,14 @@ def plan_queue(plan: Plan, npv_host: np.ndarray, L: int, item_per_wave: bool):
 def te_args(prep: Prepared, plan: Plan, out: torch.Tensor | None, out_mode: int, ld_out: int = 0,
             pair_offset: int = 0, n_pairs_local: int = 0, stats=None, stage_vars: int = 0,
             stage_stride: int = 0, item_per_wave: bool | None = None,
-            persistent: bool | None = None, counts=None) -> _lib.TeArgs:
+            persistent: bool | None = None, counts=None, var_L=None) -> _lib.TeArgs:
     """stats: (sum, sumsq, count_ge, observed) public-layout tensors for
     TNX_OUT_STATS; stage_vars / stage_stride for TNX_OUT_STAGE.  item_per_wave None: chosen from the
     plan (item_per_wave_default).  persistent None: one persistent launch
     whenever persistent_ok.  counts = (map int32 (n, n), out int32 (rows,
-    ld)) turns on the kernels' joint-count dump (persistent launch)."""
+    ld)) turns on the kernels' joint-count dump (persistent launch).  var_L
+    (device int32 (n_vars,)): per-target series lengths of stacked copies
+    prepared with different lags (persistent launch)."""
     i = plan.info
     if item_per_wave is None:
         item_per_wave = item_per_wave_default(plan)
@@ -510,6 +512,8 @@ def te_args(prep: Prepared, plan: Plan, out: torch.Tensor | None, out_mode: int,
         persistent = persistent_ok(prep, plan)
     if counts is not None and not persistent:
         raise ValueError("the joint-count dump needs the persistent launch")
+    if var_L is not None and not persistent:
+        raise ValueError("per-variable series lengths need the persistent launch")
     q = plan_queue(plan, prep.npv_host, prep.L, item_per_wave) if persistent else (None,) * 3
     cm, cd = (counts[0], counts[1]) if counts is not None else (None, None)
     st = stats if stats is not None else (None, None, None, None)
@@ -526,7 +530,8 @@ def te_args(prep: Prepared, plan: Plan, out: torch.Tensor | None, out_mode: int,
         item_per_wave=int(bool(item_per_wave)), queue_items=ptr(q[0]), queue_off=ptr(q[1]),
         queue_ctr=ptr(q[2]), counts_map=ptr(cm), counts_dbg=ptr(cd),
         counts_ld=int(cd.shape[1]) if cd is not None else 0,
-        xlogx_fx=ptr(xlogx_fx_table(prep.L, prep.device)), fx_shift=fx_shift(prep.L))
+        xlogx_fx=ptr(xlogx_fx_table(prep.L, prep.device)), fx_shift=fx_shift(prep.L),
+        var_L=ptr(var_L))
 
 
 WAVE_ITEM_PAIRS = 32   # pairs of a one-target item that a single wave takes whole
@@ -1013,6 +1018,113 @@ def _stacked_copies(prep: Prepared, plan: Plan, K: int):
     args = te_args(sp, planK, stage, _lib.TNX_OUT_STAGE, ld_out=n, stage_vars=n,
                    stage_stride=n * n)
     return sp, args, stage
+
+
+# ── several lags in one launch ───────────────────────────────────────────
+
+def _lag_stack(prep: Prepared, plan: Plan, taus, K: int):
+    """Buffers for K copies of prep's variables at each lag of taus, stacked
+    lag-major (variable (l K + k) n + j is copy k of variable j at lag
+    taus[l]; prep's tau is the smallest lag): the stacked Prepared, the
+    per-variable series lengths, TE args for the K len(taus)-fold plan
+    writing a (K len(taus), n, n) staging slab, and the slab.  None when the
+    plan cannot run as one persistent launch."""
+    n, NL = prep.n_vars, len(taus)
+    KT = K * NL
+    sp = Prepared(bins=torch.empty((KT * n, prep.ld), dtype=prep.bins.dtype, device=prep.device),
+                  npv=prep.npv.repeat(KT).contiguous(), npv_host=np.tile(prep.npv_host, KT),
+                  T=prep.T, tau=prep.tau,
+                  states=torch.empty((KT * n, prep.lds), dtype=prep.states.dtype,
+                                     device=prep.device),
+                  S1=torch.empty(KT * n, dtype=torch.float64, device=prep.device),
+                  S2a=torch.empty(KT * n, dtype=torch.float64, device=prep.device),
+                  feat=torch.empty((1, prep.feat.shape[1]), dtype=torch.float32,
+                                   device=prep.device),
+                  xlogx=prep.xlogx, hot_zero=prep.hot_zero)
+    planK = plan.cache.get(("copies", KT))
+    if planK is None:
+        planK = replicate_plan(plan, KT, n)
+        plan.cache[("copies", KT)] = planK
+    if not persistent_ok(sp, planK):
+        return None
+    var_L = h2d(np.repeat(np.array([prep.T - t for t in taus], np.int32), K * n), prep.device)
+    stage = torch.empty(KT * n * n, dtype=torch.float32, device=prep.device)
+    args = te_args(sp, planK, stage, _lib.TNX_OUT_STAGE, ld_out=n, stage_vars=n,
+                   stage_stride=n * n, persistent=True, var_L=var_L)
+    return sp, args, stage, var_L
+
+
+def _lag_prep(sp: Prepared, taus, K: int, n: int):
+    """tnx_gene_prep of each lag's block of stacked copies with that lag."""
+    bmax = int(sp.npv_host.max(initial=1))
+    for l, tau in enumerate(taus):
+        r0, r1 = l * K * n, (l + 1) * K * n
+        call("tnx_gene_prep", ptr(sp.bins[r0:r1]), r1 - r0, sp.T, sp.ld, int(tau),
+             ptr(sp.npv[r0:r1]), ptr(sp.states[r0:r1]), sp.lds, ptr(sp.S1[r0:r1]),
+             ptr(sp.S2a[r0:r1]), None, bmax, _stream(sp.device))
+
+
+def multilag_observed(prep: Prepared, plan: Plan, taus) -> list | None:
+    """Internal-layout TE matrices of plan's pairs at every lag of taus in
+    one launch (prep prepared with min(taus)); None when the stacked plan
+    cannot run persistently."""
+    n, NL = prep.n_vars, len(taus)
+    got = _lag_stack(prep, plan, taus, 1)
+    if got is None:
+        return None
+    sp, args, stage, _ = got
+    for l in range(NL):
+        sp.bins[l * n:(l + 1) * n].copy_(prep.bins)
+    _lag_prep(sp, taus, 1, n)
+    stage.zero_()
+    te_run(sp, None, None, _lib.TNX_OUT_STAGE, args=args)
+    slab = stage.view(NL, n, n)       # [lag, source, target]
+    return [slab[l].t().contiguous() for l in range(NL)]
+
+
+def multilag_surrogate_stats(prep: Prepared, plan: Plan, taus, block_lengths, observed,
+                             n_surr: int, shuffle_method: str, seed: int, rng_device):
+    """Surrogate statistics of plan's pairs at every lag of taus with each
+    batch of surrogates in one TE launch for all lags: lag l's copies are
+    block-shuffled with block_lengths[l] (the same orders as a run at that
+    lag alone), prepared with taus[l] and counted with T - taus[l] samples.
+    observed: per lag, public-layout float32 (n, n) or None (no counting).
+    Returns per lag (sum, sumsq, count_ge) public layout, or None when the
+    stacked plan cannot run persistently."""
+    n, NL = prep.n_vars, len(taus)
+    dev = prep.device
+    kb = max(1, copies_per_launch(prep, n_surr, n * n) // NL)
+    slots = _plan_slots(plan, n, dev)
+    out = []
+    for l in range(NL):
+        s = torch.zeros((n, n), dtype=torch.float64, device=dev)
+        out.append((s, torch.zeros_like(s), torch.zeros((n, n), dtype=torch.int32, device=dev)))
+    bufs = {}
+    for k0 in range(0, n_surr, kb):
+        k1 = min(k0 + kb, n_surr)
+        kk = k1 - k0
+        b = bufs.get(kk)
+        if b is None:
+            b = _lag_stack(prep, plan, taus, kk)
+            if b is None:
+                return None
+            bufs[kk] = b
+        sp, args, stage, _ = b
+        for l in range(NL):
+            perm, eff = copy_perms(n, prep.T, shuffle_method, int(block_lengths[l]), k0, k1, 0,
+                                   n_surr, seed, rng_device, dev)
+            r0 = l * kk * n
+            block_shuffle(prep.bins, prep.T, perm.reshape(kk * n, -1), eff,
+                          sp.bins[r0:r0 + kk * n], kk)
+        _lag_prep(sp, taus, kk, n)
+        te_run(sp, None, None, _lib.TNX_OUT_STAGE, args=args)
+        for l in range(NL):
+            s, s2, cg = out[l]
+            obs = observed[l] if observed is not None else None
+            call("tnx_stats_accumulate", ptr(stage[l * kk * n * n:]), kk, n * n, ptr(slots),
+                 slots.numel(), ptr(obs), ptr(s), ptr(s2), ptr(cg if obs is not None else None),
+                 _stream(dev))
+    return out
 
 
 def copy_perms(n_vars: int, T: int, shuffle_method: str, block_length: int, k0: int, k1: int,

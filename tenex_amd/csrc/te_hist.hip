@@ -528,6 +528,7 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
         return launch_persist(a, ni, s);
     }
     TNX_CHECK_ARG(a.counts_map == nullptr, "tnx_te_run: counts_map needs the persistent launch");
+    TNX_CHECK_ARG(a.var_L == nullptr, "tnx_te_run: var_L needs the persistent launch");
     if (a.n_wave_items > 0) {
         TNX_CHECK_ARG(a.work, "tnx_te_run: null work");
         size_t shm = (size_t)kWavesPerWG * wave_slice(a.cap_wave) * 4;

@@ -350,6 +350,8 @@ __device__ __forceinline__ void persist_wave_item(const tnx_te_args_t &a, const 
     TargetRegs<NI> regs;
     int cur_x = -1;
     double s12 = 0.0;
+    int Lx = L;            // the current target's samples (var_L) and normaliser
+    double invNx = invN;
     const uint32_t row_bytes = (uint32_t)a.ld;
     uint4 b0[GS], b1[GS];
     load_group<GS>(b0, row_rsrc(a.bins + (int64_t)y * a.ld, row_bytes), 0, lane);
@@ -365,8 +367,12 @@ __device__ __forceinline__ void persist_wave_item(const tnx_te_args_t &a, const 
             byn = a.n_per_var[yn];
         }
         if (x != cur_x) {
+            if (a.var_L) {
+                Lx = min(max(a.var_L[x], 1), L);
+                invNx = 1.0 / (double)Lx;
+            }
             load_target_buf<NI>(regs, row_rsrc(a.states + (int64_t)x * a.lds, 2u * (uint32_t)a.lds),
-                                L, lane);
+                                Lx, lane);
             s12 = a.S1[x] - a.S2a[x];
             cur_x = x;
         }
@@ -382,12 +388,12 @@ __device__ __forceinline__ void persist_wave_item(const tnx_te_args_t &a, const 
         const uint32_t trash_off = (uint32_t)(a.cap_wave + lane - (lane & (cp.G - 1))) * 4u;
         pipe_count<NI, 0, HZ>(b0, b1, regs, row_rsrc(a.bins + (int64_t)y * a.ld, row_bytes),
                               row_rsrc(a.bins + (int64_t)yn * a.ld, row_bytes), more, run, hc,
-                              kmul, ysh, zero_row + lane, trash_off, L, lane);
+                              kmul, ysh, zero_row + lane, trash_off, Lx, lane);
         if (run) {
             wave_lds_fence();
             if constexpr (HZ) {
                 fold_zero_key<HZ>(hist, zero_row, lane);
-                if (lane == 0) hist[0] -= pad_count(L);
+                if (lane == 0) hist[0] -= pad_count(Lx);
                 wave_lds_fence();
             } else {
                 zero_row[lane] = 0u;  // the padded samples' counts
@@ -395,7 +401,7 @@ __device__ __forceinline__ void persist_wave_item(const tnx_te_args_t &a, const 
             if (a.counts_map) dump_wave_counts(a, x, y, hist, cells, cp, lane);
             const int64_t d = reduce_wave_fx(hist, bx, by, cp, xf, lane);
             wave_lds_fence();
-            if (lane == 0) emit_te(a, slot, (ldexp((double)d, -a.fx_shift) + s12) * invN);
+            if (lane == 0) emit_te(a, slot, (ldexp((double)d, -a.fx_shift) + s12) * invNx);
         } else if (want && lane == 0) {
             emit_te(a, slot, __builtin_nan(""));  // plan mismatch: the pair is not counted
         }
@@ -553,10 +559,12 @@ __device__ __forceinline__ void persist_block_item(const tnx_te_args_t &a, const
         const int x = a.tgt_order[it.tgt_beg + ti];
         const int bx = a.n_per_var[x];
         const uint16_t *st = a.states + (int64_t)x * a.lds;
+        const int Lx = a.var_L ? min(max(a.var_L[x], 1), L) : L;  // this target's samples
+        const double invNx = a.var_L ? 1.0 / (double)Lx : invN;
 #pragma unroll
         for (int i = 0; i < NB; i++) {
             const int t = 16 * tid + 16 * kPersistThreads * i;
-            if (t < L) {
+            if (t < Lx) {
                 const uint4 u = *reinterpret_cast<const uint4 *>(st + t);
                 const uint4 v = *reinterpret_cast<const uint4 *>(st + t + 8);
                 regs.s[8 * i + 0] = u.x; regs.s[8 * i + 1] = u.y;
@@ -568,13 +576,13 @@ __device__ __forceinline__ void persist_block_item(const tnx_te_args_t &a, const
         const double s12 = a.S1[x] - a.S2a[x];
         int y = a.src_order[it.src_beg];
         int by = a.n_per_var[y];
-        load_row256<NB>(ycur, a.bins + (int64_t)y * a.ld, L);
+        load_row256<NB>(ycur, a.bins + (int64_t)y * a.ld, Lx);
         for (int sj = it.src_beg; sj < it.src_end; sj++) {
             int yn = 0, byn = 0;
             if (sj + 1 < it.src_end) {
                 yn = a.src_order[sj + 1];
                 byn = a.n_per_var[yn];
-                load_row256<NB>(ynext, a.bins + (int64_t)yn * a.ld, L);
+                load_row256<NB>(ynext, a.bins + (int64_t)yn * a.ld, Lx);
             }
             int64_t slot = 0;
             const bool want = (x != y) && out_slot(a, x, y, slot);
@@ -587,8 +595,8 @@ __device__ __forceinline__ void persist_block_item(const tnx_te_args_t &a, const
                         cdbg = a.counts_dbg + (int64_t)r * a.counts_ld;
                 }
                 const int64_t d = block_pair<NB, HZ>(hist, c2b, zero_row, trash, capB, regs, ycur,
-                                                     bx, by, L, xf, red, cdbg);
-                if (tid == 0) emit_te(a, slot, (ldexp((double)d, -a.fx_shift) + s12) * invN);
+                                                     bx, by, Lx, xf, red, cdbg);
+                if (tid == 0) emit_te(a, slot, (ldexp((double)d, -a.fx_shift) + s12) * invNx);
             } else if (want && !fits && tid == 0) {
                 emit_te(a, slot, __builtin_nan(""));  // plan mismatch
             }

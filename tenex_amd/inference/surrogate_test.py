@@ -378,3 +378,85 @@ class SurrogateTestMethod(InferenceMethod):
             shuffle_method=shuffle_method,
             block_length=block_length if shuffle_method == "block" else None,
             p_method=p_method, fdr=fdr)
+
+
+def surrogate_test_lags(bin_data, n_per_var, variable_names, taus, device, **kwargs):
+    """The surrogate test at every lag of taus with all lags in each launch
+    (the observed TE of every lag in one launch, then per batch of
+    surrogates one TE launch over the copies of every lag).  Per lag it
+    returns what SurrogateTestMethod.infer returns for that tau with the
+    observed matrix of that lag: the same block orders (block_length
+    defaults to max(2, sqrt(T - tau)) per lag), the same statistics and
+    decision.  Returns None for options it does not cover (a target
+    restriction, the learned null, an explicit fused / loop path) or a plan
+    that cannot run as one persistent launch; the caller then runs the lags
+    one by one."""
+    if kwargs.get("learned") is not None or kwargs.get("fused") is not None or \
+            kwargs.get("targets") is not None:
+        return None
+    n_surrogates = int(kwargs.get("n_surrogates", 100))
+    shuffle_method = kwargs.get("shuffle_method", "block")
+    block_length = kwargs.get("block_length")
+    p_method = kwargs.get("p_method", "parametric")
+    fdr = float(kwargs.get("fdr", 0.01))
+    seed = int(kwargs.get("seed", 42))
+    sources = kwargs.get("sources")
+    if shuffle_method not in ("block", "random") or n_surrogates < 2 or \
+            p_method not in ("parametric", "mc"):
+        return None   # the per-lag path raises the reference's errors
+    b = np.asarray(bin_data)
+    if b.ndim == 3:
+        if b.shape[2] != 1:
+            return None
+        b = b[:, :, 0]
+    n, T = b.shape
+    taus = [int(t) for t in taus]
+    if min(taus) < 1 or T - max(taus) < 2:
+        return None
+    dev = core.require_gpu(device if device is not None and torch.device(device).type == "cuda"
+                           else None)
+    rng_device = kwargs.get("rng_device") or dev
+    bls = [int(block_length) if block_length is not None else max(2, int(np.sqrt(T - t)))
+           for t in taus]
+    bins, npv_d, _, _ = core.upload_bins(b, dev)
+    npv_in = core.h2d(np.asarray(n_per_var, dtype=np.int32), dev)
+    if npv_in.shape[0] == n and bool((npv_in >= npv_d).all()):
+        npv_d = npv_in.contiguous()
+    prep = core.prepare(bins, npv_d, T, min(taus))
+    names_arr = np.asarray(variable_names)
+    s_rows = None
+    if sources is not None:
+        _, s_rows, _ = np.intersect1d(names_arr, sources, return_indices=True)
+    src_idx = None if s_rows is None else np.sort(s_rows).astype(np.int32)
+    plan = core.surrogate_plan(prep, None, src_idx)
+    obs_int = core.multilag_observed(prep, plan, taus)
+    if obs_int is None:
+        return None
+    obs_pub = [o.t().contiguous() for o in obs_int]
+    counted = p_method == "mc"
+    stats = core.multilag_surrogate_stats(prep, plan, taus, bls, obs_pub if counted else None,
+                                          n_surrogates, shuffle_method, seed, rng_device)
+    if stats is None:
+        return None
+    mask = torch.zeros((n, n), dtype=torch.bool, device=dev)
+    if s_rows is not None:
+        mask[core.h2d(s_rows.astype(np.int64), dev)] = True
+    else:
+        mask.fill_(True)
+    mask.fill_diagonal_(False)
+    out = []
+    for l, tau in enumerate(taus):
+        s, s2, cg = stats[l]
+        cg.diagonal().add_(n_surrogates)   # as the single-lag default path
+        mean, std, eff, p, sig = surrogate_decision(obs_pub[l], s, s2, cg, n_surrogates,
+                                                    p_method, fdr, mask, dev, host=False)
+        edges = torch.nonzero(sig)
+        grn = make_grn(names_arr, edges.cpu().numpy().astype(np.int64),
+                       eff[edges[:, 0], edges[:, 1]].cpu().numpy())
+        out.append(SurrogateTestResult(
+            observed_te=obs_pub[l], mean_surrogate_te=mean, std_surrogate_te=std,
+            effective_te=eff, p_values=p, grn=grn, n_surrogates=n_surrogates,
+            shuffle_method=shuffle_method,
+            block_length=bls[l] if shuffle_method == "block" else None,
+            p_method=p_method, fdr=fdr))
+    return out

@@ -10,6 +10,7 @@ import torch
 
 from ._log import set_verbose, vprint
 from .inference import get_method
+from .inference.surrogate_test import surrogate_test_lags
 from .result import TransferEntropyResult
 from .transferentropy import TransferEntropyEngine
 
@@ -118,7 +119,11 @@ class Pipeline:
     def _infer_multi_lag(self, taus, combine="max", inference_method="surrogate_test", **kwargs):
         """Per-lag surrogate tests combined per pair (tenex/pipeline.py:252-409):
         'max' (largest effective TE), 'adaptive_select' (smallest p) or
-        'weighted_sum'; Bonferroni across the K lags, BH on off-diagonal."""
+        'weighted_sum'; Bonferroni across the K lags, BH on off-diagonal.
+        By default every lag shares each launch (surrogate_test_lags: the
+        observed TE of all lags in one launch, each batch of surrogates of
+        all lags in one); batched_lags=False, or options that path does not
+        cover, run the lags one by one as the reference does."""
         if not taus:
             raise ValueError("taus must be a non-empty list of time-lags")
         if combine not in ("adaptive_select", "weighted_sum", "max"):
@@ -127,14 +132,29 @@ class Pipeline:
         if inference_method != "surrogate_test":
             raise NotImplementedError(f"multi_lag combine for {inference_method!r} is not "
                                       f"supported; only 'surrogate_test' is implemented")
-        results = []
-        original_tau = self._compute_kwargs.get("tau", 1)
-        for tau in taus:
-            self.configure(tau=tau)
-            self.clear_cache()
-            vprint(f"[TENEX] multi_lag: computing tau={tau}")
-            results.append(self.infer(inference_method, **kwargs))
-        self.configure(tau=original_tau)
+        results = None
+        if kwargs.get("batched_lags", True):
+            # every lag in each launch (observed TE, then each surrogate batch)
+            te = self.te_result
+            params = {**self._defaults, **kwargs}
+            src = getattr(self.engine, "_sources", None)
+            if src is not None:
+                params.setdefault("sources", src)
+            params.pop("batched_lags", None)
+            device = params.pop("device", None)
+            vprint(f"[TENEX] multi_lag: taus={list(taus)} in shared launches")
+            results = surrogate_test_lags(te.bin_arrs, te.n_per_var, te.variable_names, taus,
+                                          device, **params)
+        if results is None:
+            kwargs.pop("batched_lags", None)
+            results = []
+            original_tau = self._compute_kwargs.get("tau", 1)
+            for tau in taus:
+                self.configure(tau=tau)
+                self.clear_cache()
+                vprint(f"[TENEX] multi_lag: computing tau={tau}")
+                results.append(self.infer(inference_method, **kwargs))
+            self.configure(tau=original_tau)
         device = torch.device("cuda:0" if torch.cuda.is_available() else "cpu")
         return combine_surrogate_multi_lag(results, taus, combine, self.engine.variable_names,
                                            device)

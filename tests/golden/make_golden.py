@@ -35,6 +35,12 @@ trace_80x700.npz       NetWeaver(result).infer("trace", device="cpu") on 80
                        thresholded network and its GRN (100 surrogate
                        entries, significance 2.0, the reference's CPU
                        hash-entropy path).
+multilag_50x500.npz    Pipeline(engine, fdr=0.05).infer("surrogate_test",
+                       taus=[1, 2, 3]) on the CPU (50 genes x 500 cells, seed
+                       33, dependencies planted at lags 1, 2 and 3; 20
+                       surrogates, seed 42, default block length per lag):
+                       the combined statistics and edges of lag_combine
+                       'max', and the edges of 'adaptive_select'.
 """
 import os
 import sys
@@ -66,6 +72,8 @@ def main(which):
         write_fdr_300(tnx)
     if "trace_80x700" in which:
         write_trace_80(tnx)
+    if "multilag_50x500" in which:
+        write_multilag_50(tnx)
     print("golden fixtures written:", ", ".join(which))
 
 
@@ -173,7 +181,35 @@ def write_trace_80(tnx):
                         significance=2.0)
 
 
-ALL = ("te_c0", "surrogate_40x400", "surrogate_120x1000", "fdr_300x600", "trace_80x700")
+def write_multilag_50(tnx):
+    x6 = zero_inflated_lognormal(50, 500, 33)
+    rng = np.random.default_rng(33)
+    g = rng.choice(50, 12, replace=False)
+    for i, lag in enumerate((1, 2, 3, 1, 2, 3)):
+        s_, t_ = g[2 * i], g[2 * i + 1]
+        x6[t_, lag:] = x6[s_, :-lag] * (0.5 + rng.random(500 - lag))
+    x6 = x6.astype(np.float32)
+    names6 = np.array([f"g{i}" for i in range(50)])
+    out = {}
+    for combine in ("max", "adaptive_select"):
+        eng = tnx.TransferEntropyEngine(x6, names6)
+        pipe = tnx.Pipeline(eng, fdr=0.05).configure(accelerator="cpu")
+        r = pipe.infer("surrogate_test", taus=[1, 2, 3], lag_combine=combine, device="cpu",
+                       n_surrogates=20, seed=42)
+        out[combine] = r
+    r = out["max"]
+    ra = out["adaptive_select"]
+    np.savez_compressed(os.path.join(HERE, "multilag_50x500.npz"), data=x6,
+                        observed=r.observed_te, mean=r.mean_surrogate_te,
+                        std=r.std_surrogate_te, effective=r.effective_te, p=r.p_values,
+                        edges=np.asarray(r.grn.pairs, np.int64),
+                        edges_adaptive=np.asarray(ra.grn.pairs, np.int64),
+                        effective_adaptive=ra.effective_te, taus=np.array([1, 2, 3]),
+                        n_surrogates=20, seed=42, fdr=0.05)
+
+
+ALL = ("te_c0", "surrogate_40x400", "surrogate_120x1000", "fdr_300x600", "trace_80x700",
+       "multilag_50x500")
 
 if __name__ == "__main__":
     # python make_golden.py [fixture ...]   (default: all of ALL)
