@@ -104,6 +104,16 @@ TNX_API int tnx_coarsen_bins(int8_t *bins, int64_t n_vars, int64_t T, int64_t ld
 /* c*log2(c) for c in [0, N] as float hi/lo pairs (tbl[2c], tbl[2c+1]) with
  * hi + lo == the FP64 value to ~2^-48 relative.  tbl: float (2*(N+1)). */
 TNX_API int tnx_xlogx_table(float *tbl, int64_t N, void *stream);
+/* Binarization into {0, 1} bins (replaces the binarizers of tenex/binarize.py:
+ * 51-200 and mean_binarize / threshold_binarize of
+ * tenex/kernels/binarize_kernel.py): per variable a threshold thr_out[g]
+ * (device float[n_vars]), then bins_out[g * ld_out + t] = x[g * ldx + t] >
+ * thr_out[g] for t < T, zero padding up to ld_out.  method 0: thr = 0
+ * ('nonzero'); 1: numpy's float32 mean ('mean'); 2: numpy's median
+ * ('median'); 3: numpy's linear quantile q in float32 ('quantile').  x:
+ * device float32 rows of stride ldx. */
+TNX_API int tnx_binarize(const float *x, int64_t n_vars, int64_t T, int64_t ldx, int32_t method,
+                         float q, int8_t *bins_out, int64_t ld_out, float *thr_out, void *stream);
 
 /* c*log2(c) for c in [0, N] as 64-bit fixed point: tbl[c] = round(c log2 c
  * * 2^shift) (the FP64 value tnx_xlogx_table splits).  Sums of entries are

@@ -100,6 +100,7 @@ SIGNATURES = {
                               C.POINTER(PlanInfo)]),
     "tnx_te_run": (C.c_int, [C.POINTER(TeArgs), p_]),
     "tnx_te_run_noalloc": (C.c_int, [C.POINTER(TeArgs), p_]),
+    "tnx_binarize": (C.c_int, [p_, i64, i64, i64, i32, C.c_float, p_, i64, p_, p_]),
     "tnx_trace_entropy": (C.c_int, [p_, i64, i64, i64, i32, i32, p_, p_, p_, i64, p_, i32, p_,
                                     p_, p_]),
     "tnx_te_queue": (C.c_int, [p_, i64, i64, p_, i64, p_, p_, i64, i32, i32, p_, i64, p_,

@@ -12,6 +12,8 @@
 #include <stdarg.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace tnx {
@@ -386,6 +388,144 @@ __global__ __launch_bounds__(256) void gene_prep_kernel(
 
 using namespace tnx;
 
+// ── binarization (tenex/binarize.py:51-200, kernels/binarize_kernel.py) ──
+// (no contraction into FMA: the thresholds round as numpy's separate ops)
+#pragma clang fp contract(off)
+// Per-variable threshold thr[g], then bins = 1[x > thr[g]]:
+//   0 nonzero  thr = 0
+//   1 mean     numpy's float32 mean (pairwise sum over the row, / T)
+//   2 median   numpy's median: the middle order statistic, or the float32
+//              mean (a + b) / 2 of the two middle ones
+//   3 quantile numpy's linear quantile in float32: v = (T - 1) q,
+//              g = v - floor(v), a = x_(floor v), b = x_(floor v + 1),
+//              g >= 0.5 ? b - (b - a)(1 - g) : a + (b - a) g
+// Order statistics by an exact radix select on order-preserving keys, one
+// workgroup per variable, four 8-bit passes over the row.
+__device__ __forceinline__ uint32_t order_key(float v) {
+    const uint32_t u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_value(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+constexpr int kBinThreads = 256;
+
+// Key of the k-th smallest element (0-based) of row[0..T).
+__device__ uint32_t select_kth(const float *__restrict__ row, int64_t T, int64_t k,
+                               uint32_t *hist, int64_t *s_k, int *s_bin) {
+    const int tid = threadIdx.x;
+    uint32_t prefix = 0, mask = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int i = tid; i < 256; i += kBinThreads) hist[i] = 0;
+        __syncthreads();
+        for (int64_t t = tid; t < T; t += kBinThreads) {
+            const uint32_t key = order_key(row[t]);
+            if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int64_t c = 0;
+            int b = 0;
+            for (; b < 255; b++) {
+                if (c + (int64_t)hist[b] > k) break;
+                c += hist[b];
+            }
+            *s_bin = b;
+            *s_k = k - c;
+        }
+        __syncthreads();
+        prefix |= (uint32_t)(*s_bin) << shift;
+        mask |= 255u << shift;
+        k = *s_k;
+        __syncthreads();
+    }
+    return prefix;
+}
+
+// Key of the (k+1)-th smallest element given the k-th smallest key kk.
+__device__ uint32_t select_next(const float *__restrict__ row, int64_t T, int64_t k, uint32_t kk,
+                                uint32_t *s_min, unsigned long long *s_le) {
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        *s_min = 0xffffffffu;
+        *s_le = 0ull;
+    }
+    __syncthreads();
+    uint32_t mn = 0xffffffffu;
+    unsigned long long le = 0;
+    for (int64_t t = tid; t < T; t += kBinThreads) {
+        const uint32_t key = order_key(row[t]);
+        if (key <= kk) le++;
+        else mn = key < mn ? key : mn;
+    }
+    atomicMin(s_min, mn);
+    atomicAdd(s_le, le);
+    __syncthreads();
+    const uint32_t r = (*s_le > (unsigned long long)(k + 1)) ? kk : *s_min;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kBinThreads) void binarize_thr_kernel(
+    const float *__restrict__ x, int64_t T, int64_t ldx, int32_t method, float q,
+    float *__restrict__ thr) {
+    __shared__ uint32_t hist[256];
+    __shared__ int64_t s_k;
+    __shared__ int s_bin;
+    __shared__ uint32_t s_min;
+    __shared__ unsigned long long s_le;
+    const int64_t g = blockIdx.x;
+    const float *row = x + g * ldx;
+    float th = 0.0f;
+    if (method == 1) {
+        if (threadIdx.x == 0) {
+            auto ident = [row](int64_t i) { return row[i]; };
+            const float sum = np_rowsum(ident, T);
+            thr[g] = (float)((double)sum / (double)T);
+        }
+        return;
+    }
+    if (method == 2) {
+        const int64_t k = (T - 1) / 2;
+        const uint32_t ka = select_kth(row, T, k, hist, &s_k, &s_bin);
+        if (T % 2 == 1) {
+            th = key_value(ka);
+        } else {
+            const uint32_t kb = select_next(row, T, k, ka, &s_min, &s_le);
+            th = (key_value(ka) + key_value(kb)) / 2.0f;
+        }
+    } else if (method == 3) {
+        const float v = (float)(T - 1) * q;
+        int64_t lo = (int64_t)floorf(v);
+        lo = lo < 0 ? 0 : (lo > T - 1 ? T - 1 : lo);
+        const float gm = v - (float)lo;
+        const uint32_t ka = select_kth(row, T, lo, hist, &s_k, &s_bin);
+        const uint32_t kb = lo + 1 < T ? select_next(row, T, lo, ka, &s_min, &s_le) : ka;
+        const float a = key_value(ka), b = key_value(kb);
+        const float d = b - a;
+        th = gm >= 0.5f ? b - d * (1.0f - gm) : a + d * gm;
+    }
+    if (threadIdx.x == 0) thr[g] = th;
+}
+
+// out[g, t] = 1[x[g, t] > thr[g]] for t < T, zero padding up to ldo.
+__global__ void binarize_apply_kernel(const float *__restrict__ x, int64_t n, int64_t T,
+                                      int64_t ldx, const float *__restrict__ thr,
+                                      int8_t *__restrict__ out, int64_t ldo) {
+    const int64_t g = blockIdx.y;
+    const float th = thr[g];
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ldo;
+         t += (int64_t)gridDim.x * blockDim.x)
+        out[g * ldo + t] = (t < T && x[g * ldx + t] > th) ? (int8_t)1 : (int8_t)0;
+}
+#pragma clang fp contract(on)
+
+__global__ void fill_f32_kernel(float *__restrict__ p, int64_t n, float v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 extern "C" {
 
 int tnx_abi_version(void) { return TNX_ABI_VERSION; }
@@ -500,6 +640,35 @@ int tnx_gene_prep(const int8_t *bins, int64_t n_vars, int64_t T, int64_t ld, int
     hipLaunchKernelGGL(gene_prep_kernel, dim3((unsigned)n_vars), dim3(256), shm, s, bins, n_vars,
                        T, ld, tau, n_per_var, states, lds, S1, S2a, feat, bcap);
     TNX_CHECK_LAUNCH("gene_prep_kernel");
+    return TNX_OK;
+}
+
+int tnx_binarize(const float *x, int64_t n_vars, int64_t T, int64_t ldx, int32_t method, float q,
+                 int8_t *bins_out, int64_t ld_out, float *thr_out, void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(x && bins_out && thr_out, "tnx_binarize: null pointer");
+    TNX_CHECK_ARG(method >= 0 && method <= 3,
+                  "tnx_binarize: method must be 0 (nonzero), 1 (mean), 2 (median) or 3 (quantile)");
+    TNX_CHECK_ARG(method != 3 || (q >= 0.0f && q <= 1.0f),
+                  "tnx_binarize: quantile q must be in [0, 1]");
+    TNX_CHECK_ARG(n_vars >= 0 && T >= 1 && ldx >= T && ld_out >= T,
+                  "tnx_binarize: need T >= 1, ldx >= T and ld_out >= T");
+    TNX_CHECK_ARG(n_vars <= 65535, "tnx_binarize: at most 65535 variables per call");
+    if (n_vars == 0) return TNX_OK;
+    hipStream_t s = as_stream(stream);
+    if (method == 0) {
+        hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)((n_vars + 255) / 256)), dim3(256), 0, s,
+                           thr_out, n_vars, 0.0f);
+        TNX_CHECK_LAUNCH("fill_f32_kernel");
+    } else {
+        hipLaunchKernelGGL(binarize_thr_kernel, dim3((unsigned)n_vars), dim3(kBinThreads), 0, s, x,
+                           T, ldx, method, q, thr_out);
+        TNX_CHECK_LAUNCH("binarize_thr_kernel");
+    }
+    const unsigned gx = (unsigned)std::min<int64_t>((ld_out + 255) / 256, 64);
+    hipLaunchKernelGGL(binarize_apply_kernel, dim3(gx, (unsigned)n_vars), dim3(256), 0, s, x,
+                       n_vars, T, ldx, thr_out, bins_out, ld_out);
+    TNX_CHECK_LAUNCH("binarize_apply_kernel");
     return TNX_OK;
 }
 

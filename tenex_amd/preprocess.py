@@ -1,4 +1,5 @@
-"""Discretisation on the GPU (FSBW-L, dense remap, coarsening).
+"""Cell alignment (host) and discretisation on the GPU (FSBW-L, dense remap,
+coarsening).
 
 discretize() restates tenex/preprocess.py:554-688 for method 'FSBW-L':
 numpy-exact statistics (the reference's bin_fsbw_l_numpy, :61-88) computed by
@@ -15,6 +16,23 @@ import torch
 from . import core
 
 BINNING_METHODS = ("fsbw-l", "default")
+
+
+def align_data(exp_data: np.ndarray, trajectory: np.ndarray, branch: np.ndarray,
+               branch_id: int = 1) -> np.ndarray:
+    """Cells of one branch in pseudotime order (tenex/preprocess.py:16-45):
+    (n_genes, n_cells) -> (n_genes, n_selected).  The order is numpy's
+    default argsort of the selected pseudotimes, as in the reference, so
+    tied pseudotimes keep the reference's arrangement."""
+    n_cells = exp_data.shape[1]
+    if len(trajectory) != n_cells or len(branch) != n_cells:
+        raise ValueError(f"Dimension mismatch: exp_data has {n_cells} cells, trajectory has "
+                         f"{len(trajectory)}, branch has {len(branch)}")
+    mask = np.asarray(branch) == branch_id
+    if mask.sum() == 0:
+        raise ValueError(f"No cells found with branch_id={branch_id}")
+    order = np.argsort(np.asarray(trajectory)[mask])
+    return np.asarray(exp_data)[:, mask][:, order]
 
 
 def discretize(arr, method: str = "fsbw-l", kp: float = 0.5, device=None,

@@ -143,6 +143,32 @@ class TransferEntropyEngine:
         self._bin_arrs_cache[key] = (bins, n_bins, b_max, npv, T)
         return self._bin_arrs_cache[key]
 
+    def _binarize(self, method, device, backend, params):
+        """{0, 1} bins of the data (tnx_binarize), cached per method, backend,
+        parameters and device: (bins (n, ld) int8, kernel levels (n,) =
+        max + 1, b_max, reported levels (n,) = max - min + 1, T)."""
+        from .binarize import binarize_gpu, _BACKENDS, _METHODS
+        if method.lower() not in _METHODS:
+            raise ValueError(f"unknown binarization '{method}'. available: {sorted(_METHODS)}")
+        if backend is not None and backend.lower() not in _BACKENDS:
+            raise ValueError(f"unknown backend '{backend}'. available: {list(_BACKENDS)}")
+        if backend is not None and backend.lower() == "cupy":
+            raise ValueError("the cupy backend is not available in this build")
+        key = ("BIN:" + method.upper(), tuple(sorted(params.items())), str(device))
+        hit = self._bin_arrs_cache.get(key)
+        if hit is not None:
+            return hit
+        x = torch.from_numpy(self._data).to(device)
+        bins, _ = binarize_gpu(x, method, params.get("q", 0.5))
+        T = self._data.shape[1]
+        rows = bins[:, :T]
+        hi = rows.amax(dim=1).to(torch.int32)
+        npv = (hi + 1).contiguous()                   # levels 0..max for the kernels
+        report = (hi - rows.amin(dim=1).to(torch.int32) + 1).contiguous()
+        b_max = int(report.max().item())
+        self._bin_arrs_cache[key] = (bins, npv, b_max, report, T)
+        return self._bin_arrs_cache[key]
+
     def _source_indices(self) -> Optional[np.ndarray]:
         if self._sources is None:
             return None
@@ -167,6 +193,9 @@ class TransferEntropyEngine:
         and the key-0 diversion on a sample of targets and runs the fastest
         (core.autotune).  coarsening=True applies the reference's
         lossy coarsening; the default keeps counts exact at any b_max <= 127.
+        binarization ('nonzero', 'mean', 'median', 'quantile' with
+        binarization_kwargs={'q': ...}) replaces binning by {0, 1} bins
+        (tnx_binarize), which route to GEMM-B2 without a source filter.
         """
         set_verbose(verbose)
         t0 = time.time()
@@ -178,10 +207,7 @@ class TransferEntropyEngine:
         if len(device_ids) > 1:
             vprint(f"[TENEX] one GPU per process: using cuda:{device_ids[0]}")
         dev = core.require_gpu(f"cuda:{device_ids[0]}")
-        if binarization is not None:
-            raise NotImplementedError("binarization is outside this build's scope; pass "
-                                      "binary data (it discretises to b_max=2 -> GEMM-B2)")
-        if binning_method.upper() not in ("FSBW-L", "DEFAULT"):
+        if binarization is None and binning_method.upper() not in ("FSBW-L", "DEFAULT"):
             raise ValueError(f"binning_method {binning_method!r} is not implemented (FSBW-L only)")
         vprint(f"[TENEX] accelerator: gpu:[{dev.index}]")
         ev = None
@@ -202,7 +228,13 @@ class TransferEntropyEngine:
             return self._wrap_result()
         if profile:
             ev[1].record(stream)
-        bins, n_bins, b_max, npv, T = self._discretize(binning_method, kp, dev, use_numpy_bins)
+        report_npv = None
+        if binarization is not None:
+            bins, npv, b_max, report_npv, T = self._binarize(
+                binarization, dev, binarization_backend, binarization_kwargs or {})
+        else:
+            bins, n_bins, b_max, npv, T = self._discretize(binning_method, kp, dev,
+                                                           use_numpy_bins)
         prep = core.prepare(bins, npv, T, int(tau))
         if profile:
             ev[2].record(stream)
@@ -212,10 +244,12 @@ class TransferEntropyEngine:
         kernel_obj = self._select_kernel(kernel, b_max, True, smem_optin, smem_bytes, npv,
                                          source_filter)
         run_info = {"n_vars": n_vars, "T": T, "K": 1, "tau": tau, "n_pairs": n_vars * (n_vars - 1),
-                    "device_ids": [dev.index], "n_gpus": 1, "binning_method": binning_method,
+                    "device_ids": [dev.index], "n_gpus": 1,
+                    "binning_method": binning_method if binarization is None else binarization,
                     "kp": kp, "use_numpy_bins": use_numpy_bins, "b_max_before_coarsen": b_max,
                     "b_max": b_max, "kernel": kernel_obj.name, "smem_optin": smem_optin,
                     "smem_bytes": smem_bytes, "source_filter": source_filter,
+                    "binarization": binarization,
                     "bin_dtype": "torch.int8", "bin_bytes": n_vars * T}
         if profile:
             ev[3].record(stream)
@@ -257,6 +291,7 @@ class TransferEntropyEngine:
                                    "assemble": ms(6, 7)}
         self._last_run_info = run_info
         self._last_prep = prep
+        self._last_report_npv = None if report_npv is None else report_npv.cpu().numpy()
         el = time.time() - t0
         vprint(f"[TENEX] done -- {el:.3f}s  ({n_vars * (n_vars - 1) / el:,.0f} pairs/s)")
         return self._wrap_result()
@@ -287,6 +322,8 @@ class TransferEntropyEngine:
         if prep is not None:
             bin_arrs = prep.bins[:, : prep.T].cpu().numpy()[:, :, None]
             n_per_var = prep.npv_host.copy()
+            if getattr(self, "_last_report_npv", None) is not None:
+                n_per_var = self._last_report_npv.copy()   # binarized: max - min + 1
         return TransferEntropyResult(matrix=self._result_matrix,
                                      variable_names=self._variable_names, bin_arrs=bin_arrs,
                                      n_per_var=n_per_var, b_max=info.get("b_max", 0),

@@ -41,6 +41,14 @@ multilag_50x500.npz    Pipeline(engine, fdr=0.05).infer("surrogate_test",
                        surrogates, seed 42, default block length per lag):
                        the combined statistics and edges of lag_combine
                        'max', and the edges of 'adaptive_select'.
+binarize_60x400.npz    engine.compute(accelerator="cpu", binarization=...) for
+                       'median', 'quantile' (q 0.8), 'mean' and 'nonzero' on
+                       60 genes x 400 cells (seed 44, a constant variable and
+                       a tied one): bins, n_per_var and the TE matrix.
+scrna_small.npz        load_scrna on the CSV / pseudotime / branch / sources
+                       files of scrna_small/ (7 genes x 40 cells, tied
+                       pseudotimes): the aligned branch-1 data with sources,
+                       the binarised branch-2 data, and the .npy cache names.
 """
 import os
 import sys
@@ -74,6 +82,10 @@ def main(which):
         write_trace_80(tnx)
     if "multilag_50x500" in which:
         write_multilag_50(tnx)
+    if "binarize_60x400" in which:
+        write_binarize_60(tnx)
+    if "scrna_small" in which:
+        write_scrna_small(tnx)
     print("golden fixtures written:", ", ".join(which))
 
 
@@ -208,8 +220,66 @@ def write_multilag_50(tnx):
                         n_surrogates=20, seed=42, fdr=0.05)
 
 
+def write_binarize_60(tnx):
+    x7 = zero_inflated_lognormal(60, 400, 44)
+    x7[5] = 1.0                                  # a constant variable
+    x7[9, ::2] = x7[9, 1::2]                     # ties around the median
+    names7 = np.array([f"g{i}" for i in range(60)])
+    out = {"data": x7}
+    for tag, method, kw in (("median", "median", None), ("q80", "quantile", {"q": 0.8}),
+                            ("mean", "mean", None), ("nonzero", "nonzero", None)):
+        r = tnx.TransferEntropyEngine(x7, names7).compute(
+            accelerator="cpu", binarization=method, binarization_kwargs=kw)
+        out[f"bins_{tag}"] = np.asarray(r.bin_arrs)[:, :, 0].astype(np.int8)
+        out[f"npv_{tag}"] = np.asarray(r.n_per_var, np.int32)
+        out[f"matrix_{tag}"] = np.asarray(r.matrix, np.float32)
+    np.savez_compressed(os.path.join(HERE, "binarize_60x400.npz"), **out)
+
+
+def write_scrna_small(tnx):
+    """A tiny CSV / pseudotime / branch set under tests/golden/scrna_small
+    and the reference's load_scrna output for it (run on a scratch copy, so
+    the reference's .npy caches do not land in the fixture folder)."""
+    import shutil
+    import tempfile
+    d = os.path.join(HERE, "scrna_small")
+    os.makedirs(d, exist_ok=True)
+    rng = np.random.default_rng(55)
+    n_genes, n_cells = 7, 40
+    expr = np.round(rng.lognormal(0, 1, (n_cells, n_genes)) * (rng.random((n_cells, n_genes)) < 0.6),
+                    3)
+    genes = [f"G{i}" for i in range(n_genes)]
+    with open(os.path.join(d, "expression.csv"), "w") as f:
+        f.write("cell," + ",".join(genes) + "\n")
+        for c in range(n_cells):
+            f.write(f"c{c}," + ",".join(f"{v:.3f}" for v in expr[c]) + "\n")
+    pt = np.round(rng.random(n_cells), 2)          # ties on purpose
+    br = rng.integers(1, 3, n_cells)
+    np.savetxt(os.path.join(d, "pseudotime.txt"), pt, fmt="%.2f")
+    np.savetxt(os.path.join(d, "branch.txt"), br, fmt="%d")
+    with open(os.path.join(d, "sources.txt"), "w") as f:
+        f.write("G1\nG4\n")
+    from tenex.io import load_scrna
+    tmp = tempfile.mkdtemp()
+    try:
+        for fn in ("expression.csv", "pseudotime.txt", "branch.txt", "sources.txt"):
+            shutil.copy(os.path.join(d, fn), tmp)
+        r = load_scrna(os.path.join(tmp, "expression.csv"), os.path.join(tmp, "pseudotime.txt"),
+                       os.path.join(tmp, "branch.txt"), branch_id=1,
+                       sources=os.path.join(tmp, "sources.txt"))
+        rb = load_scrna(os.path.join(tmp, "expression.csv"), os.path.join(tmp, "pseudotime.txt"),
+                        os.path.join(tmp, "branch.txt"), branch_id=2, make_binary=True)
+        caches = sorted(f for f in os.listdir(tmp) if f.endswith(".npy"))
+    finally:
+        shutil.rmtree(tmp)
+    np.savez_compressed(os.path.join(HERE, "scrna_small.npz"), data=r.data,
+                        gene_names=np.asarray(r.gene_names).astype("U8"),
+                        sources=np.asarray(r.sources).astype("U8"), data_b2=rb.data,
+                        caches=np.array(caches, dtype="U64"))
+
+
 ALL = ("te_c0", "surrogate_40x400", "surrogate_120x1000", "fdr_300x600", "trace_80x700",
-       "multilag_50x500")
+       "multilag_50x500", "binarize_60x400", "scrna_small")
 
 if __name__ == "__main__":
     # python make_golden.py [fixture ...]   (default: all of ALL)
