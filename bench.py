@@ -117,7 +117,7 @@ def cpu_baseline_pairs(bins_np, npv, pairs, gpu_te_internal=None):
     }, err
 
 
-TRAFFIC_FILE = "profiles/te_wave_c2_4000x10000_traffic.md"
+TRAFFIC_FILE = "profiles/te_persist_c2_4000x10000_traffic.md"
 
 
 def measured_traffic():
