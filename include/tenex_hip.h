@@ -46,7 +46,7 @@ extern "C" {
 #define TNX_API
 #endif
 
-#define TNX_ABI_VERSION 8
+#define TNX_ABI_VERSION 9
 
 /* Status codes. */
 #define TNX_OK 0
@@ -92,6 +92,19 @@ TNX_API int tnx_fsbw_l(const float *x, int64_t n_vars, int64_t T, float kp, int3
  * Replaces remap_bins + _compact_dtype (tenex/preprocess.py:401-459, :335-354). */
 TNX_API int tnx_remap_dense(const int32_t *raw, int64_t n_vars, int64_t T, int8_t *bins_out,
                     int64_t ld, int32_t *n_per_var_out, int32_t *status, void *stream);
+
+/* tnx_remap_dense into int16 rows for variables of up to 32,767 levels
+ * (the reference's int16 bins, tenex/preprocess.py:335-354): status bit 2
+ * (4) when a variable has more than 32,767 levels. */
+TNX_API int tnx_remap_dense16(const int32_t *raw, int64_t n_vars, int64_t T, int16_t *bins_out,
+                              int64_t ld, int32_t *n_per_var_out, int32_t *status, void *stream);
+
+/* tnx_coarsen_bins from int16 rows into int8 rows bins_out (same ld): the
+ * reference's coarsening of variables above the LDS bin limit applied to
+ * bins of up to 32,767 levels (tenex/preprocess.py:462-551). */
+TNX_API int tnx_coarsen_bins16(const int16_t *bins16, int64_t n_vars, int64_t T, int64_t ld,
+                               int32_t *n_per_var, int32_t limit, int8_t *bins_out,
+                               void *stream);
 
 /* In-place lossy coarsening: for variables with n_per_var > limit,
  * bin = bin * limit / n_per_var and n_per_var = limit.
@@ -249,6 +262,11 @@ typedef struct {
      * in one launch; tau is then the smallest lag and xlogx_fx the table
      * for N = T - tau. */
     const int32_t *var_L;
+    /* From ABI 9: cap_sort > 0 leaves every pair of more than cap_sort
+     * b_t^2 b_s cells unwritten by the block role (their dense histograms
+     * would scan mostly empty cells); the caller computes them with
+     * tnx_te_sort_pairs.  Must exceed cap_wave.  0: every pair here. */
+    int64_t cap_sort;
 } tnx_te_args_t;
 
 #define TNX_QUEUES 8            /* work queues of the persistent launch (one per XCD) */
@@ -277,6 +295,29 @@ TNX_API int tnx_te_queue(const tnx_work_t *work, int64_t n_wave_items, int64_t n
  * (tenex/csrc/full_smem.cu:319-381, adaptive_smem.cu:307-376,
  * gmem.cu:190-230). */
 TNX_API int tnx_te_run(const tnx_te_args_t *args, void *stream);
+
+/* Transfer entropy of explicit [target, source] pairs by sorting their
+ * occupied joint states (no dense histogram): pairs (device int32
+ * [n_pairs][2]) on int16 bins (device, rows of stride ld, levels < 32768);
+ * out[target * ld_out + source] (ld_out > 0) or out[p] (ld_out == 0) =
+ * (2^-fx_shift (S3 - S2b) + S1[target] - S2a[target]) / (T - tau), S3 and
+ * S2b exact fixed-point sums from xlogx_fx (tnx_xlogx_table_fx with N = T -
+ * tau).  Counterpart of the reference's global-memory overflow kernel
+ * te_gmem_launch (tenex/csrc/gmem.cu:190-230) for pairs whose b_t^2 b_s
+ * dense counters dwarf the series, and the only exact path for variables
+ * of more than 127 levels.  Needs T - tau <= 16384. */
+TNX_API int tnx_te_sort_pairs(const int16_t *bins16, int64_t n_vars, int64_t T, int64_t ld,
+                              int32_t tau, const int32_t *pairs, int64_t n_pairs,
+                              const double *S1, const double *S2a, const int64_t *xlogx_fx,
+                              int32_t fx_shift, float *out, int64_t ld_out, void *stream);
+
+/* S1 and S2a (as tnx_gene_prep) of the variables vars[0..n_sel) from sorted
+ * (x_t, x_{t+tau}) keys of their int16 rows: the per-variable terms of
+ * variables whose levels exceed tnx_gene_prep's bound. */
+TNX_API int tnx_gene_prep_sort(const int16_t *bins16, int64_t n_vars, int64_t T, int64_t ld,
+                               int32_t tau, const int32_t *vars, int64_t n_sel,
+                               const int64_t *xlogx_fx, int32_t fx_shift, double *S1,
+                               double *S2a, void *stream);
 
 /* TRACE OutTE / InTE for n_jobs variables (replaces fused_outte_4way_launch
  * and fused_inte_4way_launch, tenex/csrc/trace_sort_entropy.cu:191-304, and

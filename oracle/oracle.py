@@ -234,6 +234,28 @@ def te_pairs(bins, npv, pairs, tau=1, direct=False):
     return (te, td) if direct else te
 
 
+def te_pairs_sparse(bins, pairs, tau=1):
+    """FP64 TE of [target, source] pairs from the occupied joint states only
+    (np.unique counts of (x_{t+tau}, x_t, y_t), (x_t, y_t), (x_{t+tau}, x_t)
+    and x_t; S-form), for level counts whose dense b_t^2 b_s tables would not
+    fit.  Pinned against te_pairs (dense counts) in the tests."""
+    bins = np.asarray(bins)
+    if bins.ndim == 3:
+        bins = bins[:, :, 0]
+    T = bins.shape[1]
+    L = T - tau
+
+    def S(*cols):
+        c = np.unique(np.stack(cols), axis=1, return_counts=True)[1].astype(np.float64)
+        c = c[c > 1]
+        return float(np.sum(c * np.log2(c)))
+    out = np.empty(len(pairs))
+    for i, (t, s_) in enumerate(np.asarray(pairs)):
+        xf, xp, yp = bins[t, tau:], bins[t, :L], bins[s_, :L]
+        out[i] = (S(xf, xp, yp) + S(xp) - S(xf, xp) - S(xp, yp)) / L
+    return out
+
+
 def te_matrix(bins, npv, tau=1):
     """FP64 public-layout matrix M[i, j] = TE(i -> j), diagonal 0."""
     n = bins.shape[0]

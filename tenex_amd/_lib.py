@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libtenex_hip.so")
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 TNX_OUT_PAIRFREE = 0
 TNX_OUT_MATRIX = 1
@@ -55,7 +55,8 @@ class TeArgs(C.Structure):
                 ("item_per_wave", C.c_int32),
                 ("queue_items", p_), ("queue_off", p_), ("queue_ctr", p_),
                 ("counts_map", p_), ("counts_dbg", p_), ("counts_ld", C.c_int64),
-                ("xlogx_fx", p_), ("fx_shift", C.c_int32), ("var_L", p_)]
+                ("xlogx_fx", p_), ("fx_shift", C.c_int32), ("var_L", p_),
+                ("cap_sort", i64)]
 
 
 TNX_QUEUES = 8
@@ -100,6 +101,11 @@ SIGNATURES = {
                               C.POINTER(PlanInfo)]),
     "tnx_te_run": (C.c_int, [C.POINTER(TeArgs), p_]),
     "tnx_te_run_noalloc": (C.c_int, [C.POINTER(TeArgs), p_]),
+    "tnx_remap_dense16": (C.c_int, [p_, i64, i64, p_, i64, p_, p_, p_]),
+    "tnx_coarsen_bins16": (C.c_int, [p_, i64, i64, i64, p_, i32, p_, p_]),
+    "tnx_te_sort_pairs": (C.c_int, [p_, i64, i64, i64, i32, p_, i64, p_, p_, p_, i32, p_, i64,
+                                    p_]),
+    "tnx_gene_prep_sort": (C.c_int, [p_, i64, i64, i64, i32, p_, i64, p_, i32, p_, p_, p_]),
     "tnx_binarize": (C.c_int, [p_, i64, i64, i64, i32, C.c_float, p_, i64, p_, p_]),
     "tnx_trace_entropy": (C.c_int, [p_, i64, i64, i64, i32, i32, p_, p_, p_, i64, p_, i32, p_,
                                     p_, p_]),

@@ -88,6 +88,64 @@ def remap_gpu(raw: torch.Tensor):
     return bins, npv, b_max
 
 
+def remap_any_gpu(raw: torch.Tensor):
+    """Dense relabelling into int8 rows, or, when a variable has more than
+    127 levels, into int16 rows as well (the reference's int16 bins).
+    Returns (bins int8 (n, ld), bins16 int16 (n, ld) or None, n_per_var,
+    b_max)."""
+    try:
+        bins, npv, b_max = remap_gpu(raw)
+        return bins, None, npv, b_max
+    except ValueError as e:
+        if "more than 127" not in str(e):
+            raise
+    b16, b8, npv, b_max = remap16_gpu(raw)
+    return b8, b16, npv, b_max
+
+
+def upload_bins_any(bin_arr, device):
+    """upload_bins for dense bins of up to 32,767 levels: returns (bins int8,
+    bins16 int16 or None (all levels <= 127), npv, b_max, T)."""
+    t = torch.as_tensor(np.asarray(bin_arr) if not isinstance(bin_arr, torch.Tensor) else bin_arr)
+    if t.dim() == 3:
+        if t.shape[2] != 1:
+            raise ValueError(f"only K=1 bin arrays are supported (got K={t.shape[2]})")
+        t = t[:, :, 0]
+    if t.numel() and int(t.max().item()) > MAX_LEVELS_INT8 - 1:
+        if t.is_floating_point() or int(t.min().item()) < 0 or \
+                int(t.max().item()) > MAX_LEVELS_INT16 - 1:
+            raise ValueError(f"bins must satisfy 0 <= bin < n_per_var <= {MAX_LEVELS_INT16}")
+        n, T = t.shape
+        ld = round_up(T, ROW_ALIGN)
+        b16 = torch.zeros((n, ld), dtype=torch.int16, device=device)
+        b16[:, :T] = t.to(device=device, dtype=torch.int16)
+        b8 = b16.clamp(max=MAX_LEVELS_INT8).to(torch.int8)
+        npv = (b16[:, :T].to(torch.int32).amax(dim=1) + 1).to(torch.int32).contiguous()
+        return b8, b16, npv, int(npv.max().item()), T
+    bins, npv, b_max, T = upload_bins(t, device)
+    return bins, None, npv, b_max, T
+
+
+def remap16_gpu(raw: torch.Tensor):
+    """Dense relabelling into padded int16 rows (variables of up to 32,767
+    levels).  Returns (bins16 int16 (n, ld), bins int8 (n, ld) with levels
+    above 127 clamped, n_per_var, b_max)."""
+    n, T = raw.shape
+    ld = round_up(T, ROW_ALIGN)
+    b16 = torch.empty((n, ld), dtype=torch.int16, device=raw.device)
+    npv = torch.empty(n, dtype=torch.int32, device=raw.device)
+    status = torch.zeros(1, dtype=torch.int32, device=raw.device)
+    call("tnx_remap_dense16", ptr(raw.contiguous()), n, T, ptr(b16), ld, ptr(npv), ptr(status),
+         _stream(raw.device))
+    st = int(status.item())
+    if st & 1:
+        raise ValueError("remap: a raw bin index is negative or >= 65536")
+    if st & 4:
+        raise ValueError(f"remap: a variable has more than {MAX_LEVELS_INT16} distinct levels")
+    b8 = b16.clamp(max=MAX_LEVELS_INT8).to(torch.int8)
+    return b16, b8, npv, int(npv.max().item()) if n else 0
+
+
 def upload_bins(bin_arr, device) -> tuple[torch.Tensor, torch.Tensor, int, int]:
     """Dense bins (n, T) or (n, T, 1) from the host or device -> padded int8
     device rows.  Bins must already be dense per variable (0..b-1 with every
@@ -169,6 +227,12 @@ class Prepared:
     xlogx: torch.Tensor
     plans: dict = field(default_factory=dict)
     hot_zero: bool = False   # key 0 frequent enough to divert (see key0_fraction)
+    bins16: torch.Tensor | None = None   # int16 (n, ld) when some variable has > 127 levels
+
+    @property
+    def wide(self) -> np.ndarray:
+        """Indices of the variables with more than 127 levels."""
+        return np.nonzero(self.npv_host > MAX_LEVELS_INT8)[0]
 
     @property
     def n_vars(self) -> int:
@@ -191,7 +255,17 @@ class Prepared:
         return self.bins.device
 
 
-def prepare(bins: torch.Tensor, npv: torch.Tensor, T: int, tau: int) -> Prepared:
+MAX_LEVELS_INT8 = 127     # levels an int8 bin row holds (the histogram kernels' bound)
+MAX_LEVELS_INT16 = 32767  # levels of int16 rows (the occupied-key kernels)
+
+
+def prepare(bins: torch.Tensor, npv: torch.Tensor, T: int, tau: int,
+            bins16: torch.Tensor | None = None) -> Prepared:
+    """Per-variable preparation (tnx_gene_prep).  With variables of more than
+    127 levels, bins16 (int16 rows, same ld) holds every variable's levels
+    (taken from bins when absent): their S1 / S2a come from
+    tnx_gene_prep_sort and their pairs run on the occupied-key kernel (the
+    int8 rows of those variables are not read)."""
     n, ld = bins.shape
     if not (1 <= tau < T):
         raise ValueError(f"need 1 <= tau < T (tau={tau}, T={T})")
@@ -203,12 +277,54 @@ def prepare(bins: torch.Tensor, npv: torch.Tensor, T: int, tau: int) -> Prepared
     S2a = torch.empty(n, dtype=torch.float64, device=dev)
     feat = torch.empty((n, 8), dtype=torch.float32, device=dev)
     npv_host = npv.cpu().numpy().astype(np.int32)
+    wide = np.nonzero(npv_host > MAX_LEVELS_INT8)[0]
+    if len(wide) and bins16 is None:
+        bins16 = bins.to(torch.int16)   # level counts above the values present
     call("tnx_gene_prep", ptr(bins), n, T, ld, int(tau), ptr(npv), ptr(states), lds, ptr(S1),
-         ptr(S2a), ptr(feat), int(npv_host.max(initial=1)), _stream(dev))
+         ptr(S2a), ptr(feat), int(min(npv_host.max(initial=1), MAX_LEVELS_INT8)), _stream(dev))
+    if len(wide):
+        if L > TRACE_MAX_L:
+            raise NotImplementedError(f"variables of more than {MAX_LEVELS_INT8} levels need "
+                                      f"T - tau <= {TRACE_MAX_L} (got {L})")
+        wd = h2d(wide.astype(np.int64), dev)
+        states.index_fill_(0, wd, 0)
+        feat.index_fill_(0, wd, 0.0)
+        # device arrays held in locals until the launch is queued: a freed
+        # temporary's memory could go to the table allocated in between
+        tbl = xlogx_fx_table(L, dev)
+        wide_d = h2d(wide.astype(np.int32), dev)
+        call("tnx_gene_prep_sort", ptr(bins16), n, T, ld, int(tau), ptr(wide_d), len(wide),
+             ptr(tbl), fx_shift(L), ptr(S1), ptr(S2a), _stream(dev))
     return Prepared(bins=bins, npv=npv, npv_host=npv_host, T=T,
                     tau=tau, states=states, S1=S1, S2a=S2a, feat=feat,
                     xlogx=xlogx_table(L, dev),
-                    hot_zero=key0_fraction(states, bins, L) >= HOT_ZERO_FRACTION)
+                    hot_zero=key0_fraction(states, bins, L) >= HOT_ZERO_FRACTION,
+                    bins16=bins16)
+
+
+def require_narrow(prep: Prepared, what: str):
+    """Raise for paths that read int8 rows only when prep has wide variables."""
+    if len(prep.wide):
+        raise NotImplementedError(
+            f"{what} takes variables of at most {MAX_LEVELS_INT8} levels (this data has "
+            f"{len(prep.wide)} above); use coarsening=True or a smaller kp")
+
+
+def te_sort_pairs(prep: Prepared, pairs: np.ndarray, out: torch.Tensor, ld_out: int):
+    """TE of [target, source] pairs (host int, target != source) by the
+    occupied-key kernel into out[target * ld_out + source] (ld_out > 0) or
+    out[p] (ld_out == 0)."""
+    pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+    if len(pairs) == 0:
+        return
+    if prep.L > TRACE_MAX_L:
+        raise NotImplementedError(f"the occupied-key kernel takes T - tau <= {TRACE_MAX_L}")
+    b16 = prep.bins16 if prep.bins16 is not None else prep.bins.to(torch.int16)
+    tbl = xlogx_fx_table(prep.L, prep.device)
+    pairs_d = h2d(pairs, prep.device)      # held until the launch is queued
+    call("tnx_te_sort_pairs", ptr(b16), prep.n_vars, prep.T, prep.ld, prep.tau, ptr(pairs_d),
+         len(pairs), ptr(prep.S1), ptr(prep.S2a), ptr(tbl), fx_shift(prep.L), ptr(out),
+         int(ld_out), _stream(prep.device))
 
 
 # A histogram key held by a fraction p of the samples puts about 64 p lanes
@@ -437,7 +553,16 @@ def get_plan(prep: Prepared, targets=None, sources=None, cap_wave: int = 0,
            _digest(None if targets is None else np.asarray(targets, np.int32)),
            _digest(None if sources is None else np.asarray(sources, np.int32)),
            int(cap_wave), int(cap_block))
-    return _cached_plan(key, lambda: make_plan(prep.npv_host, prep.L, prep.device, targets,
+    # variables above 127 levels never enter a histogram plan (their pairs run
+    # on the occupied-key kernel); the planner sees them clamped
+    npv = np.minimum(prep.npv_host, MAX_LEVELS_INT8) if len(prep.wide) else prep.npv_host
+    if len(prep.wide):
+        isw = prep.npv_host > MAX_LEVELS_INT8
+        for lst in (targets, sources):
+            if lst is None or isw[np.asarray(lst, dtype=np.int64)].any():
+                raise ValueError("a histogram plan cannot take variables of more than "
+                                 f"{MAX_LEVELS_INT8} levels")
+    return _cached_plan(key, lambda: make_plan(npv, prep.L, prep.device, targets,
                                                sources, cap_wave, cap_block))
 
 
@@ -497,7 +622,8 @@ def plan_queue(plan: Plan, npv_host: np.ndarray, L: int, item_per_wave: bool):
 def te_args(prep: Prepared, plan: Plan, out: torch.Tensor | None, out_mode: int, ld_out: int = 0,
             pair_offset: int = 0, n_pairs_local: int = 0, stats=None, stage_vars: int = 0,
             stage_stride: int = 0, item_per_wave: bool | None = None,
-            persistent: bool | None = None, counts=None, var_L=None) -> _lib.TeArgs:
+            persistent: bool | None = None, counts=None, var_L=None,
+            cap_sort: int = 0) -> _lib.TeArgs:
     """stats: (sum, sumsq, count_ge, observed) public-layout tensors for
     TNX_OUT_STATS; stage_vars / stage_stride for TNX_OUT_STAGE.  item_per_wave None: chosen from the
     plan (item_per_wave_default).  persistent None: one persistent launch
@@ -531,7 +657,7 @@ def te_args(prep: Prepared, plan: Plan, out: torch.Tensor | None, out_mode: int,
         queue_ctr=ptr(q[2]), counts_map=ptr(cm), counts_dbg=ptr(cd),
         counts_ld=int(cd.shape[1]) if cd is not None else 0,
         xlogx_fx=ptr(xlogx_fx_table(prep.L, prep.device)), fx_shift=fx_shift(prep.L),
-        var_L=ptr(var_L))
+        var_L=ptr(var_L), cap_sort=int(cap_sort))
 
 
 WAVE_ITEM_PAIRS = 32   # pairs of a one-target item that a single wave takes whole
@@ -633,14 +759,109 @@ def capture_te_step(prep: Prepared, plan: Plan, out: torch.Tensor, out_mode: int
     return g
 
 
+# Per-pair cost of the occupied-key kernel in microseconds of one CU by sort
+# size (measured on MI355X: 39,800 pairs per launch), and of a block-role
+# pair (block_pair_cost's units, te_persist.hip, at 1.04e-4 us per unit).
+SORT_US = {1024: 4.6, 2048: 9.5, 4096: 20.0, 8192: 55.0, 16384: 134.0}
+DENSE_US_PER_UNIT = 1.04e-4
+
+
+_SORT_THRESHOLDS: dict = {}
+
+
+def sort_threshold(L: int, cap_wave: int = 0) -> int:
+    """Smallest b_t^2 b_s above which the occupied-key kernel is expected to
+    beat the block role's dense passes by 1.5x for series of L samples (0:
+    never, including every series the LDS sort cannot hold).  Cached."""
+    cap_wave = int(cap_wave) or WAVE_CAP
+    key = (int(L), cap_wave)
+    if key not in _SORT_THRESHOLDS:
+        _SORT_THRESHOLDS[key] = _sort_threshold(int(L), cap_wave)
+    return _SORT_THRESHOLDS[key]
+
+
+def _sort_threshold(L: int, cap_wave: int) -> int:
+    if L > TRACE_MAX_L:
+        return 0
+    P = max(1024, 1 << (max(1, int(L)) - 1).bit_length())
+    sort_us = SORT_US[P]
+    cap_b = 4 * _wave_slice(cap_wave) - 2 * 64 - 128
+    best = 0
+    for bx in range(2, MAX_LEVELS_INT8 + 1):
+        for by in range(2, MAX_LEVELS_INT8 + 1):
+            cells = bx * bx * by
+            if cells <= cap_wave:
+                continue
+            ncol = bx * by
+            per = max(1, cap_b // ncol)
+            passes = -(-bx // per)
+            dense = (L * (1.8 + 0.35 * passes) + 2.0 * cells + 6000.0) * DENSE_US_PER_UNIT
+            if dense > 1.5 * sort_us and (best == 0 or cells < best):
+                best = cells
+    return max(best - 1, cap_wave + 1) if best else 0
+
+
+def _sort_split(prep: Prepared, tg: np.ndarray, sr: np.ndarray, cap: int) -> np.ndarray:
+    """[target, source] pairs of tg x sr (target != source) with more than
+    cap histogram cells."""
+    npv = prep.npv_host.astype(np.int64)
+    order = np.argsort(npv[sr], kind="stable")
+    s_sorted = sr[order]
+    b_sorted = npv[s_sorted]
+    out = []
+    for t in tg:
+        lim = cap // (npv[t] * npv[t])              # sources with by > lim exceed cap
+        k = np.searchsorted(b_sorted, lim, side="right")
+        if k < len(s_sorted):
+            s = s_sorted[k:]
+            s = s[s != t]
+            if len(s):
+                out.append(np.stack([np.full(len(s), t), s], 1))
+    return np.concatenate(out).astype(np.int64) if out else np.zeros((0, 2), np.int64)
+
+
 def te_matrix_internal(prep: Prepared, targets=None, sources=None, cap_wave: int = 0,
                        cap_block: int = 0) -> torch.Tensor:
     """Internal-layout TE matrix M[target, source] = TE(source -> target) on the
     device (zeros where not computed)."""
     n = prep.n_vars
-    plan = get_plan(prep, targets, sources, cap_wave, cap_block)
     out = torch.zeros((n, n), dtype=torch.float32, device=prep.device)
-    te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n)
+    if len(prep.wide):
+        # pairs touching a variable of more than 127 levels: occupied keys;
+        # the rest: the histogram kernels
+        isw = np.zeros(n, dtype=bool)
+        isw[prep.wide] = True
+        tg = np.arange(n) if targets is None else np.asarray(targets, dtype=np.int64)
+        sr = np.arange(n) if sources is None else np.asarray(sources, dtype=np.int64)
+        tn, sn = tg[~isw[tg]], sr[~isw[sr]]
+        if len(tn) and len(sn):
+            plan = get_plan(prep, tn.astype(np.int32), sn.astype(np.int32), cap_wave, cap_block)
+            te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n)
+        tw, sw = tg[isw[tg]], sr[isw[sr]]
+        parts = [np.stack(np.meshgrid(tw, sr, indexing="ij"), -1).reshape(-1, 2),
+                 np.stack(np.meshgrid(tn, sw, indexing="ij"), -1).reshape(-1, 2)]
+        wp = np.concatenate(parts)
+        te_sort_pairs(prep, wp[wp[:, 0] != wp[:, 1]], out, n)
+        return out
+    plan = get_plan(prep, targets, sources, cap_wave, cap_block)
+    cap = sort_threshold(prep.L, int(plan.info.cap_wave)) if plan.info.n_block_items else 0
+    if cap:
+        tg = np.arange(n) if targets is None else np.asarray(targets, dtype=np.int64)
+        sr = np.arange(n) if sources is None else np.asarray(sources, dtype=np.int64)
+        npv = prep.npv_host.astype(np.int64)
+        if npv[tg].max(initial=0) ** 2 * npv[sr].max(initial=0) <= cap:
+            cap = 0                         # no pair of the grid is that large
+        else:
+            sp = _sort_split(prep, tg, sr, cap)
+            if len(sp) == 0:
+                cap = 0
+    if not cap:
+        te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n)
+        return out
+    # pairs whose dense histograms dwarf the series: occupied keys
+    args = te_args(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n, cap_sort=cap)
+    te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, args=args)
+    te_sort_pairs(prep, sp, out, n)
     return out
 
 
@@ -656,6 +877,15 @@ def te_pair_list(prep: Prepared, pairs: np.ndarray, cap_wave: int = 0) -> torch.
     if P and (p.min() < 0 or p.max() >= prep.n_vars):
         raise ValueError(f"pair indices outside [0, {prep.n_vars})")
     valid = p[:, 0] != p[:, 1]
+    if len(prep.wide):
+        isw = np.zeros(prep.n_vars, dtype=bool)
+        isw[prep.wide] = True
+        wmask = valid & (isw[p[:, 0]] | isw[p[:, 1]])
+        if wmask.any():
+            tmp = torch.empty(int(wmask.sum()), dtype=torch.float32, device=prep.device)
+            te_sort_pairs(prep, p[wmask], tmp, 0)
+            out[h2d(np.nonzero(wmask)[0], prep.device)] = tmp
+        valid &= ~wmask
     if not valid.any():
         return out
     pv = p[valid]
@@ -715,6 +945,7 @@ def trace_te(bins: torch.Tensor, T: int, tau: int, mode: int, var_ids, conn_list
 def te_pairfree(prep: Prepared, pair_offset: int = 0, n_pairs_local: int | None = None,
                 cap_wave: int = 0, cap_block: int = 0) -> torch.Tensor:
     """TE in pair-free linear order for ids [pair_offset, pair_offset + n_local)."""
+    require_narrow(prep, "the pair-free launch")
     n = prep.n_vars
     total = n * (n - 1)
     if n_pairs_local is None:
@@ -738,6 +969,7 @@ def counts_plan(prep: Prepared, plan: Plan, pairs: np.ndarray, max_cells: int):
     launch of `plan` counts them (the persistent kernel's count dump): int32
     (P, max_cells) in layout (a * b_t + b) * b_s + c, and the launch's TE
     matrix (internal layout)."""
+    require_narrow(prep, "the joint-count dump")
     pairs = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
     n, dev = prep.n_vars, prep.device
     cells = prep.npv_host[pairs[:, 0]].astype(np.int64) ** 2 * prep.npv_host[pairs[:, 1]]
@@ -1270,6 +1502,7 @@ def surrogate_stats_pairs(prep: Prepared, pairs: np.ndarray, observed: torch.Ten
 
 
 def surrogate_plan(prep: Prepared, targets=None, sources=None) -> Plan:
+    require_narrow(prep, "the surrogate test")
     cap = surrogate_cap(prep.npv_host, targets, sources)
     if cap > SURR_MAX_CELLS:
         raise NotImplementedError(

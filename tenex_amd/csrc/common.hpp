@@ -122,8 +122,20 @@ constexpr int kSmallTbl = 256;
 __device__ __forceinline__ void xlogx_add_l(DF &acc, const float2 *ltbl,
                                             const float2 *__restrict__ tbl, uint32_t c) {
     if (c > 1) {
-        const float2 t = c < (uint32_t)kSmallTbl ? ltbl[c] : tbl[c];
-        df_add(acc, t.x, t.y);
+        // the LDS copy through an LDS-typed pointer: a select between the two
+        // tables' pointers would become one flat load
+        typedef const float __attribute__((address_space(3))) *lds_f;
+        float hi, lo;
+        if (c < (uint32_t)kSmallTbl) {
+            const lds_f lp = (lds_f)(const void *)ltbl;
+            hi = lp[2 * c];
+            lo = lp[2 * c + 1];
+        } else {
+            const float2 t = tbl[c];
+            hi = t.x;
+            lo = t.y;
+        }
+        df_add(acc, hi, lo);
     }
 }
 

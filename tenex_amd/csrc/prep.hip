@@ -160,11 +160,15 @@ __global__ void fsbw_bins_kernel(const float *__restrict__ x, int64_t n_vars, in
 constexpr int kRemapBits = 65536;
 constexpr int kRemapWords = kRemapBits / 32;
 
+// OUT = int8_t: levels above 127 set status bit 2 (values clamped);
+// OUT = int16_t: levels above 32767 set status bit 4.
+template <typename OUT>
 __global__ __launch_bounds__(256) void remap_kernel(const int32_t *__restrict__ raw,
                                                     int64_t n_vars, int64_t T,
-                                                    int8_t *__restrict__ out, int64_t ld,
+                                                    OUT *__restrict__ out, int64_t ld,
                                                     int32_t *__restrict__ npv,
                                                     int32_t *__restrict__ status) {
+    constexpr int32_t kMaxLevel = sizeof(OUT) == 1 ? 127 : 32767;
     __shared__ uint32_t bm[kRemapWords];
     __shared__ uint32_t base[kRemapWords];
     __shared__ uint32_t part[256];
@@ -205,9 +209,9 @@ __global__ __launch_bounds__(256) void remap_kernel(const int32_t *__restrict__ 
     const uint32_t total = part[255];
     if (tid == 0) {
         npv[g] = (int32_t)total;
-        if (total > 127) atomicOr(status, 2);
+        if (total > (uint32_t)kMaxLevel) atomicOr(status, sizeof(OUT) == 1 ? 2 : 4);
     }
-    int8_t *o = out + g * ld;
+    OUT *o = out + g * ld;
     for (int64_t t = tid; t < ld; t += 256) {
         int32_t d = 0;
         if (t < T) {
@@ -217,7 +221,7 @@ __global__ __launch_bounds__(256) void remap_kernel(const int32_t *__restrict__ 
                 d = (int32_t)(base[w] + __popc(bm[w] & ((1u << b) - 1u)));
             }
         }
-        o[t] = (int8_t)(d > 127 ? 127 : d);
+        o[t] = (OUT)(d > kMaxLevel ? kMaxLevel : d);
     }
 }
 
@@ -230,6 +234,20 @@ __global__ void coarsen_kernel(int8_t *__restrict__ bins, int64_t n_vars, int64_
          t += (int64_t)gridDim.x * blockDim.x) {
         int32_t v = bins[g * ld + t];
         bins[g * ld + t] = (int8_t)((v * limit) / ng);
+    }
+}
+
+// int16 rows -> int8 rows: new = old * limit / ng where ng > limit, else
+// old (every variable then has at most max(limit, 127) levels).
+__global__ void coarsen16_kernel(const int16_t *__restrict__ in, int64_t T, int64_t ld,
+                                 const int32_t *__restrict__ npv, int32_t limit,
+                                 int8_t *__restrict__ out) {
+    const int64_t g = blockIdx.y;
+    const int32_t ng = npv[g];
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ld;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = t < T ? (int32_t)in[g * ld + t] : 0;
+        out[g * ld + t] = (int8_t)(ng > limit ? (v * limit) / ng : v);
     }
 }
 
@@ -571,9 +589,39 @@ int tnx_remap_dense(const int32_t *raw, int64_t n_vars, int64_t T, int8_t *bins_
     TNX_CHECK_ARG(raw && bins_out && n_per_var_out && status, "tnx_remap_dense: null pointer");
     TNX_CHECK_ARG(ld >= T && T >= 1, "tnx_remap_dense: need ld >= T >= 1");
     if (n_vars == 0) return TNX_OK;
-    hipLaunchKernelGGL(remap_kernel, dim3((unsigned)n_vars), dim3(256), 0, as_stream(stream), raw,
-                       n_vars, T, bins_out, ld, n_per_var_out, status);
+    hipLaunchKernelGGL(remap_kernel<int8_t>, dim3((unsigned)n_vars), dim3(256), 0,
+                       as_stream(stream), raw, n_vars, T, bins_out, ld, n_per_var_out, status);
     TNX_CHECK_LAUNCH("remap_kernel");
+    return TNX_OK;
+}
+
+int tnx_remap_dense16(const int32_t *raw, int64_t n_vars, int64_t T, int16_t *bins_out,
+                      int64_t ld, int32_t *n_per_var_out, int32_t *status, void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(raw && bins_out && n_per_var_out && status, "tnx_remap_dense16: null pointer");
+    TNX_CHECK_ARG(ld >= T && T >= 1, "tnx_remap_dense16: need ld >= T >= 1");
+    if (n_vars == 0) return TNX_OK;
+    hipLaunchKernelGGL(remap_kernel<int16_t>, dim3((unsigned)n_vars), dim3(256), 0,
+                       as_stream(stream), raw, n_vars, T, bins_out, ld, n_per_var_out, status);
+    TNX_CHECK_LAUNCH("remap_kernel");
+    return TNX_OK;
+}
+
+int tnx_coarsen_bins16(const int16_t *bins16, int64_t n_vars, int64_t T, int64_t ld,
+                       int32_t *n_per_var, int32_t limit, int8_t *bins_out, void *stream) {
+    clear_error();
+    TNX_CHECK_ARG(bins16 && n_per_var && bins_out, "tnx_coarsen_bins16: null pointer");
+    TNX_CHECK_ARG(limit >= 2 && limit <= 127, "tnx_coarsen_bins16: limit must be in [2,127]");
+    TNX_CHECK_ARG(ld >= T && n_vars <= 65535, "tnx_coarsen_bins16: need ld >= T, n <= 65535");
+    if (n_vars == 0) return TNX_OK;
+    hipStream_t s = as_stream(stream);
+    const unsigned gx = (unsigned)std::min<int64_t>((ld + 255) / 256, 64);
+    hipLaunchKernelGGL(coarsen16_kernel, dim3(gx, (unsigned)n_vars), dim3(256), 0, s, bins16, T,
+                       ld, n_per_var, limit, bins_out);
+    TNX_CHECK_LAUNCH("coarsen16_kernel");
+    hipLaunchKernelGGL(coarsen_npv_kernel, dim3((unsigned)((n_vars + 255) / 256)), dim3(256), 0, s,
+                       n_per_var, n_vars, limit);
+    TNX_CHECK_LAUNCH("coarsen_npv_kernel");
     return TNX_OK;
 }
 

@@ -32,6 +32,11 @@ __host__ __device__ inline int wave_slice(int cap) { return (cap + kWave + 3) & 
 // float32 rounding added to the FP64 sum and sum of squares and the count of
 // values >= the observed TE (float32 compare), as the reference accumulates
 // its per-surrogate float32 TE matrices.
+// Whether a pair of b_x^2 b_y cells is left to the occupied-key kernel.
+__device__ __forceinline__ bool sort_pair(const tnx_te_args_t &a, int bx, int by) {
+    return a.cap_sort > 0 && (int64_t)bx * bx * by > a.cap_sort;
+}
+
 __device__ __forceinline__ void emit_te(const tnx_te_args_t &a, int64_t slot, double te) {
     if (a.out_mode == TNX_OUT_STATS) {
         const float tf = (float)te;

@@ -225,7 +225,8 @@ __global__ __launch_bounds__(kBlockThreads) void te_block_kernel(tnx_te_args_t a
                 }
                 if constexpr (!kPipe) load_block_row<NB>(ycur, a.bins + (int64_t)y * a.ld, L);
                 int64_t slot;
-                const bool want = (x != y) && out_slot(a, x, y, slot);
+                // pairs above cap_sort are left to the occupied-key kernel
+                const bool want = (x != y) && !sort_pair(a, bx, by) && out_slot(a, x, y, slot);
                 const bool fits = bx * by <= a.max_block_cols;
                 if (want && !fits && tid == 0) emit_te(a, slot, __builtin_nan(""));  // plan mismatch
                 if (want && fits) {
@@ -519,6 +520,8 @@ int tnx_te_run(const tnx_te_args_t *args, void *stream) {
                   "tnx_te_run: TNX_OUT_STAGE needs stage_vars >= 1, ld_out >= stage_vars and "
                   "stage_stride >= stage_vars * ld_out");
     TNX_CHECK_ARG(a.cap_wave >= 8 && a.cap_wave <= 8192, "tnx_te_run: cap_wave out of range");
+    TNX_CHECK_ARG(a.cap_sort == 0 || a.cap_sort > a.cap_wave,
+                  "tnx_te_run: cap_sort must be 0 or exceed cap_wave");
     TNX_CHECK_ARG((size_t)kWavesPerWG * wave_slice(a.cap_wave) * 4 <= 160 * 1024,
                   "tnx_te_run: cap_wave too large");
     hipStream_t s = as_stream(stream);

@@ -585,7 +585,8 @@ __device__ __forceinline__ void persist_block_item(const tnx_te_args_t &a, const
                 load_row256<NB>(ynext, a.bins + (int64_t)yn * a.ld, Lx);
             }
             int64_t slot = 0;
-            const bool want = (x != y) && out_slot(a, x, y, slot);
+            // pairs above cap_sort are left to the occupied-key kernel
+            const bool want = (x != y) && !sort_pair(a, bx, by) && out_slot(a, x, y, slot);
             const bool fits = bx * by <= a.max_block_cols;
             if (want && fits) {
                 int32_t *cdbg = nullptr;

@@ -25,14 +25,15 @@
 // fused_outte_4way_launch / fused_inte_4way_launch
 // (tenex/csrc/trace_sort_entropy.cu:191-304).
 #include "common.hpp"
+#include "sort_lds.hpp"
 #include "te_device.hpp"
 
 namespace tnx {
 
 constexpr uint64_t kTraceHashP = 1000000007ull;
-constexpr int kTraceE = 16;              // keys per thread
-constexpr int kTraceMinP = 1024;         // sort sizes: powers of two in [1024, 16384]
-constexpr int kTraceMaxP = 16384;
+constexpr int kTraceE = kSortE;          // keys per thread
+constexpr int kTraceMinP = kSortMinP;
+constexpr int kTraceMaxP = kSortMaxP;
 
 // Bytes [sh, sh + 16) of the 32-byte window (a, b); sh uniform in [0, 16).
 __device__ __forceinline__ void window16(const uint4 &a, const uint4 &b, int sh, uint32_t (&w)[4]) {
@@ -83,66 +84,6 @@ __device__ __forceinline__ void hash_single(uint64_t (&h)[kTraceE], const int8_t
 #pragma unroll
     for (int e = 0; e < kTraceE; ++e) h[e] = 0;
     hash_row(h, bins, ld, n_vars, var, base, off);
-}
-
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
-    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m);
-    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// Ascending bitonic sort of P = 16 * nt keys, thread tid holding elements
-// 16 tid .. 16 tid + 15 in r; lds holds P keys (element i of thread t at
-// lds[(i % 16) * nt + t]).
-__device__ void bitonic_sort(uint64_t (&r)[kTraceE], uint64_t *lds, int P, int nt, int tid) {
-    const int base = tid * kTraceE;
-    for (int k = 2; k <= P; k <<= 1) {
-        // partners in other threads: j >= 16
-        for (int j = k >> 1; j >= kTraceE; j >>= 1) {
-            const int m = j / kTraceE;             // partner thread = tid ^ m
-            const bool keep_min = (((tid & m) == 0) == ((base & k) == 0));
-            if (m < kWave) {
-#pragma unroll
-                for (int e = 0; e < kTraceE; ++e) {
-                    const uint64_t o = shfl_xor_u64(r[e], m);
-                    r[e] = keep_min ? (o < r[e] ? o : r[e]) : (o > r[e] ? o : r[e]);
-                }
-            } else {
-                __syncthreads();
-#pragma unroll
-                for (int e = 0; e < kTraceE; ++e) lds[e * nt + tid] = r[e];
-                __syncthreads();
-#pragma unroll
-                for (int e = 0; e < kTraceE; ++e) {
-                    const uint64_t o = lds[e * nt + (tid ^ m)];
-                    r[e] = keep_min ? (o < r[e] ? o : r[e]) : (o > r[e] ? o : r[e]);
-                }
-            }
-        }
-        // partners in the same thread: j < 16
-#pragma unroll
-        for (int j = kTraceE / 2; j >= 1; j >>= 1) {
-            if (j < k) {
-#pragma unroll
-                for (int e = 0; e < kTraceE; ++e) {
-                    if ((e & j) == 0) {
-                        const int p = e | j;
-                        const bool asc = ((base + e) & k) == 0;
-                        const uint64_t x = r[e], y = r[p];
-                        const uint64_t lo = x < y ? x : y, hi = x < y ? y : x;
-                        r[e] = asc ? lo : hi;
-                        r[p] = asc ? hi : lo;
-                    }
-                }
-            }
-        }
-    }
-}
-
-__device__ __forceinline__ int64_t wave_sum_i64_t(int64_t v) {
-#pragma unroll
-    for (int m = 1; m < kWave; m <<= 1) v += (int64_t)shfl_xor_u64((uint64_t)v, m);
-    return v;
 }
 
 // One (job, entropy term): blockIdx.x = job, blockIdx.y = term.  mode 0:

@@ -33,7 +33,7 @@ def to_prepared(bin_arrs, n_per_var, tau, device) -> core.Prepared:
             return core.prepare(bin_arrs.bins, bin_arrs.npv, bin_arrs.T, tau)
         return bin_arrs
     dev = core.require_gpu(device)
-    bins, npv, _, T = core.upload_bins(bin_arrs, dev)
+    bins, bins16, npv, _, T = core.upload_bins_any(bin_arrs, dev)
     if n_per_var is not None:
         npv_in = torch.as_tensor(np.asarray(n_per_var.cpu() if isinstance(n_per_var, torch.Tensor)
                                             else n_per_var), dtype=torch.int32).to(dev)
@@ -41,7 +41,7 @@ def to_prepared(bin_arrs, n_per_var, tau, device) -> core.Prepared:
             if bool((npv_in < npv).any()):
                 raise ValueError("n_per_var is smaller than the levels present in bin_arrs")
             npv = npv_in.contiguous()
-    return core.prepare(bins, npv, T, tau)
+    return core.prepare(bins, npv, T, tau, bins16=bins16)
 
 
 class _LDSKernel(TEKernel):
@@ -97,7 +97,17 @@ class AdaptiveSMEMKernel(_LDSKernel):
         return "Adaptive-SMEM"
 
     def supports(self, b_max, on_cuda, smem_optin, smem_bytes, n_per_var, source_filter):
-        return bool(on_cuda) and b_max <= 127
+        # any level count (tenex/kernels/adaptive_smem.py:1227-1229): pairs
+        # of variables above 127 levels run on the occupied-key kernel
+        return bool(on_cuda) and b_max <= core.MAX_LEVELS_INT16
+
+    @staticmethod
+    def bin_limit(L: int, smem_optin: int) -> int:
+        """The reference's coarsening limit for a series of L samples: the
+        LDS bin limit of a next_pow2(L)-thread block clamped to [128, 1024]
+        (adaptive_smem.py:1289-1290)."""
+        warps = max(128, min(1024, 1 << (max(1, int(L)) - 1).bit_length())) // 32
+        return compute_smem_bin_limit(int(smem_optin or LDS_BYTES_GFX950), warps=warps)
 
     def prepare(self, bin_arrs, n_per_var, b_max, **kwargs):
         """coarsening=True applies the reference's lossy coarsening
@@ -110,16 +120,18 @@ class AdaptiveSMEMKernel(_LDSKernel):
         smem_optin = kwargs.get("smem_optin") or LDS_BYTES_GFX950
         if not isinstance(bin_arrs, core.Prepared):
             raise TypeError("prepare() expects core.Prepared data")
-        # the limit the reference derives for its block of next_pow2(L)
-        # threads, clamped to [128, 1024] (adaptive_smem.py:1289-1290)
-        L = max(1, int(bin_arrs.L))
-        warps = max(128, min(1024, 1 << (L - 1).bit_length())) // 32
-        limit = compute_smem_bin_limit(int(smem_optin), warps=warps)
+        limit = self.bin_limit(bin_arrs.L, smem_optin)
         if b_max <= limit:
             return bin_arrs, n_per_var, b_max
         vprint(f"[TENEX] bin coarsening: b_max={b_max}->{limit}")
-        bins = bin_arrs.bins.clone()
         npv = bin_arrs.npv.clone()
-        core.coarsen_gpu(bins, npv, bin_arrs.T, limit)
+        if bin_arrs.bins16 is not None:     # levels above 127: from the int16 rows
+            bins = torch.empty_like(bin_arrs.bins)
+            core.call("tnx_coarsen_bins16", core.ptr(bin_arrs.bins16), bin_arrs.n_vars,
+                      bin_arrs.T, bin_arrs.ld, core.ptr(npv), int(limit), core.ptr(bins),
+                      core._stream(bin_arrs.device))
+        else:
+            bins = bin_arrs.bins.clone()
+            core.coarsen_gpu(bins, npv, bin_arrs.T, limit)
         prep = core.prepare(bins, npv, bin_arrs.T, bin_arrs.tau)
         return prep, npv, int(prep.npv_host.max())

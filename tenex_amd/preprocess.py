@@ -52,6 +52,25 @@ def discretize(arr, method: str = "fsbw-l", kp: float = 0.5, device=None,
     return bins, n_bins, b_max, npv, T
 
 
+def discretize_any(arr, method: str = "fsbw-l", kp: float = 0.5, device=None,
+                   use_numpy_bins: bool = True):
+    """discretize() for any level count up to 32,767: returns (bins int8,
+    bins16 int16 (or None when every variable has at most 127 levels),
+    n_bins, b_max, n_per_var, T).  The reference keeps int16 bins above 127
+    levels (tenex/preprocess.py:335-354)."""
+    if method.lower() not in BINNING_METHODS:
+        raise ValueError(f"Unknown binning method '{method}'. This build implements "
+                         f"{list(BINNING_METHODS)}")
+    dev = core.require_gpu(device)
+    x = torch.as_tensor(np.ascontiguousarray(arr, dtype=np.float32)).to(dev)
+    n, T = x.shape
+    if T < 2:
+        raise ValueError("need at least 2 time points")
+    raw, n_bins = core.fsbw_l_gpu(x, kp)
+    bins, bins16, npv, b_max = core.remap_any_gpu(raw)
+    return bins, bins16, n_bins, b_max, npv, T
+
+
 def remap_bins(bin_arr):
     """Per-variable dense relabelling of an (n, T[, 1]) integer tensor on the
     GPU (tenex/preprocess.py:401-459).  Returns (bins (n, T, 1) int8 device,

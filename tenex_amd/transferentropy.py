@@ -137,10 +137,10 @@ class TransferEntropyEngine:
         hit = self._bin_arrs_cache.get(key)
         if hit is not None:
             return hit
-        from .preprocess import discretize
-        bins, n_bins, b_max, npv, T = discretize(self._data, binning_method, kp, device,
-                                                 use_numpy_bins)
-        self._bin_arrs_cache[key] = (bins, n_bins, b_max, npv, T)
+        from .preprocess import discretize_any
+        bins, bins16, n_bins, b_max, npv, T = discretize_any(self._data, binning_method, kp,
+                                                             device, use_numpy_bins)
+        self._bin_arrs_cache[key] = (bins, bins16, n_bins, b_max, npv, T)
         return self._bin_arrs_cache[key]
 
     def _binarize(self, method, device, backend, params):
@@ -229,13 +229,25 @@ class TransferEntropyEngine:
         if profile:
             ev[1].record(stream)
         report_npv = None
+        bins16 = None
         if binarization is not None:
             bins, npv, b_max, report_npv, T = self._binarize(
                 binarization, dev, binarization_backend, binarization_kwargs or {})
         else:
-            bins, n_bins, b_max, npv, T = self._discretize(binning_method, kp, dev,
-                                                           use_numpy_bins)
-        prep = core.prepare(bins, npv, T, int(tau))
+            bins, bins16, n_bins, b_max, npv, T = self._discretize(binning_method, kp, dev,
+                                                                   use_numpy_bins)
+        b_max_before = b_max
+        if bins16 is not None and coarsening is True:
+            # the reference's lossy coarsening, from int16 levels into int8 rows
+            from .kernels.lds_hist import AdaptiveSMEMKernel
+            lim = AdaptiveSMEMKernel.bin_limit(T - int(tau), query_smem_optin([dev.index]))
+            bins = torch.empty_like(bins)
+            npv = npv.clone()
+            core.call("tnx_coarsen_bins16", core.ptr(bins16), n_vars, T, bins16.shape[1],
+                      core.ptr(npv), int(lim), core.ptr(bins), core._stream(dev))
+            bins16 = None
+            b_max = int(npv.max().item())
+        prep = core.prepare(bins, npv, T, int(tau), bins16=bins16)
         if profile:
             ev[2].record(stream)
         smem_bytes = compute_smem_bytes(b_max, T, 1, tau)
@@ -246,11 +258,12 @@ class TransferEntropyEngine:
         run_info = {"n_vars": n_vars, "T": T, "K": 1, "tau": tau, "n_pairs": n_vars * (n_vars - 1),
                     "device_ids": [dev.index], "n_gpus": 1,
                     "binning_method": binning_method if binarization is None else binarization,
-                    "kp": kp, "use_numpy_bins": use_numpy_bins, "b_max_before_coarsen": b_max,
+                    "kp": kp, "use_numpy_bins": use_numpy_bins, "b_max_before_coarsen": b_max_before,
                     "b_max": b_max, "kernel": kernel_obj.name, "smem_optin": smem_optin,
                     "smem_bytes": smem_bytes, "source_filter": source_filter,
                     "binarization": binarization,
-                    "bin_dtype": "torch.int8", "bin_bytes": n_vars * T}
+                    "bin_dtype": "torch.int16" if bins16 is not None else "torch.int8",
+                    "bin_bytes": n_vars * T * (2 if bins16 is not None else 1)}
         if profile:
             ev[3].record(stream)
         src_idx = self._source_indices()
@@ -265,7 +278,7 @@ class TransferEntropyEngine:
             prep2, npv2, b_max2 = kernel_obj.prepare(prep, npv, b_max, use_coarsening=coarsening,
                                                      on_cuda=True, n_vars=n_vars, tau=tau,
                                                      smem_optin=smem_optin)
-            run_info.update(b_max=b_max2, coarsened=b_max2 != b_max,
+            run_info.update(b_max=b_max2, coarsened=b_max2 != b_max_before,
                             mode="pair-free" if src_idx is None else "pair-based",
                             pair_free=src_idx is None)
             prep = prep2
@@ -273,7 +286,7 @@ class TransferEntropyEngine:
                 ev[4].record(stream)
                 ev[5].record(stream)
             cw, cb = kernel_obj.cap_wave, kernel_obj.cap_block
-            if autotune:
+            if autotune and not len(prep.wide):
                 tuned = core.autotune(prep, None, src_idx)
                 cw, cb = tuned["cap_wave"], tuned["cap_block"]
                 run_info["autotune"] = tuned
@@ -320,7 +333,8 @@ class TransferEntropyEngine:
         prep = self._last_prep
         bin_arrs = n_per_var = None
         if prep is not None:
-            bin_arrs = prep.bins[:, : prep.T].cpu().numpy()[:, :, None]
+            src = prep.bins16 if prep.bins16 is not None else prep.bins
+            bin_arrs = src[:, : prep.T].cpu().numpy()[:, :, None]
             n_per_var = prep.npv_host.copy()
             if getattr(self, "_last_report_npv", None) is not None:
                 n_per_var = self._last_report_npv.copy()   # binarized: max - min + 1

@@ -276,3 +276,15 @@ def test_smem_bin_limit_brute_force():
                        if (B ** 3 + 2 * B ** 2 + B + warps) * 4 <= optin)
             assert O.smem_bin_limit(optin, warps) == best
     assert O.smem_bin_limit(160 * 1024, 32) == 33
+
+
+def test_sparse_te_oracle_equals_dense():
+    """te_pairs_sparse (occupied states) equals te_pairs (dense counts)."""
+    rng = np.random.default_rng(4)
+    b = np.stack([rng.integers(0, k, 700) for k in (2, 5, 9, 17, 3, 30)]).astype(np.int32)
+    npv = (b.max(axis=1) + 1).astype(np.int32)
+    pairs = O.pairfree_pairs(6)
+    for tau in (1, 3):
+        d = O.te_pairs(b, npv, pairs, tau=tau)
+        sp = O.te_pairs_sparse(b, pairs, tau=tau)
+        assert np.abs(d - sp).max() < 1e-12
