@@ -13,6 +13,9 @@ Other workloads (their own JSON line, same fields):
   c3  1,000 x 3,000 plain surrogate test, 100 block shuffles (block 50)
   c4  6,000 x 3,000, 200 TF x 2,000 target pairs: learned vs plain null
   dpi 4,000-gene FDR network (1% of ordered pairs): DPI trimming
+  trace     TRACE OutTE / InTE, 1,000 x 3,000 with planted hubs
+  multilag  the c3 surrogate test at lags 1-3, every lag in each launch
+  sortpairs 400 x 1,000 with 2-120 levels: large pairs on occupied keys
 
 Prints one JSON line with the throughput, the roofline of the dominant
 kernel and the CPU baseline (the torch-CPU restatement of the reference's
@@ -543,12 +546,209 @@ def run_dpi(args):
     return result
 
 
+LDS_PEAK_BYTES = 256 * 128 * 2.4e9   # 128 B per clock per CU, 256 CUs, 2.4 GHz
+
+
+def sort_lds_bytes(L: int) -> tuple[int, int]:
+    """(P, LDS bytes per sort) of the in-LDS bitonic network of sort_lds.hpp
+    for L keys: stages whose partner is in another wave store and load 16 B
+    per key, lane-shuffle stages move 8 B per key (two 4-byte permutes),
+    register stages none, plus the 8-byte store of the sorted keys."""
+    P = max(1024, 1 << (max(1, L) - 1).bit_length())
+    n_lds = n_shfl = 0
+    k = 2
+    while k <= P:
+        j = k >> 1
+        while j >= 1:
+            if j >= 16 * 64:
+                n_lds += 1
+            elif j >= 16:
+                n_shfl += 1
+            j >>= 1
+        k <<= 1
+    return P, P * (16 * n_lds + 8 * n_shfl + 8)
+
+
+def run_trace(args):
+    """TRACE OutTE / InTE on 1,000 genes x 3,000 cells (the surrogate
+    workload's data with 40 hubs driving 5 targets each)."""
+    import torch
+    import tenex_amd as tnx
+    from oracle import oracle as O
+    from tenex_amd import core, synthetic
+    from tenex_amd.inference.trace import _outgoing, build_causal_network, trace_scores
+    dev = core.require_gpu("cuda:0")
+    n, T = 1000, 3000
+    x = synthetic.zero_inflated_lognormal(n, T, seed=3)
+    rng = np.random.default_rng(3)
+    hubs = rng.choice(n, 240, replace=False)
+    for h in range(40):
+        s = hubs[6 * h]
+        for t in hubs[6 * h + 1: 6 * h + 6]:
+            x[t, 1:] = 0.5 * x[t, 1:] + x[s, :-1] * (0.5 + rng.random(T - 1).astype(np.float32))
+    names = np.array([f"g{i}" for i in range(n)])
+    res = tnx.TransferEntropyEngine(x, names).compute(accelerator="gpu")
+    nw = tnx.NetWeaver(res)
+
+    def step():
+        return nw.infer("trace")
+
+    dt = timed_steps(step, args.steps, args.warmup)
+    tr = step()
+    network = build_causal_network(res.matrix, 100, 2.0, 42, dev)
+    outgoing, src, _ = _outgoing(network, n)
+    bins, _, _, _ = core.upload_bins(res.bin_arrs, dev)
+    t_k = time_launches(lambda: trace_scores(bins, T, 1, network, outgoing), max(3, args.steps))
+    L = T - 1
+    P, per_sort = sort_lds_bytes(L)
+    jobs = sum(1 for i in range(n) if len(outgoing[i])) + sum(1 for i in range(n)
+                                                              if len(network[i]))
+    lds_bytes = 4 * jobs * per_sort
+    d = res.bin_arrs[:, :, 0].astype(np.int32)
+    ids = np.nonzero(tr.outte > 0)[0][:20]
+    err = max(abs(float(tr.outte[i]) - O.trace_te(d, int(i), sorted(outgoing[i]), 0))
+              for i in ids) if len(ids) else None
+    result = {
+        "metric": "TRACE seconds (network + OutTE + InTE), 1000 genes x 3000 cells",
+        "value": dt, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int64", "data": "synthetic zero-inflated log-normal "
+        "(seed 3), 40 hubs x 5 planted targets",
+        "config": {"workload": "TRACE OutTE / InTE", "n_genes": n, "n_cells": T,
+                   "edges": int(len(src)), "entropy_jobs": 4 * jobs, "sort_size": P},
+        "roofline": {"bound": "lds", "kernel": "trace_entropy_kernel",
+                     "achieved": lds_bytes / t_k / 1e9, "peak": LDS_PEAK_BYTES / 1e9,
+                     "unit": "GB/s", "frac": lds_bytes / t_k / LDS_PEAK_BYTES,
+                     "traffic": None, "lds_bytes_per_launch": lds_bytes,
+                     "definition": "LDS bytes of the bitonic network (16 B per key per "
+                                   "cross-wave stage, 8 B per key per lane-shuffle stage)"},
+        "kernels_ms": {"trace_scores": t_k * 1e3},
+        "max_abs_err_outte_vs_fp64_oracle_sample": err,
+    }
+    if not args.no_cpu_baseline:
+        ids = [i for i in range(n) if len(outgoing[i])][:12]
+        t0 = time.perf_counter()
+        for i in ids:
+            O.trace_te(d, i, sorted(outgoing[i]), 0)
+            O.trace_te(d, i, network[i], 1) if len(network[i]) else None
+        tc = (time.perf_counter() - t0) * jobs / (2 * len(ids))
+        result["cpu_baseline"] = {"value": tc, "unit": "s", "cores": 1, "kind": "port",
+                                  "sample": f"the reference's CPU hash + np.unique entropies "
+                                            f"(compute_outte_cpu / compute_inte_cpu) for "
+                                            f"{len(ids)} variables, extrapolated to {jobs} jobs"}
+    return result
+
+
+def run_multilag(args):
+    """The C3 surrogate test at lags 1, 2 and 3, every lag in each launch."""
+    import tenex_amd as tnx
+    from tenex_amd import core, synthetic
+    dev = core.require_gpu("cuda:0")
+    n, T = 1000, 3000
+    x = synthetic.zero_inflated_lognormal(n, T, seed=3)
+    names = np.array([f"g{i}" for i in range(n)])
+    eng = tnx.TransferEntropyEngine(x, names)
+    pipe = tnx.Pipeline(eng, fdr=0.05)
+    taus = [1, 2, 3]
+
+    def step():
+        return pipe.infer("surrogate_test", taus=taus, n_surrogates=100, block_length=50)
+
+    dt = timed_steps(step, args.steps, args.warmup)
+    r = step()
+    upd = n * (n - 1) * 101 * sum(T - t for t in taus)
+    lds_peak, lds_rand = lds_peaks(dev)
+    result = {
+        "metric": "multi-lag surrogate-test seconds, 1000 genes x 3000 cells, taus 1-3, "
+                  "100 block shuffles",
+        "value": dt, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic zero-inflated log-normal (seed 3)",
+        "config": {"workload": "multi-lag surrogate test (lag_combine max)", "taus": taus,
+                   "n_surrogates": 100, "block_length": 50, "edges": len(r.grn)},
+        "roofline": {"bound": "lds_atomic", "kernel": "whole multi-lag test",
+                     "achieved": upd / dt / 1e9, "peak": lds_peak / 1e9, "unit": "Gupdates/s",
+                     "frac": upd / dt / lds_peak, "updates_per_test": upd,
+                     "peak_random_keys": lds_rand / 1e9, "traffic": None},
+    }
+    if not args.no_cpu_baseline:
+        res = eng.compute(accelerator="gpu")
+        d = res.bin_arrs[:, :, 0].astype(np.int32)
+        cb = surrogate_cpu_baseline(d, res.n_per_var, n * (n - 1), 101 * len(taus), 2048,
+                                    args.seed)
+        result["cpu_baseline"] = cb
+    return result
+
+
+def run_sortpairs(args):
+    """Occupied-key TE: 400 variables x 1,000 cells with 2..120 levels, the
+    TE matrix through the engine's matrix path (pairs above the sort
+    threshold on te_sort_pair_kernel, the rest on the histogram launch)."""
+    import torch
+    from oracle import oracle as O
+    from tenex_amd import core
+    dev = core.require_gpu("cuda:0")
+    n, T = 400, 1000
+    rng = np.random.default_rng(args.seed)
+    levels = rng.integers(2, 121, n)
+    d = np.stack([rng.integers(0, b, T) for b in levels]).astype(np.int32)
+    for g, b in enumerate(levels):
+        d[g, rng.choice(T, b, replace=False)] = np.arange(b)
+    b8, b16, npv, _, _ = core.upload_bins_any(d, dev)
+    prep = core.prepare(b8, npv, T, 1)
+    cap = core.sort_threshold(prep.L)
+    pairs = np.stack(np.meshgrid(np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 2)
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    npvh = prep.npv_host.astype(np.int64)
+    big = pairs[npvh[pairs[:, 0]] ** 2 * npvh[pairs[:, 1]] > cap]
+
+    def step():
+        return core.te_matrix_internal(prep)
+
+    dt = timed_steps(step, args.steps, args.warmup)
+    m = step().cpu().numpy()
+    out = torch.zeros((n, n), dtype=torch.float32, device=dev)
+    t_k = time_launches(lambda: core.te_sort_pairs(prep, big, out, n), max(3, args.steps))
+    plan = core.get_plan(prep)
+    dense = torch.zeros((n, n), dtype=torch.float32, device=dev)
+    t_d = time_launches(lambda: core.te_run(prep, plan, dense, 1, ld_out=n), max(3, args.steps))
+    P, per_sort = sort_lds_bytes(T - 1)
+    lds_bytes = len(big) * per_sort
+    smp = big[rng.choice(len(big), 200, replace=False)]
+    t0 = time.perf_counter()
+    te64 = O.te_pairs_sparse(d, smp)
+    tc = time.perf_counter() - t0
+    err = float(np.abs(m[smp[:, 0], smp[:, 1]].astype(np.float64) - te64).max())
+    result = {
+        "metric": "TE matrix seconds, 400 variables x 1000 cells, 2-120 levels",
+        "value": dt, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int64", "data": "synthetic uniform levels (seeded)",
+        "config": {"workload": "occupied-key routing of large pairs", "n_vars": n,
+                   "n_cells": T, "sort_threshold_cells": cap, "sorted_pairs": int(len(big)),
+                   "all_pairs": int(len(pairs)), "sort_size": P},
+        "roofline": {"bound": "lds", "kernel": "te_sort_pair_kernel",
+                     "achieved": lds_bytes / t_k / 1e9, "peak": LDS_PEAK_BYTES / 1e9,
+                     "unit": "GB/s", "frac": lds_bytes / t_k / LDS_PEAK_BYTES,
+                     "traffic": None, "lds_bytes_per_launch": lds_bytes},
+        "kernels_ms": {"te_sort_pairs (large pairs)": t_k * 1e3,
+                       "histogram launch, all pairs dense": t_d * 1e3},
+        "max_abs_err_vs_fp64_oracle_sample": err,
+        "cpu_baseline": {"value": len(smp) / tc, "unit": "pairs/s", "cores": 1,
+                         "kind": "port", "sample": "200 large pairs, numpy occupied-state "
+                                                   "entropies (oracle te_pairs_sparse)"},
+    }
+    return result
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3", "c4", "dpi"])
+    ap.add_argument("--workload", default="c2",
+                    choices=["c0", "c1", "c2", "c3", "c4", "dpi", "trace", "multilag",
+                             "sortpairs"])
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=20480)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -567,7 +767,8 @@ def main():
         run_traffic_probe(args)
         return
     fn = {"c0": run_c0, "c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4,
-          "dpi": run_dpi}[args.workload]
+          "dpi": run_dpi, "trace": run_trace, "multilag": run_multilag,
+          "sortpairs": run_sortpairs}[args.workload]
     res = fn(args)
     print(json.dumps(res), flush=True)
 

@@ -11,7 +11,8 @@
  *    synchronises: callers own every buffer and pass a hipStream_t as void*.
  *    Kernels run on that stream, on the calling thread's current device.
  *  - Bins are dense per variable (0 <= bins[v, t] < n_per_var[v] <= 127),
- *    int8, K = 1, row stride `ld` bytes (ld >= T, ld % 16 == 0).
+ *    int8, K = 1, row stride `ld` bytes (ld >= T, ld % 16 == 0); the entry
+ *    points taking int16 rows (levels < 32768) say so.
  *  - Pair convention of the reference: a pair is (target, source); its TE is
  *    TE(source -> target).  Pair-free linear order:
  *       id = target * (n - 1) + s,   source = s + (s >= target)
@@ -30,6 +31,9 @@
  *  - full/adaptive_smem_surrogate_test_launch
  *                                        tenex/csrc/full_smem_surrogate_test.cu:297-394
  *                                        tenex/csrc/adaptive_smem_surrogate_test.cu:306-400
+ *  - fused_outte/inte_4way_launch        tenex/csrc/trace_sort_entropy.cu:191-304
+ *  - the binarizers                      tenex/binarize.py:51-200
+ *  - the DPI mediator loop of _trim      tenex/inference/fdr.py:104-110
  */
 #ifndef TENEX_HIP_H
 #define TENEX_HIP_H
@@ -233,10 +237,10 @@ typedef struct {
      * or with item_per_wave wave item e whole), queues TNX_QUEUES .. hold
      * block entries (the workgroup takes one at a time: block item e).  A
      * workgroup drains its own XCD's queues first, then the others.  Needs
-     * T - tau <= 12288, four cap_wave slices within 62 KiB and a block
-     * histogram that holds max_block_cols; wave pairs give the counts and TE
-     * of the two-launch path, block pairs the same counts (TE summed in
-     * another order). */
+     * T - tau <= 12288, four cap_wave slices within 156 KiB and a block
+     * histogram (the four slices) that holds max_block_cols twice plus 128
+     * words; pairs give the counts of the two-launch path, and TE from
+     * exact fixed-point sums (xlogx_fx below). */
     const int32_t *queue_items;
     const int32_t *queue_off;  /* device int32[2 * TNX_QUEUES + 1] */
     int32_t *queue_ctr;        /* device int32[TNX_QUEUE_CTR_INTS] scratch: the queues' take
@@ -288,9 +292,10 @@ TNX_API int tnx_te_queue(const tnx_work_t *work, int64_t n_wave_items, int64_t n
                          int32_t item_per_wave, int32_t *qitems, int64_t qcap, int32_t *qoff,
                          int64_t *n_wave_entries);
 
-/* Pairwise TE for every pair of the plan: FP32 entropy terms with
- * compensated summation, FP64 final combination
- * (S3 + S1 - S2a - S2b) / N.  Replaces te_smem_launch(_pairfree),
+/* Pairwise TE for every pair of the plan, (S3 + S1 - S2a - S2b) / N with
+ * S = sum c log2 c: in the persistent launch S3 - S2b is an exact 64-bit
+ * fixed-point sum (xlogx_fx), in the two-launch kernels FP32 terms with
+ * compensated summation; the final combination is FP64.  Replaces te_smem_launch(_pairfree),
  * te_adaptive_smem_launch(_pairfree) and te_gmem_launch
  * (tenex/csrc/full_smem.cu:319-381, adaptive_smem.cu:307-376,
  * gmem.cu:190-230). */

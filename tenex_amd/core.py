@@ -1325,7 +1325,11 @@ def multilag_surrogate_stats(prep: Prepared, plan: Plan, taus, block_lengths, ob
     stacked plan cannot run persistently."""
     n, NL = prep.n_vars, len(taus)
     dev = prep.device
-    kb = max(1, copies_per_launch(prep, n_surr, n * n) // NL)
+    # copies of every lag share the launch's staging and copy budget; the
+    # surrogates are split into even batches (no small tail launch)
+    kmax = max(1, copies_per_launch(prep, n_surr * NL, n * n) // NL)
+    n_batches = -(-n_surr // kmax)
+    kb = -(-n_surr // n_batches)
     slots = _plan_slots(plan, n, dev)
     out = []
     for l in range(NL):

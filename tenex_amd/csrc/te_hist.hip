@@ -578,7 +578,7 @@ int tnx_te_counts_debug(const int8_t *bins, const uint16_t *states, const int32_
     TNX_CHECK_ARG(bins && states && n_per_var && S1 && S2a && xlogx && pairs && counts_out && te_out,
                   "tnx_te_counts_debug: null pointer");
     TNX_CHECK_ARG(max_cells >= 8 && max_cells <= 8192 && max_cells % 2 == 0, "tnx_te_counts_debug: max_cells in [8, 8192]");
-    TNX_CHECK_ARG(ld % 16 == 0 && lds % 8 == 0 && tau >= 1 && tau < T,
+    TNX_CHECK_ARG(ld % 16 == 0 && lds % 16 == 0 && lds >= T - tau && tau >= 1 && tau < T,
                   "tnx_te_counts_debug: bad strides / tau");
     if (n_pairs == 0) return TNX_OK;
     size_t shm = (size_t)kWavesPerWG * wave_slice(max_cells) * 4;

@@ -1,8 +1,9 @@
 // te_persist.hip -- every pair of a plan in one persistent launch.
 //
 // One grid of resident workgroups (256 threads, four waves, three per CU)
-// takes entries from eight work queues with an atomic take counter each.  A
-// workgroup drains the queue of its own XCD first (the planner deals entries
+// takes entries from sixteen work queues (a wave queue and a block queue per
+// XCD) with an atomic take counter each.  A workgroup drains the queues of
+// its own XCD first (the planner deals entries
 // to queues by source chunk, so an XCD's workgroups share source rows in its
 // L2), then the other queues, so the launch ends when the last entry does.
 // An entry is either
@@ -21,8 +22,8 @@
 // first) is requested.  The reduction reads each column's cells four at a
 // time and takes c*log2(c) of counts below 256 from an LDS copy of the
 // table.  Counts are those of the two-launch kernels (te_wave_reg_kernel /
-// te_block_kernel); TE differs from theirs only by the order of the
-// compensated sums.
+// te_block_kernel); S3 - S2b is an exact 64-bit fixed-point sum of the
+// table's entries, so TE does not depend on the order of the additions.
 //
 // Replaces the pair-free launches te_smem_launch_pairfree
 // (tenex/csrc/full_smem.cu:350-381) and te_adaptive_smem_launch_pairfree
