@@ -44,6 +44,10 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def onehot_kp(L):
+    return -(-int(L) // 128) * 128
+
+
 def time_launches(fn, reps):
     """Average duration (s) of fn() over reps launches, HIP events on torch's
     current stream (the stream the library launches on)."""
@@ -178,37 +182,53 @@ def run_c2(args):
         tuned = core.autotune(prep)
         cw, cb = tuned["cap_wave"], tuned["cap_block"]
         log(f"autotune: {tuned}")
+    from tenex_amd import onehot
     t0 = time.perf_counter()
-    plan = core.get_plan(prep, cap_wave=cw, cap_block=cb)
+    oh = (core.get_onehot_plan(prep, thr=args.onehot_thr)
+          if core.onehot_enabled() and args.onehot_thr > 0 else None)
+    plan = core.get_plan(prep, cap_wave=cw, cap_block=cb,
+                         src_skip=None if oh is None else oh.src_skip)
     persistent = core.persistent_ok(prep, plan)
     targs = core.te_args(prep, plan, None, _lib.TNX_OUT_MATRIX, ld_out=n)  # builds the queues
     plan_s = time.perf_counter() - t0
     info = plan.info
     n_pairs = n * (n - 1)
-    assert info.n_pairs == n_pairs
+    n_oh = 0 if oh is None else oh.n_pairs
+    assert info.n_pairs + n_oh == n_pairs, (info.n_pairs, n_oh, n_pairs)
     out = torch.zeros((n, n), dtype=torch.float32, device=dev)
     targs = core.te_args(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n)
+    stream = _lib.stream_handle(dev)
+
+    def run_oh():
+        if oh is not None:
+            onehot.run(prep, oh, targs, stream)
 
     def step():
         core.gene_prep_into(prep)
+        run_oh()
         core.te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, args=targs)
 
     if not args.no_graph:  # the same launches captured once in a HIP graph, replayed per step
-        graph = core.capture_te_step(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n)
+        graph = core.capture_te_step(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n, oh=oh)
         step = graph.replay  # noqa: F811
 
-    log(f"plan: {info.n_wave_items} wave items, {info.n_block_items} block items, "
+    log(f"plan: {n_oh} one-hot pairs in {0 if oh is None else oh.n_tiles} tiles, "
+        f"{info.n_wave_items} wave items, {info.n_block_items} block items, "
         f"{'one persistent launch' if persistent else 'two launches'}, {plan_s * 1e3:.1f} ms host")
     dt = timed_steps(step, args.steps, args.warmup)
     reps = max(2, args.steps)
-    # the TE launch alone (HIP events on the stream it runs on), and the
+    # the launches alone (HIP events on the stream they run on), and the
     # earlier two-launch kernels on the same plan for comparison
-    t_te = time_launches(lambda: core.te_run(prep, plan, out, 1, args=targs), reps)
+    t_pers = time_launches(lambda: core.te_run(prep, plan, out, 1, args=targs), reps)
+    t_oh = time_launches(run_oh, reps) if oh is not None else 0.0
+    t_te = t_pers + t_oh
     t_prep = time_launches(lambda: core.gene_prep_into(prep), reps)
     a2 = core.te_args(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n, persistent=False)
     t_two = time_launches(lambda: core.te_run(prep, plan, out, 1, args=a2), reps)
+    run_oh()
     core.te_run(prep, plan, out, 1, args=targs)
-    log(f"TE launch {t_te * 1e3:.2f} ms ({'persistent' if persistent else 'two launches'}), "
+    log(f"TE launches {t_te * 1e3:.2f} ms (one-hot pack + MFMA {t_oh * 1e3:.2f} ms, "
+        f"{'persistent' if persistent else 'two launches'} {t_pers * 1e3:.2f} ms), "
         f"two-launch kernels {t_two * 1e3:.2f} ms, gene_prep {t_prep * 1e3:.3f} ms")
     L = T - tau
     upd = n_pairs * L            # LDS histogram updates of one TE launch
@@ -242,8 +262,12 @@ def run_c2(args):
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": alg_bytes / t_te / 1e9 / HBM_PEAK_GBS,
                          "algorithmic_bytes": alg_bytes, "traffic": measured_traffic()},
-        "kernels_ms": {"te_launch": t_te * 1e3, "te_two_launch_kernels": t_two * 1e3,
-                       "gene_prep_kernel": t_prep * 1e3},
+        "kernels_ms": {"te_launches": t_te * 1e3, "persistent_launch": t_pers * 1e3,
+                       "onehot_pack_and_mfma": t_oh * 1e3,
+                       "te_two_launch_kernels": t_two * 1e3, "gene_prep_kernel": t_prep * 1e3},
+        "onehot": {"pairs": n_oh, "tiles": 0 if oh is None else oh.n_tiles,
+                   "threshold": args.onehot_thr,
+                   "fp4_mfma_ops": 0 if oh is None else 2 * oh.cells * onehot_kp(prep.L)},
         "plan_host_ms": plan_s * 1e3,
     }
     if tuned is not None:
@@ -750,6 +774,9 @@ def main():
                     choices=["c0", "c1", "c2", "c3", "c4", "dpi", "trace", "multilag",
                              "sortpairs"])
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--onehot-thr", type=int, default=None,
+                    help="largest (b_x^2 - 1) * b_y a pair may have to be counted on INT8 MFMA "
+                         "(C2; 0: every pair in the histogram kernels)")
     ap.add_argument("--cpu-sample", type=int, default=20480)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
@@ -763,6 +790,9 @@ def main():
         print("bench.py measures one GPU: this build runs one MI355X per process and has no "
               "multi-GPU path (--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+    if args.onehot_thr is None:
+        from tenex_amd import onehot
+        args.onehot_thr = onehot.ONEHOT_THR
     if args.traffic_probe:
         run_traffic_probe(args)
         return

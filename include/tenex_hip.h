@@ -50,7 +50,7 @@ extern "C" {
 #define TNX_API
 #endif
 
-#define TNX_ABI_VERSION 9
+#define TNX_ABI_VERSION 10
 
 /* Status codes. */
 #define TNX_OK 0
@@ -182,11 +182,15 @@ typedef struct {
  * block kernel's histogram so that two workgroups share a CU's LDS; an
  * explicit cap_block must be a multiple of 4 and >= the widest block
  * pair's b_t * b_s (reported as max_block_cols).
+ * src_skip (host, n_vars, or NULL): target v leaves out the first
+ * src_skip[v] sources of src_order (sources are ordered by level count,
+ * ties by list position), the pairs a one-hot plan (tnx_onehot_te) counts.
  * This is the scheduling half of te_(adaptive_)smem_launch_pairfree
  * (tenex/csrc/full_smem.cu:350-381, tenex/csrc/adaptive_smem.cu:351-376). */
 TNX_API int tnx_te_plan(const int32_t *n_per_var, int64_t n_vars, int64_t L,
                 const int32_t *tgt_list, int64_t n_tgt, const int32_t *src_list,
                 int64_t n_src, int32_t cap_wave, int32_t cap_block,
+                const int32_t *src_skip,
                 int32_t *tgt_order, int32_t *src_order, tnx_work_t *work, int64_t work_cap,
                 tnx_plan_info_t *info);
 
@@ -396,6 +400,53 @@ TNX_API int tnx_te_gemm_b2(const int8_t *bins, int64_t n, int64_t T, int64_t ld,
                    const double *S1, const double *S2a, const float *xlogx,
                    void *workspace, int64_t workspace_bytes, float *te_out, int64_t ld_out,
                    int32_t *counts_dbg, void *stream);
+
+/* ── One-hot INT8 MFMA counts for pairs of few levels ───────────────── */
+
+/* The joint counts of pair (x, y) are a product of one-hot indicator rows:
+ * target x contributes b_x^2 - 1 rows over t < L (states (a, b) =
+ * (x[t+tau], x[t]) with a < b_x - 1 at row a*b_x + b, then x_t = b for
+ * b < b_x - 1 at row (b_x - 1)*b_x + b), source y its b_y rows y[t] = c.
+ * Counts with a = b_x - 1, and of x_t = b_x - 1, follow from the
+ * per-variable marginals, so every joint count is an exact integer.
+ * Rows are grouped in 128-row (target) and 128-column (source) groups that
+ * never split a variable; a tile is two row groups x two column groups
+ * (one 128 x 128 sub-tile per wave, mask bit w = 2*wr + wc).
+ * Operands are int8 in 32-row x 32-byte blocks in MFMA lane order (block
+ * (row / 32) * (Kp / 32) + k / 32; row r, byte k at ((k % 32) / 16 * 32 +
+ * r % 32) * 16 + k % 16 inside the block).  Each pair's TE is written
+ * through the tnx_te_args_t output modes (out_mode, out, ld_out, ...),
+ * with S3 - S2b summed in the exact fixed point of xlogx_fx.  No reference
+ * counterpart: the reference counts every pair with per-sample atomics
+ * (tenex/csrc/full_smem.cu:31-142, tenex/csrc/adaptive_smem.cu:30-149). */
+typedef struct {
+    int8_t *A, *B;                 /* operands (rows_a x Kp, rows_b x Kp), tiled */
+    int64_t Kp;                    /* padded series length: multiple of 64, >= L */
+    int64_t rows_a, rows_b;        /* multiples of 128 */
+    const int32_t *arow;           /* (rows_a, 2): variable (-1: padding), (a << 8) | b
+                                    * (a = 255: an x_t row) */
+    const int32_t *brow;           /* (rows_b, 2): variable (-1: padding), level c */
+    int32_t *marg;                 /* per marginal variable: n(a, b) (b^2), n(c) (b) */
+    const int64_t *moff;           /* (n_vars): offset of the variable's marginals, -1: none */
+    const int32_t *tiles;          /* (n_tiles, 8): rg0, rg1, cg0, cg1, mask, 0, 0, 0 */
+    int64_t n_tiles;
+    const int32_t *rg_off;         /* (n_rg + 1): targets of row group g: rg_off[g] .. */
+    const int32_t *t_var, *t_row;  /* target variable, first row inside its group */
+    const int32_t *cg_off;         /* (n_cg + 1) */
+    const int32_t *s_var, *s_col;  /* source variable, first column inside its group */
+    const int32_t *mvars;          /* (n_mvars): variables whose marginals are counted */
+    int64_t n_mvars;
+} tnx_onehot_args_t;
+
+/* Marginals of mvars and the A, B operands from bins (int8, K = 1) for the
+ * series length L = T - tau (o->Kp >= L; b <= 64 for marginal variables). */
+TNX_API int tnx_onehot_pack(const tnx_onehot_args_t *o, const int8_t *bins, int64_t T,
+                            int64_t ld, int32_t tau, const int32_t *n_per_var, void *stream);
+
+/* TE of every pair of the tiles (target != source) through te's output mode;
+ * te supplies n_per_var, S1, S2a, xlogx_fx, fx_shift, T, tau and the
+ * output fields (the histogram plan fields are not read). */
+TNX_API int tnx_onehot_te(const tnx_onehot_args_t *o, const tnx_te_args_t *te, void *stream);
 
 /* ── Surrogate test ──────────────────────────────────────────────────── */
 

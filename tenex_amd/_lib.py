@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libtenex_hip.so")
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 TNX_OUT_PAIRFREE = 0
 TNX_OUT_MATRIX = 1
@@ -59,6 +59,15 @@ class TeArgs(C.Structure):
                 ("cap_sort", i64)]
 
 
+class OnehotArgs(C.Structure):
+    _fields_ = [("A", p_), ("B", p_), ("Kp", i64), ("rows_a", i64), ("rows_b", i64),
+                ("arow", p_), ("brow", p_), ("marg", p_), ("moff", p_),
+                ("tiles", p_), ("n_tiles", i64),
+                ("rg_off", p_), ("t_var", p_), ("t_row", p_),
+                ("cg_off", p_), ("s_var", p_), ("s_col", p_),
+                ("mvars", p_), ("n_mvars", i64)]
+
+
 TNX_QUEUES = 8
 TNX_QUEUE_CTR_INTS = 256
 TNX_NOALLOC_SCRATCH_INTS = 256
@@ -97,8 +106,10 @@ SIGNATURES = {
     "tnx_xlogx_table": (C.c_int, [p_, i64, p_]),
     "tnx_xlogx_table_fx": (C.c_int, [p_, i64, i32, p_]),
     "tnx_gene_prep": (C.c_int, [p_, i64, i64, i64, i32, p_, p_, i64, p_, p_, p_, i32, p_]),
-    "tnx_te_plan": (C.c_int, [p_, i64, i64, p_, i64, p_, i64, i32, i32, p_, p_, p_, i64,
+    "tnx_te_plan": (C.c_int, [p_, i64, i64, p_, i64, p_, i64, i32, i32, p_, p_, p_, p_, i64,
                               C.POINTER(PlanInfo)]),
+    "tnx_onehot_pack": (C.c_int, [C.POINTER(OnehotArgs), p_, i64, i64, i32, p_, p_]),
+    "tnx_onehot_te": (C.c_int, [C.POINTER(OnehotArgs), C.POINTER(TeArgs), p_]),
     "tnx_te_run": (C.c_int, [C.POINTER(TeArgs), p_]),
     "tnx_te_run_noalloc": (C.c_int, [C.POINTER(TeArgs), p_]),
     "tnx_remap_dense16": (C.c_int, [p_, i64, i64, p_, i64, p_, p_, p_]),

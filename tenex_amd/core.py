@@ -503,14 +503,18 @@ def _merge_items(items: np.ndarray, per_pairs: int) -> np.ndarray:
 
 
 def make_plan(npv_host: np.ndarray, L: int, device, targets=None, sources=None,
-              cap_wave: int = 0, cap_block: int = 0) -> Plan:
+              cap_wave: int = 0, cap_block: int = 0, src_skip=None) -> Plan:
+    """Histogram plan (tnx_te_plan).  src_skip (int32 (n_vars,)): target v
+    leaves out its first src_skip[v] sources in level-count order (the pairs
+    a one-hot plan counts)."""
     npv_host = np.ascontiguousarray(npv_host, dtype=np.int32)
     n = len(npv_host)
     tg = None if targets is None else np.ascontiguousarray(targets, dtype=np.int32)
     sr = None if sources is None else np.ascontiguousarray(sources, dtype=np.int32)
+    sk = None if src_skip is None else np.ascontiguousarray(src_skip, dtype=np.int32)
     info = _lib.PlanInfo()
     args = (ptr(npv_host), n, int(L), ptr(tg), 0 if tg is None else len(tg), ptr(sr),
-            0 if sr is None else len(sr), int(cap_wave), int(cap_block))
+            0 if sr is None else len(sr), int(cap_wave), int(cap_block), ptr(sk))
     call("tnx_te_plan", *args, None, None, None, 0, C.byref(info))
     n_items = info.n_wave_items + info.n_block_items
     tgt = np.empty(info.n_tgt, np.int32)
@@ -548,11 +552,12 @@ def _digest(a) -> bytes | None:
 
 
 def get_plan(prep: Prepared, targets=None, sources=None, cap_wave: int = 0,
-             cap_block: int = 0) -> Plan:
+             cap_block: int = 0, src_skip=None) -> Plan:
     key = ("te", _digest(prep.npv_host), int(prep.L), str(prep.device),
            _digest(None if targets is None else np.asarray(targets, np.int32)),
            _digest(None if sources is None else np.asarray(sources, np.int32)),
-           int(cap_wave), int(cap_block))
+           int(cap_wave), int(cap_block),
+           _digest(None if src_skip is None else np.asarray(src_skip, np.int32)))
     # variables above 127 levels never enter a histogram plan (their pairs run
     # on the occupied-key kernel); the planner sees them clamped
     npv = np.minimum(prep.npv_host, MAX_LEVELS_INT8) if len(prep.wide) else prep.npv_host
@@ -563,7 +568,7 @@ def get_plan(prep: Prepared, targets=None, sources=None, cap_wave: int = 0,
                 raise ValueError("a histogram plan cannot take variables of more than "
                                  f"{MAX_LEVELS_INT8} levels")
     return _cached_plan(key, lambda: make_plan(npv, prep.L, prep.device, targets,
-                                               sources, cap_wave, cap_block))
+                                               sources, cap_wave, cap_block, src_skip))
 
 
 # ── TE run ────────────────────────────────────────────────────────────────
@@ -740,23 +745,51 @@ def autotune(prep: Prepared, targets=None, sources=None, sample_targets: int = 2
 
 
 def capture_te_step(prep: Prepared, plan: Plan, out: torch.Tensor, out_mode: int,
-                    ld_out: int = 0) -> torch.cuda.CUDAGraph:
-    """The per-variable preparation and the plan's TE launches captured once
-    in a HIP graph (torch.cuda.CUDAGraph): replay() re-runs them on prep's
-    current bins with no host launch work.  The launches are issued once
-    eagerly first (kernel attributes set outside the capture); nothing in
-    them is a memset, and the buffers are fixed, so every replay is
-    equivalent to the eager sequence."""
+                    ld_out: int = 0, oh=None) -> torch.cuda.CUDAGraph:
+    """The per-variable preparation and the plan's TE launches (with oh, a
+    one-hot plan, its pack and MFMA launches too) captured once in a HIP
+    graph (torch.cuda.CUDAGraph): replay() re-runs them on prep's current
+    bins with no host launch work.  The launches are issued once eagerly
+    first (kernel attributes set outside the capture); nothing in them is a
+    memset, and the buffers are fixed, so every replay is equivalent to the
+    eager sequence."""
+    from . import onehot
     args = te_args(prep, plan, out, out_mode, ld_out=ld_out)
-    gene_prep_into(prep)
-    te_run(prep, plan, out, out_mode, args=args)
+
+    def step():
+        gene_prep_into(prep)
+        if oh is not None:
+            onehot.run(prep, oh, args, _stream(prep.device))
+        te_run(prep, plan, out, out_mode, args=args)
+
+    step()
     torch.cuda.synchronize(prep.device)
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g):
-        gene_prep_into(prep)
-        te_run(prep, plan, out, out_mode, args=args)
-    g._tnx_keep = (args, plan, out)  # the graph's kernels read these
+        step()
+    g._tnx_keep = (args, plan, out, oh)  # the graph's kernels read these
     return g
+
+
+def onehot_enabled() -> bool:
+    """Whether TE matrices count the pairs of few levels on INT8 MFMA
+    (onehot.py); TNX_ONEHOT=0 in the environment counts every pair in the
+    histogram kernels."""
+    return os.environ.get("TNX_ONEHOT", "1") != "0"
+
+
+def get_onehot_plan(prep: Prepared, targets=None, sources=None, thr: int | None = None):
+    """Cached one-hot plan of prep's narrow variables (None when no pair
+    qualifies or when the data hold variables of more than 127 levels)."""
+    from . import onehot
+    thr = onehot.ONEHOT_THR if thr is None else int(thr)
+    if len(prep.wide) or thr <= 0:
+        return None
+    key = ("onehot", _digest(prep.npv_host), int(prep.L), str(prep.device),
+           _digest(None if targets is None else np.asarray(targets, np.int32)),
+           _digest(None if sources is None else np.asarray(sources, np.int32)), thr)
+    return _cached_plan(key, lambda: onehot.plan(prep.npv_host, prep.L, prep.device, targets,
+                                                 sources, thr))
 
 
 # Per-pair cost of the occupied-key kernel in microseconds of one CU by sort
@@ -843,7 +876,13 @@ def te_matrix_internal(prep: Prepared, targets=None, sources=None, cap_wave: int
         wp = np.concatenate(parts)
         te_sort_pairs(prep, wp[wp[:, 0] != wp[:, 1]], out, n)
         return out
-    plan = get_plan(prep, targets, sources, cap_wave, cap_block)
+    oh = get_onehot_plan(prep, targets, sources) if onehot_enabled() else None
+    plan = get_plan(prep, targets, sources, cap_wave, cap_block,
+                    src_skip=None if oh is None else oh.src_skip)
+    if oh is not None:
+        from . import onehot
+        onehot.run(prep, oh, te_args(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n),
+                   _stream(prep.device))
     cap = sort_threshold(prep.L, int(plan.info.cap_wave)) if plan.info.n_block_items else 0
     if cap:
         tg = np.arange(n) if targets is None else np.asarray(targets, dtype=np.int64)
@@ -964,11 +1003,12 @@ def te_pairfree(prep: Prepared, pair_offset: int = 0, n_pairs_local: int | None 
     return out
 
 
-def counts_plan(prep: Prepared, plan: Plan, pairs: np.ndarray, max_cells: int):
+def counts_plan(prep: Prepared, plan: Plan, pairs: np.ndarray, max_cells: int, oh=None):
     """Joint counts of the given [target, source] pairs as the production
-    launch of `plan` counts them (the persistent kernel's count dump): int32
-    (P, max_cells) in layout (a * b_t + b) * b_s + c, and the launch's TE
-    matrix (internal layout)."""
+    launches count them (the persistent kernel's count dump; with oh, a
+    one-hot plan, its MFMA launch's too): int32 (P, max_cells) in layout
+    (a * b_t + b) * b_s + c, and the launches' TE matrix (internal
+    layout)."""
     require_narrow(prep, "the joint-count dump")
     pairs = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
     n, dev = prep.n_vars, prep.device
@@ -982,6 +1022,9 @@ def counts_plan(prep: Prepared, plan: Plan, pairs: np.ndarray, max_cells: int):
     out = torch.zeros((n, n), dtype=torch.float32, device=dev)
     args = te_args(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n, persistent=True,
                    counts=(cmap, counts))
+    if oh is not None:
+        from . import onehot
+        onehot.run(prep, oh, args, _stream(dev))
     te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, args=args)
     return counts[h2d(inv.reshape(-1), dev)], out
 

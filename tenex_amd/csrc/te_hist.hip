@@ -377,8 +377,8 @@ extern "C" {
 
 int tnx_te_plan(const int32_t *npv, int64_t n_vars, int64_t L, const int32_t *tgt_list,
                 int64_t n_tgt, const int32_t *src_list, int64_t n_src, int32_t cap_wave,
-                int32_t cap_block, int32_t *tgt_order, int32_t *src_order, tnx_work_t *work,
-                int64_t work_cap, tnx_plan_info_t *info) {
+                int32_t cap_block, const int32_t *src_skip, int32_t *tgt_order,
+                int32_t *src_order, tnx_work_t *work, int64_t work_cap, tnx_plan_info_t *info) {
     clear_error();
     TNX_CHECK_ARG(npv && info, "tnx_te_plan: null pointer");
     TNX_CHECK_ARG(n_vars >= 0 && n_vars < (1LL << 31), "tnx_te_plan: bad n_vars");
@@ -399,19 +399,26 @@ int tnx_te_plan(const int32_t *npv, int64_t n_vars, int64_t L, const int32_t *tg
     const int64_t nt = (int64_t)tg.size(), ns = (int64_t)sr.size();
     std::vector<int32_t> sb(ns);
     for (int64_t j = 0; j < ns; j++) sb[j] = npv[sr[j]];
-    // kw[i]: number of sources handled by the wave kernel for target tg[i].
-    std::vector<int64_t> kw(nt);
+    // kw[i]: number of sources handled by the wave kernel for target tg[i];
+    // sk[i]: sources left to a one-hot plan (a prefix of the sorted sources).
+    std::vector<int64_t> kw(nt), sk(nt, 0);
     for (int64_t i = 0; i < nt; i++) {
         int64_t bx = npv[tg[i]];
         int64_t lim = cap_wave / (bx * bx);
         kw[i] = std::upper_bound(sb.begin(), sb.end(), (int32_t)std::min<int64_t>(lim, 1 << 30)) -
                 sb.begin();
+        if (src_skip) {
+            TNX_CHECK_ARG(src_skip[tg[i]] >= 0 && src_skip[tg[i]] <= ns,
+                          "tnx_te_plan: src_skip[%d] = %d not in [0, %lld]", tg[i], src_skip[tg[i]],
+                          (long long)ns);
+            sk[i] = src_skip[tg[i]];
+        }
     }
     // Widest (x_t, y_t) marginal among block-kernel pairs: a pass of the
     // block kernel holds at least one value of x_{t+tau}, i.e. b_x * b_y cells.
     int32_t maxcols = 0;
     for (int64_t i = 0; i < nt; i++)
-        if (kw[i] < ns) maxcols = std::max<int32_t>(maxcols, npv[tg[i]] * sb[ns - 1]);
+        if (std::max(kw[i], sk[i]) < ns) maxcols = std::max<int32_t>(maxcols, npv[tg[i]] * sb[ns - 1]);
     if (cap_block <= 0) {
         cap_block = std::min(kDefaultCapBlock, kBlockLdsTarget / 4 - maxcols) & ~3;
         cap_block = std::max(cap_block, (maxcols + 3) & ~3);
@@ -421,31 +428,47 @@ int tnx_te_plan(const int32_t *npv, int64_t n_vars, int64_t L, const int32_t *tg
                   "pair's b_t*b_s = %d", cap_block, maxcols);
     std::vector<tnx_work_t> wv, bk;
     double cw = 0, cb = 0;
-    int64_t pw = 0;
-    for (int64_t i = 0; i < nt; i++) pw += kw[i];
-    const int64_t pb = nt * ns - pw;
+    int64_t pw = 0, pb = 0;
+    for (int64_t i = 0; i < nt; i++) {
+        pw += std::max<int64_t>(0, kw[i] - sk[i]);
+        pb += ns - std::max(kw[i], sk[i]);
+    }
     // pairs per item: as many as kWaveItemPairs / kBlockItemPairs, fewer when
     // the plan is too small to fill the chip otherwise
     const int64_t wave_pairs = std::clamp<int64_t>((pw + kFillWaveItems - 1) / kFillWaveItems,
                                                    kWavesPerWG, kWaveItemPairs);
     const int64_t block_pairs = std::clamp<int64_t>((pb + kFillBlockItems - 1) / kFillBlockItems,
                                                     1, kBlockItemPairs);
+    auto wave_range = [&](int64_t i, int64_t j0, int64_t j1) {
+        for (int64_t j = j0; j < j1; j += wave_pairs)
+            wv.push_back({(int32_t)i, 1, (int32_t)j, (int32_t)std::min(j1, j + wave_pairs)});
+    };
     for (int64_t g0 = 0; g0 < nt; g0 += kWavesPerWG) {
         int64_t g1 = std::min<int64_t>(nt, g0 + kWavesPerWG);
         int32_t cnt = (int32_t)(g1 - g0);
-        int64_t kmin = ns;
-        for (int64_t i = g0; i < g1; i++) kmin = std::min(kmin, kw[i]);
+        // the group's common wave range [smax, kmin); each target's own
+        // sources outside it go to one-target items
+        int64_t kmin = ns, smax = 0;
+        for (int64_t i = g0; i < g1; i++) {
+            kmin = std::min(kmin, kw[i]);
+            smax = std::max(smax, sk[i]);
+        }
         int64_t ch = std::max<int64_t>(1, wave_pairs / cnt);
-        for (int64_t j = 0; j < kmin; j += ch)
+        for (int64_t j = smax; j < kmin; j += ch)
             wv.push_back({(int32_t)g0, cnt, (int32_t)j, (int32_t)std::min(kmin, j + ch)});
         for (int64_t i = g0; i < g1; i++) {
             int64_t bx = npv[tg[i]];
-            for (int64_t j = kmin; j < kw[i]; j += wave_pairs)
-                wv.push_back({(int32_t)i, 1, (int32_t)j, (int32_t)std::min(kw[i], j + wave_pairs)});
-            for (int64_t j = 0; j < kw[i]; j++) cw += (double)L + (double)(bx * bx * sb[j]);
-            for (int64_t j = kw[i]; j < ns; j += block_pairs)
+            if (smax < kmin) {
+                wave_range(i, sk[i], smax);
+                wave_range(i, kmin, kw[i]);
+            } else {
+                wave_range(i, sk[i], kw[i]);
+            }
+            const int64_t b0 = std::max(kw[i], sk[i]);
+            for (int64_t j = sk[i]; j < kw[i]; j++) cw += (double)L + (double)(bx * bx * sb[j]);
+            for (int64_t j = b0; j < ns; j += block_pairs)
                 bk.push_back({(int32_t)i, 1, (int32_t)j, (int32_t)std::min<int64_t>(ns, j + block_pairs)});
-            for (int64_t j = kw[i]; j < ns; j++) {
+            for (int64_t j = b0; j < ns; j++) {
                 int64_t cols = bx * sb[j];
                 int64_t passes = (bx * cols + cap_block - 1) / cap_block;
                 cb += (double)L * (double)passes + (double)(bx * cols);
@@ -465,14 +488,17 @@ int tnx_te_plan(const int32_t *npv, int64_t n_vars, int64_t L, const int32_t *tg
         return u.tgt_beg < v.tgt_beg;
     };
     std::stable_sort(wv.begin(), wv.end(), by_chunk);
-    // pairs (excluding x == y)
-    std::vector<char> in_src(n_vars, 0);
-    for (int32_t v : sr) in_src[v] = 1;
-    int64_t self = 0;
-    for (int32_t v : tg) self += in_src[v];
+    // pairs (excluding x == y and the skipped sources)
+    std::vector<int64_t> src_pos(n_vars, -1);
+    for (int64_t j = 0; j < ns; j++) src_pos[sr[j]] = j;
+    int64_t kept = 0;
+    for (int64_t i = 0; i < nt; i++) {
+        kept += ns - sk[i];
+        if (src_pos[tg[i]] >= sk[i]) kept -= 1;
+    }
     info->n_wave_items = (int64_t)wv.size();
     info->n_block_items = (int64_t)bk.size();
-    info->n_pairs = nt * ns - self;
+    info->n_pairs = kept;
     info->cap_wave = cap_wave;
     info->cap_block = cap_block;
     info->max_block_cols = maxcols;

@@ -1,0 +1,191 @@
+"""One-hot INT8 MFMA counts for the pairs of few levels (csrc/onehot.hip).
+
+A pair (x, y) whose (b_x^2 - 1) * b_y indicator cells are few costs far less
+as an exact INT8 contraction of indicator rows than as one LDS atomic per
+sample.  The host plan here
+
+  * orders the sources by level count as the histogram planner does
+    (tnx_te_plan: ascending, ties by list position) and packs them into
+    128-column groups (b_y columns each, no source split);
+  * packs the targets, ascending by level count, into 128-row groups
+    (b_x^2 - 1 rows each);
+  * gives row group r the leading column groups g whose pairs all satisfy
+    (BX_r^2 - 1) * BY_g <= thr (BX, BY the groups' largest level counts),
+    so each target's one-hot sources are a prefix of the sorted sources:
+    src_skip[x] tells the histogram planner to leave exactly those out;
+  * deals tiles of two row groups x two column groups to the workgroups.
+
+Every pair keeps the same float TE on either path (exact integer counts and
+the same fixed-point entropy sums), so the split changes only the time.
+No reference counterpart: the reference counts every pair with per-sample
+atomics (tenex/csrc/full_smem.cu:31-142, tenex/csrc/adaptive_smem.cu:30-149).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, ptr
+
+GROUP = 128            # rows (columns) of a row (column) group
+ONEHOT_THR = 200       # largest (b_x^2 - 1) * b_y a one-hot pair may have
+ONEHOT_MAX_LEVELS = 64  # sources (a target's b_x^2 - 1 rows fit one 128-row group: b_x <= 11)
+ONEHOT_MAX_L = 65535   # counts are staged as u16
+
+
+@dataclass
+class OnehotPlan:
+    args: _lib.OnehotArgs
+    src_skip: np.ndarray          # int32 (n_vars,): sources left to this plan per target
+    n_pairs: int                  # pairs counted here (target != source)
+    n_tiles: int
+    cells: int                    # indicator cells of all tiles' active sub-tiles
+    keep: list = field(default_factory=list)   # device arrays the args point into
+
+
+def _groups(sizes: np.ndarray, cap: int = GROUP):
+    """Consecutive items into groups of at most cap rows: (group of each
+    item, first row of each item inside its group, number of groups)."""
+    g = np.empty(len(sizes), np.int64)
+    off = np.empty(len(sizes), np.int64)
+    cur, used = 0, 0
+    for i, sz in enumerate(sizes):
+        if used + sz > cap:
+            cur += 1
+            used = 0
+        g[i] = cur
+        off[i] = used
+        used += sz
+    return g, off, (cur + 1 if len(sizes) else 0)
+
+
+def plan(npv_host: np.ndarray, L: int, device, targets=None, sources=None,
+         thr: int = ONEHOT_THR) -> OnehotPlan | None:
+    """One-hot plan for the pairs of targets x sources that qualify, or None
+    when none does (thr <= 0, L too long, or no small pairs)."""
+    if thr <= 0 or L < 1 or L > ONEHOT_MAX_L:
+        return None
+    npv = np.asarray(npv_host, dtype=np.int64)
+    n = len(npv)
+    tg = np.arange(n, dtype=np.int64) if targets is None else np.asarray(targets, np.int64)
+    sr = np.arange(n, dtype=np.int64) if sources is None else np.asarray(sources, np.int64)
+    s_sorted = sr[np.argsort(npv[sr], kind="stable")]
+    # sources that can pair with the cheapest targets (b_x = 2: 3 rows)
+    bs = npv[s_sorted]
+    n_src = int(np.searchsorted(bs, min(thr // 3, ONEHOT_MAX_LEVELS), side="right"))
+    if n_src == 0:
+        return None
+    s_use = s_sorted[:n_src]
+    s_g, s_col, n_cg = _groups(npv[s_use])
+    cg_max = np.zeros(n_cg, np.int64)
+    np.maximum.at(cg_max, s_g, npv[s_use])
+    cg_end = np.searchsorted(s_g, np.arange(n_cg), side="right")   # sources before group end
+    # targets: 2 <= b_x, b_x^2 - 1 <= GROUP, (b_x^2 - 1) * b_min <= thr
+    bmin = int(bs[0])
+    t_ok = tg[(npv[tg] >= 2) & (npv[tg] ** 2 - 1 <= GROUP) &
+              ((npv[tg] ** 2 - 1) * max(bmin, 1) <= thr)]
+    t_use = t_ok[np.argsort(npv[t_ok], kind="stable")]
+    if len(t_use) == 0:
+        return None
+    t_g, t_row, n_rg = _groups(npv[t_use] ** 2 - 1)
+    rg_max = np.zeros(n_rg, np.int64)
+    np.maximum.at(rg_max, t_g, npv[t_use])
+    # column groups of each row group: the leading g with (BX^2 - 1) BY <= thr
+    ncg = np.array([int(np.searchsorted((rg_max[r] ** 2 - 1) * cg_max, thr, side="right"))
+                    for r in range(n_rg)], np.int64)
+    # drop row groups with no column group (only trailing ones: ncg falls as r grows)
+    keep_rg = int(np.count_nonzero(ncg > 0))
+    if keep_rg == 0:
+        return None
+    t_sel = t_g < keep_rg
+    t_use, t_g, t_row = t_use[t_sel], t_g[t_sel], t_row[t_sel]
+    n_rg, ncg, rg_max = keep_rg, ncg[:keep_rg], rg_max[:keep_rg]
+    src_skip = np.zeros(n, np.int32)
+    src_skip[t_use] = cg_end[ncg[t_g] - 1]
+    # pairs covered (target != source)
+    pos = np.full(n, -1, np.int64)
+    pos[s_use] = np.arange(n_src)
+    n_pairs = int(src_skip[t_use].astype(np.int64).sum() -
+                  np.count_nonzero((pos[t_use] >= 0) & (pos[t_use] < src_skip[t_use])))
+    # tiles: row panels (2i, 2i + 1) x column panels (2j, 2j + 1)
+    tiles = []
+    cells = 0
+    for i in range(0, n_rg, 2):
+        r = [i, i + 1 if i + 1 < n_rg else -1]
+        for j in range(0, int(ncg[i]), 2):
+            c = [j, j + 1 if j + 1 < n_cg else -1]
+            mask = 0
+            for wr in range(2):
+                for wc in range(2):
+                    if r[wr] >= 0 and c[wc] >= 0 and c[wc] < ncg[r[wr]]:
+                        mask |= 1 << (2 * wr + wc)
+                        cells += GROUP * GROUP
+            if mask:
+                tiles.append([r[0], r[1], c[0], c[1], mask, 0, 0, 0])
+    tiles = np.asarray(tiles, np.int32).reshape(-1, 8)
+    # operand row descriptions
+    rows_a, rows_b = n_rg * GROUP, n_cg * GROUP
+    arow = np.zeros((rows_a, 2), np.int32)
+    arow[:, 0] = -1
+    for x, g, r0 in zip(t_use, t_g, t_row):
+        bx = int(npv[x])
+        k = np.arange(bx * bx - 1)
+        code = np.where(k < (bx - 1) * bx, ((k // bx) << 8) | (k % bx),
+                        (255 << 8) | (k - (bx - 1) * bx))
+        base = g * GROUP + r0
+        arow[base: base + len(k), 0] = x
+        arow[base: base + len(k), 1] = code
+    brow = np.zeros((rows_b, 2), np.int32)
+    brow[:, 0] = -1
+    for y, g, c0 in zip(s_use, s_g, s_col):
+        by = int(npv[y])
+        base = g * GROUP + c0
+        brow[base: base + by, 0] = y
+        brow[base: base + by, 1] = np.arange(by)
+    rg_off = np.searchsorted(t_g, np.arange(n_rg + 1)).astype(np.int32)
+    cg_off = np.searchsorted(s_g, np.arange(n_cg + 1)).astype(np.int32)
+    mvars = np.unique(np.concatenate([t_use, s_use])).astype(np.int32)
+    moff = np.full(n, -1, np.int64)
+    sizes = npv[mvars] ** 2 + npv[mvars]
+    moff[mvars] = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    Kp = -(-int(L) // 128) * 128   # two 64-sample FP4 chunks per K stage
+    h = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+    keep = [torch.empty(rows_a * Kp, dtype=torch.int8, device=device),
+            torch.empty(rows_b * Kp, dtype=torch.int8, device=device),
+            h(arow), h(brow), torch.zeros(int(sizes.sum()), dtype=torch.int32, device=device),
+            h(moff), h(tiles), h(rg_off), h(t_use.astype(np.int32)), h(t_row.astype(np.int32)),
+            h(cg_off), h(s_use.astype(np.int32)), h(s_col.astype(np.int32)), h(mvars)]
+    A, B, ar, br, marg, mo, tl, ro, tv, tr, co, sv, sc, mv = keep
+    args = _lib.OnehotArgs(
+        A=ptr(A), B=ptr(B), Kp=Kp, rows_a=rows_a, rows_b=rows_b, arow=ptr(ar), brow=ptr(br),
+        marg=ptr(marg), moff=ptr(mo), tiles=ptr(tl), n_tiles=len(tiles), rg_off=ptr(ro),
+        t_var=ptr(tv), t_row=ptr(tr), cg_off=ptr(co), s_var=ptr(sv), s_col=ptr(sc),
+        mvars=ptr(mv), n_mvars=len(mvars))
+    return OnehotPlan(args=args, src_skip=src_skip, n_pairs=n_pairs, n_tiles=len(tiles),
+                      cells=cells, keep=keep)
+
+
+def covered_pairs(op: OnehotPlan, npv_host: np.ndarray, sources=None) -> np.ndarray:
+    """[target, source] pairs the plan counts (host; tests and planning)."""
+    npv = np.asarray(npv_host, np.int64)
+    n = len(npv)
+    sr = np.arange(n, dtype=np.int64) if sources is None else np.asarray(sources, np.int64)
+    s_sorted = sr[np.argsort(npv[sr], kind="stable")]
+    out = []
+    for x in np.nonzero(op.src_skip)[0]:
+        s = s_sorted[: op.src_skip[x]]
+        s = s[s != x]
+        out.append(np.stack([np.full(len(s), x), s], 1))
+    return np.concatenate(out).astype(np.int64) if out else np.zeros((0, 2), np.int64)
+
+
+def run(prep, op: OnehotPlan, te_args: _lib.TeArgs, stream: int):
+    """Operands and marginals from prep's bins, then the TE of every pair of
+    the plan through te_args' output mode."""
+    call("tnx_onehot_pack", C.byref(op.args), ptr(prep.bins), prep.T, prep.ld, prep.tau,
+         ptr(prep.npv), stream)
+    call("tnx_onehot_te", C.byref(op.args), C.byref(te_args), stream)

@@ -29,9 +29,10 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_error_reporting():
     lib = _lib.load()
-    assert lib.tnx_abi_version() == _lib.ABI_VERSION == 9
+    assert lib.tnx_abi_version() == _lib.ABI_VERSION == 10
     info = _lib.PlanInfo()
-    rc = lib.tnx_te_plan(None, 3, 10, None, 0, None, 0, 0, 0, None, None, None, 0, C.byref(info))
+    rc = lib.tnx_te_plan(None, 3, 10, None, 0, None, 0, 0, 0, None, None, None, None, 0,
+                         C.byref(info))
     assert rc == -1
     assert b"null pointer" in lib.tnx_last_error()
 
@@ -48,6 +49,8 @@ int main(void) {{
   printf("%zu %zu %zu %zu %zu\\n", offsetof(tnx_te_args_t, out), offsetof(tnx_surr_args_t, sum_te),
          offsetof(tnx_plan_info_t, est_cost_wave), offsetof(tnx_te_args_t, queue_items),
          offsetof(tnx_te_args_t, counts_ld));
+  printf("%zu %zu %zu\\n", sizeof(tnx_onehot_args_t), offsetof(tnx_onehot_args_t, tiles),
+         offsetof(tnx_onehot_args_t, n_mvars));
   return 0;
 }}''')
     exe = tmp_path / "sz"
@@ -56,7 +59,8 @@ int main(void) {{
     got = [C.sizeof(_lib.DeviceProps), C.sizeof(_lib.Work), C.sizeof(_lib.PlanInfo),
            C.sizeof(_lib.TeArgs), C.sizeof(_lib.SurrArgs), C.sizeof(_lib.Mlp),
            C.sizeof(_lib.Adam), _lib.TeArgs.out.offset, _lib.SurrArgs.sum_te.offset, _lib.PlanInfo.est_cost_wave.offset,
-           _lib.TeArgs.queue_items.offset, _lib.TeArgs.counts_ld.offset]
+           _lib.TeArgs.queue_items.offset, _lib.TeArgs.counts_ld.offset,
+           C.sizeof(_lib.OnehotArgs), _lib.OnehotArgs.tiles.offset, _lib.OnehotArgs.n_mvars.offset]
     assert [int(v) for v in out] == got
 
 
@@ -99,7 +103,7 @@ def test_te_queue_partitions_every_entry():
     n, L = 700, 2999
     npv = rng.integers(2, 40, size=n).astype(np.int32)
     info = _lib.PlanInfo()
-    args = (npv.ctypes.data, n, L, None, 0, None, 0, 0, 0)
+    args = (npv.ctypes.data, n, L, None, 0, None, 0, 0, 0, None)
     assert _lib.call("tnx_te_plan", *args, None, None, None, 0, C.byref(info)) == 0
     nw, nb = info.n_wave_items, info.n_block_items
     assert nw > 0 and nb > 0

@@ -14,13 +14,14 @@ from tenex_amd.inference import (bh_fdr_gpu, build_pairs, fdr_threshold_cpu, mak
 from tenex_amd.kernels import auto_select, get_kernel
 
 
-def _plan_host(npv, L, tg=None, sr=None, cap_wave=0, cap_block=0):
+def _plan_host(npv, L, tg=None, sr=None, cap_wave=0, cap_block=0, skip=None):
     npv = np.ascontiguousarray(npv, np.int32)
     info = _lib.PlanInfo()
     tgp = None if tg is None else np.ascontiguousarray(tg, np.int32)
     srp = None if sr is None else np.ascontiguousarray(sr, np.int32)
     args = (_lib.ptr(npv), len(npv), L, _lib.ptr(tgp), 0 if tg is None else len(tg),
-            _lib.ptr(srp), 0 if sr is None else len(sr), cap_wave, cap_block)
+            _lib.ptr(srp), 0 if sr is None else len(sr), cap_wave, cap_block,
+            _lib.ptr(skip))
     _lib.call("tnx_te_plan", *args, None, None, None, 0, C.byref(info))
     n_items = info.n_wave_items + info.n_block_items
     tgo = np.empty(info.n_tgt, np.int32)
