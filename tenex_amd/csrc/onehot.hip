@@ -47,7 +47,7 @@ constexpr int kOhStageBytes = 16 * 1024 * kOhNK;  // 8 NK A + 8 NK B blocks of 1
 constexpr int kOhStr = 130;                   // u16 row stride of a staged sub-tile
 constexpr int kOhSubBytes = kOhGroup * kOhStr * 2;
 constexpr int kOhTbl = 2048;                  // c log2 c entries kept in LDS
-constexpr int kOhLdsBytes = 4 * kOhSubBytes + kOhTbl * 8;
+constexpr int kOhLdsBytes = 4 * kOhSubBytes + kOhTbl * 8 + 4 * kOhGroup * 4;  // + n_y(c) by column
 static_assert(2 * kOhStageBytes <= 4 * kOhSubBytes, "stages alias the staging area");
 static_assert(kOhLdsBytes <= 160 * 1024, "LDS");
 
@@ -208,6 +208,60 @@ __device__ __forceinline__ int64_t oh_f(const int64_t *lt, const int64_t *__rest
     return u < (uint32_t)kOhTbl ? lt[u] : gt[u < L ? u : L];
 }
 
+// S3 - S2b of one pair (target of BX levels) from its staged counts: G
+// points at the target's first row in the pair's first column (row stride
+// kOhStr), nyc at the source's n_y(c).  Per column, the counts of a batch
+// of x_t levels are read first, then all their c log2 c lookups, so a
+// lane has many independent LDS reads in flight.
+template <int BX>
+__device__ __forceinline__ int64_t oh_pair(const uint16_t *G, const int32_t *nyc, int by,
+                                           const int64_t *lfx, const int64_t *__restrict__ gfx,
+                                           uint32_t uL, int32_t *cdbg) {
+    constexpr int R1 = BX - 1;
+    constexpr int BB = BX <= 4 ? BX : 4;  // x_t levels per batch
+    int64_t s3 = 0, s2 = 0;
+    for (int c = 0; c < by; c++) {
+        const int32_t ny = nyc[c];
+        int32_t sumM = 0;  // M[b, c] over b < R1
+#pragma unroll
+        for (int b0 = 0; b0 < BX; b0 += BB) {
+            int32_t v[BB][R1], M[BB];
+#pragma unroll
+            for (int u = 0; u < BB; u++) {
+                const int b = b0 + u;
+                if (b < BX) {
+#pragma unroll
+                    for (int aa = 0; aa < R1; aa++) v[u][aa] = G[(aa * BX + b) * kOhStr + c];
+                    M[u] = b < R1 ? (int32_t)G[(R1 * BX + b) * kOhStr + c] : 0;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < BB; u++) {
+                const int b = b0 + u;
+                if (b < BX) {
+                    if (b == R1) M[u] = ny - sumM;  // the last x_t level from n_y(c)
+                    else sumM += M[u];
+                    int32_t sa = 0;
+                    int64_t f = 0;
+#pragma unroll
+                    for (int aa = 0; aa < R1; aa++) {
+                        f += oh_f(lfx, gfx, v[u][aa], uL);
+                        sa += v[u][aa];
+                    }
+                    s3 += f + oh_f(lfx, gfx, M[u] - sa, uL);
+                    s2 += oh_f(lfx, gfx, M[u], uL);
+                    if (cdbg) {
+#pragma unroll
+                        for (int aa = 0; aa < R1; aa++) cdbg[(aa * BX + b) * by + c] = v[u][aa];
+                        cdbg[(R1 * BX + b) * by + c] = M[u] - sa;
+                    }
+                }
+            }
+        }
+    }
+    return s3 - s2;
+}
+
 // The K loop and the per-pair sums of one tile (see the file comment).
 __global__ __launch_bounds__(256) void oh_mfma_kernel(tnx_onehot_args_t o, tnx_te_args_t a) {
     extern __shared__ __attribute__((aligned(16))) uint8_t oh_lds[];
@@ -226,8 +280,11 @@ __global__ __launch_bounds__(256) void oh_mfma_kernel(tnx_onehot_args_t o, tnx_t
     // fetches blocks w * 4 NK .. +4 NK.
     const int64_t kblk = o.Kp / kOhKChunk;  // 1 KB blocks per row block
     constexpr int PER = 4 * kOhNK;  // blocks per wave per stage
+    // An absent group (-1) reads group 0 instead: every load is issued
+    // unconditionally (the data of an absent group is never used), so the
+    // compiler counts the loads in flight and waits only for the oldest
+    // stage.
     const int8_t *src[PER];
-    bool have[PER];
 #pragma unroll
     for (int i = 0; i < PER; i++) {
         const int j = wave * PER + i;
@@ -235,7 +292,6 @@ __global__ __launch_bounds__(256) void oh_mfma_kernel(tnx_onehot_args_t o, tnx_t
         const int jj = isB ? j - 8 * kOhNK : j;
         const int g = jj / (4 * kOhNK), rb = (jj / kOhNK) % 4, kc = jj % kOhNK;
         const int grp = isB ? (g ? cg1 : cg0) : (g ? rg1 : rg0);
-        have[i] = grp >= 0;
         const int8_t *base = isB ? o.B : o.A;
         src[i] = base + ((int64_t)(grp >= 0 ? grp : 0) * 4 + rb) * kblk * 1024 + (int64_t)kc * 1024 +
                  16 * lane;
@@ -244,8 +300,7 @@ __global__ __launch_bounds__(256) void oh_mfma_kernel(tnx_onehot_args_t o, tnx_t
     auto fetch = [&](v4i (&r)[PER], int64_t s) {
 #pragma unroll
         for (int i = 0; i < PER; i++)
-            r[i] = have[i] ? *reinterpret_cast<const v4i *>(src[i] + s * kOhNK * 1024)
-                           : v4i{0, 0, 0, 0};
+            r[i] = *reinterpret_cast<const v4i *>(src[i] + s * kOhNK * 1024);
     };
     auto put = [&](const v4i (&r)[PER], int buf) {
 #pragma unroll
@@ -284,20 +339,22 @@ __global__ __launch_bounds__(256) void oh_mfma_kernel(tnx_onehot_args_t o, tnx_t
     // two register sets, two LDS buffers: stage s is written to buffer s & 1
     // and stage s + 2 fetched into the freed registers before stage s is
     // computed; one barrier per stage
+    // (fetches past the last stage re-read the last stage: no branch around
+    // a load, so the loads of the newer stage stay in flight at each put)
+    const int64_t last = n_st - 1;
     v4i r0[PER], r1[PER];
     fetch(r0, 0);
-    if (n_st > 1) fetch(r1, 1);
+    __builtin_amdgcn_sched_barrier(0);  // keep stage 0's loads ahead of stage 1's
+    fetch(r1, last < 1 ? last : 1);
     for (int64_t s = 0; s < n_st; s += 2) {
         put(r0, 0);
-        if (s + 2 < n_st) fetch(r0, s + 2);
+        fetch(r0, s + 2 < last ? s + 2 : last);
         __syncthreads();
         compute(0);
-        if (s + 1 < n_st) {
-            put(r1, 1);
-            if (s + 3 < n_st) fetch(r1, s + 3);
-            __syncthreads();
-            compute(1);
-        }
+        put(r1, 1);
+        fetch(r1, s + 3 < last ? s + 3 : last);
+        __syncthreads();
+        if (s + 1 < n_st) compute(1);
     }
     __syncthreads();  // the stages are dead: the staging area reuses them
 
@@ -319,6 +376,15 @@ __global__ __launch_bounds__(256) void oh_mfma_kernel(tnx_onehot_args_t o, tnx_t
     __syncthreads();  // staging and table complete
     if (!active) return;
     const int grow = wr ? rg1 : rg0, gcol = wc ? cg1 : cg0;
+    // n_y(c) of this column group's columns, indexed by column
+    int32_t *mcol = reinterpret_cast<int32_t *>(oh_lds + 4 * kOhSubBytes + kOhTbl * 8) + wave * kOhGroup;
+    for (int cc = lane; cc < kOhGroup; cc += kWave) {
+        const int64_t br = (int64_t)gcol * kOhGroup + cc;
+        const int v = o.brow[2 * br];
+        mcol[cc] = v >= 0 ? o.marg[o.moff[v] + a.n_per_var[v] * a.n_per_var[v] + o.brow[2 * br + 1]]
+                          : 0;
+    }
+    wave_lds_fence();
     const int tb = o.rg_off[grow], nt = o.rg_off[grow + 1] - tb;
     const int sb = o.cg_off[gcol], ns = o.cg_off[gcol + 1] - sb;
     const double invN = 1.0 / (double)L;
@@ -330,48 +396,23 @@ __global__ __launch_bounds__(256) void oh_mfma_kernel(tnx_onehot_args_t o, tnx_t
         if (x == y || !out_slot(a, x, y, slot)) continue;
         const int bx = a.n_per_var[x], by = a.n_per_var[y];
         const uint16_t *G = stg + o.t_row[ti] * kOhStr + o.s_col[sj];
-        const int32_t *ny = o.marg + o.moff[y] + by * by;  // n_y(c)
-        const int R1 = bx - 1;
+        const int32_t *nyc = mcol + o.s_col[sj];
         int32_t *cdbg = nullptr;  // test hook: this pair's joint counts N[a, b, c]
         if (a.counts_map) {
             const int r = a.counts_map[(int64_t)x * a.n_vars + y];
             if (r >= 0 && bx * bx * by <= a.counts_ld) cdbg = a.counts_dbg + (int64_t)r * a.counts_ld;
         }
-        int64_t s3 = 0, s2 = 0;
-        for (int c = 0; c < by; c++) {
-            int32_t sumM = 0;
-            for (int b = 0; b < bx; b++) {
-                const int32_t M = b < R1 ? (int32_t)G[(R1 * bx + b) * kOhStr + c] : ny[c] - sumM;
-                sumM += M;
-                int32_t sa = 0;
-                int64_t f3 = 0;
-                int a0 = 0;
-                for (; a0 + 4 <= R1; a0 += 4) {
-                    int32_t v[4];
-#pragma unroll
-                    for (int u = 0; u < 4; u++) v[u] = G[((a0 + u) * bx + b) * kOhStr + c];
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        f3 += oh_f(lfx, a.xlogx_fx, v[u], uL);
-                        sa += v[u];
-                    }
-                }
-                for (; a0 < R1; a0++) {
-                    const int32_t v = G[(a0 * bx + b) * kOhStr + c];
-                    f3 += oh_f(lfx, a.xlogx_fx, v, uL);
-                    sa += v;
-                }
-                s3 += f3 + oh_f(lfx, a.xlogx_fx, M - sa, uL);
-                s2 += oh_f(lfx, a.xlogx_fx, M, uL);
-                if (cdbg) {
-                    for (int u = 0; u < R1; u++)
-                        cdbg[(u * bx + b) * by + c] = G[(u * bx + b) * kOhStr + c];
-                    cdbg[(R1 * bx + b) * by + c] = M - sa;
-                }
-            }
+        int64_t d = 0;
+        switch (bx) {
+#define TNX_OH_PAIR(BX) \
+    case BX: d = oh_pair<BX>(G, nyc, by, lfx, a.xlogx_fx, uL, cdbg); break;
+            TNX_OH_PAIR(2) TNX_OH_PAIR(3) TNX_OH_PAIR(4) TNX_OH_PAIR(5) TNX_OH_PAIR(6)
+            TNX_OH_PAIR(7) TNX_OH_PAIR(8) TNX_OH_PAIR(9) TNX_OH_PAIR(10) TNX_OH_PAIR(11)
+#undef TNX_OH_PAIR
+            default: break;  // the planner takes 2 <= b_x <= 11
         }
         const double s12 = a.S1[x] - a.S2a[x];
-        emit_te(a, slot, (ldexp((double)(s3 - s2), -a.fx_shift) + s12) * invN);
+        emit_te(a, slot, (ldexp((double)d, -a.fx_shift) + s12) * invN);
     }
 }
 

@@ -32,9 +32,10 @@ from . import _lib
 from ._lib import call, ptr
 
 GROUP = 128            # rows (columns) of a row (column) group
-ONEHOT_THR = 200       # largest (b_x^2 - 1) * b_y a one-hot pair may have
+ONEHOT_THR = 400       # largest (b_x^2 - 1) * b_y a one-hot pair may have
 ONEHOT_MAX_LEVELS = 64  # sources (a target's b_x^2 - 1 rows fit one 128-row group: b_x <= 11)
 ONEHOT_MAX_L = 65535   # counts are staged as u16
+BAND = 4               # row panels per band of the tile order
 
 
 @dataclass
@@ -111,21 +112,29 @@ def plan(npv_host: np.ndarray, L: int, device, targets=None, sources=None,
     pos[s_use] = np.arange(n_src)
     n_pairs = int(src_skip[t_use].astype(np.int64).sum() -
                   np.count_nonzero((pos[t_use] >= 0) & (pos[t_use] < src_skip[t_use])))
-    # tiles: row panels (2i, 2i + 1) x column panels (2j, 2j + 1)
+    # tiles: row panels (2i, 2i + 1) x column panels (2j, 2j + 1), in bands
+    # of BAND row panels crossed with the column panels, so that the tiles
+    # an XCD runs at the same time (a contiguous run of this order) share
+    # both operands in its L2
     tiles = []
     cells = 0
-    for i in range(0, n_rg, 2):
-        r = [i, i + 1 if i + 1 < n_rg else -1]
-        for j in range(0, int(ncg[i]), 2):
+    panels = list(range(0, n_rg, 2))
+    for b0 in range(0, len(panels), BAND):
+        band = panels[b0: b0 + BAND]
+        for j in range(0, int(ncg[band[0]]), 2):
             c = [j, j + 1 if j + 1 < n_cg else -1]
-            mask = 0
-            for wr in range(2):
-                for wc in range(2):
-                    if r[wr] >= 0 and c[wc] >= 0 and c[wc] < ncg[r[wr]]:
-                        mask |= 1 << (2 * wr + wc)
-                        cells += GROUP * GROUP
-            if mask:
-                tiles.append([r[0], r[1], c[0], c[1], mask, 0, 0, 0])
+            for i in band:
+                if j >= ncg[i]:
+                    continue
+                r = [i, i + 1 if i + 1 < n_rg else -1]
+                mask = 0
+                for wr in range(2):
+                    for wc in range(2):
+                        if r[wr] >= 0 and c[wc] >= 0 and c[wc] < ncg[r[wr]]:
+                            mask |= 1 << (2 * wr + wc)
+                            cells += GROUP * GROUP
+                if mask:
+                    tiles.append([r[0], r[1], c[0], c[1], mask, 0, 0, 0])
     tiles = np.asarray(tiles, np.int32).reshape(-1, 8)
     # operand row descriptions
     rows_a, rows_b = n_rg * GROUP, n_cg * GROUP
