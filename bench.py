@@ -792,7 +792,7 @@ def main():
         sys.exit(2)
     if args.onehot_thr is None:
         from tenex_amd import onehot
-        args.onehot_thr = onehot.ONEHOT_THR
+        args.onehot_thr = onehot.default_threshold(10000 - 1)
     if args.traffic_probe:
         run_traffic_probe(args)
         return

@@ -778,11 +778,37 @@ def onehot_enabled() -> bool:
     return os.environ.get("TNX_ONEHOT", "1") != "0"
 
 
+def get_te_plan(prep: Prepared, targets=None, sources=None, cap_wave: int = 0, cap_block: int = 0,
+                thr: int | None = None) -> Plan:
+    """Histogram plan for targets x sources that leaves the pairs of few
+    levels to a one-hot plan (onehot.py), kept as plan.cache["onehot"]
+    (absent when onehot_enabled() is off or no pair qualifies).  thr: the
+    one-hot threshold (None: onehot.ONEHOT_THR)."""
+    oh = get_onehot_plan(prep, targets, sources, thr) if onehot_enabled() else None
+    plan = get_plan(prep, targets, sources, cap_wave, cap_block,
+                    src_skip=None if oh is None else oh.src_skip)
+    if oh is not None:
+        plan.cache["onehot"] = oh
+    return plan
+
+
+def plan_all_pairs(plan: Plan, npv_host: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """(target, source) of every pair a plan counts: its histogram items and
+    its one-hot plan's pairs (target != source)."""
+    x, y = plan_pairs(plan)
+    oh = plan.cache.get("onehot")
+    if oh is not None:
+        from . import onehot
+        cov = onehot.covered_pairs(oh, npv_host)
+        x, y = np.concatenate([x, cov[:, 0]]), np.concatenate([y, cov[:, 1]])
+    return x, y
+
+
 def get_onehot_plan(prep: Prepared, targets=None, sources=None, thr: int | None = None):
     """Cached one-hot plan of prep's narrow variables (None when no pair
     qualifies or when the data hold variables of more than 127 levels)."""
     from . import onehot
-    thr = onehot.ONEHOT_THR if thr is None else int(thr)
+    thr = onehot.default_threshold(prep.L) if thr is None else int(thr)
     if len(prep.wide) or thr <= 0:
         return None
     key = ("onehot", _digest(prep.npv_host), int(prep.L), str(prep.device),
@@ -876,9 +902,8 @@ def te_matrix_internal(prep: Prepared, targets=None, sources=None, cap_wave: int
         wp = np.concatenate(parts)
         te_sort_pairs(prep, wp[wp[:, 0] != wp[:, 1]], out, n)
         return out
-    oh = get_onehot_plan(prep, targets, sources) if onehot_enabled() else None
-    plan = get_plan(prep, targets, sources, cap_wave, cap_block,
-                    src_skip=None if oh is None else oh.src_skip)
+    plan = get_te_plan(prep, targets, sources, cap_wave, cap_block)
+    oh = plan.cache.get("onehot")
     if oh is not None:
         from . import onehot
         onehot.run(prep, oh, te_args(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n),
@@ -1210,11 +1235,7 @@ def surrogate_stats_materialized(prep: Prepared, plan: Plan, perm: torch.Tensor,
     n_k = perm.shape[0]
     kb = copies_per_launch(prep, n_k, n * n)
     if kb >= 2:
-        slots = plan.cache.get("slots")
-        if slots is None:
-            x, y = plan_pairs(plan)
-            slots = h2d(np.unique(y * n + x), prep.device)
-            plan.cache["slots"] = slots
+        slots = _plan_slots(plan, n, prep.device, prep.npv_host)
         for k0 in range(0, n_k, kb):
             kk = min(kb, n_k - k0)
             _staged_batch(prep, plan, perm[k0:k0 + kk], block_length, work, slots, observed,
@@ -1236,9 +1257,13 @@ def surrogate_stats_materialized(prep: Prepared, plan: Plan, perm: torch.Tensor,
         observed = torch.zeros_like(sum_te, dtype=torch.float32)
     args = te_args(sp, plan, None, _lib.TNX_OUT_STATS, ld_out=n,
                    stats=(sum_te, sumsq_te, count_ge, observed))
+    oh = plan.cache.get("onehot")
     for kk in range(n_k):
         block_shuffle(prep.bins, prep.T, perm[kk], block_length, sp.bins)
         gene_prep_into(sp, features=False)
+        if oh is not None:
+            from . import onehot
+            onehot.run(sp, oh, args, _stream(prep.device))
         te_run(sp, plan, None, _lib.TNX_OUT_STATS, args=args)
 
 
@@ -1261,9 +1286,12 @@ def _staged_batch(prep: Prepared, plan: Plan, perm: torch.Tensor, block_length: 
     if bw is None:
         bw = _stacked_copies(prep, plan, kk)
         work[("batch", kk)] = bw
-    sp, args, stage = bw
+    sp, args, stage, ohK = bw
     block_shuffle(prep.bins, prep.T, perm.reshape(kk * n, -1), block_length, sp.bins, kk)
     gene_prep_into(sp, features=False)
+    if ohK is not None:
+        from . import onehot
+        onehot.run(sp, ohK, args, _stream(prep.device))
     te_run(sp, None, None, _lib.TNX_OUT_STAGE, args=args)
     call("tnx_stats_accumulate", ptr(stage), kk, n * n, ptr(slots), slots.numel(),
          ptr(observed), ptr(sum_te), ptr(sumsq_te), ptr(count_ge), _stream(prep.device))
@@ -1292,7 +1320,15 @@ def _stacked_copies(prep: Prepared, plan: Plan, K: int):
     stage = torch.empty(K * n * n, dtype=torch.float32, device=prep.device)
     args = te_args(sp, planK, stage, _lib.TNX_OUT_STAGE, ld_out=n, stage_vars=n,
                    stage_stride=n * n)
-    return sp, args, stage
+    ohK = None
+    oh = plan.cache.get("onehot")
+    if oh is not None:   # the one-hot pairs of each copy, all copies in one launch
+        ohK = plan.cache.get(("onehot_copies", K))
+        if ohK is None:
+            from . import onehot
+            ohK = onehot.replicate(oh, K, n, prep.device)
+            plan.cache[("onehot_copies", K)] = ohK
+    return sp, args, stage, ohK
 
 
 # ── several lags in one launch ───────────────────────────────────────────
@@ -1373,7 +1409,7 @@ def multilag_surrogate_stats(prep: Prepared, plan: Plan, taus, block_lengths, ob
     kmax = max(1, copies_per_launch(prep, n_surr * NL, n * n) // NL)
     n_batches = -(-n_surr // kmax)
     kb = -(-n_surr // n_batches)
-    slots = _plan_slots(plan, n, dev)
+    slots = _plan_slots(plan, n, dev, prep.npv_host)
     out = []
     for l in range(NL):
         s = torch.zeros((n, n), dtype=torch.float64, device=dev)
@@ -1452,12 +1488,13 @@ def var_subset(prep: Prepared, vars_host: np.ndarray, cache: bool = True) -> Pre
     return sub
 
 
-def _plan_slots(plan: Plan, m: int, device) -> torch.Tensor:
-    """Sorted public slots source * m + target of the plan's pairs over m
-    variables (device int64).  Cached on the plan."""
+def _plan_slots(plan: Plan, m: int, device, npv_host: np.ndarray) -> torch.Tensor:
+    """Sorted public slots source * m + target of the plan's pairs (its
+    one-hot plan's too) over m variables (device int64).  Cached on the
+    plan."""
     got = plan.cache.get("slots")
     if got is None:
-        x, y = plan_pairs(plan)
+        x, y = plan_all_pairs(plan, npv_host)
         got = h2d(np.unique(y * m + x), device)
         plan.cache["slots"] = got
     return got
@@ -1472,7 +1509,7 @@ def _stats_run(sub: Prepared, plan: Plan, rows, n_vars: int, observed, s, s2, cg
     m = sub.n_vars
     k_end = k_start + n_surr
     kb = max(1, copies_per_launch(sub, n_surr, m * m))
-    slots = _plan_slots(plan, m, sub.device)
+    slots = _plan_slots(plan, m, sub.device, sub.npv_host)
     work = {}
     for k0 in range(k_start, k_end, kb):
         k1 = min(k0 + kb, k_end)

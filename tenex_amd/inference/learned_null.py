@@ -202,8 +202,8 @@ def plain_stats_grid(prep, targets, sources, obs_pub, n_surr, shuffle_method, bl
     sr = np.asarray(sources, dtype=np.int64)
     involved = np.union1d(tg, sr)
     sub = core.var_subset(prep, involved)
-    plan = core.get_plan(sub, np.searchsorted(involved, tg).astype(np.int32),
-                         np.searchsorted(involved, sr).astype(np.int32))
+    plan = core.get_te_plan(sub, np.searchsorted(involved, tg).astype(np.int32),
+                            np.searchsorted(involved, sr).astype(np.int32))
     ss, ss2, _ = core.surrogate_stats_subset(prep, involved, plan, obs_pub, n_surr,
                                              shuffle_method, block_length, seed, rng_device,
                                              perm_batch=perm_batch)

@@ -120,9 +120,9 @@ def surrogate_accumulate(prep: core.Prepared, observed_pub: torch.Tensor, n_surr
         involved = np.union1d(tg, sr)
         if len(involved) < n:
             # shuffle and re-prepare only the variables the pairs touch
-            plan = core.get_plan(core.var_subset(prep, involved),
-                                 np.searchsorted(involved, tg).astype(np.int32),
-                                 np.searchsorted(involved, sr).astype(np.int32))
+            plan = core.get_te_plan(core.var_subset(prep, involved),
+                                    np.searchsorted(involved, tg).astype(np.int32),
+                                    np.searchsorted(involved, sr).astype(np.int32))
             ss, ss2, scg = core.surrogate_stats_subset(prep, involved, plan, observed_pub,
                                                        n_surrogates, shuffle_method,
                                                        block_length, seed, rng_device, k_start,
@@ -132,7 +132,7 @@ def surrogate_accumulate(prep: core.Prepared, observed_pub: torch.Tensor, n_surr
             s2[vd[:, None], vd[None, :]] = ss2
             cg[vd[:, None], vd[None, :]] = scg
             return s, s2, cg
-        plan = core.get_plan(prep, targets, sources)
+        plan = core.get_te_plan(prep, targets, sources)
         work = {}
         kb = max(perm_batch, core.copies_per_launch(prep, n_surrogates, n * n))
         for k0 in range(k_start, k_end, kb):

@@ -32,7 +32,9 @@ from . import _lib
 from ._lib import call, ptr
 
 GROUP = 128            # rows (columns) of a row (column) group
-ONEHOT_THR = 400       # largest (b_x^2 - 1) * b_y a one-hot pair may have
+ONEHOT_THR = 400       # largest (b_x^2 - 1) * b_y a one-hot pair may have (long series)
+ONEHOT_THR_SHORT = 1000   # the same for series of at most SHORT_L samples
+SHORT_L = 6000
 ONEHOT_MAX_LEVELS = 64  # sources (a target's b_x^2 - 1 rows fit one 128-row group: b_x <= 11)
 ONEHOT_MAX_L = 65535   # counts are staged as u16
 BAND = 4               # row panels per band of the tile order
@@ -46,6 +48,15 @@ class OnehotPlan:
     n_tiles: int
     cells: int                    # indicator cells of all tiles' active sub-tiles
     keep: list = field(default_factory=list)   # device arrays the args point into
+    host: dict = field(default_factory=dict)   # host copies of the plan arrays
+    sources: np.ndarray | None = None          # the source list the plan was made for
+
+
+def default_threshold(L: int) -> int:
+    """One-hot threshold for series of L samples: the histogram kernels'
+    per-pair work (clearing and reducing the histogram) weighs more against
+    the counting on short series, which favours the contraction there."""
+    return ONEHOT_THR_SHORT if L <= SHORT_L else ONEHOT_THR
 
 
 def _groups(sizes: np.ndarray, cap: int = GROUP):
@@ -162,26 +173,84 @@ def plan(npv_host: np.ndarray, L: int, device, targets=None, sources=None,
     sizes = npv[mvars] ** 2 + npv[mvars]
     moff[mvars] = np.concatenate([[0], np.cumsum(sizes)[:-1]])
     Kp = -(-int(L) // 128) * 128   # two 64-sample FP4 chunks per K stage
+    host = dict(arow=arow, brow=brow, moff=moff, msize=int(sizes.sum()), tiles=tiles,
+                rg_off=rg_off, t_var=t_use.astype(np.int32), t_row=t_row.astype(np.int32),
+                cg_off=cg_off, s_var=s_use.astype(np.int32), s_col=s_col.astype(np.int32),
+                mvars=mvars, Kp=Kp, n_rg=n_rg, n_cg=n_cg)
+    args, keep = _device_args(host, device)
+    return OnehotPlan(args=args, src_skip=src_skip, n_pairs=n_pairs, n_tiles=len(tiles),
+                      cells=cells, keep=keep, host=host,
+                      sources=None if sources is None else np.asarray(sources, np.int64))
+
+
+def _device_args(host: dict, device):
+    """Device arrays (operand buffers, marginals and the plan) and the
+    OnehotArgs pointing into them."""
+    Kp, rows_a, rows_b = host["Kp"], len(host["arow"]), len(host["brow"])
     h = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
-    keep = [torch.empty(rows_a * Kp, dtype=torch.int8, device=device),
-            torch.empty(rows_b * Kp, dtype=torch.int8, device=device),
-            h(arow), h(brow), torch.zeros(int(sizes.sum()), dtype=torch.int32, device=device),
-            h(moff), h(tiles), h(rg_off), h(t_use.astype(np.int32)), h(t_row.astype(np.int32)),
-            h(cg_off), h(s_use.astype(np.int32)), h(s_col.astype(np.int32)), h(mvars)]
+    keep = [torch.empty(rows_a * Kp // 2, dtype=torch.int8, device=device),
+            torch.empty(rows_b * Kp // 2, dtype=torch.int8, device=device),
+            h(host["arow"]), h(host["brow"]),
+            torch.zeros(host["msize"], dtype=torch.int32, device=device),
+            h(host["moff"]), h(host["tiles"]), h(host["rg_off"]), h(host["t_var"]),
+            h(host["t_row"]), h(host["cg_off"]), h(host["s_var"]), h(host["s_col"]),
+            h(host["mvars"])]
     A, B, ar, br, marg, mo, tl, ro, tv, tr, co, sv, sc, mv = keep
     args = _lib.OnehotArgs(
         A=ptr(A), B=ptr(B), Kp=Kp, rows_a=rows_a, rows_b=rows_b, arow=ptr(ar), brow=ptr(br),
-        marg=ptr(marg), moff=ptr(mo), tiles=ptr(tl), n_tiles=len(tiles), rg_off=ptr(ro),
-        t_var=ptr(tv), t_row=ptr(tr), cg_off=ptr(co), s_var=ptr(sv), s_col=ptr(sc),
-        mvars=ptr(mv), n_mvars=len(mvars))
-    return OnehotPlan(args=args, src_skip=src_skip, n_pairs=n_pairs, n_tiles=len(tiles),
-                      cells=cells, keep=keep)
+        marg=ptr(marg), moff=ptr(mo), tiles=ptr(tl), n_tiles=len(host["tiles"]),
+        rg_off=ptr(ro), t_var=ptr(tv), t_row=ptr(tr), cg_off=ptr(co), s_var=ptr(sv),
+        s_col=ptr(sc), mvars=ptr(mv), n_mvars=len(host["mvars"]))
+    return args, keep
+
+
+def replicate(op: OnehotPlan, K: int, m: int, device) -> OnehotPlan:
+    """The plan of op (over m variables) for K stacked copies, variable
+    k * m + j being copy k of variable j: every copy's pairs stay inside
+    the copy (row group g of copy k is group k * n_rg + g, likewise the
+    column groups)."""
+    H = op.host
+    n_rg, n_cg = H["n_rg"], H["n_cg"]
+    ks = np.arange(K, dtype=np.int64)
+
+    def rows(r):   # variable ids of copy k shifted by k * m; padding (-1) kept
+        out = np.tile(r, (K, 1)).reshape(K, -1, 2)
+        v = out[:, :, 0]
+        out[:, :, 0] = np.where(v >= 0, v + (ks * m)[:, None], -1)
+        return out.reshape(-1, 2).astype(np.int32)
+
+    tiles = np.tile(H["tiles"], (K, 1)).reshape(K, -1, 8)
+    for c, step in ((0, n_rg), (1, n_rg), (2, n_cg), (3, n_cg)):
+        t = tiles[:, :, c]
+        tiles[:, :, c] = np.where(t >= 0, t + (ks * step)[:, None], -1)
+    nt0, ns0 = len(H["t_var"]), len(H["s_var"])
+    moff = np.tile(H["moff"], K).reshape(K, -1)
+    moff = np.where(moff >= 0, moff + (ks * H["msize"])[:, None], -1)
+    host = dict(
+        arow=rows(H["arow"]), brow=rows(H["brow"]), moff=moff.reshape(-1),
+        msize=K * H["msize"], tiles=tiles.reshape(-1, 8).astype(np.int32),
+        rg_off=np.concatenate([H["rg_off"][:-1] + k * nt0 for k in range(K)] +
+                              [[K * nt0]]).astype(np.int32),
+        t_var=(np.tile(H["t_var"], (K, 1)) + (ks * m)[:, None]).reshape(-1).astype(np.int32),
+        t_row=np.tile(H["t_row"], K).astype(np.int32),
+        cg_off=np.concatenate([H["cg_off"][:-1] + k * ns0 for k in range(K)] +
+                              [[K * ns0]]).astype(np.int32),
+        s_var=(np.tile(H["s_var"], (K, 1)) + (ks * m)[:, None]).reshape(-1).astype(np.int32),
+        s_col=np.tile(H["s_col"], K).astype(np.int32),
+        mvars=(np.tile(H["mvars"], (K, 1)) + (ks * m)[:, None]).reshape(-1).astype(np.int32),
+        Kp=H["Kp"], n_rg=K * n_rg, n_cg=K * n_cg)
+    args, keep = _device_args(host, device)
+    return OnehotPlan(args=args, src_skip=np.tile(op.src_skip, K), n_pairs=K * op.n_pairs,
+                      n_tiles=len(host["tiles"]), cells=K * op.cells, keep=keep, host=host)
 
 
 def covered_pairs(op: OnehotPlan, npv_host: np.ndarray, sources=None) -> np.ndarray:
-    """[target, source] pairs the plan counts (host; tests and planning)."""
+    """[target, source] pairs the plan counts (host; tests and planning).
+    sources defaults to the list the plan was made for."""
     npv = np.asarray(npv_host, np.int64)
     n = len(npv)
+    if sources is None:
+        sources = op.sources
     sr = np.arange(n, dtype=np.int64) if sources is None else np.asarray(sources, np.int64)
     s_sorted = sr[np.argsort(npv[sr], kind="stable")]
     out = []

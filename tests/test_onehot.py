@@ -238,3 +238,22 @@ def test_onehot_matrix_equals_histogram_path(dev, case, monkeypatch):
         np.fill_diagonal(mask, False)
         assert np.abs(m1[mask] - ref[mask]).max() < 1e-6
         assert np.all(m1[~mask] == 0)
+
+
+def test_onehot_replicate_stays_inside_copies():
+    npv = _mixed_levels(70, 31)
+    m, K, T, tau = len(npv), 3, 257, 1
+    op = onehot.plan(npv, T - tau, "cpu", thr=300)
+    opk = onehot.replicate(op, K, m, "cpu")
+    cov = onehot.covered_pairs(op, npv)
+    exp = {(int(x) + k * m, int(y) + k * m) for k in range(K) for x, y in cov}
+    got = _tile_pairs(opk)
+    assert len(got) == len(set(got)) == opk.n_pairs and set(got) == exp
+    # the stacked copies' counts (different data per copy) through the scheme
+    npvK = np.tile(npv, K)
+    binsK = np.concatenate([_bins(npv, T, 40 + k) for k in range(K)])
+    counts = _emulate_counts(opk, binsK, npvK, T - tau, tau)
+    d = binsK.astype(np.int32)
+    for (x, y), N in counts.items():
+        assert x // m == y // m
+        assert np.array_equal(N, O.joint_counts(d, npvK, x, y, tau=tau)), (x, y)
