@@ -369,14 +369,25 @@ def h2d(a, device) -> torch.Tensor:
     return t.pin_memory().to(device, non_blocking=True)
 
 
+def d2h_pinned(t: torch.Tensor) -> np.ndarray:
+    """Device tensor -> host numpy array backed by pinned memory (the array
+    keeps the pinned buffer alive), so a later upload of the same array
+    (PendingUpload) needs no staging copy."""
+    out = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    out.copy_(t)
+    return out.numpy()
+
+
 _SIDE_STREAMS: dict = {}
 
 
 class PendingUpload:
     """A host array's copy to the device that overlaps other work: a
-    background thread stages it in pinned memory and starts the copy on a
-    side stream (default priority); get() waits for the thread and makes
-    the current stream wait for the copy, then returns the device tensor."""
+    background thread starts the copy on a side stream (default priority);
+    get() waits for the thread and makes the current stream wait for the
+    copy, then returns the device tensor.  An array in pinned memory (as
+    the engine returns its TE matrix, d2h_pinned) is copied directly;
+    others are staged in pinned memory first."""
 
     def __init__(self, a, device):
         self.device = torch.device(device)
@@ -392,11 +403,19 @@ class PendingUpload:
         self._thread = threading.Thread(target=self._run, daemon=True)
         self._thread.start()
 
+    CHUNK = 16 << 20   # bytes per copy: small copies of other streams slip in between
+
     def _run(self):
         try:
-            pinned = self._host.pin_memory()
+            h = self._host
+            pinned = h if h.is_pinned() else h.pin_memory()
             with torch.cuda.device(self.device), torch.cuda.stream(self._stream):
-                self._out = pinned.to(self.device, non_blocking=True)
+                out = torch.empty(h.shape, dtype=h.dtype, device=self.device)
+                src, dst = pinned.reshape(-1), out.reshape(-1)
+                step = max(1, self.CHUNK // max(1, h.element_size()))
+                for o in range(0, src.numel(), step):
+                    dst[o:o + step].copy_(src[o:o + step], non_blocking=True)
+                self._out = out
                 self._ev.record(self._stream)
         except BaseException as e:   # re-raised by get()
             self._err = e

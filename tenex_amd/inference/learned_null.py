@@ -121,12 +121,13 @@ class NullMLP:
         call("tnx_mlp_adam", C.byref(m), ptr(self.master), ptr(grad), ptr(self.m1), ptr(self.m2),
              self.n_params, float(lr), float(b1), float(b2), float(eps), self.t, self._s())
 
-    def fit(self, X, T, steps=300, lr=3e-3, lr_min=1e-5, batch=65536, seed=0):
+    def fit(self, X, T, steps=300, lr=3e-3, lr_min=1e-5, batch=65536, seed=0, lazy=False):
         """Adam with a cosine learning-rate schedule from lr down to lr_min,
         in one native call (tnx_mlp_fit: no host round trip per step).  Up
         to `batch` rows train full batch; larger sets are shuffled once
         (seeded) and each step takes the next `batch` rows of the shuffled
-        order.  Returns a few of the step losses."""
+        order.  Returns a few of the step losses (lazy: a callable that
+        reads them, so the caller picks when the host waits)."""
         n = X.shape[0]
         X = X.contiguous().to(torch.float32)
         T = T.contiguous().to(torch.float32)
@@ -146,8 +147,11 @@ class NullMLP:
              self.n_params, ptr(X), ptr(T), n, C.byref(opt), ptr(grad), ptr(ws),
              ws.numel() * 4, ptr(hist), self._s())
         self.t += steps
-        h = hist.cpu().numpy()
-        return [float(v) for v in h[:: max(1, steps // 8)]] + [float(h[-1])]
+
+        def losses():
+            h = hist.cpu().numpy()
+            return [float(v) for v in h[:: max(1, steps // 8)]] + [float(h[-1])]
+        return losses if lazy else losses()
 
 
 def pair_features(prep: core.Prepared, tgt: np.ndarray, src: np.ndarray) -> torch.Tensor:
@@ -253,8 +257,8 @@ def _tested_pairs(rows: np.ndarray, cols: np.ndarray, device=None):
 
 def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogates,
                            shuffle_method, block_length, seed, rng_device, p_method, fdr, mask,
-                           src_idx, dev, train_frac: float = 0.05, band: float = 1.0,
-                           steps: int = 150, lr: float = 6e-3, min_train: int = 2000,
+                           src_idx, dev, train_frac: float = 0.025, band: float = 1.0,
+                           steps: int = 90, lr: float = 1e-2, min_train: int = 2000,
                            model_seed: int = 0, band_sigmas: float = 5.0,
                            fit_batch: int = 4096, tested=None):
     """Learned-null surrogate test; returns a SurrogateTestResult with an
@@ -268,6 +272,11 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     from .surrogate_test import (SurrogateTestResult, surrogate_decision,
                                  surrogate_decision_pairs)
     t0 = time.perf_counter()
+    # phases are timed with events on the stream, read after the final wait:
+    # no host wait between phases, so host work overlaps the GPU's
+    stream = torch.cuda.current_stream(dev)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+    ev[0].record(stream)
     obs_src = obs_d       # a device tensor, or a core.PendingUpload still in flight
     n = prep.n_vars
     if tested is not None and (src_idx is None or
@@ -309,8 +318,7 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     ls, lt = lmap[pub_d[trd, 0]], lmap[pub_d[trd, 1]]
     n_train = int(trd.numel())
     mean_tr, std_tr = _moments(ss[ls, lt], ss2[ls, lt], n_surrogates)
-    torch.cuda.synchronize(dev)
-    t_train_plain = time.perf_counter() - t0
+    ev[1].record(stream)
     # 2. train the MLP on (mean, log std)
     X_all = pair_features(prep, pub_d[:, 1], pub_d[:, 0])
     mu, sd = _norm_stats(X_all)
@@ -322,8 +330,8 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     Yg[trd] = (Y - ymu) / ysd
     model = NullMLP(Xn.shape[1], 2, dev, seed=model_seed)
     losses = model.fit(Xn[fit_rows], Yg[fit_rows].to(torch.float32), steps=steps, lr=lr,
-                       batch=fit_batch, seed=seed)
-    t_fit = time.perf_counter() - t0 - t_train_plain
+                       batch=fit_batch, seed=seed, lazy=True)
+    ev[2].record(stream)
     # 3. predicted null for every pair, plain statistics where measured
     Yp = model.forward(Xn).to(torch.float64) * ysd + ymu
     mean_p = Yp[:, 0].contiguous()
@@ -351,17 +359,16 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     p_pred = (0.5 * torch.special.erfc(z / math.sqrt(2.0))).to(torch.float32)
     sig = bh_reject_gpu(p_pred, fdr) & (obs - mean_p > 0)
     z_sig_min = torch.where(sig, z, torch.full_like(z, math.inf)).min()
-    z_err, z_min = (float(v) for v in torch.stack([z_err_d, z_sig_min]).cpu())
-    band = max(band, band_sigmas * z_err)
-    if math.isfinite(z_min):
-        z_thr = z_min
-    else:
-        z_thr = float(-math.log(fdr / max(P, 1)))  # no predicted edge: a high bar
-    border = torch.abs(z - z_thr) < band
+    # band max(band, band_sigmas * z_err) and threshold z_min (no predicted
+    # edge: a high bar) stay on the device; the host reads them at the end
+    band_d = torch.clamp(band_sigmas * z_err_d, min=float(band))
+    z_thr_d = torch.where(torch.isfinite(z_sig_min), z_sig_min,
+                          torch.full_like(z_sig_min, float(-math.log(fdr / max(P, 1)))))
+    border = torch.abs(z - z_thr_d) < band_d
     border[trd] = False
     rr_d = torch.nonzero(border).flatten()
     rr = rr_d.cpu().numpy()
-    t_predict = time.perf_counter() - t0 - t_train_plain - t_fit
+    ev[3].record(stream)
     # the sums the decision reads: predicted pairs from the predicted moments,
     # measured pairs (grid, re-runs) their plain sums as they are, so that
     # their statistics equal the plain test's bit for bit
@@ -376,8 +383,7 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
                                          shuffle_method, block_length, seed, rng_device)
         S[rr_d] = s_b.to(torch.float64)
         S2[rr_d] = s2_b.to(torch.float64)
-    torch.cuda.synchronize(dev)
-    t_rerun = time.perf_counter() - t0 - t_train_plain - t_fit - t_predict
+    ev[4].record(stream)
     # 5. decision with the reference arithmetic on the merged null
     obs_d = obs_src.get() if isinstance(obs_src, core.PendingUpload) else obs_src
     if P == n_mask:
@@ -396,6 +402,9 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
     grn = make_grn(np.asarray(variable_names), edges.cpu().numpy().astype(np.int64),
                    eff[edges[:, 0], edges[:, 1]].cpu().numpy())
     t_total = time.perf_counter() - t0
+    t_train_plain, t_fit, t_predict, t_rerun = (ev[i].elapsed_time(ev[i + 1]) / 1e3
+                                                for i in range(4))
+    z_err, band, z_thr = (float(v) for v in torch.stack([z_err_d, band_d, z_thr_d]).cpu())
     vprint(f"[TENEX] learned null: trained on {n_train} pairs, re-ran {len(rr)} of {P}; "
            f"{len(grn)} edges")
     res = SurrogateTestResult(observed_te=obs_d, mean_surrogate_te=mean_m,
@@ -407,6 +416,6 @@ def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogate
                             z_err_holdout=z_err,
                             plain_fraction=(n_train + len(rr)) / max(P, 1), z_threshold=z_thr,
                             t_train_plain=t_train_plain, t_fit=t_fit, t_predict=t_predict,
-                            t_rerun=t_rerun, t_total=t_total, losses=losses,
+                            t_rerun=t_rerun, t_total=t_total, losses=losses(),
                             rerun_pairs=rr_pub)
     return res

@@ -293,7 +293,8 @@ class TransferEntropyEngine:
             internal = core.te_matrix_internal(prep, None, src_idx, cw, cb)
         if profile:
             ev[6].record(stream)
-        self._result_matrix = internal.t().contiguous().cpu().numpy()
+        # pinned host memory: NetWeaver's tests upload it again without staging
+        self._result_matrix = core.d2h_pinned(internal.t().contiguous())
         if profile:
             ev[7].record(stream)
             torch.cuda.synchronize(dev)

@@ -3,7 +3,9 @@
   4,000 x 10,000 mixed cardinality (seed 2): TE of 20,480 stratified pairs
       within 1e-6 of the FP64 oracle; raw joint counts of 256 sampled wave-
       and block-role pairs (single and multi-pass) bit-exact, dumped from
-      the production launch of the whole matrix.
+      the production launch of the whole matrix; with the one-hot split,
+      counts of 128 MFMA pairs bit-exact and the whole matrix equal to the
+      histogram-only one bit for bit.
   1,000 x 3,000 surrogate test (100 block shuffles of length 50, seed 42,
       torch CPU generator): per-pair FP64 sums, sums of squares and counts of
       2,000 sampled pairs against the oracle's shuffle-and-count.
@@ -73,6 +75,22 @@ def test_c2_mixed_cardinality_4000x10000(dev):
     err = np.abs(m[smp[:, 0], smp[:, 1]].astype(np.float64) - te64)
     assert err.max() < TOL, err.max()
     assert np.all(np.diag(m) == 0)
+    # the production split: pairs of few levels on the one-hot MFMA kernel
+    # (counts of 128 of them dumped from that launch), the rest in the
+    # histogram launch; the matrix equals the histogram-only one bit for bit
+    from tenex_amd import onehot
+    plan2 = core.get_te_plan(prep)
+    oh = plan2.cache["onehot"]
+    assert oh.n_pairs > 5_000_000 and plan2.info.n_pairs + oh.n_pairs == n * (n - 1)
+    cov = onehot.covered_pairs(oh, prep.npv_host)
+    sel2 = cov[np.random.default_rng(12).choice(len(cov), 128, replace=False)]
+    mc2 = int((npvh[sel2[:, 0]] ** 2 * npvh[sel2[:, 1]]).max())
+    counts2, m2 = core.counts_plan(prep, plan2, sel2, mc2, oh=oh)
+    counts2 = counts2.cpu().numpy()
+    for r, (t, s) in enumerate(sel2):
+        exp = O.joint_counts(d, prep.npv_host, int(t), int(s)).ravel()
+        assert np.array_equal(counts2[r, : exp.size], exp), (t, s, exp.size)
+    assert np.array_equal(m2.cpu().numpy(), m)
 
 
 def test_c3_surrogate_stats_1000x3000(dev):
