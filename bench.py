@@ -38,6 +38,7 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 MFMA_INT8_PEAK_TOPS = 5000.0
+MFMA_FP4_PEAK_TOPS = 10100.0   # dense FP4 MFMA (published MI355X figure)
 
 
 def log(msg):

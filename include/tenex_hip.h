@@ -50,7 +50,7 @@ extern "C" {
 #define TNX_API
 #endif
 
-#define TNX_ABI_VERSION 10
+#define TNX_ABI_VERSION 11
 
 /* Status codes. */
 #define TNX_OK 0
@@ -406,9 +406,10 @@ TNX_API int tnx_te_gemm_b2(const int8_t *bins, int64_t n, int64_t T, int64_t ld,
 /* The joint counts of pair (x, y) are a product of one-hot indicator rows:
  * target x contributes b_x^2 - 1 rows over t < L (states (a, b) =
  * (x[t+tau], x[t]) with a < b_x - 1 at row a*b_x + b, then x_t = b for
- * b < b_x - 1 at row (b_x - 1)*b_x + b), source y its b_y rows y[t] = c.
- * Counts with a = b_x - 1, and of x_t = b_x - 1, follow from the
- * per-variable marginals, so every joint count is an exact integer.
+ * b < b_x - 1 at row (b_x - 1)*b_x + b), source y its b_y rows y[t] = c
+ * (with src_ie, b_y - 1 rows).  Counts with a = b_x - 1, of x_t = b_x - 1
+ * (and with src_ie of c = b_y - 1) follow from the per-variable
+ * marginals, so every joint count is an exact integer.
  * Rows are grouped in 128-row (target) and 128-column (source) groups that
  * never split a variable; a tile is two row groups x two column groups
  * (one 128 x 128 sub-tile per wave, mask bit w = 2*wr + wc).
@@ -436,6 +437,10 @@ typedef struct {
     const int32_t *s_var, *s_col;  /* source variable, first column inside its group */
     const int32_t *mvars;          /* (n_mvars): variables whose marginals are counted */
     int64_t n_mvars;
+    int32_t src_ie;                /* 1: a source of b_y >= 2 levels has b_y - 1 rows (levels
+                                    * 0 .. b_y - 2); its last level's counts follow from the
+                                    * target's n(a, b) and n(b) */
+    int32_t pad0;
 } tnx_onehot_args_t;
 
 /* Marginals of mvars and the A, B operands from bins (int8, K = 1) for the

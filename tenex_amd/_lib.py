@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libtenex_hip.so")
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 TNX_OUT_PAIRFREE = 0
 TNX_OUT_MATRIX = 1
@@ -65,7 +65,7 @@ class OnehotArgs(C.Structure):
                 ("tiles", p_), ("n_tiles", i64),
                 ("rg_off", p_), ("t_var", p_), ("t_row", p_),
                 ("cg_off", p_), ("s_var", p_), ("s_col", p_),
-                ("mvars", p_), ("n_mvars", i64)]
+                ("mvars", p_), ("n_mvars", i64), ("src_ie", i32), ("pad0", i32)]
 
 
 TNX_QUEUES = 8

@@ -684,6 +684,16 @@ def te_args(prep: Prepared, plan: Plan, out: torch.Tensor | None, out_mode: int,
         var_L=ptr(var_L), cap_sort=int(cap_sort))
 
 
+def te_out_args(prep: Prepared, out: torch.Tensor, out_mode: int, ld_out: int) -> _lib.TeArgs:
+    """TE args with prep's per-variable data and an output only (no
+    histogram plan): what the one-hot kernel reads."""
+    return _lib.TeArgs(
+        bins=ptr(prep.bins), states=ptr(prep.states), n_per_var=ptr(prep.npv), S1=ptr(prep.S1),
+        S2a=ptr(prep.S2a), xlogx=ptr(prep.xlogx), n_vars=prep.n_vars, T=prep.T, ld=prep.ld,
+        lds=prep.lds, tau=prep.tau, out_mode=out_mode, out=ptr(out), ld_out=ld_out,
+        xlogx_fx=ptr(xlogx_fx_table(prep.L, prep.device)), fx_shift=fx_shift(prep.L))
+
+
 WAVE_ITEM_PAIRS = 32   # pairs of a one-target item that a single wave takes whole
 
 
@@ -823,7 +833,8 @@ def plan_all_pairs(plan: Plan, npv_host: np.ndarray) -> tuple[np.ndarray, np.nda
     return x, y
 
 
-def get_onehot_plan(prep: Prepared, targets=None, sources=None, thr: int | None = None):
+def get_onehot_plan(prep: Prepared, targets=None, sources=None, thr: int | None = None,
+                    src_ie: bool = True):
     """Cached one-hot plan of prep's narrow variables (None when no pair
     qualifies or when the data hold variables of more than 127 levels)."""
     from . import onehot
@@ -832,9 +843,10 @@ def get_onehot_plan(prep: Prepared, targets=None, sources=None, thr: int | None 
         return None
     key = ("onehot", _digest(prep.npv_host), int(prep.L), str(prep.device),
            _digest(None if targets is None else np.asarray(targets, np.int32)),
-           _digest(None if sources is None else np.asarray(sources, np.int32)), thr)
+           _digest(None if sources is None else np.asarray(sources, np.int32)), thr,
+           bool(src_ie))
     return _cached_plan(key, lambda: onehot.plan(prep.npv_host, prep.L, prep.device, targets,
-                                                 sources, thr))
+                                                 sources, thr, src_ie))
 
 
 # Per-pair cost of the occupied-key kernel in microseconds of one CU by sort

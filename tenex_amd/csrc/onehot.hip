@@ -210,17 +210,21 @@ __device__ __forceinline__ int64_t oh_f(const int64_t *lt, const int64_t *__rest
 
 // S3 - S2b of one pair (target of BX levels) from its staged counts: G
 // points at the target's first row in the pair's first column (row stride
-// kOhStr), nyc at the source's n_y(c).  Per column, the counts of a batch
-// of x_t levels are read first, then all their c log2 c lookups, so a
-// lane has many independent LDS reads in flight.
+// kOhStr), nyc at the source's n_y(c).  The first C1 columns are stored
+// (C1 = b_y, or b_y - 1 with the source's last level left out: then that
+// column is the rows' totals, nt(a, b) and nt(b) of the target, less the
+// stored columns).  Per column, the counts of a batch of x_t levels are
+// read first, then all their c log2 c lookups, so a lane has many
+// independent LDS reads in flight.
 template <int BX>
-__device__ __forceinline__ int64_t oh_pair(const uint16_t *G, const int32_t *nyc, int by,
+__device__ __forceinline__ int64_t oh_pair(const uint16_t *G, const int32_t *nyc, int32_t ny_last,
+                                           int by, int C1, const int32_t *__restrict__ nt,
                                            const int64_t *lfx, const int64_t *__restrict__ gfx,
                                            uint32_t uL, int32_t *cdbg) {
     constexpr int R1 = BX - 1;
     constexpr int BB = BX <= 4 ? BX : 4;  // x_t levels per batch
     int64_t s3 = 0, s2 = 0;
-    for (int c = 0; c < by; c++) {
+    for (int c = 0; c < C1; c++) {
         const int32_t ny = nyc[c];
         int32_t sumM = 0;  // M[b, c] over b < R1
 #pragma unroll
@@ -257,6 +261,32 @@ __device__ __forceinline__ int64_t oh_pair(const uint16_t *G, const int32_t *nyc
                     }
                 }
             }
+        }
+    }
+    if (C1 < by) {
+        // the source's last level, left out of the operands: each row's
+        // total (n(a, b) for a state row, n(b) for an x_t row: nt[BX^2 + b])
+        // less its stored columns
+        const int c = C1;
+        auto rest = [&](int r, int32_t tot) {
+            int32_t s = 0;
+            for (int cc = 0; cc < C1; cc++) s += (int32_t)G[r * kOhStr + cc];
+            return tot - s;
+        };
+        int32_t sumM = 0;
+        for (int b = 0; b < BX; b++) {
+            const int32_t M = b < R1 ? rest(R1 * BX + b, nt[BX * BX + b]) : ny_last - sumM;
+            sumM += M;
+            int32_t sa = 0;
+            for (int aa = 0; aa < R1; aa++) {
+                const int32_t v = rest(aa * BX + b, nt[aa * BX + b]);
+                s3 += oh_f(lfx, gfx, v, uL);
+                sa += v;
+                if (cdbg) cdbg[(aa * BX + b) * by + c] = v;
+            }
+            s3 += oh_f(lfx, gfx, M - sa, uL);
+            s2 += oh_f(lfx, gfx, M, uL);
+            if (cdbg) cdbg[(R1 * BX + b) * by + c] = M - sa;
         }
     }
     return s3 - s2;
@@ -396,7 +426,11 @@ __global__ __launch_bounds__(256) void oh_mfma_kernel(tnx_onehot_args_t o, tnx_t
         if (x == y || !out_slot(a, x, y, slot)) continue;
         const int bx = a.n_per_var[x], by = a.n_per_var[y];
         const uint16_t *G = stg + o.t_row[ti] * kOhStr + o.s_col[sj];
+        // n_y(c) of the stored columns from LDS; the left-out last level's from memory
+        const int C1 = (o.src_ie && by >= 2) ? by - 1 : by;
         const int32_t *nyc = mcol + o.s_col[sj];
+        const int32_t *nt = o.marg + o.moff[x];   // the target's n(a, b), n(b)
+        const int32_t ny_last = o.marg[o.moff[y] + by * by + by - 1];
         int32_t *cdbg = nullptr;  // test hook: this pair's joint counts N[a, b, c]
         if (a.counts_map) {
             const int r = a.counts_map[(int64_t)x * a.n_vars + y];
@@ -405,7 +439,7 @@ __global__ __launch_bounds__(256) void oh_mfma_kernel(tnx_onehot_args_t o, tnx_t
         int64_t d = 0;
         switch (bx) {
 #define TNX_OH_PAIR(BX) \
-    case BX: d = oh_pair<BX>(G, nyc, by, lfx, a.xlogx_fx, uL, cdbg); break;
+    case BX: d = oh_pair<BX>(G, nyc, ny_last, by, C1, nt, lfx, a.xlogx_fx, uL, cdbg); break;
             TNX_OH_PAIR(2) TNX_OH_PAIR(3) TNX_OH_PAIR(4) TNX_OH_PAIR(5) TNX_OH_PAIR(6)
             TNX_OH_PAIR(7) TNX_OH_PAIR(8) TNX_OH_PAIR(9) TNX_OH_PAIR(10) TNX_OH_PAIR(11)
 #undef TNX_OH_PAIR

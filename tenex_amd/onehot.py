@@ -75,10 +75,18 @@ def _groups(sizes: np.ndarray, cap: int = GROUP):
     return g, off, (cur + 1 if len(sizes) else 0)
 
 
+def src_cols(b, src_ie: bool):
+    """Operand rows (stored columns) of a source of b levels."""
+    return np.maximum(b - 1, 1) if src_ie else b
+
+
 def plan(npv_host: np.ndarray, L: int, device, targets=None, sources=None,
-         thr: int = ONEHOT_THR) -> OnehotPlan | None:
+         thr: int = ONEHOT_THR, src_ie: bool = True) -> OnehotPlan | None:
     """One-hot plan for the pairs of targets x sources that qualify, or None
-    when none does (thr <= 0, L too long, or no small pairs)."""
+    when none does (thr <= 0, L too long, or no small pairs).  A pair
+    qualifies when its (b_x^2 - 1) * cols(b_y) indicator cells are at most
+    thr, cols(b) = b, or with src_ie max(b - 1, 1) (the source's last level
+    then follows from the target's marginals)."""
     if thr <= 0 or L < 1 or L > ONEHOT_MAX_L:
         return None
     npv = np.asarray(npv_host, dtype=np.int64)
@@ -88,25 +96,28 @@ def plan(npv_host: np.ndarray, L: int, device, targets=None, sources=None,
     s_sorted = sr[np.argsort(npv[sr], kind="stable")]
     # sources that can pair with the cheapest targets (b_x = 2: 3 rows)
     bs = npv[s_sorted]
-    n_src = int(np.searchsorted(bs, min(thr // 3, ONEHOT_MAX_LEVELS), side="right"))
+    cs = src_cols(bs, src_ie)
+    n_src = int(np.searchsorted(cs, thr // 3, side="right"))   # cs ascends with bs
+    n_src = int(np.searchsorted(bs[:n_src], ONEHOT_MAX_LEVELS, side="right"))
     if n_src == 0:
         return None
     s_use = s_sorted[:n_src]
-    s_g, s_col, n_cg = _groups(npv[s_use])
+    s_g, s_col, n_cg = _groups(src_cols(npv[s_use], src_ie))
     cg_max = np.zeros(n_cg, np.int64)
-    np.maximum.at(cg_max, s_g, npv[s_use])
+    np.maximum.at(cg_max, s_g, src_cols(npv[s_use], src_ie))
     cg_end = np.searchsorted(s_g, np.arange(n_cg), side="right")   # sources before group end
-    # targets: 2 <= b_x, b_x^2 - 1 <= GROUP, (b_x^2 - 1) * b_min <= thr
-    bmin = int(bs[0])
+    # targets: 2 <= b_x, b_x^2 - 1 <= GROUP, (b_x^2 - 1) * cols(b_min) <= thr
+    cmin = int(cs[0])
     t_ok = tg[(npv[tg] >= 2) & (npv[tg] ** 2 - 1 <= GROUP) &
-              ((npv[tg] ** 2 - 1) * max(bmin, 1) <= thr)]
+              ((npv[tg] ** 2 - 1) * max(cmin, 1) <= thr)]
     t_use = t_ok[np.argsort(npv[t_ok], kind="stable")]
     if len(t_use) == 0:
         return None
     t_g, t_row, n_rg = _groups(npv[t_use] ** 2 - 1)
     rg_max = np.zeros(n_rg, np.int64)
     np.maximum.at(rg_max, t_g, npv[t_use])
-    # column groups of each row group: the leading g with (BX^2 - 1) BY <= thr
+    # column groups of each row group: the leading g with (BX^2 - 1) CY <= thr
+    # (CY: the group's largest column count)
     ncg = np.array([int(np.searchsorted((rg_max[r] ** 2 - 1) * cg_max, thr, side="right"))
                     for r in range(n_rg)], np.int64)
     # drop row groups with no column group (only trailing ones: ncg falls as r grows)
@@ -162,10 +173,10 @@ def plan(npv_host: np.ndarray, L: int, device, targets=None, sources=None,
     brow = np.zeros((rows_b, 2), np.int32)
     brow[:, 0] = -1
     for y, g, c0 in zip(s_use, s_g, s_col):
-        by = int(npv[y])
+        cy = int(src_cols(int(npv[y]), src_ie))
         base = g * GROUP + c0
-        brow[base: base + by, 0] = y
-        brow[base: base + by, 1] = np.arange(by)
+        brow[base: base + cy, 0] = y
+        brow[base: base + cy, 1] = np.arange(cy)
     rg_off = np.searchsorted(t_g, np.arange(n_rg + 1)).astype(np.int32)
     cg_off = np.searchsorted(s_g, np.arange(n_cg + 1)).astype(np.int32)
     mvars = np.unique(np.concatenate([t_use, s_use])).astype(np.int32)
@@ -176,7 +187,7 @@ def plan(npv_host: np.ndarray, L: int, device, targets=None, sources=None,
     host = dict(arow=arow, brow=brow, moff=moff, msize=int(sizes.sum()), tiles=tiles,
                 rg_off=rg_off, t_var=t_use.astype(np.int32), t_row=t_row.astype(np.int32),
                 cg_off=cg_off, s_var=s_use.astype(np.int32), s_col=s_col.astype(np.int32),
-                mvars=mvars, Kp=Kp, n_rg=n_rg, n_cg=n_cg)
+                mvars=mvars, Kp=Kp, n_rg=n_rg, n_cg=n_cg, src_ie=int(bool(src_ie)))
     args, keep = _device_args(host, device)
     return OnehotPlan(args=args, src_skip=src_skip, n_pairs=n_pairs, n_tiles=len(tiles),
                       cells=cells, keep=keep, host=host,
@@ -200,7 +211,7 @@ def _device_args(host: dict, device):
         A=ptr(A), B=ptr(B), Kp=Kp, rows_a=rows_a, rows_b=rows_b, arow=ptr(ar), brow=ptr(br),
         marg=ptr(marg), moff=ptr(mo), tiles=ptr(tl), n_tiles=len(host["tiles"]),
         rg_off=ptr(ro), t_var=ptr(tv), t_row=ptr(tr), cg_off=ptr(co), s_var=ptr(sv),
-        s_col=ptr(sc), mvars=ptr(mv), n_mvars=len(host["mvars"]))
+        s_col=ptr(sc), mvars=ptr(mv), n_mvars=len(host["mvars"]), src_ie=host["src_ie"])
     return args, keep
 
 
@@ -238,7 +249,7 @@ def replicate(op: OnehotPlan, K: int, m: int, device) -> OnehotPlan:
         s_var=(np.tile(H["s_var"], (K, 1)) + (ks * m)[:, None]).reshape(-1).astype(np.int32),
         s_col=np.tile(H["s_col"], K).astype(np.int32),
         mvars=(np.tile(H["mvars"], (K, 1)) + (ks * m)[:, None]).reshape(-1).astype(np.int32),
-        Kp=H["Kp"], n_rg=K * n_rg, n_cg=K * n_cg)
+        Kp=H["Kp"], n_rg=K * n_rg, n_cg=K * n_cg, src_ie=H["src_ie"])
     args, keep = _device_args(host, device)
     return OnehotPlan(args=args, src_skip=np.tile(op.src_skip, K), n_pairs=K * op.n_pairs,
                       n_tiles=len(host["tiles"]), cells=K * op.cells, keep=keep, host=host)

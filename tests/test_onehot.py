@@ -78,7 +78,8 @@ def _tile_pairs(op):
 
 @pytest.mark.parametrize("thr", [24, 120, 400, 2000])
 @pytest.mark.parametrize("subset", [False, True])
-def test_onehot_split_covers_every_pair_once(thr, subset):
+@pytest.mark.parametrize("src_ie", [False, True])
+def test_onehot_split_covers_every_pair_once(thr, subset, src_ie):
     npv = _mixed_levels(400, 3)
     n, L = len(npv), 999
     tg = sr = None
@@ -86,13 +87,13 @@ def test_onehot_split_covers_every_pair_once(thr, subset):
         rng = np.random.default_rng(4)
         tg = np.sort(rng.choice(n, 150, replace=False)).astype(np.int32)
         sr = rng.choice(n, 220, replace=False).astype(np.int32)
-    op = onehot.plan(npv, L, "cpu", tg, sr, thr=thr)
+    op = onehot.plan(npv, L, "cpu", tg, sr, thr=thr, src_ie=src_ie)
     assert op is not None
     cov = onehot.covered_pairs(op, npv, sr)
     cov_set = {(int(x), int(y)) for x, y in cov}
     assert len(cov_set) == len(cov) == op.n_pairs
     for x, y in cov:  # every one-hot pair satisfies the rule
-        assert (npv[x] ** 2 - 1) * npv[y] <= thr
+        assert (npv[x] ** 2 - 1) * onehot.src_cols(npv[y], src_ie) <= thr
     assert sorted(_tile_pairs(op)) == sorted(cov_set)
     info, hist = _hist_pairs(npv, L, op.src_skip, tg, sr)
     assert info.n_pairs == len(hist) == len(set(hist))
@@ -124,10 +125,13 @@ def _emulate_counts(op, bins, npv, L, tau):
         return R
 
     G = rows(arow, True) @ rows(brow, False).T
-    nyc = {}
+    nyc, nab = {}, {}
     for v in range(len(npv)):
         if moff[v] >= 0:
             nyc[v] = np.bincount(d[v, :L], minlength=npv[v])
+            nab[v] = np.bincount(d[v, tau:tau + L] * npv[v] + d[v, :L],
+                                 minlength=npv[v] ** 2).reshape(npv[v], npv[v])
+    src_ie = bool(op.host["src_ie"])
     out = {}
     for rg0, rg1, cg0, cg1, mask, *_ in tiles:
         for w in range(4):
@@ -140,8 +144,12 @@ def _emulate_counts(op, bins, npv, L, tau):
                     if x == y:
                         continue
                     bx, by = int(npv[x]), int(npv[y])
+                    c1 = max(by - 1, 1) if src_ie else by
                     g = G[rg * 128 + t_row[ti]: rg * 128 + t_row[ti] + bx * bx - 1,
-                          cg * 128 + s_col[sj]: cg * 128 + s_col[sj] + by]
+                          cg * 128 + s_col[sj]: cg * 128 + s_col[sj] + c1]
+                    if c1 < by:   # the left-out level: row totals less the stored columns
+                        tot = np.concatenate([nab[x][: bx - 1].ravel(), nyc[x][: bx - 1]])
+                        g = np.concatenate([g, (tot - g.sum(1))[:, None]], 1)
                     N = np.zeros((bx, bx, by), np.int64)
                     for c in range(by):
                         sumM = 0
@@ -158,12 +166,14 @@ def _emulate_counts(op, bins, npv, L, tau):
 
 
 @pytest.mark.parametrize("tau", [1, 3])
-def test_onehot_scheme_reproduces_joint_counts(tau):
+@pytest.mark.parametrize("src_ie", [False, True])
+def test_onehot_scheme_reproduces_joint_counts(tau, src_ie):
     npv = _mixed_levels(60, 7)
+    npv[:3] = 1          # constant variables: one level, one stored column
     T = 301
     bins = _bins(npv, T, 8)
     L = T - tau
-    op = onehot.plan(npv, L, "cpu", thr=150)
+    op = onehot.plan(npv, L, "cpu", thr=150, src_ie=src_ie)
     got = _emulate_counts(op, bins, npv, L, tau)
     assert len(got) == op.n_pairs > 100
     d = bins.astype(np.int32)
