@@ -334,25 +334,34 @@ def run_c1(args):
     out = torch.empty((n, n), dtype=torch.float32, device=dev)
     s = torch.cuda.current_stream().cuda_stream
 
-    def launch():
+    def launch(precision=_lib.TNX_B2_AUTO):
         _lib.call("tnx_te_gemm_b2", prep.bins.data_ptr(), n, T, prep.ld, 1, prep.S1.data_ptr(),
                   prep.S2a.data_ptr(), prep.xlogx.data_ptr(), ws.data_ptr(), ws_bytes,
-                  out.data_ptr(), n, None, s)
+                  out.data_ptr(), n, None, precision, s)
 
-    dt = timed_steps(launch, args.steps, args.warmup)
+    dt = timed_steps(launch, args.steps, args.warmup)   # the default: FP4 operands (L < 2^24)
     t_k = time_launches(launch, max(3, args.steps))
+    te_fp4 = out.cpu().numpy()
+    t_8 = time_launches(lambda: launch(_lib.TNX_B2_INT8), max(3, args.steps))
+    dmax = float(np.abs(te_fp4 - out.cpu().numpy()).max())
+    launch()
     L = T - 1
     ops = 2.0 * 3 * n * n * L
     tops = ops / t_k / 1e12
     result = {"metric": "TE gene-pairs/sec, binary 2000 genes x 5000 cells (GEMM-B2)",
               "value": n * (n - 1) / dt, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
               "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
-              "scaling": "weak", "vs_baseline": None, "dtype": "int8 x int8 -> int32",
+              "scaling": "weak", "vs_baseline": None,
+              "dtype": "fp4 (e2m1) 0/1 operands -> fp32 sums (exact below 2^24)",
               "data": "synthetic Bernoulli with 200 planted lagged pairs (seed 7)",
               "config": {"workload": "C1 GEMM-B2 2000x5000", "n_pairs": n * (n - 1)},
-              "roofline": {"bound": "mfma", "kernel": "b2_mfma_kernel (+ b2_pack_kernel)",
-                           "achieved": tops, "peak": MFMA_INT8_PEAK_TOPS, "unit": "TOP/s",
-                           "frac": tops / MFMA_INT8_PEAK_TOPS, "traffic": None}}
+              "roofline": {"bound": "mfma", "kernel": "b2_mfma_kernel<FP4> (+ b2_pack_kernel<FP4>)",
+                           "achieved": tops, "peak": MFMA_FP4_PEAK_TOPS, "unit": "TOP/s",
+                           "frac": tops / MFMA_FP4_PEAK_TOPS, "ops_per_launch": ops,
+                           "frac_of_int8_peak": tops / MFMA_INT8_PEAK_TOPS, "traffic": None},
+              "int8_kernel": {"ms": t_8 * 1e3, "tops": ops / t_8 / 1e12,
+                              "frac_of_int8_peak": ops / t_8 / 1e12 / MFMA_INT8_PEAK_TOPS,
+                              "max_abs_diff_vs_fp4": dmax}}
     if not args.no_cpu_baseline:
         d = prep.bins[:, :T].cpu().numpy().astype(np.int32)
         pairs = sample_pairs(prep.npv_host, args.cpu_sample, args.seed, stratify=False)

@@ -50,7 +50,7 @@ extern "C" {
 #define TNX_API
 #endif
 
-#define TNX_ABI_VERSION 11
+#define TNX_ABI_VERSION 12
 
 /* Status codes. */
 #define TNX_OK 0
@@ -390,16 +390,21 @@ TNX_API int tnx_te_counts_debug(const int8_t *bins, const uint16_t *states, cons
 /* Workspace bytes for tnx_te_gemm_b2 at (n, T). */
 TNX_API int64_t tnx_gemm_b2_workspace(int64_t n, int64_t T, int32_t tau);
 
-/* All n*(n-1) TE values for binary bins (0/1) via exact INT8 MFMA
- * contractions with INT32 accumulation (C = fp.p^T, D = p.p^T, E = f.p^T)
- * and the fused 8-state epilogue.  te_out float (n, ld_out), internal
- * layout te_out[i * ld_out + j] = TE(j -> i), diagonal 0.
- * S1, S2a from tnx_gene_prep.  Replaces compute_te_gemm_b2
+/* Precision of tnx_te_gemm_b2's contractions (all exact). */
+#define TNX_B2_AUTO 0  /* FP4 when L < 2^24, else INT8 */
+#define TNX_B2_INT8 1  /* int8 x int8 -> int32 (v_mfma_i32_32x32x32_i8) */
+#define TNX_B2_FP4 2   /* 0/1 as e2m1 nibbles -> fp32 (v_mfma_scale_f32_32x32x64_f8f6f4) */
+
+/* All n*(n-1) TE values for binary bins (0/1) via exact MFMA contractions
+ * (C = fp.p^T, D = p.p^T, E = f.p^T) and the fused 8-state epilogue.
+ * te_out float (n, ld_out), internal layout te_out[i * ld_out + j] =
+ * TE(j -> i), diagonal 0.  S1, S2a from tnx_gene_prep.  counts_dbg (NULL
+ * or int32 (3, n, n)): C, D, E.  Replaces compute_te_gemm_b2
  * (tenex/kernels/gemm_b2.py:307-425, :66-187, :210-285). */
 TNX_API int tnx_te_gemm_b2(const int8_t *bins, int64_t n, int64_t T, int64_t ld, int32_t tau,
                    const double *S1, const double *S2a, const float *xlogx,
                    void *workspace, int64_t workspace_bytes, float *te_out, int64_t ld_out,
-                   int32_t *counts_dbg, void *stream);
+                   int32_t *counts_dbg, int32_t precision, void *stream);
 
 /* ── One-hot INT8 MFMA counts for pairs of few levels ───────────────── */
 

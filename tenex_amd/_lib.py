@@ -11,12 +11,16 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libtenex_hip.so")
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 TNX_OUT_PAIRFREE = 0
 TNX_OUT_MATRIX = 1
 TNX_OUT_STATS = 2
 TNX_OUT_STAGE = 3
+
+TNX_B2_AUTO = 0
+TNX_B2_INT8 = 1
+TNX_B2_FP4 = 2
 
 _lock = threading.Lock()
 _lib = None
@@ -125,7 +129,8 @@ SIGNATURES = {
     "tnx_te_counts_debug": (C.c_int, [p_, p_, p_, p_, p_, p_, i64, i64, i64, i64, i32, p_, i64,
                                       i32, p_, p_, p_]),
     "tnx_gemm_b2_workspace": (i64, [i64, i64, i32]),
-    "tnx_te_gemm_b2": (C.c_int, [p_, i64, i64, i64, i32, p_, p_, p_, p_, i64, p_, i64, p_, p_]),
+    "tnx_te_gemm_b2": (C.c_int, [p_, i64, i64, i64, i32, p_, p_, p_, p_, i64, p_, i64, p_, i32,
+                                 p_]),
     "tnx_surrogate_run": (C.c_int, [C.POINTER(SurrArgs), p_]),
     "tnx_block_shuffle": (C.c_int, [p_, i64, i64, i64, p_, i32, i32, p_, p_]),
     "tnx_block_shuffle_copies": (C.c_int, [p_, i64, i32, i64, i64, p_, i32, i32, p_, p_]),

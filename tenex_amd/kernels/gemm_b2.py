@@ -1,10 +1,11 @@
-"""GEMM-B2: binary data (b_max == 2) on exact INT8 MFMA (gemm_b2.hip).
+"""GEMM-B2: binary data (b_max == 2) on exact MFMA contractions (gemm_b2.hip).
 
 Reference: tenex/kernels/gemm_b2.py:307-425 (three FP32 torch.mm
 contractions C = fp.p^T, D = p.p^T, E = f.p^T, then the 8-state epilogue
-:210-285 in Triton).  Here one fused gfx950 kernel contracts int8 operands
-with INT32 accumulation (counts exact for any L < 2^31) and applies the
-8-state epilogue in registers.
+:210-285 in Triton).  Here one fused gfx950 kernel contracts the 0/1
+operands (FP4 e2m1 with FP32 sums below 2^24 samples, INT8 with INT32
+sums beyond: counts exact either way) and applies the 8-state epilogue in
+registers.
 """
 from __future__ import annotations
 
@@ -16,9 +17,12 @@ from ..core import call, ptr
 from . import TEKernel
 
 
-def te_gemm_b2_device(prep: core.Prepared, counts_dbg: bool = False):
+def te_gemm_b2_device(prep: core.Prepared, counts_dbg: bool = False,
+                      precision: int = _lib.TNX_B2_AUTO):
     """Internal-layout (n, n) TE matrix te[i, j] = TE(j -> i) on the device.
-    With counts_dbg=True also returns int32 (3, n, n) counts C, D, E."""
+    With counts_dbg=True also returns int32 (3, n, n) counts C, D, E.
+    precision: TNX_B2_AUTO (FP4 0/1 operands with FP32 sums below 2^24
+    samples, else INT8), TNX_B2_INT8 or TNX_B2_FP4; all exact."""
     n, T, tau = prep.n_vars, prep.T, prep.tau
     if int(prep.npv_host.max(initial=1)) > 2:
         raise ValueError("GEMM-B2 requires binary data (b_max <= 2)")
@@ -27,7 +31,7 @@ def te_gemm_b2_device(prep: core.Prepared, counts_dbg: bool = False):
     out = torch.empty((n, n), dtype=torch.float32, device=prep.device)
     cnt = torch.zeros((3, n, n), dtype=torch.int32, device=prep.device) if counts_dbg else None
     call("tnx_te_gemm_b2", ptr(prep.bins), n, T, prep.ld, int(tau), ptr(prep.S1), ptr(prep.S2a),
-         ptr(prep.xlogx), ptr(ws), ws_bytes, ptr(out), n, ptr(cnt),
+         ptr(prep.xlogx), ptr(ws), ws_bytes, ptr(out), n, ptr(cnt), int(precision),
          torch.cuda.current_stream(prep.device).cuda_stream)
     return (out, cnt) if counts_dbg else out
 
@@ -68,4 +72,4 @@ class GEMMB2Kernel(TEKernel):
         return m[pt[:, 0], pt[:, 1]].cpu().numpy()
 
     def log_description(self, **kwargs):
-        return "GEMM-B2 (INT8 MFMA, INT32 accumulate, fused 8-state epilogue, exact)"
+        return "GEMM-B2 (FP4/INT8 MFMA on 0/1 operands, exact counts, fused 8-state epilogue)"

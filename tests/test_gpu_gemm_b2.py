@@ -1,10 +1,11 @@
-"""GPU parity of GEMM-B2 (INT8 MFMA, INT32 accumulation) against the oracle."""
+"""GPU parity of GEMM-B2 (INT8 and FP4 MFMA contractions, both exact) against
+the oracle."""
 import numpy as np
 import pytest
 import torch
 
 from oracle import oracle as O
-from tenex_amd import core, synthetic
+from tenex_amd import _lib, core, synthetic
 from tenex_amd.kernels.gemm_b2 import te_gemm_b2_device
 
 pytestmark = pytest.mark.gpu
@@ -23,11 +24,12 @@ def _binary_prep(x, dev):
 # memory (double-float sums) instead of LDS
 @pytest.mark.parametrize("n,T,seed", [(150, 300, 0), (97, 1001, 1), (260, 64, 2), (33, 4097, 3),
                                       (40, 9000, 4)])
-def test_gemm_b2_vs_oracle(dev, n, T, seed):
+@pytest.mark.parametrize("precision", [_lib.TNX_B2_INT8, _lib.TNX_B2_FP4])
+def test_gemm_b2_vs_oracle(dev, n, T, seed, precision):
     x, planted = synthetic.bernoulli_planted(n, T, n_planted=min(20, n // 3), seed=seed)
     x[0] = 0.0  # a constant gene
     d, npv, prep = _binary_prep(x, dev)
-    te, cnt = te_gemm_b2_device(prep, counts_dbg=True)
+    te, cnt = te_gemm_b2_device(prep, counts_dbg=True, precision=precision)
     te = te.cpu().numpy()
     cnt = cnt.cpu().numpy()
     ref, C, D, E = O.te_b2(d, counts=True)
@@ -52,13 +54,14 @@ def test_gemm_b2_planted_closed_form(dev):
         assert abs(te[t, s] - h_cond) < TOL
 
 
-def test_gemm_b2_c1_scale_counts(dev):
+@pytest.mark.parametrize("precision", [_lib.TNX_B2_INT8, _lib.TNX_B2_FP4])
+def test_gemm_b2_c1_scale_counts(dev, precision):
     """2,000 genes x 5,000 cells with 200 planted pairs: counts of a row
     block bit-exact, marginal identities on the full matrix."""
     n, T = 2000, 5000
     x, planted = synthetic.bernoulli_planted(n, T, n_planted=200, seed=7)
     d, npv, prep = _binary_prep(x, dev)
-    te, cnt = te_gemm_b2_device(prep, counts_dbg=True)
+    te, cnt = te_gemm_b2_device(prep, counts_dbg=True, precision=precision)
     cnt = cnt.cpu().numpy()
     p = d[:, :-1].astype(np.int64)
     f = d[:, 1:].astype(np.int64)
@@ -79,3 +82,14 @@ def test_gemm_b2_c1_scale_counts(dev):
     # planted edges dominate their rows
     for s, t in planted[:20]:
         assert te[t, s] > 0.05
+
+
+def test_gemm_b2_fp4_equals_int8(dev):
+    """The default (FP4 below 2^24 samples) and the INT8 contractions give
+    the same counts, so the same TE matrix bit for bit."""
+    x, planted = synthetic.bernoulli_planted(300, 2500, n_planted=40, seed=9)
+    d, npv, prep = _binary_prep(x, dev)
+    te4, c4 = te_gemm_b2_device(prep, counts_dbg=True)
+    te8, c8 = te_gemm_b2_device(prep, counts_dbg=True, precision=_lib.TNX_B2_INT8)
+    assert torch.equal(c4, c8)
+    assert torch.equal(te4, te8)
