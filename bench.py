@@ -26,6 +26,7 @@ scatter_add path, timed on a bounded sample of the same workload).
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -47,6 +48,24 @@ def log(msg):
 
 def onehot_kp(L):
     return -(-int(L) // 128) * 128
+
+
+def onehot_roofline(dev, oh, n_pairs, thr, L, t_mfma):
+    """The one-hot MFMA launch against the FP4 matrix-core peaks (published,
+    and measured from registers in this run)."""
+    if oh is None:
+        return {"pairs": 0, "threshold": thr}
+    from tenex_amd import core
+    ops = 2.0 * oh.cells * onehot_kp(L)
+    measured = core.mfma_rate(dev, 0)
+    return {"pairs": n_pairs, "tiles": oh.n_tiles, "threshold": thr, "kernel": "oh_mfma_kernel",
+            "bound": "mfma", "fp4_mfma_ops": ops, "ms": t_mfma * 1e3,
+            "achieved": ops / t_mfma / 1e12, "unit": "TOP/s", "peak": MFMA_FP4_PEAK_TOPS,
+            "frac": ops / t_mfma / 1e12 / MFMA_FP4_PEAK_TOPS,
+            "peak_measured_registers": measured / 1e12,
+            "frac_of_measured": ops / t_mfma / measured,
+            "traffic": measured_traffic("onehot_traffic_bytes_per_launch"),
+            "traffic_source": TRAFFIC_FILE}
 
 
 def time_launches(fn, reps):
@@ -128,32 +147,38 @@ def cpu_baseline_pairs(bins_np, npv, pairs, gpu_te_internal=None):
 TRAFFIC_FILE = "profiles/te_persist_c2_4000x10000_traffic.md"
 
 
-def measured_traffic():
-    """HBM bytes per launch of the C2 wave kernel, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None if
-    absent."""
+def measured_traffic(key: str = "traffic_bytes_per_launch"):
+    """HBM bytes per launch of a C2 kernel (the persistent launch, or with
+    key onehot_traffic_bytes_per_launch the one-hot MFMA launch), from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated);
+    None if absent."""
     import re
     path = os.path.join(HERE, TRAFFIC_FILE)
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", open(path).read())
+    m = re.search(r"(?m)^" + key + r":\s*(\d+)", open(path).read())
     return int(m.group(1)) if m else None
 
 
 def run_traffic_probe(args):
-    """One C2 TE matrix (the persistent launch of the headline step), then a
-    256 MiB device copy: the program the FETCH_SIZE / WRITE_SIZE counter
-    passes run (the copy calibrates the counters)."""
+    """One C2 TE matrix as the headline step computes it (the one-hot pack
+    and MFMA launches, then the persistent launch), then a 256 MiB device
+    copy: the program the FETCH_SIZE / WRITE_SIZE counter passes run (the
+    copy calibrates the counters)."""
     import torch
-    from tenex_amd import _lib, core, synthetic
+    from tenex_amd import _lib, core, onehot, synthetic
     dev = core.require_gpu("cuda:0")
     n, T = 4000, 10000
     bins, npv, b_max, T = core.upload_bins(synthetic.mixed_cardinality_bins(n, T, seed=args.seed),
                                            dev)
     prep = core.prepare(bins, npv, T, 1)
-    plan = core.get_plan(prep)
+    oh = core.get_onehot_plan(prep, thr=args.onehot_thr) if args.onehot_thr > 0 else None
+    plan = core.get_plan(prep, src_skip=None if oh is None else oh.src_skip)
     out = torch.zeros((n, n), dtype=torch.float32, device=dev)
-    core.te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n)
+    targs = core.te_args(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n)
+    if oh is not None:
+        onehot.run(prep, oh, targs, _lib.stream_handle(dev))
+    core.te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, args=targs)
     src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
     dst.copy_(src)
@@ -222,6 +247,9 @@ def run_c2(args):
     # earlier two-launch kernels on the same plan for comparison
     t_pers = time_launches(lambda: core.te_run(prep, plan, out, 1, args=targs), reps)
     t_oh = time_launches(run_oh, reps) if oh is not None else 0.0
+    t_oh_mfma = (time_launches(lambda: _lib.call("tnx_onehot_te", ctypes.byref(oh.args),
+                                                 ctypes.byref(targs), stream), reps)
+                 if oh is not None else 0.0)
     t_te = t_pers + t_oh
     t_prep = time_launches(lambda: core.gene_prep_into(prep), reps)
     a2 = core.te_args(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=n, persistent=False)
@@ -266,9 +294,7 @@ def run_c2(args):
         "kernels_ms": {"te_launches": t_te * 1e3, "persistent_launch": t_pers * 1e3,
                        "onehot_pack_and_mfma": t_oh * 1e3,
                        "te_two_launch_kernels": t_two * 1e3, "gene_prep_kernel": t_prep * 1e3},
-        "onehot": {"pairs": n_oh, "tiles": 0 if oh is None else oh.n_tiles,
-                   "threshold": args.onehot_thr,
-                   "fp4_mfma_ops": 0 if oh is None else 2 * oh.cells * onehot_kp(prep.L)},
+        "onehot": onehot_roofline(dev, oh, n_oh, args.onehot_thr, prep.L, t_oh_mfma),
         "plan_host_ms": plan_s * 1e3,
     }
     if tuned is not None:
@@ -362,6 +388,9 @@ def run_c1(args):
               "int8_kernel": {"ms": t_8 * 1e3, "tops": ops / t_8 / 1e12,
                               "frac_of_int8_peak": ops / t_8 / 1e12 / MFMA_INT8_PEAK_TOPS,
                               "max_abs_diff_vs_fp4": dmax}}
+    fp4_reg = core.mfma_rate(dev, 0)
+    result["roofline"]["peak_measured_registers"] = fp4_reg / 1e12
+    result["roofline"]["frac_of_measured"] = ops / t_k / fp4_reg
     if not args.no_cpu_baseline:
         d = prep.bins[:, :T].cpu().numpy().astype(np.int32)
         pairs = sample_pairs(prep.npv_host, args.cpu_sample, args.seed, stratify=False)

@@ -587,6 +587,15 @@ TNX_API int tnx_mlp_fit(const tnx_mlp_t *m, float *master, float *m1, float *m2,
 TNX_API int tnx_bench_lds_atomics(int32_t mode, int64_t n_blocks, int32_t iters, uint32_t *out,
                                   void *stream);
 
+/* Matrix-core throughput from registers (no memory in the loop): mode 0
+ * FP4 (v_mfma_scale_f32_32x32x64_f8f6f4), 1 FP8, 2 INT8
+ * (v_mfma_i32_32x32x32_i8); n_blocks workgroups of four waves, each wave
+ * iters x 4 MFMAs; out: n_blocks x 256 floats (device).  The rate is the
+ * roofline peak the MFMA kernels are measured against, next to the
+ * published figure.  No reference counterpart. */
+TNX_API int tnx_bench_mfma(int32_t mode, int64_t n_blocks, int32_t iters, float *out,
+                           void *stream);
+
 #ifdef __cplusplus
 }
 #endif

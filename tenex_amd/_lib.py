@@ -145,6 +145,7 @@ SIGNATURES = {
     "tnx_mlp_adam": (C.c_int, [C.POINTER(Mlp), p_, p_, p_, p_, i64, C.c_float, C.c_float,
                                C.c_float, C.c_float, i32, p_]),
     "tnx_bench_lds_atomics": (C.c_int, [i32, i64, i32, p_, p_]),
+    "tnx_bench_mfma": (C.c_int, [i32, i64, i32, p_, p_]),
 }
 
 

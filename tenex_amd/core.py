@@ -1138,6 +1138,32 @@ def lds_atomic_rate(device, mode: int = 0, n_blocks: int | None = None, iters: i
     return float(np.median(rates))
 
 
+MFMA_OPS = {0: 32 * 32 * 64 * 2, 1: 32 * 32 * 64 * 2, 2: 32 * 32 * 32 * 2}
+
+
+def mfma_rate(device, mode: int = 0, n_blocks: int | None = None, iters: int = 2048,
+              reps: int = 3) -> float:
+    """Measured chip-wide matrix-core operations per second from registers
+    (tnx_bench_mfma: mode 0 FP4, 1 FP8, 2 INT8; median of reps)."""
+    props = device_props(device)
+    if n_blocks is None:
+        n_blocks = int(props.n_cu) * 4
+    out = torch.empty(n_blocks * 256, dtype=torch.float32, device=device)
+    s = _stream(device)
+    call("tnx_bench_mfma", int(mode), int(n_blocks), int(iters), ptr(out), s)
+    rates = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        call("tnx_bench_mfma", int(mode), int(n_blocks), int(iters), ptr(out), s)
+        e1.record()
+        e1.synchronize()
+        n_mfma = n_blocks * 4.0 * iters * 4
+        rates.append(n_mfma * MFMA_OPS[int(mode)] / (e0.elapsed_time(e1) / 1e3))
+    return float(np.median(rates))
+
+
 # ── surrogates ────────────────────────────────────────────────────────────
 
 SURR_MAX_CELLS = 32768
