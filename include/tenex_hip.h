@@ -418,16 +418,19 @@ TNX_API int tnx_te_gemm_b2(const int8_t *bins, int64_t n, int64_t T, int64_t ld,
  * Rows are grouped in 128-row (target) and 128-column (source) groups that
  * never split a variable; a tile is two row groups x two column groups
  * (one 128 x 128 sub-tile per wave, mask bit w = 2*wr + wc).
- * Operands are int8 in 32-row x 32-byte blocks in MFMA lane order (block
- * (row / 32) * (Kp / 32) + k / 32; row r, byte k at ((k % 32) / 16 * 32 +
- * r % 32) * 16 + k % 16 inside the block).  Each pair's TE is written
+ * Operands are 0/1 indicators as FP4 (e2m1, 1.0 = 0x2) in 1 KB blocks of
+ * 32 rows x 64 samples in MFMA lane order (block (row / 32) * (Kp / 64) +
+ * k / 64; inside it row r, sample k at byte ((k % 64) / 32 * 32 + r % 32)
+ * * 16 + (k % 32) / 2, even samples in the low nibble), contracted by
+ * v_mfma_scale_f32_32x32x64_f8f6f4 with FP32 sums (exact: every count is
+ * below 2^24).  Each pair's TE is written
  * through the tnx_te_args_t output modes (out_mode, out, ld_out, ...),
  * with S3 - S2b summed in the exact fixed point of xlogx_fx.  No reference
  * counterpart: the reference counts every pair with per-sample atomics
  * (tenex/csrc/full_smem.cu:31-142, tenex/csrc/adaptive_smem.cu:30-149). */
 typedef struct {
-    int8_t *A, *B;                 /* operands (rows_a x Kp, rows_b x Kp), tiled */
-    int64_t Kp;                    /* padded series length: multiple of 64, >= L */
+    int8_t *A, *B;                 /* operands (rows_a x Kp / 2, rows_b x Kp / 2 bytes), tiled */
+    int64_t Kp;                    /* padded series length: multiple of 128, >= L (L <= 65535) */
     int64_t rows_a, rows_b;        /* multiples of 128 */
     const int32_t *arow;           /* (rows_a, 2): variable (-1: padding), (a << 8) | b
                                     * (a = 255: an x_t row) */

@@ -1,13 +1,16 @@
-// onehot.hip -- exact joint counts of pairs with few levels on INT8 MFMA.
+// onehot.hip -- exact joint counts of pairs with few levels on FP4 MFMA.
 //
 // For target x (b_x levels) and source y (b_y levels) the joint counts of
 // (x_{t+tau}, x_t, y_t) over t < L are inner products of 0/1 indicator rows:
 //     N[a, b, c] = sum_t [x_{t+tau} = a][x_t = b] [y_t = c]
 //     M[b, c]    = sum_t [x_t = b] [y_t = c]                  ((x_t, y_t))
 // Target x contributes the rows (a, b) with a < b_x - 1 and the rows x_t = b
-// with b < b_x - 1 (b_x^2 - 1 rows), source y its b_y rows y_t = c.  The
-// rest follow from marginals:  M[b_x - 1, c] = n_y(c) - sum_b M[b, c] and
-// N[b_x - 1, b, c] = M[b, c] - sum_a N[a, b, c], all exact integers.
+// with b < b_x - 1 (b_x^2 - 1 rows), source y its rows y_t = c: all b_y of
+// them, or with src_ie the first b_y - 1 (at least one).  The rest follow
+// from marginals:  M[b_x - 1, c] = n_y(c) - sum_b M[b, c],
+// N[b_x - 1, b, c] = M[b, c] - sum_a N[a, b, c], and with src_ie the last
+// source level's cells N[a, b, b_y - 1] = n_x(a, b) - sum_c N[a, b, c] and
+// M[b, b_y - 1] = n_x(b) - sum_c M[b, c], all exact integers.
 // Indicators are FP4 (e2m1: 1.0 is 0x2), products accumulate in FP32, which
 // is exact for counts below 2^24: one v_mfma_scale_f32_32x32x64_f8f6f4
 // (scales 2^0) counts 32 x 32 (row, column) cells over 64 samples.  A pair

@@ -1,16 +1,18 @@
-"""One-hot INT8 MFMA counts for the pairs of few levels (csrc/onehot.hip).
+"""One-hot FP4 MFMA counts for the pairs of few levels (csrc/onehot.hip).
 
-A pair (x, y) whose (b_x^2 - 1) * b_y indicator cells are few costs far less
-as an exact INT8 contraction of indicator rows than as one LDS atomic per
-sample.  The host plan here
+A pair (x, y) whose (b_x^2 - 1) * max(b_y - 1, 1) indicator cells are few
+costs far less as an exact contraction of 0/1 indicator rows (FP4 operands,
+FP32 sums, exact below 2^24 samples) than as one LDS atomic per sample.
+The host plan here
 
   * orders the sources by level count as the histogram planner does
     (tnx_te_plan: ascending, ties by list position) and packs them into
-    128-column groups (b_y columns each, no source split);
+    128-column groups (src_cols(b_y) columns each, no source split);
   * packs the targets, ascending by level count, into 128-row groups
     (b_x^2 - 1 rows each);
   * gives row group r the leading column groups g whose pairs all satisfy
-    (BX_r^2 - 1) * BY_g <= thr (BX, BY the groups' largest level counts),
+    (BX_r^2 - 1) * src_cols(BY_g) <= thr (BX, BY the groups' largest
+    level counts),
     so each target's one-hot sources are a prefix of the sorted sources:
     src_skip[x] tells the histogram planner to leave exactly those out;
   * deals tiles of two row groups x two column groups to the workgroups.
