@@ -35,6 +35,16 @@ void clear_error() { g_err[0] = 0; }
 // Evaluated here without recursion (explicit stack, depth <= 7 per buffer).
 #pragma clang fp contract(off)
 
+// Correctly rounded float32 square root and quotient (numpy's np.sqrt and
+// true_divide on float32).  HIP's __fsqrt_rn is the native v_sqrt_f32 (not
+// correctly rounded) unless OCML_BASIC_ROUNDED_OPERATIONS is defined, so
+// both are computed in FP64 and rounded once: the FP64 result is correctly
+// rounded, and rounding it again to float32 gives the correctly rounded
+// float32 result because 53 >= 2 * 24 + 2 (double rounding is innocuous
+// for sqrt and division at these precisions).
+__device__ __forceinline__ float sqrt_rn(float v) { return (float)sqrt((double)v); }
+__device__ __forceinline__ float div_rn(float a, float b) { return (float)((double)a / (double)b); }
+
 template <typename F>
 __device__ float pw_leaf(const F &f, int64_t off, int64_t n) {
     if (n < 8) {
@@ -129,7 +139,7 @@ __global__ void fsbw_stats_kernel(const float *__restrict__ x, int64_t n_vars, i
     };
     float ss = np_rowsum(sq, T);
     float var = (float)((double)ss / (double)(T - 1));
-    float sd = __fsqrt_rn(var);
+    float sd = sqrt_rn(var);
     if (sd == 0.0f || isnan(sd)) sd = 1.0f;
     float mn = row[0], mx = row[0];
     for (int64_t t = 1; t < T; t++) {
@@ -140,7 +150,7 @@ __global__ void fsbw_stats_kernel(const float *__restrict__ x, int64_t n_vars, i
     float smin = mn - kp * sd;
     sd_out[g] = sd;
     smin_out[g] = smin;
-    if (nb_out) nb_out[g] = (int32_t)ceilf(__fdiv_rn(mx - mn, sd));
+    if (nb_out) nb_out[g] = (int32_t)ceilf(div_rn(mx - mn, sd));
 }
 
 __global__ void fsbw_bins_kernel(const float *__restrict__ x, int64_t n_vars, int64_t T,
@@ -149,7 +159,7 @@ __global__ void fsbw_bins_kernel(const float *__restrict__ x, int64_t n_vars, in
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_vars * T) return;
     int64_t g = i / T;
-    float q = __fdiv_rn(x[i] - smin[g], sd[g]);
+    float q = div_rn(x[i] - smin[g], sd[g]);
     int32_t b = (int32_t)floorf(q);
     raw[i] = b < 0 ? 0 : b;
 }

@@ -31,7 +31,8 @@ def test_discretize_bit_exact_c0(dev):
     assert np.array_equal(npv.cpu().numpy(), npv_o) and b_max == bmax_o
 
 
-@pytest.mark.parametrize("n,T,seed", [(37, 9999, 1), (64, 3000, 2), (16, 8193, 3), (50, 130, 4)])
+@pytest.mark.parametrize("n,T,seed", [(37, 9999, 1), (64, 3000, 2), (16, 8193, 3), (50, 130, 4),
+                                      (600, 10000, 2)])
 def test_discretize_bit_exact_lengths(dev, n, T, seed):
     x = synthetic.zero_inflated_lognormal(n, T, seed=seed)
     raw_g, nb_g = core.fsbw_l_gpu(torch.from_numpy(x).to(dev))
