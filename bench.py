@@ -169,8 +169,13 @@ def run_traffic_probe(args):
     from tenex_amd import _lib, core, onehot, synthetic
     dev = core.require_gpu("cuda:0")
     n, T = 4000, 10000
-    bins, npv, b_max, T = core.upload_bins(synthetic.mixed_cardinality_bins(n, T, seed=args.seed),
-                                           dev)
+    if args.continuous:   # the continuous-data variant: discretised on the GPU
+        x = torch.from_numpy(synthetic.zero_inflated_lognormal(n, T, seed=args.seed)).to(dev)
+        raw, _ = core.fsbw_l_gpu(x, 0.5)
+        bins, npv, b_max = core.remap_gpu(raw)
+    else:
+        bins, npv, b_max, T = core.upload_bins(
+            synthetic.mixed_cardinality_bins(n, T, seed=args.seed), dev)
     prep = core.prepare(bins, npv, T, 1)
     oh = core.get_onehot_plan(prep, thr=args.onehot_thr) if args.onehot_thr > 0 else None
     plan = core.get_plan(prep, src_skip=None if oh is None else oh.src_skip)
@@ -880,6 +885,8 @@ def main():
                          "(C2; 0: every pair in the histogram kernels)")
     ap.add_argument("--cpu-sample", type=int, default=20480)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--continuous", action="store_true",
+                    help="with --traffic-probe: the continuous-data variant of C2")
     ap.add_argument("--no-continuous", action="store_true",
                     help="c2: skip the continuous-data variant (discretize + TE matrix)")
     ap.add_argument("--no-graph", action="store_true",

@@ -50,7 +50,7 @@ extern "C" {
 #define TNX_API
 #endif
 
-#define TNX_ABI_VERSION 12
+#define TNX_ABI_VERSION 13
 
 /* Status codes. */
 #define TNX_OK 0
@@ -275,6 +275,16 @@ typedef struct {
      * would scan mostly empty cells); the caller computes them with
      * tnx_te_sort_pairs.  Must exceed cap_wave.  0: every pair here. */
     int64_t cap_sort;
+    /* From ABI 13, the persistent launch: hot_cw > 0 (4, 8 or 16) counts
+     * the samples of each target's four hot states (x_{t+tau}, x_t) with
+     * both levels below 2 (zero-inflated data put most samples there) in
+     * lane-private copies: hot_cw copies per hot cell in each wave slice,
+     * 16 in the block role, folded back before the reduction.  Every
+     * variable needs n_per_var <= hot_maxb <= 32.  Each wave slice grows by
+     * 4 * hot_cw * hot_maxb words after cap_wave (rounded up to 4).  Results
+     * are identical either way; only speed differs.  0: off. */
+    int32_t hot_cw;
+    int32_t hot_maxb;
 } tnx_te_args_t;
 
 #define TNX_QUEUES 8            /* work queues of the persistent launch (one per XCD) */

@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libtenex_hip.so")
-ABI_VERSION = 12
+ABI_VERSION = 13
 
 TNX_OUT_PAIRFREE = 0
 TNX_OUT_MATRIX = 1
@@ -60,7 +60,7 @@ class TeArgs(C.Structure):
                 ("queue_items", p_), ("queue_off", p_), ("queue_ctr", p_),
                 ("counts_map", p_), ("counts_dbg", p_), ("counts_ld", C.c_int64),
                 ("xlogx_fx", p_), ("fx_shift", C.c_int32), ("var_L", p_),
-                ("cap_sort", i64)]
+                ("cap_sort", i64), ("hot_cw", C.c_int32), ("hot_maxb", C.c_int32)]
 
 
 class OnehotArgs(C.Structure):

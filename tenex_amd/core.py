@@ -254,6 +254,34 @@ class Prepared:
     def device(self):
         return self.bins.device
 
+    @property
+    def hot(self) -> tuple[int, int]:
+        """(hot_cw, hot_maxb) of the persistent launch: with a frequent key 0
+        (zero-inflated data) and at most HOT_MAX_LEVELS levels everywhere,
+        the samples of each target's four hot states (both levels below 2)
+        are counted in HOT_CW lane-private copies per cell; else (0, 0)."""
+        bmax = int(self.npv_host.max()) if len(self.npv_host) else 0
+        if self.hot_zero and HOT_CW > 0 and 1 <= bmax <= HOT_MAX_LEVELS:
+            return HOT_CW, bmax
+        return 0, 0
+
+    @property
+    def hot_words(self) -> int:
+        """LDS words a wave slice adds for the hot cells (0 when off)."""
+        cw, mb = self.hot
+        return 4 * cw * mb
+
+
+HOT_CW = 4               # wave-slice copies of a hot cell (4, 8 or 16; 0: off)
+HOT_CB = 16              # block-role copies of a hot cell (te_args.hpp kHotCB)
+HOT_MAX_LEVELS = 32      # the hot cells hold source levels below this
+
+
+def wave_cap(prep: "Prepared") -> int:
+    """Cells of a wave histogram for prep: WAVE_CAP less the hot cells'
+    words, so that a slice takes the LDS it takes without them."""
+    return WAVE_CAP - prep.hot_words
+
 
 MAX_LEVELS_INT8 = 127     # levels an int8 bin row holds (the histogram kernels' bound)
 MAX_LEVELS_INT16 = 32767  # levels of int16 rows (the occupied-key kernels)
@@ -572,6 +600,8 @@ def _digest(a) -> bytes | None:
 
 def get_plan(prep: Prepared, targets=None, sources=None, cap_wave: int = 0,
              cap_block: int = 0, src_skip=None) -> Plan:
+    if cap_wave == 0 and prep.hot_words:
+        cap_wave = wave_cap(prep)
     key = ("te", _digest(prep.npv_host), int(prep.L), str(prep.device),
            _digest(None if targets is None else np.asarray(targets, np.int32)),
            _digest(None if sources is None else np.asarray(sources, np.int32)),
@@ -609,13 +639,27 @@ def persistent_ok(prep: Prepared, plan: Plan) -> bool:
     if os.environ.get("TNX_PERSIST", "1") == "0" or prep.L > PERSIST_MAX_L:
         return False
     i = plan.info
-    slice_words = _wave_slice(int(i.cap_wave))
-    if 4 * slice_words * 4 > PERSIST_MAX_SLICES_BYTES:
+    if 4 * _wave_slice(_wave_area(prep, int(i.cap_wave))) * 4 > PERSIST_MAX_SLICES_BYTES:
         return False
     if i.n_block_items == 0:
         return True
+    return block_cap(prep, plan) >= max(int(i.max_block_cols), 4)
+
+
+def _wave_area(prep: Prepared, cap_wave: int) -> int:
+    """Words of a wave slice before its 64 lane words (te_args.hpp wave_area)."""
+    cw, mb = prep.hot
+    return ((cap_wave + 3) & ~3) + 4 * cw * mb if cw else cap_wave
+
+
+def block_cap(prep: Prepared, plan: Plan) -> int:
+    """Histogram cells of the persistent launch's block role (its four wave
+    slices less the widest marginal, two rows of lane words and the hot
+    cells): a block pair of more cells runs in several passes."""
+    i = plan.info
     mc = max(int(i.max_block_cols), 1)
-    return 4 * slice_words - mc - 128 >= max(mc, 4)
+    return (4 * _wave_slice(_wave_area(prep, int(i.cap_wave))) - mc - 128
+            - 4 * HOT_CB * prep.hot[1]) & ~3
 
 
 def plan_queue(plan: Plan, npv_host: np.ndarray, L: int, item_per_wave: bool):
@@ -681,7 +725,7 @@ def te_args(prep: Prepared, plan: Plan, out: torch.Tensor | None, out_mode: int,
         queue_ctr=ptr(q[2]), counts_map=ptr(cm), counts_dbg=ptr(cd),
         counts_ld=int(cd.shape[1]) if cd is not None else 0,
         xlogx_fx=ptr(xlogx_fx_table(prep.L, prep.device)), fx_shift=fx_shift(prep.L),
-        var_L=ptr(var_L), cap_sort=int(cap_sort))
+        var_L=ptr(var_L), cap_sort=int(cap_sort), hot_cw=prep.hot[0], hot_maxb=prep.hot[1])
 
 
 def te_out_args(prep: Prepared, out: torch.Tensor, out_mode: int, ld_out: int) -> _lib.TeArgs:
@@ -993,7 +1037,7 @@ def te_pair_list(prep: Prepared, pairs: np.ndarray, cap_wave: int = 0) -> torch.
         gene_prep_into(sub, features=False)
         loc = np.searchsorted(involved, pv)
     plan = te_pairlist_plan(sub.npv_host, np.unique(loc, axis=0), prep.device,
-                            cap_wave or WAVE_CAP)
+                            cap_wave or wave_cap(sub))
     M = torch.zeros((m, m), dtype=torch.float32, device=prep.device)
     te_run(sub, plan, M, _lib.TNX_OUT_MATRIX, ld_out=m)
     lt = h2d(loc, prep.device)
@@ -1092,7 +1136,7 @@ def counts_debug(prep: Prepared, pairs: np.ndarray, max_cells: int = 8192):
     than the persistent launch takes use tnx_te_counts_debug (wave path)."""
     pairs_np = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
     if prep.L <= PERSIST_MAX_L and os.environ.get("TNX_PERSIST", "1") != "0":
-        plan = te_pairlist_plan(prep.npv_host, pairs_np, prep.device)
+        plan = te_pairlist_plan(prep.npv_host, pairs_np, prep.device, wave_cap(prep))
         if persistent_ok(prep, plan):
             counts, m = counts_plan(prep, plan, pairs_np, max_cells)
             te = m[h2d(pairs_np[:, 0], prep.device), h2d(pairs_np[:, 1], prep.device)]
@@ -1625,7 +1669,7 @@ def surrogate_stats_pairs(prep: Prepared, pairs: np.ndarray, observed: torch.Ten
         raise ValueError("surrogate_stats_pairs: pairs must be distinct")
     sub = var_subset(prep, involved)
     vd = sub.plans["rows"]
-    plan = te_pairlist_plan(sub.npv_host, loc, dev)
+    plan = te_pairlist_plan(sub.npv_host, loc, dev, wave_cap(sub))
     s = torch.zeros((m, m), dtype=torch.float64, device=dev)
     s2 = torch.zeros_like(s)
     cg = None

@@ -25,6 +25,22 @@ constexpr int kChunkSrc = 64;        // sources per locality chunk
 // words, rounded to 16 bytes (the copy fold reads four copies at a time).
 __host__ __device__ inline int wave_slice(int cap) { return (cap + kWave + 3) & ~3; }
 
+// Hot target states (persistent launch, hot_cw > 0): words of a wave slice's
+// hot cells (4 states x hot_maxb source levels x hot_cw copies), and the
+// words before the slice's 64 lane words (histogram, then the hot cells
+// from a 16-byte boundary).
+constexpr int kHotCB = 16;  // block-role copies of a hot cell
+__host__ __device__ inline int hot_wave_words(const tnx_te_args_t &a) {
+    return a.hot_cw > 0 ? 4 * a.hot_cw * a.hot_maxb : 0;
+}
+__host__ __device__ inline int hot_block_words(const tnx_te_args_t &a) {
+    return a.hot_cw > 0 ? 4 * kHotCB * a.hot_maxb : 0;
+}
+__host__ __device__ inline int hot_base(const tnx_te_args_t &a) { return (a.cap_wave + 3) & ~3; }
+__host__ __device__ inline int wave_area(const tnx_te_args_t &a) {
+    return a.hot_cw > 0 ? hot_base(a) + hot_wave_words(a) : a.cap_wave;
+}
+
 // Item of workgroup slot w in [0, n): workgroup w runs on XCD w % 8, and XCD
 // x takes the x-th contiguous slice of the items, in order (a bijection of
 // [0, n)).

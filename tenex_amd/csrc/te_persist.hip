@@ -81,18 +81,56 @@ __device__ __forceinline__ void load_target_buf(TargetRegs<NI> &r, __amdgpu_buff
     }
 }
 
+// Hot target states (HS): a target's states (a, b) = (x_{t+tau}, x_t) with
+// a < 2 and b < 2 hold most samples of zero-inflated data, so many lanes of
+// a wave would add to the same few cells.  Their samples go to lane-private
+// copies instead: in the target's registers such a state a * b_x + b
+// becomes 0x8000 | (h << csh), h = 2a + b, and source level c of it is
+// counted in copy r = lane mod C of hot cell (c, h), word (4c + h) C + r of
+// the hot cells (csh = 2 + log2 C, so h << csh is its byte offset).  Other
+// states stay below 0x8000 (b_x <= 127).  The copies are summed into the
+// histogram before the reduction, so the counts are unchanged.
+__device__ __forceinline__ uint32_t hot_encode16(uint32_t v, uint32_t bx, uint32_t csh) {
+    const uint32_t d = v - bx;  // a = 1: d = b
+    const uint32_t h = v < 2u ? v : (d < 2u ? 2u + d : 4u);
+    return h < 4u ? (0x8000u | (h << csh)) : v;
+}
+
+template <int NR>
+__device__ __forceinline__ void hot_encode_regs(uint32_t (&s)[NR], int bx, uint32_t csh) {
+#pragma unroll
+    for (int k = 0; k < NR; k++)
+        s[k] = hot_encode16(s[k] & 0xFFFFu, (uint32_t)bx, csh) |
+               (hot_encode16(s[k] >> 16, (uint32_t)bx, csh) << 16);
+}
+
+// Byte offset of sample (state sv, source level yv) from the lane's
+// histogram base: the cell (kmul = 4 G b_y, ysh = 2 + log2 G), or for a hot
+// state its hot copy (hmul = 16 C bytes per source level; hadj = the lane's
+// copy offset from the base, less the 0x8000 flag that sv carries).
+struct HotK {
+    uint32_t hmul, hadj;
+};
+template <bool HS>
+__device__ __forceinline__ uint32_t key_off(uint32_t sv, uint32_t yv, uint32_t kmul,
+                                            uint32_t ysh, const HotK &hk) {
+    const uint32_t off = __umul24(sv, kmul) + (yv << ysh);
+    if constexpr (!HS) return off;
+    const uint32_t hof = __umul24(yv, hk.hmul) + sv + hk.hadj;
+    return sv >= 0x8000u ? hof : off;
+}
+
 // Count chunks [C0, C0 + CN) of the pair (target states in regs, source
 // chunks in ys; C0 is a constant once the caller's group loop is unrolled)
-// into the privatised histogram.  Only the chunk holding sample L - 1 can be
-// partial; its samples past L are not predicated one by one (per-sample exec
-// masks cost scalar registers the register-resident target cannot spare)
-// but sent by a select: with HZ to the lane's key-0 word, whose surplus is
-// taken off cell 0 after the fold (pad_count); without HZ to `trash`, a
-// lane-private word the caller clears.  Chunks wholly
-// past L, or past NI, are skipped (wave-uniform).
-template <int NI, int CN, bool HZ>
+// into the privatised histogram (with HS, hot states into their copies).
+// Only the chunk holding sample L - 1 can be partial; its samples past L
+// are not predicated one by one (per-sample exec masks cost scalar
+// registers the register-resident target cannot spare) but sent by a select
+// to `trash`, a lane-private word the caller clears.  Chunks wholly past L,
+// or past NI, are skipped (wave-uniform).
+template <int NI, int CN, bool HS>
 __device__ __forceinline__ void count_group(uint32_t *hist, TargetRegs<NI> &r, const uint4 *ys,
-                                            int C0, uint32_t kmul, uint32_t ysh, uint32_t *zero,
+                                            int C0, uint32_t kmul, uint32_t ysh, const HotK &hk,
                                             uint32_t trash_off, int L, int lane) {
 #pragma unroll
     for (int k = 0; k < NI * 8; k++) asm volatile("" : "+v"(r.s[k]));
@@ -107,34 +145,23 @@ __device__ __forceinline__ void count_group(uint32_t *hist, TargetRegs<NI> &r, c
             for (int q = 0; q < 16; q++) {
                 const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
                 const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                lds_inc_key0<HZ>(hist, sv, kmul, yv, ysh, zero);
+                lds_inc(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) +
+                                                     key_off<HS>(sv, yv, kmul, ysh, hk)));
             }
         } else {
             // samples past L (state 0 past L, but the source bytes between
-            // L and T are real): with HZ to the lane's key-0 word, taken off
-            // cell 0 after the fold; without HZ to the trash word
+            // L and T are real) go to the trash word
             const int nv = L - (16 * lane + 1024 * i);  // valid samples of this lane's 16
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const uint32_t sv = (sreg[8 * i + (q >> 1)] >> (16 * (q & 1))) & 0xFFFFu;
                 const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                const uint32_t off = __umul24(sv, kmul) + (yv << ysh);
-                if constexpr (HZ) {
-                    const uint32_t o = q < nv ? off : 0u;
-                    lds_inc(o ? reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) + o)
-                              : zero);
-                } else {
-                    lds_inc(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) +
-                                                         (q < nv ? off : trash_off)));
-                }
+                const uint32_t off = key_off<HS>(sv, yv, kmul, ysh, hk);
+                lds_inc(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) +
+                                                     (q < nv ? off : trash_off)));
             }
         }
     }
-}
-
-// Samples past L that a register-resident pass counts in the key-0 words (HZ).
-__device__ __forceinline__ uint32_t pad_count(int L) {
-    return (uint32_t)(((L + 1023) / 1024) * 1024 - L);
 }
 
 // Source-row pipeline of a wave pair: NG groups of GS chunks in two
@@ -151,13 +178,13 @@ struct SrcPipe {
     static constexpr int GS = (NI + NG - 1) / NG;
 };
 
-template <int NI, int G, bool HZ>
+template <int NI, int G, bool HS>
 __device__ __forceinline__ void pipe_count(uint4 (&b0)[SrcPipe<NI>::GS],
                                            uint4 (&b1)[SrcPipe<NI>::GS], TargetRegs<NI> &regs,
                                            __amdgpu_buffer_rsrc_t yr, __amdgpu_buffer_rsrc_t yr_next,
                                            bool more, bool run,
                                            uint32_t *hc, uint32_t kmul, uint32_t ysh,
-                                           uint32_t *zero, uint32_t trash_off, int L, int lane) {
+                                           const HotK &hk, uint32_t trash_off, int L, int lane) {
     constexpr int NG = SrcPipe<NI>::NG, GS = SrcPipe<NI>::GS;
     if constexpr (G < NG) {
         uint4(&cur)[GS] = (G & 1) ? b1 : b0;
@@ -167,8 +194,8 @@ __device__ __forceinline__ void pipe_count(uint4 (&b0)[SrcPipe<NI>::GS],
         } else {
             if (more) load_group<GS>(nxt, yr_next, 0, lane);
         }
-        if (run) count_group<NI, GS, HZ>(hc, regs, cur, G * GS, kmul, ysh, zero, trash_off, L, lane);
-        pipe_count<NI, G + 1, HZ>(b0, b1, regs, yr, yr_next, more, run, hc, kmul, ysh, zero,
+        if (run) count_group<NI, GS, HS>(hc, regs, cur, G * GS, kmul, ysh, hk, trash_off, L, lane);
+        pipe_count<NI, G + 1, HS>(b0, b1, regs, yr, yr_next, more, run, hc, kmul, ysh, hk,
                                   trash_off, L, lane);
     }
 }
@@ -333,7 +360,7 @@ __device__ __forceinline__ void dump_wave_counts(const tnx_te_args_t &a, int x, 
 // The pairs q = pfirst, pfirst + pstep, ... of a wave item (pair q is
 // target tgt_order[tgt_beg + q % cnt] against source src_order[src_beg +
 // q / cnt]), one wave.
-template <int NI, bool HZ>
+template <int NI, bool HS>
 __device__ __forceinline__ void persist_wave_item(const tnx_te_args_t &a, const tnx_work_t it,
                                                   int pfirst, int pstep, uint32_t *hist,
                                                   uint32_t *zero_row, const XfLookup &xf, int L,
@@ -374,6 +401,7 @@ __device__ __forceinline__ void persist_wave_item(const tnx_te_args_t &a, const 
             }
             load_target_buf<NI>(regs, row_rsrc(a.states + (int64_t)x * a.lds, 2u * (uint32_t)a.lds),
                                 Lx, lane);
+            if constexpr (HS) hot_encode_regs(regs.s, bx, 2u + (uint32_t)__builtin_ctz(a.hot_cw));
             s12 = a.S1[x] - a.S2a[x];
             cur_x = x;
         }
@@ -383,22 +411,38 @@ __device__ __forceinline__ void persist_wave_item(const tnx_te_args_t &a, const 
         const bool fits = cells <= a.cap_wave;
         const bool run = want && fits;
         const WaveCopies cp = wave_copies(cells, a.cap_wave, TNX_WAVE_COPY_BUDGET);
-        uint32_t *hc = hist + (lane & (cp.G - 1));
+        const int gl = lane & (cp.G - 1);
+        uint32_t *hc = hist + gl;
         const uint32_t kmul = (uint32_t)by << (2 + cp.gsh), ysh = 2 + cp.gsh;
-        // without HZ the lane's key-0 word is free: padded samples go there
-        const uint32_t trash_off = (uint32_t)(a.cap_wave + lane - (lane & (cp.G - 1))) * 4u;
-        pipe_count<NI, 0, HZ>(b0, b1, regs, row_rsrc(a.bins + (int64_t)y * a.ld, row_bytes),
+        // the lane's word after the slice's histogram (and hot cells) takes
+        // the padded samples
+        const uint32_t trash_off = (uint32_t)((zero_row - hist) + lane - gl) * 4u;
+        HotK hk{0u, 0u};
+        if constexpr (HS)
+            hk = HotK{16u * (uint32_t)a.hot_cw,
+                      (uint32_t)(hot_base(a) + (lane & (a.hot_cw - 1)) - gl) * 4u - 0x8000u};
+        pipe_count<NI, 0, HS>(b0, b1, regs, row_rsrc(a.bins + (int64_t)y * a.ld, row_bytes),
                               row_rsrc(a.bins + (int64_t)yn * a.ld, row_bytes), more, run, hc,
-                              kmul, ysh, zero_row + lane, trash_off, Lx, lane);
+                              kmul, ysh, hk, trash_off, Lx, lane);
         if (run) {
             wave_lds_fence();
-            if constexpr (HZ) {
-                fold_zero_key<HZ>(hist, zero_row, lane);
-                if (lane == 0) hist[0] -= pad_count(Lx);
+            if constexpr (HS) {  // hot copies into copy 0 of their cells (empty until now)
+                uint32_t *hot = hist + hot_base(a);
+                const int C = a.hot_cw;
+                for (int m = lane; m < 4 * by; m += kWave) {
+                    uint4 *q = reinterpret_cast<uint4 *>(hot + m * C);
+                    uint32_t v = 0;
+                    for (int r = 0; r < C / 4; r++) {
+                        const uint4 u = q[r];
+                        q[r] = make_uint4(0u, 0u, 0u, 0u);
+                        v += (u.x + u.y) + (u.z + u.w);
+                    }
+                    const int h = m & 3, svh = (h >> 1) * bx + (h & 1);
+                    if (v) hist[(svh * by + (m >> 2)) << cp.gsh] += v;
+                }
                 wave_lds_fence();
-            } else {
-                zero_row[lane] = 0u;  // the padded samples' counts
             }
+            zero_row[lane] = 0u;  // the padded samples' counts
             if (a.counts_map) dump_wave_counts(a, x, y, hist, cells, cp, lane);
             const int64_t d = reduce_wave_fx(hist, bx, by, cp, xf, lane);
             wave_lds_fence();
@@ -411,18 +455,20 @@ __device__ __forceinline__ void persist_wave_item(const tnx_te_args_t &a, const 
 }
 
 // One pair, whole workgroup (hist_pair_block's passes over ranges of
-// x_{t+tau}) with a branch-free accumulate and a cell-parallel scan: samples of the pair's other passes and
-// samples past L are sent to `trash`, a thread-private word, by a select
-// instead of an exec mask per sample.  hist, c2b and zero_row are zero on
-// entry and exit; trash holds garbage (the caller clears it before the
-// region serves wave slices again).
-template <int NB, bool HZ>
-__device__ __forceinline__ int64_t block_pair(uint32_t *hist, uint32_t *c2b, uint32_t *zero_row,
+// x_{t+tau}) with a branch-free accumulate and a cell-parallel scan: samples
+// of the pair's other passes and samples past L are sent to `trash`, a
+// thread-private word, by a select instead of an exec mask per sample.
+// With HS the hot states' samples are counted in the first pass, in kHotCB
+// copies of their hot cells at `hot`, and each pass adds the copies of its
+// hot cells to its histogram before the scan.  hist, c2b and the hot cells
+// are zero on entry and exit; trash holds garbage (the caller clears it
+// before the region serves wave slices again).
+template <int NB, bool HS>
+__device__ __forceinline__ int64_t block_pair(uint32_t *hist, uint32_t *c2b, uint32_t *hot,
                                          uint32_t *trash, int cap, const TargetRegs<NB> &regs,
                                          const uint4 *ys, int bx, int by, int L,
                                          const XfLookup &xf, int64_t *red, int32_t *cdbg) {
     const int tid = threadIdx.x, nth = kPersistThreads;
-    uint32_t *zero = zero_row + (tid & 63);
     const int ncol = bx * by;
     const uint32_t by4 = (uint32_t)by << 2;
     int apass = cap / ncol;
@@ -432,10 +478,13 @@ __device__ __forceinline__ int64_t block_pair(uint32_t *hist, uint32_t *c2b, uin
         const int a1 = (a0 + apass < bx) ? (a0 + apass) : bx;
         const uint32_t lo = (uint32_t)(a0 * bx), span = (uint32_t)((a1 - a0) * bx);
         uint32_t *hc = hist - a0 * ncol;  // cell index of the full pair
-        // byte offsets from hc of this thread's trash and key-0 words
+        // byte offset from hc of this thread's trash word
         const uint32_t trash_rel = (uint32_t)((trash + (tid & 63)) - hc) * 4u;
-        const uint32_t zero_rel = (uint32_t)(zero - hc) * 4u;
         const bool single = a1 - a0 == bx;
+        HotK hk{0u, 0u};
+        if constexpr (HS)
+            hk = HotK{16u * kHotCB,
+                      (uint32_t)((hot + (tid & (kHotCB - 1))) - hc) * 4u - 0x8000u};
 #pragma unroll
         for (int i = 0; i < NB; i++) {
             const int c0 = 16 * nth * i;  // first sample of chunk row i
@@ -449,12 +498,15 @@ __device__ __forceinline__ int64_t block_pair(uint32_t *hist, uint32_t *c2b, uin
             asm volatile("" : "+v"(yw[0]), "+v"(yw[1]), "+v"(yw[2]), "+v"(yw[3]));
             asm volatile("" : "+v"(sr[0]), "+v"(sr[1]), "+v"(sr[2]), "+v"(sr[3]), "+v"(sr[4]),
                          "+v"(sr[5]), "+v"(sr[6]), "+v"(sr[7]));
+            // a wave whose 1024 samples of this row all lie past L skips it
+            if (c0 + 16 * (tid & ~(kWave - 1)) >= L) continue;  // wave-uniform
             if (single && c0 + 16 * nth <= L) {
 #pragma unroll
                 for (int q = 0; q < 16; q++) {
                     const uint32_t sv = (sr[q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
                     const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-                    lds_inc_key0<HZ>(hc, sv, by4, yv, 2u, zero);
+                    lds_inc(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hc) +
+                                                         key_off<HS>(sv, yv, by4, 2u, hk)));
                 }
             } else {
                 const int nv = L - (16 * tid + c0);  // valid samples of this thread's 16
@@ -464,14 +516,28 @@ __device__ __forceinline__ int64_t block_pair(uint32_t *hist, uint32_t *c2b, uin
                     const uint32_t yv = (yw[q >> 2] >> (8 * (q & 3))) & 0xFFu;
                     const uint32_t off = __umul24(sv, by4) + (yv << 2);
                     uint32_t rel = (sv - lo < span && q < nv) ? off : trash_rel;
-                    if (HZ) rel = rel ? rel : zero_rel;  // key 0 (first pass only)
+                    if constexpr (HS) {  // hot states: counted once, in the first pass
+                        const uint32_t hof = __umul24(yv, hk.hmul) + sv + hk.hadj;
+                        rel = (a0 == 0 && sv >= 0x8000u && q < nv) ? hof : rel;
+                    }
                     lds_inc(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hc) + rel));
                 }
             }
         }
         __syncthreads();
-        if (HZ && a0 == 0) {  // key 0 only occurs in the first pass
-            if (tid < kWave) fold_zero_key<HZ>(hist, zero_row, tid);
+        if constexpr (HS) {  // this pass's hot cells: copies summed into the histogram
+            const int m = tid, h = m & 3, ah = h >> 1;
+            if (m < 4 * by && ah >= a0 && ah < a1) {
+                uint4 *q = reinterpret_cast<uint4 *>(hot + m * kHotCB);
+                uint32_t v = 0;
+#pragma unroll
+                for (int r = 0; r < kHotCB / 4; r++) {
+                    const uint4 u = q[r];
+                    q[r] = make_uint4(0u, 0u, 0u, 0u);
+                    v += (u.x + u.y) + (u.z + u.w);
+                }
+                if (v) hist[(ah * bx + (h & 1)) * by + (m >> 2) - a0 * ncol] += v;
+            }
             __syncthreads();
         }
         if (cdbg) {  // test hook: the pass's counts, read-only, before the scan clears them
@@ -543,7 +609,7 @@ __device__ __forceinline__ void load_row256(uint4 *ys, const int8_t *__restrict_
 // A block item: its targets one by one against sources [src_beg, src_end),
 // the whole workgroup on each pair (hist_pair_block), the next source row
 // loaded while the current pair is counted.
-template <int NI, bool HZ>
+template <int NI, bool HS>
 __device__ __forceinline__ void persist_block_item(const tnx_te_args_t &a, const tnx_work_t it,
                                                    uint32_t *region, int capB,
                                                    const XfLookup &xf, int64_t *red, int L,
@@ -551,9 +617,9 @@ __device__ __forceinline__ void persist_block_item(const tnx_te_args_t &a, const
     constexpr int NB = (NI + 3) / 4;  // 16-sample chunks per thread (4096 samples per chunk row)
     const int tid = threadIdx.x;
     uint32_t *hist = region;
-    uint32_t *c2b = region + capB;
-    uint32_t *zero_row = c2b + a.max_block_cols;
-    uint32_t *trash = zero_row + kWave;
+    uint32_t *hot = region + capB;  // hot cells (HS): 4 states x b_y levels x kHotCB copies
+    uint32_t *c2b = hot + hot_block_words(a);
+    uint32_t *trash = c2b + a.max_block_cols + kWave;
     TargetRegs<NB> regs;
     uint4 ycur[NB], ynext[NB];
     for (int ti = 0; ti < it.tgt_cnt; ti++) {
@@ -574,6 +640,7 @@ __device__ __forceinline__ void persist_block_item(const tnx_te_args_t &a, const
                 regs.s[8 * i + 6] = v.z; regs.s[8 * i + 7] = v.w;
             }
         }
+        if constexpr (HS) hot_encode_regs(regs.s, bx, 2u + (uint32_t)__builtin_ctz(kHotCB));
         const double s12 = a.S1[x] - a.S2a[x];
         int y = a.src_order[it.src_beg];
         int by = a.n_per_var[y];
@@ -596,7 +663,7 @@ __device__ __forceinline__ void persist_block_item(const tnx_te_args_t &a, const
                     if (r >= 0 && (int64_t)bx * bx * by <= a.counts_ld)
                         cdbg = a.counts_dbg + (int64_t)r * a.counts_ld;
                 }
-                const int64_t d = block_pair<NB, HZ>(hist, c2b, zero_row, trash, capB, regs, ycur,
+                const int64_t d = block_pair<NB, HS>(hist, c2b, hot, trash, capB, regs, ycur,
                                                      bx, by, Lx, xf, red, cdbg);
                 if (tid == 0) emit_te(a, slot, (ldexp((double)d, -a.fx_shift) + s12) * invNx);
             } else if (want && !fits && tid == 0) {
@@ -628,7 +695,7 @@ __device__ __forceinline__ int take_entry(const tnx_te_args_t &a, int q0, int ho
     return -1;
 }
 
-template <int NI, bool HZ>
+template <int NI, bool HS>
 __global__ __launch_bounds__(kPersistThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void
 te_persist_kernel(tnx_te_args_t a) {
     // 16-byte aligned: the copy fold reads four copies per ds_read_b128
@@ -644,10 +711,11 @@ te_persist_kernel(tnx_te_args_t a) {
     const double invN = 1.0 / (double)L;
     for (int i = tid; i < kSmallTbl; i += kPersistThreads) lfx[i] = i <= L ? a.xlogx_fx[i] : 0;
     const XfLookup xf{lfx, row_rsrc(a.xlogx_fx, (uint32_t)(8 * (L + 1)))};
-    const int slice = wave_slice(a.cap_wave);
+    const int slice = wave_slice(wave_area(a));
     for (int i = tid; i < kWavesPerWG * slice; i += kPersistThreads) region[i] = 0;
     __syncthreads();
-    const int capB = (kWavesPerWG * slice - a.max_block_cols - 2 * kWave) & ~3;
+    const int capB =
+        (kWavesPerWG * slice - a.max_block_cols - 2 * kWave - hot_block_words(a)) & ~3;
     const int home = (int)(blockIdx.x % TNX_QUEUES);  // this workgroup's XCD
     // Block entries: the whole workgroup takes one at a time (thread 0 takes,
     // LDS broadcast), its own XCD's block queue first.
@@ -659,7 +727,7 @@ te_persist_kernel(tnx_te_args_t a) {
             const int e = __builtin_amdgcn_readfirstlane(s_entry);  // uniform: scalar loads
             __syncthreads();  // s_entry is rewritten by the next take
             if (e < 0) break;
-            persist_block_item<NI, HZ>(a, a.work[a.n_wave_items + e], region, capB, xf, red, L,
+            persist_block_item<NI, HS>(a, a.work[a.n_wave_items + e], region, capB, xf, red, L,
                                        invN);
         }
     };
@@ -672,7 +740,7 @@ te_persist_kernel(tnx_te_args_t a) {
     // item_per_wave, item e whole).
     {
         uint32_t *hist = region + wave * slice;
-        uint32_t *zero_row = hist + a.cap_wave;
+        uint32_t *zero_row = hist + wave_area(a);  // the lanes' trash words
         const bool per_wave = a.item_per_wave != 0;
         int tries = 0;
         while (true) {
@@ -682,7 +750,7 @@ te_persist_kernel(tnx_te_args_t a) {
             tries = __builtin_amdgcn_readfirstlane(tries);
             if (e < 0) break;
             const int64_t wi = per_wave ? e : e / kWavesPerWG;
-            persist_wave_item<NI, HZ>(a, a.work[wi], per_wave ? 0 : e % kWavesPerWG,
+            persist_wave_item<NI, HS>(a, a.work[wi], per_wave ? 0 : e % kWavesPerWG,
                                       per_wave ? 1 : kWavesPerWG, hist, zero_row, xf, L, invN,
                                       lane);
         }
@@ -717,11 +785,11 @@ static int persist_grid(const void *kfn, size_t shm) {
     return grid;
 }
 
-template <bool HZ>
+template <bool HS>
 static const void *persist_fn(int ni) {
     switch (ni) {
 #define TNX_PERSIST_FN(NI) \
-    case NI: return (const void *)te_persist_kernel<NI, HZ>;
+    case NI: return (const void *)te_persist_kernel<NI, HS>;
         TNX_PERSIST_FN(1) TNX_PERSIST_FN(2) TNX_PERSIST_FN(3) TNX_PERSIST_FN(4)
         TNX_PERSIST_FN(6) TNX_PERSIST_FN(8) TNX_PERSIST_FN(10) TNX_PERSIST_FN(12)
 #undef TNX_PERSIST_FN
@@ -736,19 +804,24 @@ int launch_persist(const tnx_te_args_t &a, int ni, hipStream_t s) {
     TNX_CHECK_ARG(ni >= 1 && ni <= 12 && a.T - a.tau <= kPersistMaxL,
                   "tnx_te_run: the persistent launch takes T - tau <= %lld",
                   (long long)kPersistMaxL);
-    const size_t slice = (size_t)wave_slice(a.cap_wave);
+    TNX_CHECK_ARG(a.hot_cw == 0 || ((a.hot_cw == 4 || a.hot_cw == 8 || a.hot_cw == 16) &&
+                                    a.hot_maxb >= 1 && a.hot_maxb <= 32),
+                  "tnx_te_run: hot_cw must be 0, 4, 8 or 16 and 1 <= hot_maxb <= 32 (got %d, %d)",
+                  a.hot_cw, a.hot_maxb);
+    const size_t slice = (size_t)wave_slice(wave_area(a));
     const size_t shm = (size_t)kWavesPerWG * slice * 4;  // + the static 2 KiB table
     TNX_CHECK_ARG(shm <= 156 * 1024,
                   "tnx_te_run: cap_wave %d too large for the persistent launch", a.cap_wave);
     if (a.n_block_items > 0)
         TNX_CHECK_ARG(a.max_block_cols >= 1 &&
-                          kWavesPerWG * (int64_t)slice - a.max_block_cols - 2 * kWave >=
+                          kWavesPerWG * (int64_t)slice - a.max_block_cols - 2 * kWave -
+                                  hot_block_words(a) >=
                               std::max<int64_t>(a.max_block_cols, 4),
                       "tnx_te_run: max_block_cols %d too large for the persistent launch",
                       a.max_block_cols);
     TNX_CHECK_ARG(!a.counts_map || (a.counts_dbg && a.counts_ld >= 1),
                   "tnx_te_run: counts_map needs counts_dbg and counts_ld");
-    const void *kfn = a.hot_zero ? persist_fn<true>(ni) : persist_fn<false>(ni);
+    const void *kfn = a.hot_cw > 0 ? persist_fn<true>(ni) : persist_fn<false>(ni);
     TNX_CHECK_ARG(kfn != nullptr, "tnx_te_run: no persistent variant for NI = %d", ni);
     TNX_CHECK_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     const int grid = persist_grid(kfn, shm);

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_error_reporting():
     lib = _lib.load()
-    assert lib.tnx_abi_version() == _lib.ABI_VERSION == 12
+    assert lib.tnx_abi_version() == _lib.ABI_VERSION == 13
     info = _lib.PlanInfo()
     rc = lib.tnx_te_plan(None, 3, 10, None, 0, None, 0, 0, 0, None, None, None, None, 0,
                          C.byref(info))

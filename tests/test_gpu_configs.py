@@ -60,7 +60,7 @@ def test_c2_mixed_cardinality_4000x10000(dev):
     rng = np.random.default_rng(11)
     cand = pairs[rng.choice(len(pairs), 400000, replace=False)]
     cells = npvh[cand[:, 0]] ** 2 * npvh[cand[:, 1]]
-    cap_b = 4 * core._wave_slice(int(plan.info.cap_wave)) - int(plan.info.max_block_cols) - 128
+    cap_b = core.block_cap(prep, plan)
     wave = cand[cells <= plan.info.cap_wave][:128]
     single = cand[(cells > plan.info.cap_wave) & (cells <= cap_b)][:96]
     multi = cand[cells > cap_b][:32]

@@ -111,7 +111,7 @@ def test_counts_dump_wave_and_block_roles(dev, hot):
     npv = prep.npv_host.astype(np.int64)
     pairs = O.pairfree_pairs(n)
     cells = npv[pairs[:, 0]] ** 2 * npv[pairs[:, 1]]
-    capB = 4 * core._wave_slice(128) - int(plan.info.max_block_cols) - 128
+    capB = core.block_cap(prep, plan)
     rng = np.random.default_rng(1)
     pick = np.concatenate([rng.choice(np.nonzero(cells <= 128)[0], 60, replace=False),
                            rng.choice(np.nonzero((cells > 128) & (cells <= capB))[0], 30,
