@@ -597,7 +597,7 @@ def run_c4(args):
            "vs_baseline": None, "dtype": "int32 counts; bf16 weights / fp32 accumulate MLP",
            "data": "synthetic zero-inflated log-normal with 100 planted TF->target edges",
            "config": {"workload": "C4 learned-null surrogate test", "n_surrogates": 100,
-                      "block_length": 50, "band": "max(1.0, 5 x held-out RMS z error)"},
+                      "block_length": 50, "band": "max(1.0, 3.5 x held-out RMS z error)"},
            "roofline": {"bound": "lds_atomic", "kernel": "plain surrogate test (the learned "
                         "test measures a fraction of these histograms)",
                         "achieved": upd / t_plain / 1e9, "peak": lds_peak / 1e9,

@@ -158,17 +158,21 @@ def upload_bins(bin_arr, device) -> tuple[torch.Tensor, torch.Tensor, int, int]:
     n, T = t.shape
     if t.is_floating_point() or t.dtype == torch.bool or t.is_complex():
         raise ValueError(f"bins must be an integer array (got {t.dtype})")
+    ld = round_up(T, ROW_ALIGN)
+    bins = torch.zeros((n, ld), dtype=torch.int8, device=device)
     if n and T:
-        lo, hi = int(t.min().item()), int(t.max().item())
+        # the range is checked on the device, after the upload, with one read back
+        td = t.to(device=device, non_blocking=t.is_pinned())
+        mm = torch.stack([td.min(), td.max()]).to(torch.int64)
+        bins[:, :T] = td.to(torch.int8)
+        npv = (bins[:, :T].to(torch.int32).amax(dim=1) + 1).to(torch.int32)
+        lo, hi, b_max = (int(v) for v in torch.cat([mm, npv.max().view(1).to(torch.int64)]).cpu())
         if lo < 0 or hi > 126:
             raise ValueError(f"bins must satisfy 0 <= bin < n_per_var <= 127 (got values in "
                              f"[{lo}, {hi}]); int8 bins cannot hold them")
-    ld = round_up(T, ROW_ALIGN)
-    bins = torch.zeros((n, ld), dtype=torch.int8, device=device)
-    bins[:, :T] = t.to(device=device, dtype=torch.int8)
-    npv = (bins[:, :T].to(torch.int32).amax(dim=1) + 1).to(torch.int32) if T else \
-        torch.ones(n, dtype=torch.int32, device=device)
-    b_max = int(npv.max().item()) if n else 0
+    else:
+        npv = torch.ones(n, dtype=torch.int32, device=device)
+        b_max = 1 if n else 0
     return bins, npv.contiguous(), b_max, T
 
 

@@ -257,9 +257,9 @@ def _tested_pairs(rows: np.ndarray, cols: np.ndarray, device=None):
 
 def learned_surrogate_test(prep, observed_te, obs_d, variable_names, n_surrogates,
                            shuffle_method, block_length, seed, rng_device, p_method, fdr, mask,
-                           src_idx, dev, train_frac: float = 0.025, band: float = 1.0,
+                           src_idx, dev, train_frac: float = 0.015, band: float = 1.0,
                            steps: int = 90, lr: float = 1e-2, min_train: int = 2000,
-                           model_seed: int = 0, band_sigmas: float = 5.0,
+                           model_seed: int = 0, band_sigmas: float = 3.5,
                            fit_batch: int = 4096, tested=None):
     """Learned-null surrogate test; returns a SurrogateTestResult with an
     extra `learned_info` dict (training size, re-run size, timings, losses).

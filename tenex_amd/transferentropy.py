@@ -335,7 +335,8 @@ class TransferEntropyEngine:
         bin_arrs = n_per_var = None
         if prep is not None:
             src = prep.bins16 if prep.bins16 is not None else prep.bins
-            bin_arrs = src[:, : prep.T].cpu().numpy()[:, :, None]
+            # pinned host memory: an inference step uploads these bins again
+            bin_arrs = core.d2h_pinned(src[:, : prep.T].contiguous())[:, :, None]
             n_per_var = prep.npv_host.copy()
             if getattr(self, "_last_report_npv", None) is not None:
                 n_per_var = self._last_report_npv.copy()   # binarized: max - min + 1
