@@ -365,3 +365,20 @@ def test_learned_null_requires_parametric_p_values():
               kernel="Adaptive-SMEM", learned="mlp", p_method="mc")
     with pytest.raises(ValueError, match="parametric"):
         SurrogateTestMethod().infer(np.zeros((3, 3), np.float32), names, None, **kw)
+
+
+def test_upload_bins_rejects_out_of_range_levels():
+    """Bins outside 0 <= bin <= 126 cannot be int8 histogram keys: upload_bins
+    raises instead of passing wrapped bytes to the kernels (checked here with
+    the host as the device)."""
+    from tenex_amd import core
+    ok = np.array([[0, 1, 2, 1], [3, 0, 0, 1]], np.int32)
+    bins, npv, b_max, T = core.upload_bins(ok, "cpu")
+    assert T == 4 and b_max == 4 and npv.tolist() == [3, 4]
+    assert np.array_equal(bins[:, :4].numpy(), ok)
+    for bad in (np.array([[0, -1, 2]], np.int32), np.array([[0, 127, 2]], np.int32),
+                np.array([[0, 200, 1]], np.int64)):
+        with pytest.raises(ValueError, match="int8 bins cannot hold"):
+            core.upload_bins(bad, "cpu")
+    with pytest.raises(ValueError, match="integer"):
+        core.upload_bins(np.zeros((2, 3), np.float32), "cpu")
