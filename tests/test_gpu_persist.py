@@ -183,3 +183,37 @@ def test_persistent_graph_replays(dev):
         got = out.clone()
         eager = core.te_matrix_internal(prep, cap_wave=1536)
         assert torch.equal(got, eager), rep
+
+
+@pytest.mark.parametrize("cap_wave", [0, 256])
+def test_hot_states_edge_levels(dev, cap_wave):
+    """Hot states with targets of one and two levels (every state hot, or
+    the only state hot), sources of one level, a short last chunk, and with a
+    small wave histogram the block role in one and several passes: the hot
+    kernels equal the plain ones bit for bit and the FP64 oracle within
+    1e-6."""
+    n, T = 40, 2345
+    d, nb, npv, bm = O.discretize(synthetic.zero_inflated_lognormal(n, T, seed=21))
+    d = d.astype(np.int32)
+    d[0] = 0                                   # constant (one level)
+    d[1] = (np.arange(T) % 3 == 0)             # two levels
+    d[2] = (np.arange(T) % 7 == 0) * 2         # levels {0, 2}: relabelled to {0, 1} below
+    d[2] = np.unique(d[2], return_inverse=True)[1]
+    prep = _prep(d, dev)
+    assert prep.hot_zero and prep.hot[0] > 0
+    plan = core.get_plan(prep, cap_wave=cap_wave)
+    assert core.persistent_ok(prep, plan)
+    if cap_wave:
+        npvh = prep.npv_host.astype(np.int64)
+        cells = npvh[:, None] ** 2 * npvh[None, :]
+        assert plan.info.n_block_items > 0 and (cells > core.block_cap(prep, plan)).any()
+    m_hot = _matrix(prep, plan, True)
+    prep.hot_zero = False
+    assert prep.hot == (0, 0)
+    plan0 = core.get_plan(prep, cap_wave=cap_wave)
+    m_plain = _matrix(prep, plan0, True)
+    assert np.array_equal(m_hot, m_plain)
+    pairs = O.pairfree_pairs(n)
+    te64 = O.te_pairs(d, prep.npv_host, pairs)
+    err = np.abs(m_hot[pairs[:, 0], pairs[:, 1]].astype(np.float64) - te64)
+    assert err.max() < TOL, err.max()
