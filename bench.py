@@ -271,8 +271,10 @@ def run_c2(args):
     # states 2 B / sample, source 1 B / sample) + 4 B per TE written
     alg_bytes = n * L * 3 + 4 * n_pairs
     te_host = out.cpu().numpy()
-    kname = (f"te_persist_kernel<{-(-L // 1024)}, {str(prep.hot_zero).lower()}>" if persistent
+    kname = (f"te_persist_kernel<{-(-L // 1024)}, {str(prep.hot[0] > 0).lower()}>" if persistent
              else "te_wave_reg_kernel + te_block_kernel")
+    if oh is not None:   # the step's TE launches: the one-hot split, then the histogram launch
+        kname = "oh_pack_kernel + oh_mfma_kernel, then " + kname
     result = {
         "metric": "TE gene-pairs/sec at 4k genes x 10k cells (mixed-cardinality, all ordered pairs)",
         "value": n_pairs / dt, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
