@@ -217,22 +217,25 @@ struct XfLookup {
 // cannot be merged with the LDS read into a generic (flat) load.
 template <int N>
 __device__ __forceinline__ int64_t xf_sum(const XfLookup &xf, const uint32_t (&c)[N]) {
+    // every count reads entry min(c, kSmallTbl - 1) from LDS (one min, no
+    // compare and select); a count >= kSmallTbl then swaps that entry for
+    // its own, read from memory
     int64_t s = 0;
-    bool big = false;
+    uint32_t any = 0;
 #pragma unroll
     for (int u = 0; u < N; u++) {
-        const bool small = c[u] < (uint32_t)kSmallTbl;
-        s += xf.lfx[small ? c[u] : 0u];
-        big |= !small;
+        s += xf.lfx[min(c[u], (uint32_t)kSmallTbl - 1u)];
+        any |= c[u];
     }
-    if (__builtin_amdgcn_ballot_w64(big)) {  // wave-uniform
+    if (__builtin_amdgcn_ballot_w64(any >= (uint32_t)kSmallTbl)) {  // wave-uniform
         int64_t w[N];
 #pragma unroll
         for (int u = 0; u < N; u++)
             w[u] = (int64_t)__builtin_amdgcn_raw_buffer_load_b64(
                 xf.gfx, (int)(c[u] < (uint32_t)kSmallTbl ? 0x80000000u : c[u] * 8u), 0, 0);
 #pragma unroll
-        for (int u = 0; u < N; u++) s += w[u];
+        for (int u = 0; u < N; u++)
+            s += c[u] < (uint32_t)kSmallTbl ? 0 : w[u] - xf.lfx[kSmallTbl - 1];
     }
     return s;
 }
