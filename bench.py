@@ -894,8 +894,10 @@ def main():
                     help="c2: skip the continuous-data variant (discretize + TE matrix)")
     ap.add_argument("--no-graph", action="store_true",
                     help="c2: launch the step eagerly instead of replaying it from a HIP graph")
-    ap.add_argument("--autotune", action="store_true",
-                    help="c2: time the launch candidates on a sample first (core.autotune)")
+    ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
+                    help="c2: time the launch candidates on a sample before the timed "
+                         "region (core.autotune; it keeps the defaults unless a candidate "
+                         "is 2%% faster)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run one C2 matrix and a 256 MiB copy (for rocprofv3 --pmc), no JSON")
     args = ap.parse_args()
