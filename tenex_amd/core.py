@@ -777,7 +777,7 @@ _TUNE_CACHE: dict = {}
 def autotune(prep: Prepared, targets=None, sources=None, sample_targets: int = 256,
              reps: int = 3, min_gain: float = 0.02) -> dict:
     """Launch parameters for the TE kernels on prep, chosen by timing: each
-    (cap_wave, cap_block) of TUNE_CAPS, with and without the key-0
+    (cap_wave, cap_block) of TUNE_CAPS, with and without the hot-state (key-0)
     diversion, runs the plan of an evenly spaced sample of at most
     sample_targets targets against all sources; a candidate replaces the
     defaults only when it is min_gain faster.  Cached per (levels, L,
@@ -799,7 +799,9 @@ def autotune(prep: Prepared, targets=None, sources=None, sample_targets: int = 2
         for hz in (hz0, not hz0):
             prep.hot_zero = hz
             for cw, cb in TUNE_CAPS:
-                plan = make_plan(prep.npv_host, prep.L, prep.device, tg, sources, cw, cb)
+                # cw 0: the default cap, less the hot cells' words when they are on
+                plan = make_plan(prep.npv_host, prep.L, prep.device, tg, sources,
+                                 cw or wave_cap(prep), cb)
                 args = te_args(prep, plan, out, _lib.TNX_OUT_MATRIX, ld_out=prep.n_vars)
                 te_run(prep, plan, out, _lib.TNX_OUT_MATRIX, args=args)
                 e0 = torch.cuda.Event(enable_timing=True)
